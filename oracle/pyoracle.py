@@ -1,0 +1,206 @@
+"""ctypes binding of the CPU restatement in cntc_oracle.c.
+
+Test infrastructure only: used by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg as the checker.  The product (contact_amd) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libcntc_oracle.so")
+
+EXTER, ADHES, SLIP, PLAST = 0, 1, 2, 3
+
+
+class OrcInput(ctypes.Structure):
+    _fields_ = [
+        ("tang", ctypes.c_int), ("norm", ctypes.c_int), ("pvtime", ctypes.c_int),
+        ("iestim", ctypes.c_int), ("gausei", ctypes.c_int),
+        ("poiss1", ctypes.c_double), ("poiss2", ctypes.c_double),
+        ("gg1", ctypes.c_double), ("gg2", ctypes.c_double),
+        ("fstat", ctypes.c_double),
+        ("ipotcn", ctypes.c_int), ("mx", ctypes.c_int), ("my", ctypes.c_int),
+        ("xc1", ctypes.c_double), ("yc1", ctypes.c_double),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+        ("hz_a1", ctypes.c_double), ("hz_b1", ctypes.c_double),
+        ("hz_aa", ctypes.c_double), ("hz_bb", ctypes.c_double), ("hz_scale", ctypes.c_double),
+        ("ibase", ctypes.c_int), ("prmudf", ctypes.c_double * 6),
+        ("h_elem", ctypes.POINTER(ctypes.c_double)),
+        ("pen", ctypes.c_double), ("fn", ctypes.c_double), ("cksi", ctypes.c_double),
+        ("ceta", ctypes.c_double), ("cphi", ctypes.c_double), ("chi", ctypes.c_double),
+        ("dq", ctypes.c_double), ("veloc", ctypes.c_double),
+        ("maxgs", ctypes.c_int), ("maxin", ctypes.c_int), ("maxnr", ctypes.c_int),
+        ("maxout", ctypes.c_int), ("eps", ctypes.c_double),
+        ("fdecay", ctypes.c_double), ("betath", ctypes.c_double), ("d_ifc", ctypes.c_double),
+        ("d_lin", ctypes.c_double), ("d_cns", ctypes.c_double), ("d_slp", ctypes.c_double),
+        ("pow_s", ctypes.c_double), ("kdowfb", ctypes.c_int),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        L.orc_new.restype = vp
+        L.orc_free.argtypes = [vp]
+        L.orc_default_input.argtypes = [ctypes.POINTER(OrcInput)]
+        L.orc_calculate.argtypes = [vp, ctypes.POINTER(OrcInput)]
+        L.orc_calculate.restype = ctypes.c_int
+        L.orc_get_mx.argtypes = [vp]
+        L.orc_get_my.argtypes = [vp]
+        L.orc_get_grid.argtypes = [vp, dp, dp, dp, dp]
+        L.orc_get_forces.argtypes = [vp, dp, dp, dp, dp]
+        L.orc_get_tractions.argtypes = [vp, dp, dp, dp]
+        L.orc_get_slip.argtypes = [vp, dp, dp]
+        L.orc_get_eldiv.argtypes = [vp, ip]
+        L.orc_get_h.argtypes = [vp, dp]
+        L.orc_get_pen.argtypes = [vp]
+        L.orc_get_pen.restype = ctypes.c_double
+        L.orc_get_hertz.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.orc_get_iters.argtypes = [vp, ip, ip, ip, ip]
+        L.orc_opt_fft_size.argtypes = [ctypes.c_int]
+        L.orc_opt_fft_size.restype = ctypes.c_int
+        L.orc_elascf_pcwcns.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, dp]
+        L.orc_conv_block.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, ctypes.c_int]
+        L.orc_make_prec_block.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, dp]
+        L.orc_ellip_bd.argtypes = [ctypes.c_double, dp, dp]
+        L.orc_fft.argtypes = [ctypes.c_int, dp, dp, dp, dp, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def default_input():
+    inp = OrcInput()
+    lib().orc_default_input(ctypes.byref(inp))
+    return inp
+
+
+class Problem:
+    """One (ire, icp) contact problem solved by the CPU restatement."""
+
+    def __init__(self):
+        self._p = lib().orc_new()
+        self._h = None
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            lib().orc_free(self._p)
+            self._p = None
+
+    def calculate(self, inp, h_elem=None):
+        if h_elem is not None:
+            self._h = np.ascontiguousarray(h_elem, dtype=np.float64)
+            inp.h_elem = _dptr(self._h)
+        return lib().orc_calculate(self._p, ctypes.byref(inp))
+
+    @property
+    def shape(self):
+        return lib().orc_get_mx(self._p), lib().orc_get_my(self._p)
+
+    def forces(self):
+        v = [ctypes.c_double() for _ in range(4)]
+        lib().orc_get_forces(self._p, *[ctypes.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def tractions(self):
+        mx, my = self.shape
+        pn, px, py = (np.zeros(mx * my) for _ in range(3))
+        lib().orc_get_tractions(self._p, _dptr(pn), _dptr(px), _dptr(py))
+        return pn, px, py
+
+    def slip(self):
+        mx, my = self.shape
+        sx, sy = np.zeros(mx * my), np.zeros(mx * my)
+        lib().orc_get_slip(self._p, _dptr(sx), _dptr(sy))
+        return sx, sy
+
+    def eldiv(self):
+        mx, my = self.shape
+        el = np.zeros(mx * my, dtype=np.int32)
+        lib().orc_get_eldiv(self._p, el.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return el
+
+    def h(self):
+        mx, my = self.shape
+        h = np.zeros(mx * my)
+        lib().orc_get_h(self._p, _dptr(h))
+        return h
+
+    def pen(self):
+        return lib().orc_get_pen(self._p)
+
+    def grid(self):
+        v = [ctypes.c_double() for _ in range(4)]
+        lib().orc_get_grid(self._p, *[ctypes.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def hertz(self):
+        v = [ctypes.c_double() for _ in range(6)]
+        lib().orc_get_hertz(self._p, *[ctypes.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def iters(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        lib().orc_get_iters(self._p, *[ctypes.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+
+def opt_fft_size(n):
+    return lib().orc_opt_fft_size(int(n))
+
+
+def elascf_pcwcns(akv, nuv, mx, my, dx, dy, xshft=0.0, yshft=0.0):
+    cf = np.zeros(9 * (2 * mx) * (2 * my))
+    lib().orc_elascf_pcwcns(akv, nuv, mx, my, dx, dy, xshft, yshft, _dptr(cf))
+    return cf.reshape(3, 3, 2 * my, 2 * mx)
+
+
+def conv_block(cfblock, p, method=1):
+    my2, mx2 = cfblock.shape
+    mx, my = mx2 // 2, my2 // 2
+    cfb = np.ascontiguousarray(cfblock, dtype=np.float64)
+    pp = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+    u = np.zeros(mx * my)
+    lib().orc_conv_block(mx, my, _dptr(cfb), _dptr(pp), _dptr(u), method)
+    return u
+
+
+def make_prec_block(cfblock, ga):
+    my2, mx2 = cfblock.shape
+    cfb = np.ascontiguousarray(cfblock, dtype=np.float64)
+    m = np.zeros_like(cfb)
+    lib().orc_make_prec_block(mx2 // 2, my2 // 2, ga, _dptr(cfb), _dptr(m))
+    return m
+
+
+def ellip_bd(mc):
+    b, d = ctypes.c_double(), ctypes.c_double()
+    lib().orc_ellip_bd(mc, ctypes.byref(b), ctypes.byref(d))
+    return b.value, d.value
+
+
+def fft(x, sign=-1):
+    x = np.asarray(x, dtype=np.complex128)
+    re, im = np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag)
+    ro, io = np.zeros(len(x)), np.zeros(len(x))
+    lib().orc_fft(len(x), _dptr(re), _dptr(im), _dptr(ro), _dptr(io), sign)
+    return ro + 1j * io
