@@ -1,0 +1,273 @@
+"""ctypes mirror of the reference's python_intfc over libcntc_amd.so.
+
+Function names, arguments and return values follow python_intfc/cntc_*.py (module 3);
+arrays come back as numpy arrays of shape (my, mx) like the reference wrapper.  The library
+must have been built (contact_amd/build_lib.py); loading fails loudly otherwise.
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, "libcntc_amd.so")
+
+CNTC = dict(if_units=1933, un_cntc=1934, un_spck=1935, un_si=1936, ic_pvtime=1970,
+            ic_bound=1971, ic_tang=1972, ic_norm=1973, ic_force=1974, ic_iestim=1983,
+            ic_output=1984, ic_flow=1985, if_idebug=2000, if_precision=2009,
+            fld_h=1, fld_mu=2, fld_px=3, fld_py=4, fld_pn=5, fld_sx=15, fld_sy=16,
+            err_allow=-12, err_norm=-27, err_tang=-28, err_icp=-31, err_other=-99)
+
+_lib = None
+_pi, _pd = POINTER(c_int), POINTER(c_double)
+
+_PROTOS = {
+    "cntc_initializefirst": [_pi, _pi, _pi, c_char_p, c_char_p, c_char_p, _pi, _pi, _pi],
+    "cntc_initialize": [_pi, _pi, _pi, _pi, c_char_p, _pi],
+    "cntc_setglobalflags": [_pi, _pi, _pi],
+    "cntc_setflags": [_pi, _pi, _pi, _pi, _pi],
+    "cntc_setsolverflags": [_pi, _pi, _pi, _pi, _pi, _pi, _pd],
+    "cntc_setmaterialparameters": [_pi, _pi, _pi, _pi, _pd],
+    "cntc_settimestep": [_pi, _pi, _pd],
+    "cntc_setreferencevelocity": [_pi, _pi, _pd],
+    "cntc_setrollingstepsize": [_pi, _pi, _pd, _pd],
+    "cntc_setfrictionmethod": [_pi, _pi, _pi, _pi, _pd],
+    "cntc_sethertzcontact": [_pi, _pi, _pi, _pi, _pd],
+    "cntc_setpotcontact": [_pi, _pi, _pi, _pi, _pd],
+    "cntc_setpenetration": [_pi, _pi, _pd],
+    "cntc_setnormalforce": [_pi, _pi, _pd],
+    "cntc_setundeformeddistc": [_pi, _pi, _pi, _pi, _pd],
+    "cntc_setcreepages": [_pi, _pi, _pd, _pd, _pd],
+    "cntc_calculate": [_pi, _pi, _pi],
+    "cntc_calculatebatch": [_pi, _pi, _pi, _pi],
+    "cntc_gethertzcontact": [_pi, _pi, _pi, _pd],
+    "cntc_getnumelements": [_pi, _pi, _pi, _pi],
+    "cntc_getgriddiscretization": [_pi, _pi, _pd, _pd],
+    "cntc_getpotcontact": [_pi, _pi, _pi, _pd],
+    "cntc_getpenetration": [_pi, _pi, _pd],
+    "cntc_getcreepages": [_pi, _pi, _pd, _pd, _pd],
+    "cntc_getcontactforces": [_pi, _pi, _pd, _pd, _pd, _pd],
+    "cntc_getcontactpatchareas": [_pi, _pi, _pd, _pd, _pd],
+    "cntc_getelementdivision": [_pi, _pi, _pi, _pi],
+    "cntc_getmaximumpressure": [_pi, _pi, _pd],
+    "cntc_getmaximumtraction": [_pi, _pi, _pd],
+    "cntc_getfielddata": [_pi, _pi, _pi, _pi, _pd],
+    "cntc_gettractions": [_pi, _pi, _pi, _pd, _pd, _pd],
+    "cntc_getmicroslip": [_pi, _pi, _pi, _pd, _pd],
+    "cntc_getcalculationtime": [_pi, _pi, _pd, _pd],
+    "cntc_resetcalculationtime": [_pi, _pi],
+    "cntc_finalize": [_pi],
+    "cntc_finalizelast": [],
+}
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise RuntimeError("libcntc_amd.so is not built: run contact_amd/build_lib.py")
+        L = ctypes.CDLL(library_path)
+        for name, args in _PROTOS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = None
+        _lib = L
+    return _lib
+
+
+def _i(v):
+    return ctypes.byref(c_int(int(v)))
+
+
+def _d(v):
+    return ctypes.byref(c_double(float(v)))
+
+
+def _iarr(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_pi)
+
+
+def _darr(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_pd)
+
+
+def initialize(ire, imodul=3, outdir=" ", idebug=1):
+    ifcver, ierr = c_int(0), c_int(0)
+    s = outdir.encode()
+    lib().cntc_initialize(_i(ire), _i(imodul), ctypes.byref(ifcver), ctypes.byref(ierr), s, _i(len(s)))
+    return [ifcver.value, ierr.value]
+
+
+def setglobalflags(params, values):
+    p, pp = _iarr(params)
+    v, vp = _iarr(values)
+    lib().cntc_setglobalflags(_i(len(p)), pp, vp)
+
+
+def setflags(ire, icp, params, values):
+    p, pp = _iarr(params)
+    v, vp = _iarr(values)
+    lib().cntc_setflags(_i(ire), _i(icp), _i(len(p)), pp, vp)
+
+
+def setsolverflags(ire, icp, imeth, iparam, rparam):
+    ia, iap = _iarr(iparam)
+    ra, rap = _darr(rparam)
+    lib().cntc_setsolverflags(_i(ire), _i(icp), _i(imeth), _i(len(ia)), iap, _i(len(ra)), rap)
+
+
+def setmaterialparameters(ire, icp, m_digit, rparam):
+    ra, rap = _darr(rparam)
+    lib().cntc_setmaterialparameters(_i(ire), _i(icp), _i(m_digit), _i(len(ra)), rap)
+
+
+def settimestep(ire, icp, dt):
+    lib().cntc_settimestep(_i(ire), _i(icp), _d(dt))
+
+
+def setreferencevelocity(ire, icp, veloc):
+    lib().cntc_setreferencevelocity(_i(ire), _i(icp), _d(veloc))
+
+
+def setrollingstepsize(ire, icp, chi, dq):
+    lib().cntc_setrollingstepsize(_i(ire), _i(icp), _d(chi), _d(dq))
+
+
+def setfrictionmethod(ire, icp, imeth, params):
+    ra, rap = _darr(params)
+    lib().cntc_setfrictionmethod(_i(ire), _i(icp), _i(imeth), _i(len(ra)), rap)
+
+
+def sethertzcontact(ire, icp, ipotcn, params):
+    ra, rap = _darr(params)
+    lib().cntc_sethertzcontact(_i(ire), _i(icp), _i(ipotcn), _i(len(ra)), rap)
+
+
+def setpotcontact(ire, icp, ipotcn, params):
+    ra, rap = _darr(params)
+    lib().cntc_setpotcontact(_i(ire), _i(icp), _i(ipotcn), _i(len(ra)), rap)
+
+
+def setpenetration(ire, icp, pen):
+    lib().cntc_setpenetration(_i(ire), _i(icp), _d(pen))
+
+
+def setnormalforce(ire, icp, fn):
+    lib().cntc_setnormalforce(_i(ire), _i(icp), _d(fn))
+
+
+def setundeformeddistc(ire, icp, ibase, prmudf):
+    ra, rap = _darr(np.ravel(prmudf))
+    lib().cntc_setundeformeddistc(_i(ire), _i(icp), _i(ibase), _i(len(ra)), rap)
+
+
+def setcreepages(ire, icp, vx, vy, phi):
+    lib().cntc_setcreepages(_i(ire), _i(icp), _d(vx), _d(vy), _d(phi))
+
+
+def calculate(ire=1, icp=1, idebug=1):
+    ierr = c_int(-999)
+    lib().cntc_calculate(_i(ire), _i(icp), ctypes.byref(ierr))
+    return ierr.value
+
+
+def calculatebatch(ire_list, icp_list):
+    """Solve the cases of several (ire, icp) problems in one batched GPU run."""
+    ire, irp = _iarr(ire_list)
+    icp, icpp = _iarr(icp_list)
+    err = np.zeros(len(ire), dtype=np.int32)
+    lib().cntc_calculatebatch(_i(len(ire)), irp, icpp, err.ctypes.data_as(_pi))
+    return err
+
+
+def gethertzcontact(ire=1, icp=1, lenarr=7):
+    v = np.zeros(lenarr)
+    lib().cntc_gethertzcontact(_i(ire), _i(icp), _i(lenarr), v.ctypes.data_as(_pd))
+    return v
+
+
+def getnumelements(ire=1, icp=1):
+    mx, my = c_int(0), c_int(0)
+    lib().cntc_getnumelements(_i(ire), _i(icp), ctypes.byref(mx), ctypes.byref(my))
+    return mx.value, my.value
+
+
+def getgriddiscretization(ire=1, icp=1):
+    dx, dy = c_double(0), c_double(0)
+    lib().cntc_getgriddiscretization(_i(ire), _i(icp), ctypes.byref(dx), ctypes.byref(dy))
+    return dx.value, dy.value
+
+
+def getpenetration(ire=1, icp=1):
+    v = c_double(0)
+    lib().cntc_getpenetration(_i(ire), _i(icp), ctypes.byref(v))
+    return v.value
+
+
+def getcontactforces(ire=1, icp=1):
+    v = [c_double(0) for _ in range(4)]
+    lib().cntc_getcontactforces(_i(ire), _i(icp), *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
+def getcontactpatchareas(ire=1, icp=1):
+    v = [c_double(0) for _ in range(3)]
+    lib().cntc_getcontactpatchareas(_i(ire), _i(icp), *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
+def getelementdivision(ire=1, icp=1):
+    mx, my = getnumelements(ire, icp)
+    el = np.zeros(mx * my, dtype=np.int32)
+    lib().cntc_getelementdivision(_i(ire), _i(icp), _i(mx * my), el.ctypes.data_as(_pi))
+    return el.reshape(my, mx)
+
+
+def getmaximumpressure(ire=1, icp=1):
+    v = c_double(0)
+    lib().cntc_getmaximumpressure(_i(ire), _i(icp), ctypes.byref(v))
+    return v.value
+
+
+def getfielddata(ire, icp, ifld):
+    mx, my = getnumelements(ire, icp)
+    v = np.zeros(mx * my)
+    lib().cntc_getfielddata(_i(ire), _i(icp), _i(ifld), _i(mx * my), v.ctypes.data_as(_pd))
+    return v.reshape(my, mx)
+
+
+def gettractions(ire=1, icp=1):
+    mx, my = getnumelements(ire, icp)
+    pn, px, py = (np.zeros(mx * my) for _ in range(3))
+    lib().cntc_gettractions(_i(ire), _i(icp), _i(mx * my), pn.ctypes.data_as(_pd),
+                            px.ctypes.data_as(_pd), py.ctypes.data_as(_pd))
+    return pn.reshape(my, mx), px.reshape(my, mx), py.reshape(my, mx)
+
+
+def getmicroslip(ire=1, icp=1):
+    mx, my = getnumelements(ire, icp)
+    sx, sy = np.zeros(mx * my), np.zeros(mx * my)
+    lib().cntc_getmicroslip(_i(ire), _i(icp), _i(mx * my), sx.ctypes.data_as(_pd), sy.ctypes.data_as(_pd))
+    return sx.reshape(my, mx), sy.reshape(my, mx)
+
+
+def getcalculationtime(ire=1, icp=1):
+    c, w = c_double(0), c_double(0)
+    lib().cntc_getcalculationtime(_i(ire), _i(icp), ctypes.byref(c), ctypes.byref(w))
+    return c.value, w.value
+
+
+def resetcalculationtime(ire=1, icp=1):
+    lib().cntc_resetcalculationtime(_i(ire), _i(icp))
+
+
+def finalize(ire):
+    lib().cntc_finalize(_i(ire))
+
+
+def finalizelast():
+    lib().cntc_finalizelast()

@@ -1,0 +1,626 @@
+// cntc_api.cpp -- the cntc_* C ABI (module-3 subset) over the HIP engine.
+//
+// Problem store, unit conversion and argument checks follow src/contact_addon.f90; every
+// numerical step runs on the GPU through cntc::Engine.  Calls are serialised by one mutex
+// (the engine owns a single stream); problems of different (ire, icp) keep separate state.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <vector>
+
+#include "../../include/cntc_amd.h"
+#include "engine.hpp"
+
+namespace {
+
+using cntc::CaseInput;
+using cntc::Engine;
+using cntc::Patch;
+
+// unit conventions, src/m_global_data.f90:463-526
+struct Scaling {
+    double len = 1, area = 1, forc = 1, veloc = 1, angle = 1, body = 1;
+};
+
+Scaling select_units(int iunits)
+{
+    Scaling s;
+    if (iunits == CNTC_un_spck) { s.len = 1e3; s.area = 1e6; s.veloc = 1e3; s.body = -1.0; }
+    else if (iunits == CNTC_un_si) { s.len = 1e3; s.area = 1e6; s.veloc = 1e3; }
+    return s;
+}
+
+struct Problem {
+    Patch patch;
+    Scaling scl;
+    double tcpu = 0, twall = 0;
+    bool calculated = false;
+};
+
+struct Store {
+    std::mutex mtx;
+    std::unique_ptr<Engine> engine;
+    std::map<std::pair<int, int>, std::unique_ptr<Problem>> probs;
+    std::map<int, int> ire_module;
+    int precision = 0;
+    int idebug = 0;
+};
+
+Store& store()
+{
+    static Store s;
+    return s;
+}
+
+const int MAX_REID = 999, MAX_CPID = 9;
+
+// cntc_activate, src/m_caddon_data.f90:89-225: auto-initialise, check ids
+Problem* activate(int ire, int icp)
+{
+    Store& S = store();
+    if (ire <= 0 || ire > MAX_REID || icp <= 0 || icp > MAX_CPID) return nullptr;
+    auto m = S.ire_module.find(ire);
+    if (m == S.ire_module.end()) S.ire_module[ire] = 3;
+    else if (m->second != 3) return nullptr;
+    auto key = std::make_pair(ire, icp);
+    auto it = S.probs.find(key);
+    if (it == S.probs.end()) {
+        std::unique_ptr<Problem> p(new Problem());
+        p->patch.in.precision = S.precision;
+        it = S.probs.emplace(key, std::move(p)).first;
+    }
+    return it->second.get();
+}
+
+Engine* engine()
+{
+    Store& S = store();
+    if (!S.engine) {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return nullptr;
+        S.engine.reset(new Engine());
+    }
+    return S.engine.get();
+}
+
+}  // namespace
+
+extern "C" {
+
+void cntc_initializefirst(int* ifcver, int* ierror, int* ioutput, const char*, const char*, const char*,
+                          int*, int*, int*)
+{
+    (void)ioutput;
+    if (ifcver) *ifcver = 2400;
+    if (ierror) *ierror = 0;
+}
+
+void cntc_initialize(int* ire, int* imodul, int* ifcver, int* ierror, const char*, int*)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    if (ifcver) *ifcver = 2400;
+    if (!ire || !imodul || *ire <= 0 || *ire > MAX_REID || *imodul != 3) {
+        if (ierror) *ierror = CNTC_err_other;
+        return;
+    }
+    store().ire_module[*ire] = 3;
+    if (ierror) *ierror = 0;
+}
+
+void cntc_setglobalflags(int* lenflg, int* params, int* values)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    for (int i = 0; i < *lenflg; i++) {
+        if (params[i] == CNTC_if_idebug) store().idebug = values[i];
+        if (params[i] == CNTC_if_precision) store().precision = values[i] ? 1 : 0;
+    }
+}
+
+// src/contact_addon.f90:780-973
+void cntc_setflags(int* ire, int* icp, int* lenflg, int* params, int* values)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    CaseInput& in = P->patch.in;
+    for (int i = 0; i < *lenflg; i++) {
+        int v = values[i];
+        switch (params[i]) {
+        case CNTC_if_units: P->scl = select_units(v); break;
+        case CNTC_ic_pvtime: in.pvtime = std::max(0, std::min(3, v)); break;
+        case CNTC_ic_tang: in.tang = std::max(0, std::min(3, v)); break;
+        case CNTC_ic_norm: in.norm = std::max(0, std::min(1, v)); break;
+        case CNTC_ic_iestim: in.iestim = std::max(0, std::min(3, v)); break;
+        case CNTC_if_precision: in.precision = v ? 1 : 0; break;
+        default: break;   // output, flow, file flags: no effect on the solution
+        }
+    }
+}
+
+// src/contact_addon.f90:1128-1283
+void cntc_setsolverflags(int* ire, int* icp, int* gdigit, int* nints, int* iparam, int* nreals, double* rparam)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    static const int nints_loc[7] = {4, 0, 5, 5, 5, 5, 1}, nreals_loc[7] = {1, 0, 4, 4, 2, 8, 1};
+    int gd = *gdigit;
+    if (gd < 0 || gd > 6) return;
+    if (*nints != nints_loc[gd] || *nreals != nreals_loc[gd]) return;
+    CaseInput& in = P->patch.in;
+    if (gd != 1 && gd != 6) {
+        in.gausei = gd;
+        in.eps = std::max(1e-20, rparam[0]);
+        in.maxgs = std::max(1, iparam[0]);
+        in.maxin = std::max(1, iparam[1]);
+        in.maxnr = std::max(1, iparam[2]);
+        in.maxout = std::max(1, iparam[3]);
+        if (gd == 5) {
+            in.fdecay = rparam[1];
+            in.betath = rparam[2];
+            in.kdowfb = iparam[4];
+            in.d_ifc = std::max(0.01, rparam[3]);
+            in.d_lin = rparam[4];
+            in.d_cns = std::max(0.01, rparam[5]);
+            in.d_slp = std::max(0.01, rparam[6]);
+            in.pow_s = std::max(0.01, std::min(10.0, rparam[7]));
+            if (in.d_lin * (in.d_cns - in.d_ifc) < 0.0) {
+                in.d_lin = 0.0;
+                in.d_cns = in.d_ifc;
+            }
+        }
+    }
+}
+
+// src/contact_addon.f90:1287-1473
+void cntc_setmaterialparameters(int* ire, int* icp, int* mdigit, int* nparam, double* rparam)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P || *mdigit != 0 || *nparam != 4) return;
+    CaseInput& in = P->patch.in;
+    in.poiss1 = rparam[0];
+    in.poiss2 = rparam[1];
+    in.gg1 = rparam[2] * P->scl.forc / P->scl.area;
+    in.gg2 = rparam[3] * P->scl.forc / P->scl.area;
+}
+
+void cntc_settimestep(int* ire, int* icp, double* dt)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    (void)activate(*ire, *icp);
+    (void)dt;
+}
+
+void cntc_setreferencevelocity(int* ire, int* icp, double* veloc)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    P->patch.in.veloc = std::max(1e-9, std::fabs(*veloc) * P->scl.veloc);
+}
+
+void cntc_setrollingstepsize(int* ire, int* icp, double* chi, double* dq)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    P->patch.in.chi = *chi * P->scl.angle;
+    P->patch.in.dq = *dq * P->scl.len;
+}
+
+// src/contact_addon.f90:1671+ (L=0: fstat, fkin)
+void cntc_setfrictionmethod(int* ire, int* icp, int* imeth, int* nparam, double* params)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P || *imeth != 0 || *nparam != 2) return;
+    P->patch.in.fstat = params[0];
+}
+
+// src/contact_addon.f90:1972-2093
+void cntc_sethertzcontact(int* ire, int* icp, int* ipotcn, int* nparam, double* params)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P || *nparam != 5) return;
+    CaseInput& in = P->patch.in;
+    double L = P->scl.len;
+    if (*ipotcn == -3) {
+        in.mx = std::max(1, (int)std::lround(params[0]));
+        in.my = std::max(1, (int)std::lround(params[1]));
+        in.hz_aa = std::max(1e-6, params[2]) * L;
+        in.hz_bb = std::max(1e-6, params[3]) * L;
+        in.hz_scale = std::max(1e-6, params[4]);
+    } else if (*ipotcn == -1) {
+        in.mx = std::max(1, (int)std::lround(params[0]));
+        in.my = std::max(1, (int)std::lround(params[1]));
+        in.hz_a1 = std::max(1e-12, params[2]) / L;
+        in.hz_b1 = std::max(1e-12, params[3]) / L;
+        in.hz_scale = std::max(1e-6, params[4]);
+    } else {
+        return;
+    }
+    in.ipotcn = *ipotcn;
+}
+
+// src/contact_addon.f90:2097-2257
+void cntc_setpotcontact(int* ire, int* icp, int* ipotcn, int* nparam, double* params)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P || *ipotcn < 1 || *ipotcn > 4 || *nparam != 6) return;
+    CaseInput& in = P->patch.in;
+    double L = P->scl.len;
+    in.mx = std::max(1, (int)std::lround(params[0]));
+    in.my = std::max(1, (int)std::lround(params[1]));
+    if (*ipotcn == 1) {
+        in.xl = params[2] * L; in.yl = params[3] * L;
+        in.dx = std::max(1e-12, params[4]) * L; in.dy = std::max(1e-12, params[5]) * L;
+    } else if (*ipotcn == 2) {
+        in.xl = params[2] * L; in.yl = params[3] * L; in.xh = params[4] * L; in.yh = params[5] * L;
+    } else if (*ipotcn == 3) {
+        in.xc1 = params[2] * L; in.yc1 = params[3] * L;
+        in.dx = std::max(1e-12, params[4]) * L; in.dy = std::max(1e-12, params[5]) * L;
+    } else {
+        in.xc1 = params[2] * L; in.yc1 = params[3] * L; in.xcm = params[4] * L; in.ycm = params[5] * L;
+    }
+    in.ipotcn = *ipotcn;
+}
+
+void cntc_setpenetration(int* ire, int* icp, double* pen)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    P->patch.in.norm = 0;
+    P->patch.in.pen = *pen * P->scl.len;
+}
+
+void cntc_setnormalforce(int* ire, int* icp, double* fn)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    P->patch.in.norm = 1;
+    P->patch.in.fn = *fn;
+}
+
+// src/contact_addon.f90:2361-2487
+void cntc_setundeformeddistc(int* ire, int* icp, int* ibase, int* nparam, double* prmudf)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    CaseInput& in = P->patch.in;
+    double L = P->scl.len;
+    if (*ibase == 1) {
+        if (*nparam != 6) return;
+        for (int k = 0; k < 3; k++) in.prmudf[k] = prmudf[k] / L;
+        in.prmudf[3] = prmudf[3];
+        in.prmudf[4] = prmudf[4];
+        in.prmudf[5] = prmudf[5] * L;
+    } else if (*ibase == 9) {
+        if (*nparam != in.mx * in.my) return;
+        in.h_elem.assign(prmudf, prmudf + *nparam);
+        for (auto& v : in.h_elem) v *= L;
+    } else {
+        return;
+    }
+    in.ibase = *ibase;
+}
+
+// src/contact_addon.f90:2491-2530
+void cntc_setcreepages(int* ire, int* icp, double* vx, double* vy, double* phi)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    CaseInput& in = P->patch.in;
+    if (in.tang == 1) {
+        in.cksi = *vx * P->scl.len;
+        in.ceta = *vy * P->scl.len;
+        in.cphi = *phi;
+    } else {
+        in.cksi = *vx;
+        in.ceta = *vy;
+        in.cphi = *phi / P->scl.len;
+    }
+}
+
+static void calculate_many(int n, const int* ire, const int* icp, int* ierror)
+{
+    Store& S = store();
+    Engine* E = engine();
+    std::vector<Patch*> ps;
+    std::vector<Problem*> pr;
+    for (int k = 0; k < n; k++) {
+        Problem* P = activate(ire[k], icp[k]);
+        if (!P) { ierror[k] = CNTC_err_icp; continue; }
+        if (!E) { ierror[k] = CNTC_err_other; continue; }
+        ps.push_back(&P->patch);
+        pr.push_back(P);
+    }
+    if (!E || ps.empty()) return;
+    auto t0 = std::chrono::steady_clock::now();
+    std::clock_t c0 = std::clock();
+    try {
+        E->solve(ps);
+    } catch (const std::exception& ex) {
+        if (S.idebug > 0) std::fprintf(stderr, "cntc_calculate: %s\n", ex.what());
+        for (auto* p : ps) p->out.ierror = CNTC_err_other;
+    }
+    double tw = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    double tc = double(std::clock() - c0) / CLOCKS_PER_SEC;
+    size_t j = 0;
+    for (int k = 0; k < n; k++) {
+        Problem* P = activate(ire[k], icp[k]);
+        if (!P || j >= pr.size() || P != pr[j]) continue;
+        ierror[k] = P->patch.out.ierror;
+        P->calculated = true;
+        P->twall += tw / pr.size();
+        P->tcpu += tc / pr.size();
+        j++;
+    }
+    if (S.idebug > 0 && !E->last_error.empty()) std::fprintf(stderr, "cntc_calculate: %s\n", E->last_error.c_str());
+}
+
+void cntc_calculate(int* ire, int* icp, int* ierror)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    int e = 0;
+    calculate_many(1, ire, icp, &e);
+    *ierror = e;
+}
+
+void cntc_calculatebatch(int* nprob, const int* ire, const int* icp, int* ierror)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    for (int k = 0; k < *nprob; k++) ierror[k] = 0;
+    calculate_many(*nprob, ire, icp, ierror);
+}
+
+// ---- getters -----------------------------------------------------------------------------
+
+static Problem* solved(int ire, int icp)
+{
+    auto it = store().probs.find({ire, icp});
+    if (it == store().probs.end()) return nullptr;
+    return it->second.get();
+}
+
+void cntc_gethertzcontact(int* ire, int* icp, int* lenarr, double* v)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    const auto& o = P->patch.out;
+    double L = P->scl.len;
+    double vals[7] = {o.hz_a1 * L, o.hz_b1 * L, o.hz_aa / L, o.hz_bb / L, o.hz_rho / L, o.hz_cp / L, P->patch.in.hz_scale};
+    for (int k = 0; k < std::min(*lenarr, 7); k++) v[k] = vals[k];
+}
+
+void cntc_getnumelements(int* ire, int* icp, int* mx, int* my)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    *mx = P->patch.mx;
+    *my = P->patch.my;
+}
+
+void cntc_getgriddiscretization(int* ire, int* icp, double* dx, double* dy)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    *dx = P->patch.dx / P->scl.len;
+    *dy = P->patch.dy / P->scl.len;
+}
+
+void cntc_getpotcontact(int* ire, int* icp, int* lenarr, double* v)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    const Patch& p = P->patch;
+    double L = P->scl.len;
+    double vals[6] = {(double)p.mx, (double)p.my, p.xc1 / L, p.yc1 / L, p.dx / L, p.dy / L};
+    for (int k = 0; k < std::min(*lenarr, 6); k++) v[k] = vals[k];
+}
+
+void cntc_getpenetration(int* ire, int* icp, double* pen)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    *pen = P->patch.out.pen / P->scl.len;
+}
+
+void cntc_getcreepages(int* ire, int* icp, double* vx, double* vy, double* phi)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    const CaseInput& in = P->patch.in;
+    if (in.tang == 1) { *vx = in.cksi / P->scl.len; *vy = in.ceta / P->scl.len; *phi = in.cphi; }
+    else { *vx = in.cksi; *vy = in.ceta; *phi = in.cphi * P->scl.len; }
+}
+
+// src/contact_addon.f90:5275-5318
+void cntc_getcontactforces(int* ire, int* icp, double* fn, double* tx, double* ty, double* mz)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    const auto& o = P->patch.out;
+    *fn = o.fn;
+    *tx = P->scl.body * o.fx;
+    *ty = P->scl.body * o.fy;
+    *mz = P->scl.body * o.mz / P->scl.len;
+}
+
+static std::vector<int> eldiv_of(Problem* P)
+{
+    std::vector<int> el(P->patch.n, 0);
+    Engine* E = engine();
+    if (E && P->calculated) E->get_eldiv(P->patch, el.data());
+    return el;
+}
+
+void cntc_getcontactpatchareas(int* ire, int* icp, double* carea, double* harea, double* sarea)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    auto el = eldiv_of(P);
+    int nadh = 0, nslip = 0, nplast = 0;
+    for (int e : el) { nadh += e == 1; nslip += e == 2; nplast += e == 3; }
+    double a = P->patch.dxdy / P->scl.area;
+    *carea = (double)(float)(nadh + nslip + nplast) * a;
+    *harea = (double)(float)nadh * a;
+    *sarea = (double)(float)nslip * a;
+}
+
+void cntc_getelementdivision(int* ire, int* icp, int* lenarr, int* eldiv)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    auto el = eldiv_of(P);
+    for (int k = 0; k < std::min(*lenarr, (int)el.size()); k++) eldiv[k] = el[k];
+}
+
+static std::vector<double> field_of(Problem* P, int what)
+{
+    std::vector<double> v(P->patch.n, 0.0);
+    Engine* E = engine();
+    if (E && P->calculated) E->get_field(P->patch, what, v.data());
+    return v;
+}
+
+void cntc_getmaximumpressure(int* ire, int* icp, double* pnmax)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    auto pn = field_of(P, 0);
+    double m = 0.0;
+    for (double x : pn) m = std::max(m, std::fabs(x));
+    *pnmax = m * P->scl.area;
+}
+
+void cntc_getmaximumtraction(int* ire, int* icp, double* ptmax)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    auto px = field_of(P, 1), py = field_of(P, 2);
+    double m = -1.0;
+    for (size_t k = 0; k < px.size(); k++) m = std::max(m, std::sqrt(px[k] * px[k] + py[k] * py[k]));
+    *ptmax = m * P->scl.area;
+}
+
+// src/contact_addon.f90:5672+
+void cntc_getfielddata(int* ire, int* icp, int* ifld, int* lenarr, double* fld)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    const Scaling& s = P->scl;
+    double dq = P->patch.dq;
+    std::vector<double> v;
+    double fac = 1.0;
+    switch (*ifld) {
+    case CNTC_fld_h: v = field_of(P, 5); fac = 1.0 / s.len; break;
+    case CNTC_fld_mu: v.assign(P->patch.n, P->patch.in.fstat); break;
+    case CNTC_fld_px: v = field_of(P, 1); fac = s.area * s.body; break;
+    case CNTC_fld_py: v = field_of(P, 2); fac = s.area * s.body; break;
+    case CNTC_fld_pn: v = field_of(P, 0); fac = s.area; break;
+    case CNTC_fld_sx: v = field_of(P, 3); fac = s.body / dq; break;
+    case CNTC_fld_sy: v = field_of(P, 4); fac = s.body / dq; break;
+    default: return;
+    }
+    for (int k = 0; k < std::min(*lenarr, (int)v.size()); k++) fld[k] = fac * v[k];
+}
+
+void cntc_gettractions(int* ire, int* icp, int* lenarr, double* pn, double* px, double* py)
+{
+    int f;
+    f = CNTC_fld_pn; cntc_getfielddata(ire, icp, &f, lenarr, pn);
+    f = CNTC_fld_px; cntc_getfielddata(ire, icp, &f, lenarr, px);
+    f = CNTC_fld_py; cntc_getfielddata(ire, icp, &f, lenarr, py);
+}
+
+void cntc_getmicroslip(int* ire, int* icp, int* lenarr, double* sx, double* sy)
+{
+    int f;
+    f = CNTC_fld_sx; cntc_getfielddata(ire, icp, &f, lenarr, sx);
+    f = CNTC_fld_sy; cntc_getfielddata(ire, icp, &f, lenarr, sy);
+}
+
+void cntc_getcalculationtime(int* ire, int* icp, double* tcpu, double* twall)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) { *tcpu = *twall = 0.0; return; }
+    *tcpu = P->tcpu;
+    *twall = P->twall;
+}
+
+void cntc_resetcalculationtime(int* ire, int* icp)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    P->tcpu = P->twall = 0.0;
+}
+
+// ---- profiling (no counterpart in the reference) -----------------------------------------
+
+void cntc_amdprofile(int* mode)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Engine* E = engine();
+    if (E) E->set_profile(*mode != 0);
+}
+
+void cntc_amdprofileget(int* lenarr, int* nkinds, int* kinds, double* seconds, double* bytes, double* launches)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Engine* E = engine();
+    *nkinds = 0;
+    if (!E) return;
+    std::vector<long> l(*lenarr);
+    *nkinds = E->get_profile(*lenarr, kinds, seconds, bytes, l.data());
+    for (int k = 0; k < std::min(*nkinds, *lenarr); k++) launches[k] = (double)l[k];
+}
+
+void cntc_finalize(int* ire)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Store& S = store();
+    for (auto it = S.probs.begin(); it != S.probs.end();) {
+        if (it->first.first == *ire) it = S.probs.erase(it);
+        else ++it;
+    }
+    S.ire_module.erase(*ire);
+}
+
+void cntc_finalizelast(void)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Store& S = store();
+    S.probs.clear();
+    S.ire_module.clear();
+    S.engine.reset();
+}
+
+}  // extern "C"

@@ -1,0 +1,1828 @@
+// engine.cpp -- host orchestration of the batched HIP contact solve.
+//
+// Each patch's case runs as a C++20 coroutine whose control flow follows the reference solve
+// (contac/panprc -> snorm/normcg -> stang/gdsteady|tangcg).  Device work is requested through
+// the scheduler: requests that need no result are queued and the coroutine continues; requests
+// that return reduction values suspend it.  When every coroutine is suspended the scheduler
+// launches the queued requests level by level -- one kernel per operation kind for all patches
+// that asked for it -- then synchronises once, copies the reductions back and resumes the
+// coroutines.  A single patch and a batch of thousands run through the same code.
+#include "engine.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <coroutine>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <stdexcept>
+
+#include "kernels.hpp"
+#include "ops.hpp"
+
+namespace cntc {
+
+#define HIPCHK(x)                                                                                 \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess)                                                                     \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                                     __FILE__ + ":" + std::to_string(__LINE__));                  \
+    } while (0)
+
+static const double PI = 3.14159265358979323846264338327950288;
+static const double TINY = 1e-20;   // m_globals.f90:37
+static double sp(float x) { return (double)x; }
+
+enum { ERR_NORM = -27, ERR_TANG = -28, ERR_OTHER = -99 };
+
+DeviceArena::~DeviceArena()
+{
+    if (base) (void)hipFree(base);
+}
+
+void DeviceArena::ensure(size_t n)
+{
+    if (n <= bytes) return;
+    if (base) HIPCHK(hipFree(base));
+    base = nullptr;
+    HIPCHK(hipMalloc(&base, n));
+    bytes = n;
+}
+
+// opt_fft_size, src/m_aijpj.f90:1022-1119
+int opt_fft_size(int fft_size)
+{
+    static const int kpat[9][8] = {
+        {-2, 1, 0, 0, 0, 4, 3, 3},   {-5, 3, 0, 0, 0, 32, 27, 3}, {-1, -1, 1, 0, 0, 6, 5, 5},
+        {-4, 1, 1, 0, 0, 16, 15, 5}, {0, -3, 2, 0, 0, 27, 25, 5}, {-3, 0, 0, 1, 0, 8, 7, 7},
+        {1, -1, -1, 1, 0, 15, 14, 7}, {-1, 2, -1, 0, 0, 10, 9, 5}, {-2, -1, 0, 0, 1, 12, 11, 11}};
+    int k2 = (int)std::ceil(std::log(1.0 * fft_size) / std::log(2.0));
+    long prod = 1L << k2;
+    int kf[5] = {k2, 0, 0, 0, 0};
+    bool changed = true;
+    while (changed) {
+        changed = false;
+        for (int ip = 0; ip < 9; ip++) {
+            if (kpat[ip][5] <= 0 || kpat[ip][7] > 7) continue;
+            for (;;) {
+                bool ok = true;
+                for (int f = 0; f < 5; f++)
+                    if (kf[f] + kpat[ip][f] < 0) ok = false;
+                if (!ok || (long)kpat[ip][6] * prod < (long)kpat[ip][5] * fft_size) break;
+                for (int f = 0; f < 5; f++) kf[f] += kpat[ip][f];
+                prod = (long)kpat[ip][6] * prod / kpat[ip][5];
+                changed = true;
+            }
+        }
+    }
+    return (int)prod;
+}
+
+// ------------------------------------------------------------------------------------------
+// coroutine task
+// ------------------------------------------------------------------------------------------
+
+struct Task {
+    struct promise_type {
+        std::coroutine_handle<> cont;
+        int rc = 0;
+        Task get_return_object() { return Task{std::coroutine_handle<promise_type>::from_promise(*this)}; }
+        std::suspend_always initial_suspend() noexcept { return {}; }
+        struct Final {
+            bool await_ready() noexcept { return false; }
+            std::coroutine_handle<> await_suspend(std::coroutine_handle<promise_type> h) noexcept
+            {
+                auto c = h.promise().cont;
+                return c ? c : std::noop_coroutine();
+            }
+            void await_resume() noexcept {}
+        };
+        Final final_suspend() noexcept { return {}; }
+        void return_value(int v) { rc = v; }
+        void unhandled_exception() { std::terminate(); }
+    };
+    std::coroutine_handle<promise_type> h;
+    explicit Task(std::coroutine_handle<promise_type> hh) : h(hh) {}
+    Task(Task&& o) noexcept : h(o.h) { o.h = nullptr; }
+    Task(const Task&) = delete;
+    ~Task()
+    {
+        if (h) h.destroy();
+    }
+    bool await_ready() { return false; }
+    std::coroutine_handle<> await_suspend(std::coroutine_handle<> parent)
+    {
+        h.promise().cont = parent;
+        return h;
+    }
+    int await_resume() { return h.promise().rc; }
+};
+
+// ------------------------------------------------------------------------------------------
+// scheduler
+// ------------------------------------------------------------------------------------------
+
+union AnyItem {
+    VecItem vec;
+    NcgItem ncg;
+    AijItem aij;
+    TcgItem tcg;
+    StItem st;
+    GdItem gd;
+    ConvItem conv;
+    CoefItem coef;
+    EmbedItem emb;
+    ExtractItem ext;
+    SetupItem setup;
+    AnyItem() { std::memset(this, 0, sizeof(*this)); }
+};
+
+static size_t item_size(int kind)
+{
+    switch (kind) {
+    case OP_VEC: return sizeof(VecItem);
+    case OP_NCG_NEGPN: case OP_NCG_DD: case OP_NCG_EXPAND: case OP_NCG_HMIN:
+    case OP_SN_CONTRACT: case OP_SN_EXPAND: case OP_SN_FINAL: return sizeof(NcgItem);
+    case OP_AIJ_ONES: return sizeof(AijItem);
+    case OP_TCG_NT: case OP_TCG_RSET: case OP_TCG_ZPREP: case OP_TCG_QPOST: case OP_TCG_PUPD:
+    case OP_TCG_AS: case OP_TCG_SA: return sizeof(TcgItem);
+    case OP_ST_AS: case OP_ST_SA: return sizeof(StItem);
+    case OP_GD_DP: case OP_GD_PROJECT: case OP_GD_PROBE: case OP_GD_UPDATE: case OP_GD_MOVES:
+    case OP_GD_DIAGRES: return sizeof(GdItem);
+    case OP_CONV: case OP_CONV32: return sizeof(ConvItem);
+    case OP_ELASCF: return sizeof(CoefItem);
+    case OP_EMBED: return sizeof(EmbedItem);
+    case OP_EXTRACT: return sizeof(ExtractItem);
+    case OP_SETUP: case OP_FORCES: return sizeof(SetupItem);
+    }
+    return 0;
+}
+
+struct Req {
+    int kind;
+    AnyItem item;
+    double* res = nullptr;
+};
+
+class Scheduler {
+public:
+    explicit Scheduler(hipStream_t s) : s_(s)
+    {
+        cap_ = 64ull << 20;
+        HIPCHK(hipHostMalloc((void**)&hstage_, cap_, hipHostMallocDefault));
+        HIPCHK(hipMalloc((void**)&dstage_, cap_));
+        HIPCHK(hipHostMalloc((void**)&hres_, rescap_ * sizeof(double), hipHostMallocDefault));
+        HIPCHK(hipMalloc((void**)&dres_, rescap_ * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&dpart_, partcap_ * sizeof(double)));
+    }
+    ~Scheduler()
+    {
+        for (auto ev : ev_pool) (void)hipEventDestroy(ev);
+        (void)hipHostFree(hstage_);
+        (void)hipFree(dstage_);
+        (void)hipHostFree(hres_);
+        (void)hipFree(dres_);
+        (void)hipFree(dpart_);
+    }
+    void resize(int nslots)
+    {
+        q_.assign(nslots, {});
+        waiter_.assign(nslots, nullptr);
+    }
+    void enqueue(int slot, int kind, const AnyItem& it)
+    {
+        Req r;
+        r.kind = kind;
+        r.item = it;
+        q_[slot].push_back(r);
+    }
+    void set_result_dest(int slot, double* dst) { q_[slot].back().res = dst; }
+
+    // per-kernel timing with HIP events on the scheduler's stream (enabled on request)
+    struct ProfEntry {
+        double seconds = 0, bytes = 0;
+        long launches = 0;
+    };
+    bool profile = false;
+    std::map<int, ProfEntry> prof;   // key kind*4 + stage
+    std::vector<hipEvent_t> ev_pool;
+    void set_waiter(int slot, std::coroutine_handle<> h) { waiter_[slot] = h; }
+
+    // run all tasks to completion
+    void run(std::vector<Task>& tasks, const std::vector<int>& slots)
+    {
+        for (size_t i = 0; i < tasks.size(); i++) tasks[i].h.resume();
+        for (;;) {
+            bool any = false;
+            for (auto& q : q_)
+                if (!q.empty()) any = true;
+            if (!any) break;
+            flush();
+            for (size_t i = 0; i < slots.size(); i++) {
+                int sl = slots[i];
+                if (waiter_[sl]) {
+                    auto h = waiter_[sl];
+                    waiter_[sl] = nullptr;
+                    h.resume();
+                }
+            }
+        }
+        for (auto& t : tasks)
+            if (!t.h.done()) throw std::runtime_error("scheduler: a patch solve did not finish");
+    }
+
+private:
+    struct Group {
+        int kind;
+        std::vector<int> reqs;   // indices into flat list
+    };
+    void flush()
+    {
+        // collect requests level by level
+        std::vector<Req*> flat;
+        std::vector<std::vector<Group>> levels;
+        size_t maxlen = 0;
+        for (auto& q : q_) maxlen = std::max(maxlen, q.size());
+        for (size_t lev = 0; lev < maxlen; lev++) {
+            std::map<int, Group> byk;
+            for (auto& q : q_)
+                if (lev < q.size()) {
+                    int idx = (int)flat.size();
+                    flat.push_back(&q[lev]);
+                    auto& g = byk[q[lev].kind];
+                    g.kind = q[lev].kind;
+                    g.reqs.push_back(idx);
+                }
+            std::vector<Group> gs;
+            for (auto& kv : byk) gs.push_back(kv.second);
+            levels.push_back(gs);
+        }
+        // stage items, group tables and result slots
+        struct Launch {
+            int kind, nitems, K, stage;
+            size_t items_off, gstart_off, modes_off, part_off, res_off;
+            int ngroups;
+            size_t lds;
+        };
+        std::vector<Launch> launches;
+        size_t off = 0;
+        size_t part = 0, res = 0;
+        auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        std::vector<std::pair<Req*, size_t>> resmap;
+        for (auto& lev : levels)
+            for (auto& g : lev) {
+                int kind = g.kind;
+                size_t isz = item_size(kind);
+                int nitems = (int)g.reqs.size();
+                size_t items_off = off;
+                if (off + isz * nitems > cap_) throw std::runtime_error("scheduler: staging overflow");
+                for (int i = 0; i < nitems; i++)
+                    std::memcpy(hstage_ + off + isz * i, &flat[g.reqs[i]]->item, isz);
+                off = align(off + isz * nitems);
+                int K = op_nred(kind, &flat[g.reqs[0]]->item);
+                int nst = (kind == OP_CONV || kind == OP_CONV32) ? 3 : 1;
+                for (int st = 0; st < nst; st++) {
+                    Launch L;
+                    L.kind = kind;
+                    L.nitems = nitems;
+                    L.K = K;
+                    L.stage = st;
+                    L.items_off = items_off;
+                    L.gstart_off = off;
+                    int* gs = (int*)(hstage_ + off);
+                    int tot = 0;
+                    size_t lds = 0;
+                    for (int i = 0; i < nitems; i++) {
+                        gs[i] = tot;
+                        tot += op_groups(kind, &flat[g.reqs[i]]->item, st);
+                        lds = std::max(lds, op_lds(kind, &flat[g.reqs[i]]->item, st));
+                    }
+                    gs[nitems] = tot;
+                    off = align(off + sizeof(int) * (nitems + 1));
+                    L.ngroups = tot;
+                    L.lds = lds;
+                    L.modes_off = off;
+                    int* md = (int*)(hstage_ + off);
+                    for (int i = 0; i < nitems; i++) md[i] = op_combine_mode(kind, &flat[g.reqs[i]]->item);
+                    off = align(off + sizeof(int) * nitems);
+                    L.part_off = part;
+                    L.res_off = res;
+                    if (K > 0) {
+                        part += (size_t)tot * K;
+                        for (int i = 0; i < nitems; i++)
+                            resmap.push_back({flat[g.reqs[i]], res + (size_t)i * K});
+                        res += (size_t)nitems * K;
+                    }
+                    if (off > cap_) throw std::runtime_error("scheduler: staging overflow");
+                    launches.push_back(L);
+                }
+            }
+        if (part > partcap_) {
+            HIPCHK(hipStreamSynchronize(s_));
+            HIPCHK(hipFree(dpart_));
+            partcap_ = part * 2;
+            HIPCHK(hipMalloc((void**)&dpart_, partcap_ * sizeof(double)));
+        }
+        if (res > rescap_) {
+            HIPCHK(hipStreamSynchronize(s_));
+            HIPCHK(hipHostFree(hres_));
+            HIPCHK(hipFree(dres_));
+            rescap_ = res * 2;
+            HIPCHK(hipHostMalloc((void**)&hres_, rescap_ * sizeof(double), hipHostMallocDefault));
+            HIPCHK(hipMalloc((void**)&dres_, rescap_ * sizeof(double)));
+        }
+        HIPCHK(hipMemcpyAsync(dstage_, hstage_, off, hipMemcpyHostToDevice, s_));
+        std::vector<std::pair<int, double>> pkeys;
+        size_t nev = 0;
+        if (profile) {
+            while (ev_pool.size() < 2 * launches.size()) {
+                hipEvent_t ev;
+                HIPCHK(hipEventCreate(&ev));
+                ev_pool.push_back(ev);
+            }
+        }
+        for (auto& L : launches) {
+            if (profile && L.ngroups > 0) {
+                double bytes = 0;
+                for (int i = 0; i < L.nitems; i++)
+                    bytes += op_bytes(L.kind, hstage_ + L.items_off + item_size(L.kind) * i, L.stage);
+                pkeys.push_back({L.kind * 4 + L.stage, bytes});
+                HIPCHK(hipEventRecord(ev_pool[nev++], s_));
+            }
+            launch_op(L.kind, dstage_ + L.items_off, (const int*)(dstage_ + L.gstart_off), L.nitems,
+                      L.ngroups, dpart_ + L.part_off, L.stage, L.lds, s_);
+            if (profile && L.ngroups > 0) HIPCHK(hipEventRecord(ev_pool[nev++], s_));
+            if (L.K > 0)
+                launch_finalize((const int*)(dstage_ + L.gstart_off), (const int*)(dstage_ + L.modes_off),
+                                L.nitems, L.K, dpart_ + L.part_off, dres_ + L.res_off, s_);
+        }
+        HIPCHK(hipGetLastError());
+        if (res > 0) HIPCHK(hipMemcpyAsync(hres_, dres_, res * sizeof(double), hipMemcpyDeviceToHost, s_));
+        HIPCHK(hipStreamSynchronize(s_));
+        for (size_t k = 0; k < pkeys.size(); k++) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ev_pool[2 * k], ev_pool[2 * k + 1]));
+            auto& pe = prof[pkeys[k].first];
+            pe.seconds += ms * 1e-3;
+            pe.bytes += pkeys[k].second;
+            pe.launches++;
+        }
+        for (auto& rm : resmap)
+            if (rm.first->res) {
+                int K = op_nred(rm.first->kind, &rm.first->item);
+                for (int k = 0; k < K; k++) rm.first->res[k] = hres_[rm.second + k];
+            }
+        for (auto& q : q_) q.clear();
+        nflush++;
+    }
+
+    hipStream_t s_;
+    std::vector<std::vector<Req>> q_;
+    std::vector<std::coroutine_handle<>> waiter_;
+    char* hstage_ = nullptr;
+    char* dstage_ = nullptr;
+    size_t cap_ = 0;
+    double* hres_ = nullptr;
+    double* dres_ = nullptr;
+    size_t rescap_ = 1 << 20;
+    double* dpart_ = nullptr;
+    size_t partcap_ = 4 << 20;
+
+public:
+    long nflush = 0;
+};
+
+// awaiter of a request with results
+struct Res {
+    double v[12];
+};
+
+struct ResAwait {
+    Scheduler* S;
+    int slot;
+    Res r;
+    bool await_ready() { return false; }
+    void await_suspend(std::coroutine_handle<> h)
+    {
+        S->set_result_dest(slot, r.v);
+        S->set_waiter(slot, h);
+    }
+    Res await_resume() { return r; }
+};
+
+// per-patch issuing context
+struct Ctx {
+    Scheduler* S;
+    Patch* P;
+    int slot;
+    Engine* E;
+
+    void op(int kind, const AnyItem& it) { S->enqueue(slot, kind, it); }
+    ResAwait opr(int kind, const AnyItem& it)
+    {
+        S->enqueue(slot, kind, it);
+        return ResAwait{S, slot, {}};
+    }
+    // masked BLAS-1 helpers (src/m_gridfunc.f90)
+    AnyItem vitem(int sub, int mask, double* x0, double* x1, double* y0, double* y1, double a, double b)
+    {
+        AnyItem it;
+        it.vec.n = P->n;
+        it.vec.sub = sub;
+        it.vec.mask = mask;
+        it.vec.ncomp = x1 ? 2 : 1;
+        it.vec.el = P->el;
+        it.vec.x[0] = x0;
+        it.vec.x[1] = x1;
+        it.vec.y[0] = y0;
+        it.vec.y[1] = y1;
+        it.vec.a = a;
+        it.vec.b = b;
+        return it;
+    }
+    void set(int mask, double* x, double a, double* x1 = nullptr) { op(OP_VEC, vitem(V_SET, mask, x, x1, nullptr, nullptr, a, 0)); }
+    void copy(int mask, double* x, double* y, double* x1 = nullptr, double* y1 = nullptr) { op(OP_VEC, vitem(V_COPY, mask, x, x1, y, y1, 0, 0)); }
+    void scal(int mask, double* x, double a, double* x1 = nullptr) { op(OP_VEC, vitem(V_SCAL, mask, x, x1, nullptr, nullptr, a, 0)); }
+    void axpy(int mask, double a, double* x, double* y, double* x1 = nullptr, double* y1 = nullptr) { op(OP_VEC, vitem(V_AXPY, mask, x, x1, y, y1, a, 0)); }
+    void axpby(int mask, double a, double* x, double b, double* y, double* x1 = nullptr, double* y1 = nullptr) { op(OP_VEC, vitem(V_AXPBY, mask, x, x1, y, y1, a, b)); }
+    void addc(int mask, double* x, double a) { op(OP_VEC, vitem(V_ADDC, mask, x, nullptr, nullptr, nullptr, a, 0)); }
+    ResAwait dot(int mask, double* x, double* y, double* x1 = nullptr, double* y1 = nullptr) { return opr(OP_VEC, vitem(V_DOT, mask, x, x1, y, y1, 0, 0)); }
+    ResAwait sum(int mask, double* x) { return opr(OP_VEC, vitem(V_SUM, mask, x, nullptr, nullptr, nullptr, 0, 0)); }
+    ResAwait sumsq(int mask, double* x, double* x1 = nullptr) { return opr(OP_VEC, vitem(V_SUMSQ, mask, x, x1, nullptr, nullptr, 0, 0)); }
+    ResAwait maxabs(int mask, double* x, double* x1 = nullptr) { return opr(OP_VEC, vitem(V_MAXABS, mask, x, x1, nullptr, nullptr, 0, 0)); }
+    ResAwait count(int mask) { return opr(OP_VEC, vitem(V_COUNT, mask, P->pn, nullptr, nullptr, nullptr, 0, 0)); }
+
+    // FFT product u[ik] (mask) = sum_jk C[ik][jk] * p[jk], src/m_aijpj.f90:258-451.
+    // which: 0 = cs, 1 = cv, 2 = ms; fp32 selects the single-precision spectra.
+    void conv(int which, int mask, int nout, const int* ik, double* const* out, int nin, const int* jk,
+              double* const* in, const double* const* addend = nullptr, bool fp32 = false)
+    {
+        AnyItem it;
+        ConvItem& c = it.conv;
+        c.mx = P->mx;
+        c.my = P->my;
+        c.Lx = P->Lx;
+        c.Ly = P->Ly;
+        c.nin = nin;
+        c.nout = nout;
+        c.mask = mask;
+        c.mode = 0;
+        int M = P->Lx / 2, H = M + 1;
+        size_t bsz = (size_t)H * P->Ly;
+        const char* base;
+        size_t esz = fp32 ? 8 : 16;
+        if (which == 0) base = (const char*)(fp32 ? P->spec32 : P->spec);
+        else if (which == 1) base = (const char*)P->specv;
+        else base = (const char*)(fp32 ? P->specm32 : P->specm);
+        for (int j = 0; j < nin; j++) c.in[j] = in[j];
+        for (int i = 0; i < nout; i++) {
+            c.out[i] = out[i];
+            c.addend[i] = addend ? addend[i] : nullptr;
+            for (int j = 0; j < nin; j++) {
+                int a = ik[i], b = jk[j];   // 0-based directions
+                const void* sp = nullptr;
+                if (which == 2) {
+                    if (a == b) sp = base + (size_t)a * bsz * esz;
+                } else if (P->nt_cpl || !((a == 2) != (b == 2))) {
+                    sp = base + (size_t)(a * 3 + b) * bsz * esz;
+                }
+                c.spec[i][j] = sp;
+            }
+        }
+        c.el = P->el;
+        c.wsS = P->wsS;
+        c.wsT = P->wsT;
+        c.twM = fp32 ? P->twM32 : P->twM;
+        c.twLx = fp32 ? P->twLx32 : P->twLx;
+        c.twLy = fp32 ? P->twLy32 : P->twLy;
+        c.scale = 1.0;
+        op(fp32 ? OP_CONV32 : OP_CONV, it);
+    }
+    // normal-normal product
+    void conv_n(int which, int mask, double* out, double* in, bool fp32 = false)
+    {
+        int ik[1] = {2}, jk[1] = {2};
+        double* o[1] = {out};
+        double* i[1] = {in};
+        conv(which, mask, 1, ik, o, 1, jk, i, nullptr, fp32);
+    }
+    // tangential-tangential product with optional addend
+    void conv_t(int which, int mask, double* ox, double* oy, double* ix, double* iy,
+                const double* ax = nullptr, const double* ay = nullptr, bool fp32 = false)
+    {
+        int ik[2] = {0, 1}, jk[2] = {0, 1};
+        double* o[2] = {ox, oy};
+        double* i[2] = {ix, iy};
+        const double* ad[2] = {ax, ay};
+        conv(which, mask, 2, ik, o, 2, jk, i, (ax || ay) ? ad : nullptr, fp32);
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+// engine basics
+// ------------------------------------------------------------------------------------------
+
+Engine::Engine()
+{
+    HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    sched_.reset(new Scheduler(stream_));
+}
+
+Engine::~Engine()
+{
+    sched_.reset();
+    for (auto& t : tw_) (void)hipFree(t.second);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+const void* Engine::twiddles(int L, bool fp32)
+{
+    long key = (long)L * 2 + (fp32 ? 1 : 0);
+    for (auto& t : tw_)
+        if (t.first == key) return t.second;
+    void* d = nullptr;
+    if (fp32) {
+        std::vector<float> h(2 * (size_t)L);
+        for (int e = 0; e < L; e++) {
+            long double a = -2.0L * 3.141592653589793238462643383279502884L * e / L;
+            h[2 * e] = (float)cosl(a);
+            h[2 * e + 1] = (float)sinl(a);
+        }
+        HIPCHK(hipMalloc(&d, h.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        std::vector<double> h(2 * (size_t)L);
+        for (int e = 0; e < L; e++) {
+            long double a = -2.0L * 3.141592653589793238462643383279502884L * e / L;
+            h[2 * e] = (double)cosl(a);
+            h[2 * e + 1] = (double)sinl(a);
+        }
+        HIPCHK(hipMalloc(&d, h.size() * sizeof(double)));
+        HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    tw_.push_back({key, d});
+    return d;
+}
+
+// ------------------------------------------------------------------------------------------
+// host set-up of a case: material, Hertz, grid, undeformed distance, rigid slip
+// ------------------------------------------------------------------------------------------
+
+// complete elliptic integrals B(m), D(m) (mc = 1-m) by Carlson's duplication algorithms for
+// R_F and R_D (B.C. Carlson, Numer. Algorithms 10 (1995) 13-26): K = R_F(0,mc,1),
+// D = R_D(0,mc,1)/3, B = K - D; used in place of ellip_bd (src/m_hertz.f90:503-1004)
+static double rf(double x, double y, double z)
+{
+    double a0 = (x + y + z) / 3.0;
+    double q = std::pow(3e-17, -1.0 / 6.0) * std::max({std::fabs(a0 - x), std::fabs(a0 - y), std::fabs(a0 - z)});
+    double a = a0, f = 1.0;
+    for (int it = 0; it < 60 && q * f >= std::fabs(a); it++) {
+        double sx = std::sqrt(x), sy = std::sqrt(y), sz = std::sqrt(z), lam = sx * sy + sx * sz + sy * sz;
+        x = 0.25 * (x + lam); y = 0.25 * (y + lam); z = 0.25 * (z + lam); a = 0.25 * (a + lam); f *= 0.25;
+    }
+    double X = (a - x) / a, Y = (a - y) / a, Z = -(X + Y), e2 = X * Y - Z * Z, e3 = X * Y * Z;
+    return (1.0 - e2 / 10.0 + e3 / 14.0 + e2 * e2 / 24.0 - 3.0 * e2 * e3 / 44.0) / std::sqrt(a);
+}
+static double rd(double x, double y, double z)
+{
+    double a0 = (x + y + 3.0 * z) / 5.0;
+    double q = std::pow(0.25e-17, -1.0 / 6.0) * std::max({std::fabs(a0 - x), std::fabs(a0 - y), std::fabs(a0 - z)});
+    double a = a0, f = 1.0, s = 0.0;
+    for (int it = 0; it < 60 && q * f >= std::fabs(a); it++) {
+        double sx = std::sqrt(x), sy = std::sqrt(y), sz = std::sqrt(z), lam = sx * sy + sx * sz + sy * sz;
+        s += f / (sz * (z + lam));
+        x = 0.25 * (x + lam); y = 0.25 * (y + lam); z = 0.25 * (z + lam); a = 0.25 * (a + lam); f *= 0.25;
+    }
+    double X = (a - x) / a, Y = (a - y) / a, Z = -(X + Y) / 3.0;
+    double e2 = X * Y - 6.0 * Z * Z, e3 = (3.0 * X * Y - 8.0 * Z * Z) * Z, e4 = 3.0 * (X * Y - Z * Z) * Z * Z,
+           e5 = X * Y * Z * Z * Z;
+    double ser = 1.0 - 3.0 * e2 / 14.0 + e3 / 6.0 + 9.0 * e2 * e2 / 88.0 - 3.0 * e4 / 22.0 - 9.0 * e2 * e3 / 52.0 + 3.0 * e5 / 26.0;
+    return f * ser / (a * std::sqrt(a)) + 3.0 * s;
+}
+static double eli(double elmodk, double& K, double& E)
+{
+    double mc = 1.0 - elmodk * elmodk, b, d;
+    if (1.0 - mc < 1.11e-16) { b = PI / 4; d = PI / 4; }
+    else {
+        double mcc = mc < 1.11e-32 ? 1.11e-32 : mc;
+        double kk = rf(0.0, mcc, 1.0);
+        d = rd(0.0, mcc, 1.0) / 3.0;
+        b = kk - d;
+    }
+    K = b + d;
+    E = b + mc * d;
+    return (E - b) / b;
+}
+
+// hzcalc3d, src/m_hertz.f90:267-384
+static bool hertz3d(double e_star, int ipotcn, int norm, double& a1, double& b1, double& aa, double& bb,
+                    double& pen, double& fn, double& cp, double& rho)
+{
+    double K, E, y;
+    if (ipotcn == -1) {
+        if (std::min(a1, b1) < TINY) return false;
+        bool zbla = b1 <= a1;
+        y = zbla ? b1 / a1 : a1 / b1;
+        // bisection on the modulus, src/m_hertz.f90:386-442
+        const double epshz = sp(1e-9f);
+        double xl = 0.0, xr = 1.0, x = 0.5;
+        double elr = eli(xr, K, E), ell = eli(xl, K, E);
+        while (std::fabs(xr - xl) > epshz) {
+            if ((y - elr) * (y - ell) > 0.0) return false;
+            x = 0.5 * (xl + xr);
+            double elx = eli(x, K, E);
+            if ((y - elr) * (y - elx) <= 0.0) { xl = x; ell = elx; }
+            else { xr = x; elr = elx; }
+        }
+        eli(x, K, E);
+        double g = std::sqrt(std::max(1e-40, 1.0 - x * x)), sg = std::sqrt(g);
+        rho = 2.0 / (a1 + b1);
+        if (norm == 1) {
+            cp = std::pow((3.0 * std::max(0.0, fn) * rho * E) / (4.0 * PI * e_star * sg), 1.0 / 3.0);
+            pen = 2.0 * (cp * sg) * (cp * sg) * K / (rho * E);
+        } else {
+            cp = std::sqrt((std::max(0.0, pen) * rho * E) / (2.0 * K * sg * sg));
+            fn = 4.0 * PI * cp * cp * cp * e_star * sg / (3.0 * rho * E);
+        }
+        if (zbla) { aa = cp * sg; bb = cp / sg; } else { aa = cp / sg; bb = cp * sg; }
+        return true;
+    }
+    bool zbla = bb <= aa;
+    double g = zbla ? bb / aa : aa / bb, sg = std::sqrt(g);
+    y = eli(std::sqrt(1.0 - g * g), K, E);
+    cp = std::sqrt(aa * bb);
+    if (norm == 1) {
+        rho = 4.0 * PI * cp * cp * cp * e_star * sg / (3.0 * fn * E);
+        pen = 2.0 * (cp * sg) * (cp * sg) * K / (rho * E);
+    } else {
+        pen = std::max(pen, 1e-9);
+        rho = 2.0 * (cp * sg) * (cp * sg) * K / (pen * E);
+        fn = 4.0 * PI * cp * cp * cp * e_star * sg / (3.0 * rho * E);
+    }
+    double apb = 2.0 / rho, ama = apb / (y + 1.0), ami = y * ama;
+    if (zbla) { a1 = ami; b1 = ama; } else { a1 = ama; b1 = ami; }
+    return true;
+}
+
+// Host part of contac's set-up (src/m_scontc.f90:37-176): checks, Hertz, grid, h and w.
+bool Engine::prepare(Patch& P)
+{
+    CaseInput& in = P.in;
+    CaseResult& R = P.out;
+    R = CaseResult();
+    bool hertzian = in.ipotcn == -1 || in.ipotcn == -3;
+    if (in.tang < 0 || in.tang > 3 || in.ipotcn == 0 || (!hertzian && in.ibase == 0)) {
+        R.ierror = ERR_OTHER;
+        last_error = "potential contact or undeformed distance not specified";
+        return false;
+    }
+    if (in.pvtime != 0 && in.pvtime != 2) { R.ierror = ERR_OTHER; return false; }
+    // combin_mater, src/m_hierarch_data.f90:1543-1553
+    P.ga = 2.0 / (1.0 / in.gg1 + 1.0 / in.gg2);
+    P.nu = P.ga * (in.poiss1 / in.gg1 + in.poiss2 / in.gg2) / 2.0;
+    P.ak = (P.ga / 4.0) * ((1.0 - 2.0 * in.poiss1) / in.gg1 - (1.0 - 2.0 * in.poiss2) / in.gg2);
+    P.nt_cpl = std::fabs(P.ak) >= 1e-6;
+    if (P.nt_cpl) { R.ierror = ERR_OTHER; last_error = "dissimilar materials (K != 0) are not supported"; return false; }
+    P.muscal = in.fstat;
+    P.fntrue = in.fn;
+    P.pen = in.pen;
+    // potential contact and grid (potcon_fill / potcon_hertz, src/m_hierarch_data.f90:1801-1959)
+    int mx = in.mx, my = in.my;
+    double dx = in.dx, dy = in.dy, xc1 = in.xc1, yc1 = in.yc1;
+    int ibase = in.ibase;
+    double prm[6];
+    std::memcpy(prm, in.prmudf, sizeof prm);
+    if (in.ipotcn == -1 || in.ipotcn == -3) {
+        double e_star = P.ga / (1.0 - P.nu);
+        R.hz_a1 = in.hz_a1; R.hz_b1 = in.hz_b1; R.hz_aa = in.hz_aa; R.hz_bb = in.hz_bb;
+        if (!hertz3d(e_star, in.ipotcn, in.norm, R.hz_a1, R.hz_b1, R.hz_aa, R.hz_bb, P.pen, P.fntrue,
+                     R.hz_cp, R.hz_rho)) { R.ierror = ERR_OTHER; return false; }
+        ibase = 1;
+        std::memset(prm, 0, sizeof prm);
+        prm[0] = R.hz_a1;
+        prm[2] = R.hz_b1;
+        double xl = -in.hz_scale * std::max(1e-9, R.hz_aa), yl = -in.hz_scale * std::max(1e-9, R.hz_bb);
+        dx = (-xl - xl) / mx;
+        dy = (-yl - yl) / my;
+        xc1 = xl + 0.5 * dx;
+        yc1 = yl + 0.5 * dy;
+    } else if (in.ipotcn == 1) {
+        xc1 = in.xl + 0.5 * dx;
+        yc1 = in.yl + 0.5 * dy;
+    } else if (in.ipotcn == 2) {
+        dx = (in.xh - in.xl) / mx;
+        dy = (in.yh - in.yl) / my;
+        xc1 = in.xl + 0.5 * dx;
+        yc1 = in.yl + 0.5 * dy;
+    } else if (in.ipotcn == 4) {
+        dx = (in.xcm - in.xc1) / std::max(1, mx - 1);
+        dy = (in.ycm - in.yc1) / std::max(1, my - 1);
+    } else if (in.ipotcn != 3) {
+        R.ierror = ERR_OTHER;
+        return false;
+    }
+    if (mx < 2 || my < 2) { R.ierror = ERR_OTHER; last_error = "grids need mx, my >= 2"; return false; }
+    bool newgrid = mx != P.mx || my != P.my;
+    if (newgrid && P.have_solution && (in.pvtime != 2 || in.iestim != 0)) {
+        R.ierror = ERR_OTHER;
+        last_error = "continuation needs an unchanged grid";
+        return false;
+    }
+    P.mx = mx; P.my = my; P.n = mx * my;
+    P.dx = dx; P.dy = dy; P.dxdy = dx * dy; P.xc1 = xc1; P.yc1 = yc1;
+    // check_roll_stepsize, src/m_sdis.f90:125-204
+    bool is_roll = in.tang == 2 || in.tang == 3;
+    P.chi = in.chi;
+    P.dq = in.dq > 0 ? in.dq : dx;
+    if (in.tang == 3) {
+        if (std::fabs(P.chi) > 0.01 && std::fabs(P.chi - PI) > 0.01) P.chi = 0.0;
+        P.dq = dx;
+    }
+    if (!is_roll) { P.chi = 0.0; P.dq = 1.0; }
+    if (std::fabs(P.chi) > 0.01) { R.ierror = ERR_OTHER; last_error = "rolling direction chi=0 only"; return false; }
+    if (in.tang == 2 && std::fabs(P.dq - dx) > 1e-9 * dx) { R.ierror = ERR_OTHER; last_error = "transient rolling needs dq = dx"; return false; }
+    // set_norm_rhs, src/m_sdis.f90:321-494: the quadratic gap is formed on the GPU; the host
+    // keeps a copy only where it is needed (element-wise input, or eldiv0's initial estimate)
+    int ihertz = in.ipotcn < 0 ? -in.ipotcn : 0;
+    P.ibase_eff = ibase;
+    std::memcpy(P.prm, prm, sizeof prm);
+    bool need_host_h = ibase == 9 || (ihertz == 0 && in.iestim == 0);
+    P.h_host.clear();
+    if (need_host_h) {
+        P.h_host.resize(P.n);
+        for (int iy = 0; iy < my; iy++)
+            for (int ix = 0; ix < mx; ix++) {
+                int i = ix + iy * mx;
+                double x = xc1 + ix * dx, y = yc1 + iy * dy;
+                if (ibase == 1)
+                    P.h_host[i] = prm[0] * x * x + prm[1] * x * y + prm[2] * y * y + prm[3] * x + prm[4] * y + prm[5];
+                else if (ibase == 9) {
+                    if ((int)in.h_elem.size() < P.n) { R.ierror = ERR_OTHER; return false; }
+                    P.h_host[i] = in.h_elem[i];
+                } else { R.ierror = ERR_OTHER; return false; }
+            }
+    } else if (ibase != 1) {
+        R.ierror = ERR_OTHER;
+        return false;
+    }
+    // padded sizes of the products
+    P.Lx = 2 * opt_fft_size(mx);
+    P.Ly = 2 * opt_fft_size(my);
+    // device memory
+    int M = P.Lx / 2, H = M + 1;
+    size_t n = P.n;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t nb = (size_t)(2 * mx) * (2 * my);
+    size_t bsz = (size_t)H * P.Ly;
+    size_t pH = (size_t)mx + 1, pL = 2 * (size_t)my;   // preconditioner transform (2mx x 2my)
+    size_t ws = std::max((size_t)3 * P.Ly * H, pH * pL) * 16;
+    size_t emb = std::max((size_t)P.Lx * P.Ly, (size_t)(2 * mx) * (2 * my)) * 8;
+    size_t need = al(n) + al(37 * n * 8) + al(9 * nb * 8) * 2 + al(3 * nb * 8) + al(9 * bsz * 16) * 2 +
+                  al(3 * bsz * 16) + al(9 * bsz * 8) + al(3 * bsz * 8) + al(3 * emb) + 2 * al(ws) + 4096;
+    P.mem.ensure(need);
+    char* b = P.mem.base;
+    auto take = [&](size_t bytes) { char* r = b; b += al(bytes); return r; };
+    P.el = (int8_t*)take(n);
+    double* d = (double*)take(37 * n * 8);
+    P.pn = d; P.px = d + n; P.py = d + 2 * n; P.sx = d + 3 * n; P.sy = d + 4 * n;
+    P.pvx = d + 5 * n; P.pvy = d + 6 * n; P.pvn = d + 7 * n;
+    P.h = d + 8 * n; P.hx = d + 9 * n; P.hy = d + 10 * n; P.wsx = d + 11 * n; P.wsy = d + 12 * n;
+    for (int k = 0; k < 24; k++) P.w[k] = d + (13 + k) * n;
+    P.cf = (double*)take(9 * nb * 8);
+    P.cfv = (double*)take(9 * nb * 8);
+    P.cfm = (double*)take(3 * nb * 8);
+    P.spec = take(9 * bsz * 16);
+    P.specv = take(9 * bsz * 16);
+    P.specm = take(3 * bsz * 16);
+    P.spec32 = take(9 * bsz * 8);
+    P.specm32 = take(3 * bsz * 8);
+    P.embed = (double*)take(3 * emb);
+    P.wsS = take(ws);
+    P.wsT = take(ws);
+    P.twM = twiddles(M, false); P.twLx = twiddles(P.Lx, false); P.twLy = twiddles(P.Ly, false);
+    P.twM32 = twiddles(M, true); P.twLx32 = twiddles(P.Lx, true); P.twLy32 = twiddles(P.Ly, true);
+    P.ptwM = twiddles(mx, false); P.ptwLx = twiddles(2 * mx, false); P.ptwLy = twiddles(2 * my, false);
+    if (newgrid) {
+        HIPCHK(hipMemsetAsync(P.el, 0, n, stream_));
+        HIPCHK(hipMemsetAsync(P.pn, 0, 8 * n * 8, stream_));
+        P.have_solution = false;
+    }
+    if (ibase == 9)
+        HIPCHK(hipMemcpyAsync(P.h, P.h_host.data(), n * 8, hipMemcpyHostToDevice, stream_));
+    // init_curr_data, src/m_sdis.f90:587-768: eldiv0 (src/m_sdis.f90:818-1012) on the host for
+    // a non-Hertzian zero initial estimate; the Hertzian estimate is formed on the GPU
+    if (in.iestim == 0 && ihertz == 0) {
+        const double facpen = sp(0.60f), reltol = sp(0.01f);
+        double fnscal = P.fntrue / P.ga;
+        size_t imin = 0;
+        for (size_t i = 1; i < n; i++)
+            if (P.h_host[i] < P.h_host[imin]) imin = i;
+        double hsmin = P.h_host[imin];
+        double pentru = P.pen - hsmin;
+        if (in.norm == 1) {
+            double fac = 1.0 * 0.75 * std::sqrt(PI) * (1.0 - P.nu);
+            double penmin = 0.0, fnmin = 0.0, penmax = fac * fnscal / std::sqrt(P.dxdy);
+            long ncnmax = 0;
+            for (size_t i = 0; i < n; i++)
+                if (P.h_host[i] - hsmin < facpen * penmax) ncnmax++;
+            double fnmax = penmax * std::sqrt(P.dxdy * ncnmax) / fac;
+            int iter = 0;
+            while (iter < 50 && std::fabs(fnmax - fnmin) > reltol * fnscal) {
+                iter++;
+                double penmid = 0.5 * (penmax + penmin);
+                long ncnmid = 0;
+                for (size_t i = 0; i < n; i++)
+                    if (P.h_host[i] - hsmin < facpen * penmid) ncnmid++;
+                double fnmid = penmid * std::sqrt(P.dxdy * ncnmid) / fac;
+                if (fnmid < fnscal) { penmin = penmid; fnmin = fnmid; }
+                else { penmax = penmid; fnmax = fnmid; }
+            }
+            pentru = penmax;
+            P.pen = pentru + hsmin;
+        }
+        P.el_host.resize(n);
+        for (size_t i = 0; i < n; i++) P.el_host[i] = (P.h_host[i] - hsmin < facpen * pentru) ? ADHES : EXTER;
+        HIPCHK(hipMemcpyAsync(P.el, P.el_host.data(), n, hipMemcpyHostToDevice, stream_));
+    }
+    P.hertz_init = in.iestim == 0 && ihertz >= 1;
+    P.hz_pnmax = P.hertz_init ? 3.0 * P.fntrue / (2.0 * PI * R.hz_aa * R.hz_bb) : 0.0;
+    P.xofs = is_roll ? std::cos(P.chi) * P.dq * (1.0 / 6.0) : 0.0;
+    P.yofs = is_roll ? std::sin(P.chi) * P.dq * (1.0 / 6.0) : 0.0;
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------
+// device set-up: coefficients and spectra (sgencr, src/m_visc.f90:127-376; fft_makePrec,
+// src/m_aijpj.f90:457-708)
+// ------------------------------------------------------------------------------------------
+
+static Task setup_coefficients(Ctx C, bool need_cv)
+{
+    Patch& P = *C.P;
+    int mx = P.mx, my = P.my;
+    size_t nb = (size_t)(2 * mx) * (2 * my);
+    int M = P.Lx / 2, H = M + 1;
+    size_t bsz = (size_t)H * P.Ly;
+    AnyItem it;
+    it.coef.mx = mx; it.coef.my = my;
+    it.coef.dx = P.dx; it.coef.dy = P.dy;
+    it.coef.xshft = 0.0; it.coef.yshft = 0.0;
+    it.coef.akv = P.ak; it.coef.nuv = P.nu;
+    it.coef.cf = P.cf;
+    C.op(OP_ELASCF, it);
+    if (need_cv) {
+        AnyItem iv = it;
+        iv.coef.xshft = std::cos(P.chi) * P.dq;
+        iv.coef.yshft = std::sin(P.chi) * P.dq;
+        iv.coef.cf = P.cfv;
+        C.op(OP_ELASCF, iv);
+    }
+    co_await C.count(M_ALL);   // completes the coefficient generation before the roll step
+    if (need_cv) launch_rollcoef((int)nb, P.cf, P.cfv, C.E->stream());
+
+    // spectra of the blocks: embed (wrap, scaled by 1/(G M Ly)) then forward transform
+    double scale = (1.0 / P.ga) / ((double)M * (double)P.Ly);
+    std::vector<std::pair<const double*, void*>> blocks;
+    int bl[5] = {0, 1, 3, 4, 8};   // xx, xy, yx, yy, zz (normal-tangential blocks are zero, K=0)
+    for (int b : bl) blocks.push_back({P.cf + (size_t)b * nb, (char*)P.spec + (size_t)b * bsz * 16});
+    if (need_cv)
+        for (int b : bl) blocks.push_back({P.cfv + (size_t)b * nb, (char*)P.specv + (size_t)b * bsz * 16});
+    for (size_t k0 = 0; k0 < blocks.size(); k0 += 3) {
+        int nin = (int)std::min<size_t>(3, blocks.size() - k0);
+        AnyItem cv;
+        ConvItem& c = cv.conv;
+        c.mx = mx; c.my = my; c.Lx = P.Lx; c.Ly = P.Ly;
+        c.nin = nin; c.nout = nin; c.mask = M_ALL; c.mode = 1;
+        for (int j = 0; j < nin; j++) {
+            AnyItem e;
+            e.emb.Lx = P.Lx; e.emb.Ly = P.Ly; e.emb.rx = mx; e.emb.ry = my;
+            e.emb.cf_mx = mx; e.emb.cf_my = my;
+            e.emb.cf = blocks[k0 + j].first;
+            e.emb.out = P.embed + (size_t)j * P.Lx * P.Ly;
+            e.emb.scale = scale;
+            C.op(OP_EMBED, e);
+            c.in[j] = e.emb.out;
+            c.out[j] = (double*)blocks[k0 + j].second;
+        }
+        c.wsS = P.wsS; c.wsT = P.wsT;
+        c.twM = P.twM; c.twLx = P.twLx; c.twLy = P.twLy;
+        c.scale = 1.0;
+        C.op(OP_CONV, cv);
+    }
+    if (P.in.precision == 1) {
+        co_await C.count(M_ALL);
+        for (int b : bl)
+            launch_to_float(2 * bsz, (const double*)((char*)P.spec + (size_t)b * bsz * 16),
+                            (float*)((char*)P.spec32 + (size_t)b * bsz * 8), C.E->stream());
+    }
+    co_return 0;
+}
+
+// approximate inverse of diagonal block kk: m = G^2 IFFT(1/FFT(c)) on the 2mx x 2my circulant,
+// then its spectrum at the product size
+static Task setup_prec(Ctx C, int kk)
+{
+    Patch& P = *C.P;
+    int mx = P.mx, my = P.my;
+    size_t nb = (size_t)(2 * mx) * (2 * my);
+    int M = P.Lx / 2, H = M + 1;
+    size_t bsz = (size_t)H * P.Ly;
+    int bidx = kk * 3 + kk;
+    int pLx = 2 * mx, pLy = 2 * my;
+    AnyItem e;
+    e.emb.Lx = pLx; e.emb.Ly = pLy; e.emb.rx = mx; e.emb.ry = my;
+    e.emb.cf_mx = mx; e.emb.cf_my = my;
+    e.emb.cf = P.cf + (size_t)bidx * nb;
+    e.emb.out = P.embed;
+    e.emb.scale = 1.0;
+    C.op(OP_EMBED, e);
+    AnyItem cv;
+    ConvItem& c = cv.conv;
+    c.mx = mx; c.my = my; c.Lx = pLx; c.Ly = pLy;
+    c.nin = 1; c.nout = 1; c.mask = M_ALL; c.mode = 2;
+    c.in[0] = P.embed;
+    c.out[0] = P.embed + (size_t)pLx * pLy;
+    c.wsS = P.wsS; c.wsT = P.wsT;
+    c.twM = P.ptwM; c.twLx = P.ptwLx; c.twLy = P.ptwLy;
+    c.scale = P.ga * P.ga / ((double)mx * (double)pLy);
+    C.op(OP_CONV, cv);
+    AnyItem x;
+    x.ext.Lx = pLx; x.ext.Ly = pLy; x.ext.cf_mx = mx; x.ext.cf_my = my;
+    x.ext.in = P.embed + (size_t)pLx * pLy;
+    x.ext.cf = P.cfm + (size_t)kk * nb;
+    x.ext.scale = 1.0;
+    C.op(OP_EXTRACT, x);
+    // spectrum at the product size
+    AnyItem e2;
+    e2.emb.Lx = P.Lx; e2.emb.Ly = P.Ly; e2.emb.rx = mx; e2.emb.ry = my;
+    e2.emb.cf_mx = mx; e2.emb.cf_my = my;
+    e2.emb.cf = P.cfm + (size_t)kk * nb;
+    e2.emb.out = P.embed;
+    e2.emb.scale = (1.0 / P.ga) / ((double)M * (double)P.Ly);
+    C.op(OP_EMBED, e2);
+    AnyItem cs;
+    ConvItem& c2 = cs.conv;
+    c2.mx = mx; c2.my = my; c2.Lx = P.Lx; c2.Ly = P.Ly;
+    c2.nin = 1; c2.nout = 1; c2.mask = M_ALL; c2.mode = 1;
+    c2.in[0] = P.embed;
+    c2.out[0] = (double*)((char*)P.specm + (size_t)kk * bsz * 16);
+    c2.wsS = P.wsS; c2.wsT = P.wsT;
+    c2.twM = P.twM; c2.twLx = P.twLx; c2.twLy = P.twLy;
+    c2.scale = 1.0;
+    C.op(OP_CONV, cs);
+    co_await C.count(M_ALL);
+    if (P.in.precision == 1)
+        launch_to_float(2 * bsz, (const double*)((char*)P.specm + (size_t)kk * bsz * 16),
+                        (float*)((char*)P.specm32 + (size_t)kk * bsz * 8), C.E->stream());
+    co_return 0;
+}
+
+// row-extent sum of one influence row with unit tractions, times 1/G (gf3_AijPj)
+static ResAwait aij_ones(Ctx& C, int bidx)
+{
+    Patch& P = *C.P;
+    AnyItem it;
+    it.aij.mx = P.mx; it.aij.my = P.my;
+    it.aij.ix = std::max(1, P.mx / 2);
+    it.aij.iy = std::max(1, P.my / 2);
+    it.aij.cf_mx = P.mx; it.aij.cf_my = P.my;
+    it.aij.el = P.el;
+    it.aij.cf = P.cf + (size_t)bidx * (2 * P.mx) * (2 * P.my);
+    it.aij.ga_inv = 1.0 / P.ga;
+    return C.opr(OP_AIJ_ONES, it);
+}
+
+// ------------------------------------------------------------------------------------------
+// NormCG, src/m_solvpn.f90:24-461
+// ------------------------------------------------------------------------------------------
+
+static NcgItem ncg_item(Patch& P)
+{
+    NcgItem I{};
+    I.n = P.n;
+    I.el = P.el;
+    I.p = P.pn;
+    I.rhs = P.w[0];
+    I.res = P.w[1];
+    I.dd = P.w[3];
+    I.v = P.w[5];
+    I.h = P.h;
+    I.unn = P.w[7];
+    I.hmin = P.hmin;
+    return I;
+}
+
+static Task normcg(Ctx C, int maxcg, double eps, double* pen, int* itcg_out, double* err_out)
+{
+    Patch& P = *C.P;
+    const int norm = P.in.norm;
+    const bool fp32 = P.in.precision == 1;
+    const int npot = P.n;
+    double dxdy = P.dxdy, fntrue = P.fntrue;
+    double *rhs = P.w[0], *res = P.w[1], *r_prv = P.w[2], *dd = P.w[3], *z = P.w[4], *v = P.w[5], *q = P.w[6];
+    double* ps = P.pn;
+    int numinn = npot <= 150 ? 3 : (npot <= 400 ? 2 : 1);
+    int itcg = 0, ret = 0;
+    double davg = 0.0;
+    Res r;
+
+    if (norm == 1) *pen = 0.0;
+    C.set(M_ALL, rhs, *pen);
+    C.axpy(M_ALL, -1.0, P.h, rhs);
+    C.set(M_ALL, res, 0.0);
+    C.set(M_ALL, v, 0.0);
+    C.set(M_ALL, r_prv, 0.0);
+    r = co_await C.count(M_INT);
+    long ncon = (long)r.v[0];
+    if (norm == 0) {
+        if (P.hmin - *pen >= 0.0) {
+            // no interpenetration: zero solution, empty contact area
+            C.set(M_ALL, ps, 0.0);
+            co_await C.count(M_ALL);
+            HIPCHK(hipMemsetAsync(P.el, 0, npot, C.E->stream()));
+            *itcg_out = 0;
+            *err_out = 0.0;
+            co_return 0;
+        }
+    } else {
+        if (ncon <= 0) {
+            double htrsh = P.hmin + 0.1 * std::max(P.hmax - P.hmin, 1e-10);
+            NcgItem I = ncg_item(P);
+            I.mode = 1;          // strict h < threshold, pressure kept
+            I.hmin = htrsh;
+            AnyItem it;
+            it.ncg = I;
+            r = co_await C.opr(OP_NCG_HMIN, it);
+            ncon += (long)r.v[0];
+        }
+        r = co_await C.sum(M_ALL, ps);
+        double fk = dxdy * r.v[0];
+        if (std::fabs(fk) < sp(1e-3f) * fntrue) {
+            double pn = fntrue / (dxdy * (double)ncon);
+            C.set(M_INT, ps, pn);
+        } else {
+            C.scal(M_INT, ps, fntrue / fk);
+        }
+    }
+    // res = rhs - A ps on the interior
+    C.conv_n(0, M_INT, res, ps);
+    C.axpby(M_INT, 1.0, rhs, -1.0, res);
+    if (norm == 1) {
+        r = co_await C.sum(M_INT, res);
+        C.addc(M_INT, res, -r.v[0] / std::max(1.0, r.v[1]));
+    }
+    double rz1 = 0.0, rz2 = 0.0, rms_xk = 1.0, rms_upd = 2.0 * eps, rms_upd1 = 0.0, alpha = 0.0;
+    int itinn = 0;
+    bool lchanged = false;
+    while ((lchanged || rms_upd > eps * rms_xk) && itcg < maxcg) {
+        itcg++;
+        itinn++;
+        C.conv_n(2, M_INT, z, res, fp32);
+        if (norm == 1) {
+            r = co_await C.sum(M_INT, z);
+            C.addc(M_INT, z, -r.v[0] / std::max(1.0, r.v[1]));
+        }
+        rz1 = rz2;
+        r = co_await C.dot(M_INT, z, res);
+        rz2 = r.v[0];
+        if (itcg <= 1 || rz1 < TINY) {
+            C.copy(M_INT, z, v);
+        } else {
+            r = co_await C.dot(M_INT, z, r_prv);
+            double beta = std::max(0.0, (rz2 - r.v[0]) / std::max(TINY, rz1));
+            C.axpby(M_INT, 1.0, z, beta, v);
+        }
+        if (norm == 1) {
+            r = co_await C.sum(M_INT, v);
+            C.addc(M_INT, v, -r.v[0] / std::max(1.0, r.v[1]));
+        }
+        C.conv_n(0, M_INT, q, v, fp32);
+        if (norm == 1) {
+            r = co_await C.sum(M_INT, q);
+            C.addc(M_INT, q, -r.v[0] / std::max(1.0, r.v[1]));
+        }
+        r = co_await C.dot(M_INT, res, v);
+        double rv = r.v[0];
+        r = co_await C.dot(M_INT, q, v);
+        double vav = r.v[0];
+        if (std::fabs(vav) > 1e-32 && ncon == 1) alpha = rv / vav;
+        else alpha = rv / std::max(TINY, vav);
+        C.axpy(M_INT, alpha, v, ps);
+        r = co_await C.sumsq(M_INT, v);
+        rms_upd = std::fabs(alpha) * std::sqrt(r.v[0] / std::max(1.0, r.v[1]));
+        if (itcg == 1) rms_upd1 = rms_upd;
+        if (itcg <= 3 || itcg % 10 == 0) {
+            r = co_await C.sumsq(M_INT, ps);
+            rms_xk = std::sqrt(r.v[0] / std::max(1.0, r.v[1]));
+        }
+        C.copy(M_ALL, res, r_prv);
+        if (itinn < numinn && rms_upd >= eps * rms_xk) {
+            C.axpy(M_INT, -alpha, q, res);
+        } else {
+            AnyItem it;
+            it.ncg = ncg_item(P);
+            r = co_await C.opr(OP_NCG_NEGPN, it);
+            bool lchg_negpn = r.v[0] > 0;
+            ncon = (long)r.v[1];
+            double fk = dxdy * r.v[2];
+            if (ncon <= 0) {
+                NcgItem I = ncg_item(P);
+                I.mode = 2;      // h <= hmin + 1e-5, pressure set to zero
+                I.hmin = P.hmin + 1e-5;
+                AnyItem ih;
+                ih.ncg = I;
+                r = co_await C.opr(OP_NCG_HMIN, ih);
+                ncon += (long)r.v[0];
+                lchg_negpn = true;
+                Res s2 = co_await C.sum(M_ALL, ps);
+                fk = dxdy * s2.v[0];
+            }
+            if (norm == 1 && lchg_negpn) {
+                if (std::fabs(fk) < sp(1e-3f) * fntrue) {
+                    C.set(M_INT, ps, 1.0);
+                    r = co_await C.sum(M_ALL, ps);
+                    fk = r.v[0];
+                }
+                C.scal(M_INT, ps, fntrue / fk);
+            }
+            C.conv_n(0, M_ALL, dd, ps);
+            AnyItem idd;
+            idd.ncg = ncg_item(P);
+            r = co_await C.opr(OP_NCG_DD, idd);
+            double avg = 0.0;
+            if (norm == 1) {
+                davg = r.v[0] / (double)ncon;
+                avg = r.v[0] / std::max(1.0, r.v[1]);
+            }
+            AnyItem iex;
+            iex.ncg = ncg_item(P);
+            iex.ncg.avg = avg;
+            iex.ncg.davg = davg;
+            r = co_await C.opr(OP_NCG_EXPAND, iex);
+            bool lchg_intpen = r.v[0] > 0;
+            ncon += (long)r.v[0];
+            itinn = 0;
+            lchanged = lchg_intpen || lchg_negpn;
+        }
+    }
+    if (norm == 1) *pen = davg;
+    double conv = 1.0;
+    if (rms_upd * rms_upd1 > 0.0 && itcg > 1) conv = std::exp(std::log(rms_upd / rms_upd1) / (itcg - 1));
+    if (rms_upd > rms_xk && conv > 1.0 && itcg >= maxcg) ret = ERR_NORM;
+    *itcg_out = itcg;
+    *err_out = rms_upd;
+    co_return ret;
+}
+
+// snorm, src/m_snorm.f90:31-378
+static Task snorm(Ctx C, int* itnorm_out)
+{
+    Patch& P = *C.P;
+    const int npot = P.n;
+    double* ps = P.pn;
+    double* unn = P.w[7];
+    bool use_vecfft = true;   // the product of the whole area serves the expansion test
+    (void)npot;
+    co_await setup_prec(C, 2);
+    int itnorm = 0, itcg = 0, ret = 0;
+    bool zready;
+    Res r;
+    do {
+        itnorm++;
+        zready = true;
+        C.set(M_EXT, ps, 0.0);
+        int it = 0;
+        double errpn = 0.0;
+        int rc = co_await normcg(C, P.in.maxgs, P.in.eps, &P.pen, &it, &errpn);
+        itcg += it;
+        if (rc) { ret = rc; break; }
+        AnyItem ic;
+        ic.ncg = ncg_item(P);
+        ic.ncg.errpn = errpn;
+        r = co_await C.opr(OP_SN_CONTRACT, ic);
+        if (r.v[0] > 0) zready = false;
+        if (zready) {
+            if (use_vecfft) C.conv_n(0, M_ALL, unn, ps);
+            r = co_await aij_ones(C, 8);
+            double tol = std::fabs(errpn * r.v[0] * (1.0 / P.ga));
+            AnyItem ie;
+            ie.ncg = ncg_item(P);
+            ie.ncg.pen = P.pen;
+            ie.ncg.tol = tol;
+            r = co_await C.opr(OP_SN_EXPAND, ie);
+            if (r.v[0] != 0) zready = false;
+        }
+        if (it >= P.in.maxgs) zready = false;
+    } while (!zready && itnorm < P.in.maxin);
+    if (!zready && !ret) itnorm = -1;
+    AnyItem ifin;
+    ifin.ncg = ncg_item(P);
+    co_await C.opr(OP_SN_FINAL, ifin);
+    P.out.itcg = itcg;
+    if (P.in.norm == 0) {
+        r = co_await C.sum(M_ALL, ps);
+        P.fntrue = P.dxdy * r.v[0];
+    }
+    *itnorm_out = itnorm;
+    co_return ret;
+}
+
+// ------------------------------------------------------------------------------------------
+// GDsteady, src/gdsteady.f90:9-606
+// ------------------------------------------------------------------------------------------
+
+struct GdState {
+    int gd_meth, kdown;
+};
+
+static GdItem gd_item(Patch& P)
+{
+    GdItem I{};
+    I.n = P.n; I.mx = P.mx; I.my = P.my;
+    I.el = P.el; I.px = P.px; I.py = P.py; I.pn = P.pn;
+    I.dpx = P.w[0]; I.dpy = P.w[1];
+    I.sx = P.sx; I.sy = P.sy;
+    I.wsx = P.wsx; I.wsy = P.wsy;
+    I.nx = P.w[2]; I.ny = P.w[3];
+    I.rx = P.w[4]; I.ry = P.w[5];
+    I.dvx = P.w[6]; I.dvy = P.w[7];
+    I.vx = P.w[8]; I.vy = P.w[9];
+    I.qx = P.w[10]; I.qy = P.w[11];
+    I.dscl = P.w[12];
+    I.poldx = P.w[13]; I.poldy = P.w[14];
+    I.mu = P.in.fstat;
+    I.fac_v = 1000.0;
+    I.fdecay = P.in.fdecay;
+    I.d_ifc = P.in.d_ifc; I.d_lin = P.in.d_lin; I.d_cns = P.in.d_cns; I.d_slp = P.in.d_slp; I.pow_s = P.in.pow_s;
+    return I;
+}
+
+static Task gdsteady(Ctx C, int maxgd, double eps, int* itgd_out, double* err_out, int* lstagn)
+{
+    Patch& P = *C.P;
+    const double tiny = 1e-12;
+    const int npot = P.n;
+    const bool fp32 = P.in.precision == 1;
+    GdItem G = gd_item(P);
+    {
+        // coefficients of element (0,0) times 1/G: symmetric blocks of cs
+        std::vector<double> h(4);
+        int mx = P.mx, my = P.my;
+        size_t nb = (size_t)(2 * mx) * (2 * my), k = (size_t)my * (2 * mx) + mx;
+        HIPCHK(hipMemcpyAsync(&h[0], P.cf + 0 * nb + k, 8, hipMemcpyDeviceToHost, C.E->stream()));
+        HIPCHK(hipMemcpyAsync(&h[1], P.cf + 1 * nb + k, 8, hipMemcpyDeviceToHost, C.E->stream()));
+        HIPCHK(hipMemcpyAsync(&h[2], P.cf + 3 * nb + k, 8, hipMemcpyDeviceToHost, C.E->stream()));
+        HIPCHK(hipMemcpyAsync(&h[3], P.cf + 4 * nb + k, 8, hipMemcpyDeviceToHost, C.E->stream()));
+        HIPCHK(hipStreamSynchronize(C.E->stream()));
+        G.c11 = h[0] / P.ga; G.c12 = h[1] / P.ga; G.c21 = h[2] / P.ga; G.c22 = h[3] / P.ga;
+    }
+    // gd_meth / kdown (src/contact_addon.f90:1240-1247)
+    int gd_meth = 1, kdown = 3;
+    if (P.in.fdecay >= 0.999) gd_meth = 1;
+    else if (P.in.fdecay <= 0.001) { gd_meth = 2; kdown = std::max(1, (int)std::lround(-P.in.fdecay)); }
+    else gd_meth = 3;
+    Res r;
+    r = co_await C.count(M_INT);
+    double ncon = r.v[0];
+    double facnel = (double)std::sqrt((float)npot / (float)ncon);
+    C.set(M_ALL, G.dscl, 1.0);
+    C.set(M_ALL, G.poldx, 0.0, G.poldy);
+    C.set(M_ALL, G.vx, 0.0, G.vy);
+    C.set(M_ALL, G.dvx, 0.0, G.dvy);
+    C.set(M_ALL, G.qx, 0.0, G.qy);
+    int itgd = 0, it_fb = -99;
+    bool lchanged = false;
+    double dif = 2.0, difid = 1.0, dif1 = 0.0, beta = 1.0, alpha = 0.0, alpha0 = 0.0;
+    *lstagn = 0;
+    AnyItem it;
+    it.gd = G;
+    C.op(OP_TCG_NT, [&] { AnyItem t; TcgItem& T = t.tcg; T.n = npot; T.px = P.px; T.py = P.py; T.nx = G.nx; T.ny = G.ny; return t; }());
+    it.gd.n = P.n;
+    C.op(OP_GD_DP, it);
+    C.set(M_ALL, P.sx, 0.0, P.sy);
+    C.conv_t(0, M_INT, P.sx, P.sy, G.dpx, G.dpy, G.wsx, G.wsy);
+    AnyItem id = it;
+    id.gd.first = 1;
+    co_await C.opr(OP_GD_DIAGRES, id);
+    r = co_await C.maxabs(M_ALL, G.rx, G.ry);
+    if (r.v[0] < tiny) dif = 0.0;
+
+    double resrms2 = 0.0;
+    while ((lchanged || dif > difid) && itgd < maxgd) {
+        itgd++;
+        // project_searchdir
+        AnyItem ip = it;
+        if (beta > P.in.betath || itgd - it_fb <= 1) { ip.gd.imeth = gd_meth; ip.gd.kdown = kdown; }
+        else { ip.gd.imeth = 2; ip.gd.kdown = P.in.kdowfb; it_fb = itgd; }
+        C.op(OP_GD_PROJECT, ip);
+        C.conv_t(0, M_INT, G.qx, G.qy, G.dvx, G.dvy, nullptr, nullptr, fp32);
+
+        // perform_linesearch, src/gdsteady.f90:1046-1462
+        const int MAX_J = 99;
+        const double eps_j = 0.001;
+        double tbl_alpha[MAX_J + 2], tbl_rho[MAX_J + 2], tbl_drhoda[MAX_J + 2];
+        int j = 0, itb_a = 0, itb_b = 0, itb_x = 0, itb_w = 0, itb_v = 0, itb_u;
+        bool stop_j = false, has_bracket = false, prev_bracket;
+        double alpha_j = 0.0, alpha_prv = -1.0, dst_cur = 0.0, dst_prv1 = 0.0, dst_prv2 = 0.0, ddrho;
+        double sum_dq2 = 0.0, sum_r2 = 0.0;
+        while (!stop_j) {
+            AnyItem pr = it;
+            pr.gd.alpha = alpha_j;
+            pr.gd.first = j == 0 ? 1 : 0;
+            r = co_await C.opr(OP_GD_PROBE, pr);
+            double rhonew = r.v[0];
+            double drho_da = r.v[1] + r.v[3] + alpha_j * (r.v[2] + r.v[4]) + alpha_j * alpha_j * r.v[5];
+            if (j == 0) {
+                if (itgd <= 1) alpha0 = r.v[6] / r.v[7];
+                else alpha0 = 0.5 * (alpha0 + alpha);
+                sum_dq2 = r.v[8];
+                sum_r2 = r.v[9];
+            }
+            int itb = 1;
+            while (itb <= j && alpha_j > tbl_alpha[itb]) itb++;
+            itb_u = itb;
+            for (int k = j; k >= itb; k--) {
+                tbl_alpha[k + 1] = tbl_alpha[k];
+                tbl_rho[k + 1] = tbl_rho[k];
+                tbl_drhoda[k + 1] = tbl_drhoda[k];
+            }
+            if (has_bracket) {
+                if (itb_a >= itb_u) itb_a++;
+                if (itb_b >= itb_u) itb_b++;
+                if (itb_x >= itb_u) itb_x++;
+                if (itb_w >= itb_u) itb_w++;
+                if (itb_v >= itb_u) itb_v++;
+            }
+            j++;
+            tbl_alpha[itb] = alpha_j;
+            tbl_rho[itb] = rhonew;
+            tbl_drhoda[itb] = drho_da;
+            prev_bracket = has_bracket;
+            has_bracket = false;
+            int ibrack = 1;
+            while (ibrack < j - 1 && !has_bracket) {
+                has_bracket = tbl_rho[ibrack + 1] <= std::min(tbl_rho[ibrack], tbl_rho[ibrack + 2]);
+                if (!has_bracket) ibrack++;
+            }
+            if (has_bracket && !prev_bracket) {
+                itb_a = ibrack;
+                itb_x = ibrack + 1;
+                itb_b = ibrack + 2;
+                if (tbl_rho[itb_a] < tbl_rho[itb_b]) { itb_w = itb_a; itb_v = itb_b; }
+                else { itb_w = itb_b; itb_v = itb_a; }
+                dst_prv1 = tbl_alpha[itb_x] - tbl_alpha[itb_w];
+                dst_prv2 = tbl_alpha[itb_w] - tbl_alpha[itb_v];
+            } else if (has_bracket) {
+                if (tbl_rho[itb_u] <= tbl_rho[itb_x]) {
+                    if (tbl_alpha[itb_u] >= tbl_alpha[itb_x]) itb_a = itb_x;
+                    else itb_b = itb_x;
+                    itb_v = itb_w;
+                    itb_w = itb_x;
+                    itb_x = itb_u;
+                } else {
+                    if (tbl_alpha[itb_u] < tbl_alpha[itb_x]) itb_a = itb_u;
+                    else itb_b = itb_u;
+                    if (tbl_rho[itb_u] <= tbl_rho[itb_w] || tbl_alpha[itb_w] == tbl_alpha[itb_x]) {
+                        itb_v = itb_w;
+                        itb_w = itb_u;
+                    } else if (tbl_rho[itb_u] <= tbl_rho[itb_v] || tbl_alpha[itb_v] == tbl_alpha[itb_x] ||
+                               tbl_alpha[itb_v] == tbl_alpha[itb_w]) {
+                        itb_v = itb_u;
+                    }
+                }
+                dst_prv2 = dst_prv1;
+                dst_prv1 = dst_cur;
+            }
+            alpha_prv = alpha_j;
+            if (has_bracket) {
+                double xmid = 0.5 * (tbl_alpha[itb_a] + tbl_alpha[itb_b]);
+                double tolx = eps_j * std::fabs(tbl_alpha[itb_x]) + tiny;
+                bool ldone = std::fabs(tbl_alpha[itb_x] - xmid) <= 2.0 * tolx - 0.5 * (tbl_alpha[itb_b] - tbl_alpha[itb_a]);
+                if (!ldone) {
+                    bool use_parab;
+                    if (std::fabs(dst_prv2) <= tolx) {
+                        use_parab = false;
+                    } else {
+                        double br = (tbl_alpha[itb_x] - tbl_alpha[itb_w]) * (tbl_rho[itb_x] - tbl_rho[itb_v]);
+                        double bq = (tbl_alpha[itb_x] - tbl_alpha[itb_v]) * (tbl_rho[itb_x] - tbl_rho[itb_w]);
+                        double bp = (tbl_alpha[itb_x] - tbl_alpha[itb_v]) * bq - (tbl_alpha[itb_x] - tbl_alpha[itb_w]) * br;
+                        bq = 2.0 * (bq - br);
+                        if (bq > 0.0) bp = -bp;
+                        bq = std::fabs(bq);
+                        if (std::fabs(bp) >= std::fabs(0.5 * bq * dst_prv2) || bp <= bq * (tbl_alpha[itb_a] - tbl_alpha[itb_x]) ||
+                            bp >= bq * (tbl_alpha[itb_b] - tbl_alpha[itb_x])) {
+                            use_parab = false;
+                        } else {
+                            use_parab = true;
+                            dst_cur = bp / bq;
+                            alpha_j = tbl_alpha[itb_x] + dst_cur;
+                            if (alpha_j - tbl_alpha[itb_a] < 2.0 * tolx || tbl_alpha[itb_b] - alpha_j < 2.0 * tolx)
+                                dst_cur = tolx * (xmid - tbl_alpha[itb_x] >= 0 ? 1.0 : -1.0);
+                        }
+                    }
+                    if (!use_parab) {
+                        if (tbl_alpha[itb_x] >= xmid) dst_prv1 = tbl_alpha[itb_a] - tbl_alpha[itb_x];
+                        else dst_prv1 = tbl_alpha[itb_b] - tbl_alpha[itb_x];
+                        dst_cur = 0.381966 * dst_prv1;
+                    }
+                    alpha_j = tbl_alpha[itb_x] + dst_cur;
+                }
+            } else if (j == 1) {
+                alpha_j = alpha_j + 0.6 * alpha0;
+                ddrho = 10.0;
+            } else if (tbl_rho[j] < tbl_rho[1]) {
+                double dalph = tbl_alpha[j] - tbl_alpha[j - 1];
+                if (j < 3) {
+                    ddrho = (tbl_drhoda[j] - tbl_drhoda[j - 1]) / dalph;
+                    alpha_j = tbl_alpha[j] + std::min(3.0 * dalph, -tbl_drhoda[j - 1] / ddrho);
+                } else {
+                    alpha_j = tbl_alpha[j] + 3.0 * dalph;
+                }
+            } else {
+                double dalph = tbl_alpha[2] - tbl_alpha[1];
+                if (j < 3) {
+                    ddrho = (tbl_drhoda[2] - tbl_drhoda[1]) / dalph;
+                    alpha_j = tbl_alpha[1] + std::min(-3.0 * dalph, tbl_drhoda[1] / ddrho);
+                } else {
+                    alpha_j = tbl_alpha[1] - 3.0 * dalph;
+                }
+            }
+            stop_j = (j >= MAX_J || std::fabs(alpha_j - alpha_prv) < eps_j * std::max(std::fabs(alpha_j), std::fabs(alpha_prv)));
+        }
+        (void)ddrho;
+        if (!has_bracket) {   // the reference aborts the run here
+            *itgd_out = itgd;
+            *err_out = dif;
+            co_return ERR_TANG;
+        }
+        if (j >= MAX_J) {
+            int best = 1;
+            for (int k = 2; k <= MAX_J; k++)
+                if (tbl_rho[k] < tbl_rho[best]) best = k;
+            alpha_j = tbl_alpha[best];
+        }
+        alpha = alpha_j;
+        beta = alpha * std::sqrt(sum_dq2) / std::sqrt(sum_r2);
+        // update p and the slip
+        AnyItem iu = it;
+        iu.gd.alpha = alpha;
+        C.op(OP_GD_UPDATE, iu);
+        C.set(M_ALL, P.sx, 0.0, P.sy);
+        C.conv_t(0, M_INT, P.sx, P.sy, G.dpx, G.dpy, G.wsx, G.wsy);
+        r = co_await C.opr(OP_GD_MOVES, it);
+        lchanged = r.v[0] > 0;
+        AnyItem idr = it;
+        idr.gd.first = 0;
+        r = co_await C.opr(OP_GD_DIAGRES, idr);
+        resrms2 = r.v[2];
+        double facdif;
+        if (beta >= 0.1) facdif = 1.0;
+        else if (beta >= 0.001) facdif = 0.1 / beta;
+        else facdif = 100.0;
+        dif = facnel * std::sqrt(r.v[0] / (2.0 * npot)) * facdif;
+        difid = eps * std::max(1e-6, facnel * std::sqrt(r.v[1] / (2.0 * npot)));
+        if (itgd == 1) dif1 = dif;
+    }
+    {
+        Res c2 = co_await C.count(M_INT);
+        double resrms = std::sqrt(resrms2 / std::max(1.0, 2.0 * c2.v[0]));
+        double res_dp = resrms / std::max(G.c11, G.c22);
+        double conv = 1.0;
+        if (dif * dif1 > 0.0 && itgd > 1) conv = std::exp(std::log(dif / dif1) / (itgd - 1));
+        if (lchanged && itgd >= maxgd) *lstagn = 1;
+        else if (dif > difid && conv > 1.0 && itgd >= maxgd) *lstagn = 1;
+        else if (itgd > 0 && conv < 1.0 - tiny && res_dp > 5.0 * difid / (1.0 - conv)) { itgd = -itgd; *lstagn = 1; }
+    }
+    *itgd_out = itgd;
+    *err_out = dif;
+    co_return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// TangCG, src/m_solvpt.f90:1841-2442
+// ------------------------------------------------------------------------------------------
+
+static Task tangcg(Ctx C, int maxcg, double eps, int* itcg_out, double* err_out)
+{
+    Patch& P = *C.P;
+    const int num_inn = 4;
+    const double small = 1e-6;
+    const int npot = P.n;
+    const bool fp32 = P.in.precision == 1;
+    double c11, c22;
+    {
+        int mx = P.mx, my = P.my;
+        size_t nb = (size_t)(2 * mx) * (2 * my), k = (size_t)my * (2 * mx) + mx;
+        double h[2];
+        HIPCHK(hipMemcpyAsync(&h[0], P.cf + 0 * nb + k, 8, hipMemcpyDeviceToHost, C.E->stream()));
+        HIPCHK(hipMemcpyAsync(&h[1], P.cf + 4 * nb + k, 8, hipMemcpyDeviceToHost, C.E->stream()));
+        HIPCHK(hipStreamSynchronize(C.E->stream()));
+        c11 = h[0];
+        c22 = h[1];
+    }
+    co_await setup_prec(C, 0);
+    co_await setup_prec(C, 1);
+    TcgItem T{};
+    T.n = npot; T.el = P.el; T.px = P.px; T.py = P.py; T.pn = P.pn; T.sx = P.sx; T.sy = P.sy;
+    T.nx = P.w[0]; T.ny = P.w[1]; T.rx = P.w[2]; T.ry = P.w[3]; T.zx = P.w[4]; T.zy = P.w[5];
+    T.vx = P.w[6]; T.vy = P.w[7]; T.qx = P.w[8]; T.qy = P.w[9]; T.poldx = P.w[10]; T.poldy = P.w[11];
+    T.mu = P.in.fstat; T.ga = P.ga; T.c11 = c11; T.c22 = c22;
+    AnyItem t;
+    t.tcg = T;
+    Res r;
+    r = co_await C.count(M_ADH);
+    double nadh = r.v[0];
+    r = co_await C.count(M_INT);
+    double ncon = r.v[0];
+    double nslip = ncon - nadh;
+    double facnel = (double)std::sqrt((float)npot / (float)ncon);
+    bool use_fftprec = true;
+    int itcg = 0, it_inn = 0;
+    bool lchanged = false;
+    double dif = 2.0, difid = 1.0, difinn = 0.0, trsinn = 0.0, alpha = 0.0;
+    C.set(M_ALL, T.vx, 0.0, T.vy);
+    C.set(M_ALL, T.qx, 0.0, T.qy);
+    C.set(M_ALL, T.zx, 0.0, T.zy);
+    C.set(M_ALL, T.poldx, 0.0, T.poldy);
+    C.op(OP_TCG_NT, t);
+    C.set(M_ALL, P.sx, 0.0, P.sy);
+    C.conv_t(0, M_INT, P.sx, P.sy, P.px, P.py, P.wsx, P.wsy);
+    C.op(OP_TCG_RSET, t);
+    while ((lchanged || dif > difid) && itcg < maxcg) {
+        itcg++;
+        it_inn++;
+        if (use_fftprec) {
+            if (nslip > 3 * nadh && P.my > 1) use_fftprec = false;
+            if (itcg >= maxcg / 2) use_fftprec = false;
+            if (!use_fftprec) lchanged = true;
+        }
+        if (use_fftprec) {
+            int ik[2] = {0, 1}, jk[2] = {0, 1};
+            double* o[2] = {T.zx, T.zy};
+            double* i2[2] = {T.rx, T.ry};
+            C.conv(2, M_INT, 2, ik, o, 2, jk, i2, nullptr, fp32);
+        } else {
+            C.copy(M_ALL, T.rx, T.zx, T.ry, T.zy);
+        }
+        r = co_await C.opr(OP_TCG_ZPREP, t);
+        double zq = r.v[0], vq0 = r.v[1];
+        if (itcg <= 1 || lchanged) {
+            C.copy(M_ALL, T.zx, T.vx, T.zy, T.vy);
+        } else {
+            double beta;
+            if (std::fabs(zq) < 1e-60 || std::fabs(vq0) < small * std::fabs(zq)) beta = 0.0;
+            else beta = -zq / vq0;
+            C.axpby(M_ALL, 1.0, T.zx, beta, T.vx, T.zy, T.vy);
+        }
+        C.conv_t(0, M_INT, T.qx, T.qy, T.vx, T.vy, nullptr, nullptr, fp32);
+        r = co_await C.opr(OP_TCG_QPOST, t);
+        double rv = r.v[0], vq = r.v[1];
+        if (std::fabs(rv) < 1e-60) alpha = 0.0;
+        else if (std::fabs(vq) < small * std::fabs(rv)) alpha = 1.0;
+        else alpha = rv / vq;
+        AnyItem tu = t;
+        tu.tcg.alpha = alpha;
+        r = co_await C.opr(OP_TCG_PUPD, tu);
+        dif = alpha * facnel * std::sqrt(r.v[0] / (2.0 * npot));
+        difinn += dif;
+        if (itcg <= 5 || itcg % 5 == 1) {
+            double ptang = facnel * std::sqrt(r.v[1] / (2.0 * npot));
+            difid = eps * std::max(1e-6, ptang);
+            trsinn = sp(0.01f) * ptang;
+        }
+        if (it_inn >= num_inn || dif <= difid || difinn > trsinn) {
+            lchanged = false;
+            r = co_await C.opr(OP_TCG_AS, t);
+            if (r.v[0] > 0) lchanged = true;
+            C.conv_t(0, M_INT, P.sx, P.sy, P.px, P.py, P.wsx, P.wsy);
+            r = co_await C.opr(OP_TCG_SA, t);
+            if (r.v[0] > 0) lchanged = true;
+            dif = facnel * std::sqrt(r.v[1] / (2.0 * npot));
+            it_inn = 0;
+            difinn = 0.0;
+            if (lchanged) {
+                Res a = co_await C.count(M_ADH);
+                Res b = co_await C.count(M_INT);
+                nadh = a.v[0];
+                nslip = b.v[0] - a.v[0];
+            }
+        }
+        if ((lchanged || dif > difid) && itcg < maxcg) {
+            if (it_inn <= 0) {
+                C.op(OP_TCG_NT, t);
+                C.op(OP_TCG_RSET, t);
+            } else {
+                C.axpy(M_INT, -alpha, T.qx, T.rx, T.qy, T.ry);
+            }
+        }
+    }
+    *itcg_out = itcg;
+    *err_out = dif;
+    co_return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// stang, src/m_stang.f90:28-951 (Coulomb friction, creepages prescribed)
+// ------------------------------------------------------------------------------------------
+
+static Task stang(Ctx C, int* ittang_out)
+{
+    Patch& P = *C.P;
+    Res r;
+    r = co_await C.count(M_INT);
+    if (r.v[0] <= 0) co_return ERR_OTHER;
+    // stang_rhs: ws = -facdt h_t (- A'_tt(cv) pv_t for T=2) on the interior; facdt = 1 for
+    // interior elements when dq = dx (src/m_leadedge.f90:290-318)
+    C.set(M_ALL, P.wsx, 0.0, P.wsy);
+    C.axpy(M_INT, -1.0, P.hx, P.wsx, P.hy, P.wsy);
+    if (P.in.tang == 2) {
+        double *ux = P.w[20], *uy = P.w[21];
+        C.conv_t(1, M_INT, ux, uy, P.pvx, P.pvy);
+        C.axpy(M_INT, -1.0, ux, P.wsx, uy, P.wsy);
+    }
+    int ittang = 0, itgs = 0, ret = 0;
+    bool zready = false;
+    while (!zready && ittang < P.in.maxin) {
+        ittang++;
+        zready = true;
+        int it = 0;
+        double errpt = 0.0;
+        int rc;
+        if (P.in.tang == 3) {
+            int lstagn = 0;
+            rc = co_await gdsteady(C, P.in.maxgs, P.in.eps, &it, &errpt, &lstagn);
+            P.out.gd_stagnated = lstagn;
+        } else {
+            rc = co_await tangcg(C, P.in.maxgs, P.in.eps, &it, &errpt);
+        }
+        itgs += std::abs(it);
+        if (rc) { ret = rc; break; }
+        StItem S{};
+        S.n = P.n; S.el = P.el; S.px = P.px; S.py = P.py; S.pn = P.pn; S.sx = P.sx; S.sy = P.sy;
+        S.mu = P.in.fstat;
+        S.tol = std::sqrt(2.0) * errpt;
+        S.errpt = errpt;
+        AnyItem sa;
+        sa.st = S;
+        r = co_await C.opr(OP_ST_AS, sa);
+        if (r.v[0] != 0) zready = false;
+        if (zready) {
+            Res a = co_await aij_ones(C, 0);
+            Res b = co_await aij_ones(C, 4);
+            sa.st.tol1 = errpt * 2.0 * std::fabs(a.v[0] / P.ga);
+            sa.st.tol2 = errpt * 2.0 * std::fabs(b.v[0] / P.ga);
+            r = co_await C.opr(OP_ST_SA, sa);
+            if (r.v[0] != 0) zready = false;
+        }
+    }
+    if (!zready && !ret) ittang = -1;
+    P.out.itgs = itgs;
+    *ittang_out = ittang;
+    co_return ret;
+}
+
+// ------------------------------------------------------------------------------------------
+// one case: panprc (src/m_scontc.f90:356-553) and the force integration of soutpt
+// (src/m_soutpt.f90:424-450)
+// ------------------------------------------------------------------------------------------
+
+static SetupItem setup_item(Patch& P)
+{
+    SetupItem S{};
+    S.n = P.n; S.mx = P.mx; S.my = P.my; S.ibase = P.ibase_eff;
+    S.el = P.el; S.h = P.h; S.hx = P.hx; S.hy = P.hy;
+    S.pn = P.pn; S.px = P.px; S.py = P.py; S.pvn = P.pvn; S.pvx = P.pvx; S.pvy = P.pvy;
+    S.xc1 = P.xc1; S.yc1 = P.yc1; S.dx = P.dx; S.dy = P.dy;
+    for (int k = 0; k < 6; k++) S.prm[k] = P.prm[k];
+    return S;
+}
+
+static Task calculate(Ctx C)
+{
+    Patch& P = *C.P;
+    CaseResult& R = P.out;
+    {
+        // set_prev_data, set_norm_rhs, set_tang_rhs, init_curr_data (src/m_sdis.f90:208-768)
+        AnyItem it;
+        it.setup = setup_item(P);
+        SetupItem& S = it.setup;
+        S.prev = P.in.pvtime == 0;
+        S.zero_pn = P.in.iestim == 0;
+        S.zero_pt = P.in.iestim == 0 || P.in.tang == 0;
+        S.hertz = P.hertz_init;
+        S.cksi = P.in.cksi; S.ceta = P.in.ceta; S.cphi = P.in.cphi; S.dq = P.dq;
+        S.xofs = P.xofs; S.yofs = P.yofs;
+        S.hz_aa = R.hz_aa; S.hz_bb = R.hz_bb; S.pnmax = P.hz_pnmax;
+        Res r = co_await C.opr(OP_SETUP, it);
+        P.hmax = r.v[0];
+        P.hmin = -r.v[1];
+    }
+    co_await setup_coefficients(C, P.in.tang == 2);
+    int itnorm = 0, ittang = 0;
+    int rc = co_await snorm(C, &itnorm);
+    if (rc == 0) {
+        R.itnorm = itnorm;
+        C.set(M_EXT, P.px, 0.0, P.py);
+        C.set(M_EXT, P.pn, 0.0);
+        Res r = co_await C.count(M_INT);
+        if (P.in.tang == 0 || r.v[0] <= 0) {
+            C.set(M_ALL, P.sx, 0.0, P.sy);
+        } else {
+            rc = co_await stang(C, &ittang);
+            R.ittang = ittang;
+        }
+    }
+    // forces (soutpt, src/m_soutpt.f90:424-450)
+    AnyItem fi;
+    fi.setup = setup_item(P);
+    Res s = co_await C.opr(OP_FORCES, fi);
+    if (P.in.norm == 0) P.fntrue = P.dxdy * s.v[0];
+    double den = P.fntrue * P.muscal + TINY;
+    double fxrel = P.dxdy * s.v[1] / den, fyrel = P.dxdy * s.v[2] / den;
+    R.fx = fxrel * den;
+    R.fy = fyrel * den;
+    R.fn = P.fntrue;
+    R.mz = -P.dxdy * s.v[3] + P.dxdy * s.v[4];
+    if (std::fabs(R.mz) < 0.5 * P.in.eps * (P.muscal * P.fntrue + TINY)) R.mz = 0.0;
+    R.pen = P.pen;
+    P.have_solution = true;
+    if (rc) R.ierror = rc;
+    else if (R.itnorm < 0) R.ierror = ERR_NORM;
+    else if (R.ittang < 0) R.ierror = ERR_TANG;
+    co_return R.ierror;
+}
+
+int Engine::solve(const std::vector<Patch*>& patches)
+{
+    last_error.clear();
+    std::vector<Patch*> ok;
+    for (Patch* p : patches)
+        if (prepare(*p)) ok.push_back(p);
+    HIPCHK(hipStreamSynchronize(stream_));
+    sched_->resize((int)ok.size());
+    std::vector<Task> tasks;
+    std::vector<int> slots;
+    for (size_t i = 0; i < ok.size(); i++) {
+        ok[i]->slot = (int)i;
+        tasks.push_back(calculate(Ctx{sched_.get(), ok[i], (int)i, this}));
+        slots.push_back((int)i);
+    }
+    sched_->run(tasks, slots);
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+void Engine::set_profile(bool on)
+{
+    sched_->profile = on;
+    sched_->prof.clear();
+    sched_->nflush = 0;
+}
+
+int Engine::get_profile(int len, int* kinds, double* seconds, double* bytes, long* launches) const
+{
+    int k = 0;
+    for (auto& kv : sched_->prof) {
+        if (k < len) {
+            kinds[k] = kv.first;
+            seconds[k] = kv.second.seconds;
+            bytes[k] = kv.second.bytes;
+            launches[k] = kv.second.launches;
+        }
+        k++;
+    }
+    return k;
+}
+
+long Engine::rounds() const { return sched_->nflush; }
+
+void Engine::get_field(const Patch& p, int what, double* out) const
+{
+    const double* src = nullptr;
+    switch (what) {
+    case 0: src = p.pn; break;
+    case 1: src = p.px; break;
+    case 2: src = p.py; break;
+    case 3: src = p.sx; break;
+    case 4: src = p.sy; break;
+    case 5: src = p.h; break;
+    }
+    if (!src || p.n <= 0) return;
+    HIPCHK(hipMemcpyAsync(out, src, (size_t)p.n * 8, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+}
+
+void Engine::get_eldiv(const Patch& p, int* out) const
+{
+    if (p.n <= 0) return;
+    std::vector<int8_t> e(p.n);
+    HIPCHK(hipMemcpyAsync(e.data(), p.el, p.n, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (int i = 0; i < p.n; i++) out[i] = e[i];
+}
+
+}  // namespace cntc

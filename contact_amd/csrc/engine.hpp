@@ -1,0 +1,122 @@
+// engine.hpp -- host side of the HIP contact solver: case inputs, per-patch device state and
+// the batched scheduler that runs many patch solves through shared kernel launches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace cntc {
+
+// Inputs of one case, in CONTACT's internal units (mm, N, N/mm^2).  Defaults are the
+// module-3 defaults of the cntc_* library (src/m_hierarch_data.f90:966-1014, 1728-1747,
+// src/m_global_data.f90:336-350).
+struct CaseInput {
+    int tang = 3, norm = 0, pvtime = 2, iestim = 0, gausei = 0;
+    double poiss1 = 0.28, poiss2 = 0.28, gg1 = 1.0, gg2 = 1.0;
+    double fstat = 0.30;
+    int ipotcn = 0;                 // 1..4 grid forms, -1/-3 Hertzian; 0 = not yet set
+    int mx = 1, my = 1;
+    double xl = 0, yl = 0, xh = 0, yh = 0, xc1 = 0, yc1 = 0, xcm = 0, ycm = 0, dx = 1, dy = 1;
+    double hz_a1 = 0, hz_b1 = 0, hz_aa = 0, hz_bb = 0, hz_scale = 1.0;
+    int ibase = 0;                  // 1 quadratic, 9 element-wise; 0 = not yet set
+    double prmudf[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<double> h_elem;
+    double pen = 0, fn = 0, cksi = 0, ceta = 0, cphi = 0, chi = 0, dq = -1.0, veloc = 1e3;
+    int maxgs = 999, maxin = 20, maxnr = 25, maxout = 1;
+    double eps = 1e-5;
+    double fdecay = (double)0.90f, betath = (double)0.10f, d_ifc = 1.0, d_lin = 1.0, d_cns = 1.0,
+           d_slp = 1.0, pow_s = (double)0.90f;
+    int kdowfb = 1;
+    int precision = 0;              // 0: FP64 products; 1: FP32 products in FP64 refinement
+};
+
+// Results of one case.
+struct CaseResult {
+    int ierror = 0;
+    double fn = 0, fx = 0, fy = 0, mz = 0;
+    double pen = 0;
+    int itnorm = 0, ittang = 0, itcg = 0, itgs = 0;
+    int gd_stagnated = 0;
+    double hz_a1 = 0, hz_b1 = 0, hz_aa = 0, hz_bb = 0, hz_rho = 0, hz_cp = 0;
+};
+
+struct DeviceArena {
+    char* base = nullptr;
+    size_t bytes = 0;
+    ~DeviceArena();
+    void ensure(size_t n);
+};
+
+// One contact problem (a result element / contact patch pair of the library): inputs of the
+// last case, the grid, and the device-resident grid functions that persist between cases.
+struct Patch {
+    CaseInput in;
+    CaseResult out;
+    int mx = 0, my = 0, n = 0;
+    double dx = 1, dy = 1, dxdy = 1, xc1 = 0, yc1 = 0;
+    int Lx = 0, Ly = 0;
+    bool have_solution = false;
+    int slot = -1;                   // scheduler slot during a solve
+    // material and kinematics of the current case
+    double ga = 1, nu = 0, ak = 0, muscal = 0.3, fntrue = 0, pen = 0, dq = 1, chi = 0;
+    bool nt_cpl = false;
+    std::vector<double> h_host;      // undeformed distance, host copy where needed
+    std::vector<int8_t> el_host;
+    double hmin = 0, hmax = 0;
+    int ibase_eff = 1;
+    double prm[6] = {0, 0, 0, 0, 0, 0};
+    bool hertz_init = false;
+    double hz_pnmax = 0, xofs = 0, yofs = 0;
+    // device memory
+    DeviceArena mem;
+    size_t mem_need = 0;
+    int8_t* el = nullptr;
+    double *pn = nullptr, *px = nullptr, *py = nullptr, *sx = nullptr, *sy = nullptr;
+    double *pvx = nullptr, *pvy = nullptr, *pvn = nullptr;
+    double *h = nullptr, *hx = nullptr, *hy = nullptr, *wsx = nullptr, *wsy = nullptr;
+    double* w[24] = {};              // work vectors
+    double *cf = nullptr, *cfv = nullptr, *cfm = nullptr;  // coefficient blocks cs, cv, ms
+    void *spec = nullptr, *specv = nullptr, *specm = nullptr, *spec32 = nullptr, *specm32 = nullptr;
+    double* embed = nullptr;         // padded real scratch for spectra (Lx*Ly, or 4*cf_mx*cf_my)
+    void *wsS = nullptr, *wsT = nullptr;
+    const void *twM = nullptr, *twLx = nullptr, *twLy = nullptr;
+    const void *twM32 = nullptr, *twLx32 = nullptr, *twLy32 = nullptr;
+    const void *ptwM = nullptr, *ptwLx = nullptr, *ptwLy = nullptr;   // preconditioner sizes
+};
+
+class Scheduler;
+
+// The engine owns the stream, the staging buffers and the twiddle-table cache.
+class Engine {
+public:
+    Engine();
+    ~Engine();
+    // Solve one case for every patch (each with its own inputs).  Returns 0 when every case
+    // ran; individual outcomes are in Patch::out.
+    int solve(const std::vector<Patch*>& patches);
+    hipStream_t stream() const { return stream_; }
+    // download the element data of a patch
+    void get_field(const Patch& p, int what, double* out) const;
+    void get_eldiv(const Patch& p, int* out) const;
+    // twiddle tables exp(-2 pi i e / L), e < L, cached per length and precision
+    const void* twiddles(int L, bool fp32);
+    std::string last_error;
+    // per-kernel timing (HIP events around every launch); key = kind*4 + stage
+    void set_profile(bool on);
+    int get_profile(int len, int* kinds, double* seconds, double* bytes, long* launches) const;
+    long rounds() const;
+
+private:
+    bool prepare(Patch& p);
+    hipStream_t stream_ = nullptr;
+    std::unique_ptr<Scheduler> sched_;
+    std::vector<std::pair<long, void*>> tw_;
+};
+
+int opt_fft_size(int n);
+
+}  // namespace cntc

@@ -1,0 +1,1692 @@
+// kernels.hip -- HIP kernels (gfx950) of the batched contact solve.
+//
+// Grid functions are stored per component (SoA), x fastest, element ii = ix + iy*mx
+// (src/m_gridfunc.f90:157-181).  Element states are int8.  All kernels are batched: a launch
+// serves a list of per-patch items, each owning a contiguous range of work-groups.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+
+#include "fft_core.hpp"
+#include "ops.hpp"
+#include "kernels.hpp"
+
+namespace cntc {
+
+constexpr int ELEM_THREADS = 256;
+constexpr int ELEM_PER_THREAD = 4;
+constexpr int ROW_THREADS = 64;
+constexpr int FFT_THREADS = 256;
+constexpr int MAXK = 12;
+
+// ------------------------------------------------------------------------------------------
+// item lookup and reductions
+// ------------------------------------------------------------------------------------------
+
+__device__ inline int find_item(const int* gstart, int nitems, int g)
+{
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (gstart[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ inline bool in_mask(int mask, int8_t e)
+{
+    switch (mask) {
+    case M_ALL: return true;
+    case M_INT: return e >= ADHES;
+    case M_EXT: return e == EXTER;
+    case M_ADH: return e == ADHES;
+    case M_SLP: return e == SLIP;
+    }
+    return false;
+}
+
+// combine modes: 0 sum, 1 max, 2 min
+__device__ inline double combine(double a, double b, int mode)
+{
+    return mode == 0 ? a + b : (mode == 1 ? fmax(a, b) : fmin(a, b));
+}
+
+// work-group reduction of K values; result valid in thread 0.  Wave-level steps use
+// cross-lane shuffles (DPP on CDNA), then one LDS exchange between the waves.
+template <int K>
+__device__ inline void group_reduce(double (&v)[K], int mode, double* lds)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwave = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x = combine(x, __shfl_xor(x, off), mode);
+        v[k] = x;
+    }
+    __syncthreads();
+    if (lane == 0)
+        for (int k = 0; k < K; k++) lds[wave * K + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < K; k++) {
+            double x = lds[k];
+            for (int w = 1; w < nwave; w++) x = combine(x, lds[w * K + k], mode);
+            v[k] = x;
+        }
+    }
+}
+
+template <int K>
+__device__ inline void write_partials(double (&v)[K], int mode, double* partials, int kstride)
+{
+    __shared__ double red_lds[16 * MAXK];
+    group_reduce<K>(v, mode, red_lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < K; k++) partials[(size_t)blockIdx.x * kstride + k] = v[k];
+}
+
+// finalize: one thread per (item, k) adds the partials of the item's groups in order
+__global__ void k_finalize(const int* gstart, const int* modes, int nitems, int K,
+                           const double* partials, double* results)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nitems * K) return;
+    int i = t / K, k = t - i * K;
+    int mode = modes[i];
+    double x = mode == 0 ? 0.0 : (mode == 1 ? -INFINITY : INFINITY);
+    for (int g = gstart[i]; g < gstart[i + 1]; g++) x = combine(x, partials[(size_t)g * K + k], mode);
+    results[(size_t)i * K + k] = x;
+}
+
+// ------------------------------------------------------------------------------------------
+// masked BLAS-1 (src/m_gridfunc.f90:936-1650)
+// ------------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(ELEM_THREADS) k_vec(const VecItem* items, const int* gstart,
+                                                      int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const VecItem& I = items[it];
+    int g0 = gstart[it];
+    int base = (blockIdx.x - g0) * ELEM_THREADS * ELEM_PER_THREAD;
+    int mode = I.sub == V_MAXABS || I.sub == V_MAX ? 1 : (I.sub == V_MIN ? 2 : 0);
+    double acc[2] = {mode == 0 ? 0.0 : (mode == 1 ? -INFINITY : INFINITY), 0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        if (!in_mask(I.mask, I.el ? I.el[i] : (int8_t)0)) continue;
+        for (int c = 0; c < I.ncomp; c++) {
+            switch (I.sub) {
+            case V_SET: I.x[c][i] = I.a; break;
+            case V_COPY: I.y[c][i] = I.x[c][i]; break;
+            case V_SCAL: I.x[c][i] *= I.a; break;
+            case V_AXPY: I.y[c][i] += I.a * I.x[c][i]; break;
+            case V_AXPBY: I.y[c][i] = I.a * I.x[c][i] + I.b * I.y[c][i]; break;
+            case V_ADDC: I.x[c][i] += I.a; break;
+            case V_DOT: acc[0] += I.x[c][i] * I.y[c][i]; acc[1] += 1.0; break;
+            case V_SUM: acc[0] += I.x[c][i]; acc[1] += 1.0; break;
+            case V_SUMSQ: acc[0] += I.x[c][i] * I.x[c][i]; acc[1] += 1.0; break;
+            case V_MAXABS: acc[0] = fmax(acc[0], fabs(I.x[c][i])); break;
+            case V_MIN: acc[0] = fmin(acc[0], I.x[c][i]); break;
+            case V_MAX: acc[0] = fmax(acc[0], I.x[c][i]); break;
+            case V_COUNT: acc[0] += 1.0; break;
+            }
+        }
+    }
+    if (mode != 0) {
+        double v1[1] = {acc[0]};
+        write_partials<1>(v1, mode, partials, 2);
+        if (threadIdx.x == 0) partials[(size_t)blockIdx.x * 2 + 1] = 0.0;
+    } else {
+        write_partials<2>(acc, 0, partials, 2);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// NormCG and snorm element steps
+// ------------------------------------------------------------------------------------------
+
+// interior elements with negative pressure move to the exterior (src/m_solvpn.f90:302-311);
+// returns (#moved, #interior after, sum pn)
+__global__ void __launch_bounds__(ELEM_THREADS) k_ncg_negpn(const NcgItem* items, const int* gstart,
+                                                            int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const NcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        int8_t s = I.el[i];
+        if (s >= ADHES && I.p[i] < 0.0) {
+            I.el[i] = EXTER;
+            I.p[i] = 0.0;
+            s = EXTER;
+            acc[0] += 1.0;
+        }
+        if (s >= ADHES) acc[1] += 1.0;
+        acc[2] += I.p[i];
+    }
+    write_partials<3>(acc, 0, partials, 3);
+}
+
+// elements at the deepest interpenetration enter: mode 1 at the start of NormCG with N=1
+// (src/m_solvpn.f90:140-152), mode 2 when no contact is left (src/m_solvpn.f90:316-326)
+__global__ void __launch_bounds__(ELEM_THREADS) k_ncg_hmin(const NcgItem* items, const int* gstart,
+                                                           int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const NcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[1] = {0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        bool enter = I.mode == 1 ? I.h[i] < I.hmin : I.h[i] <= I.hmin;
+        if (enter) {
+            if (I.el[i] < ADHES) acc[0] += 1.0;
+            I.el[i] = ADHES;
+            if (I.mode == 2) I.p[i] = 0.0;
+        }
+    }
+    write_partials<1>(acc, 0, partials, 1);
+}
+
+// dd = A ps - rhs on all elements; returns (sum dd, count) over the interior
+__global__ void __launch_bounds__(ELEM_THREADS) k_ncg_dd(const NcgItem* items, const int* gstart,
+                                                         int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const NcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[2] = {0.0, 0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        double d = I.dd[i] - I.rhs[i];
+        I.dd[i] = d;
+        if (I.el[i] >= ADHES) { acc[0] += d; acc[1] += 1.0; }
+    }
+    write_partials<2>(acc, 0, partials, 2);
+}
+
+// proj_avg of dd, residual, and interpenetration check of the exterior
+// (src/m_solvpn.f90:351-390); returns #elements that entered
+__global__ void __launch_bounds__(ELEM_THREADS) k_ncg_expand(const NcgItem* items, const int* gstart,
+                                                             int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const NcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[1] = {0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        int8_t s = I.el[i];
+        double d = I.dd[i];
+        if (s >= ADHES) {
+            d -= I.avg;
+            I.dd[i] = d;
+            I.res[i] = -d;
+        } else {
+            I.res[i] = 0.0;
+            if (s == EXTER && d - I.davg < 0.0) {
+                I.el[i] = ADHES;
+                s = ADHES;
+                I.res[i] = -(d - I.davg);
+                acc[0] += 1.0;
+            }
+        }
+        if (s == EXTER) I.v[i] = 0.0;
+    }
+    write_partials<1>(acc, 0, partials, 1);
+}
+
+// snorm contract step (src/m_snorm.f90:196-219): returns (#moved out, #small pn)
+__global__ void __launch_bounds__(ELEM_THREADS) k_sn_contract(const NcgItem* items, const int* gstart,
+                                                              int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const NcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[2] = {0.0, 0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        if (I.el[i] >= ADHES) {
+            if (I.p[i] < -I.errpn) {
+                I.el[i] = EXTER;
+                I.p[i] = 0.0;
+                acc[0] += 1.0;
+            }
+            if (fabs(I.p[i]) < I.errpn) acc[1] += 1.0;
+        }
+    }
+    write_partials<2>(acc, 0, partials, 2);
+}
+
+// snorm expand step (src/m_snorm.f90:253-286): returns (#moved in, #small dd)
+__global__ void __launch_bounds__(ELEM_THREADS) k_sn_expand(const NcgItem* items, const int* gstart,
+                                                            int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const NcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[2] = {0.0, 0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        if (I.el[i] == EXTER && I.h[i] - I.pen < 0.0) {
+            double d = I.h[i] - I.pen + I.unn[i];
+            if (d < -I.tol) {
+                I.el[i] = ADHES;
+                acc[0] += 1.0;
+            }
+            if (fabs(d) < I.tol) acc[1] += 1.0;
+        }
+    }
+    write_partials<2>(acc, 0, partials, 2);
+}
+
+// final regularisation (src/m_snorm.f90:315-325)
+__global__ void __launch_bounds__(ELEM_THREADS) k_sn_final(const NcgItem* items, const int* gstart,
+                                                           int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const NcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[1] = {0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        if (I.p[i] < 0.0 && I.el[i] >= ADHES) {
+            I.p[i] = 0.0;
+            I.el[i] = EXTER;
+            acc[0] += 1.0;
+        }
+    }
+    write_partials<1>(acc, 0, partials, 1);
+}
+
+// gf3_AijPj with unit tractions at one element: sum over rows jy of the coefficients for
+// jx in [row1st-1, rowlst+1] of the current contact area (src/m_aijpj.f90:158-185,
+// src/m_gridfunc.f90:582-637).  One group per item, one thread per row.
+__global__ void __launch_bounds__(ROW_THREADS) k_aij_ones(const AijItem* items, const int* gstart,
+                                                          int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const AijItem& I = items[it];
+    double acc[1] = {0.0};
+    int rows_per_group = ROW_THREADS;
+    int jy0 = (blockIdx.x - gstart[it]) * rows_per_group;
+    int jy = jy0 + threadIdx.x + 1;   // 1-based row
+    if (jy <= I.my) {
+        const int8_t* row = I.el + (size_t)(jy - 1) * I.mx;
+        int first = 1;
+        while (row[first - 1] <= EXTER && first != I.mx) first++;
+        int last = 0;
+        for (int ix = first; ix <= I.mx; ix++)
+            if (row[ix - 1] >= ADHES) last = ix;
+        int j0 = first - 1 < 1 ? 1 : first - 1;
+        int j1 = last + 1 > I.mx ? I.mx : last + 1;
+        double rowsum = 0.0;
+        for (int jx = j0; jx <= j1; jx++) {
+            int dx = I.ix - jx, dy = I.iy - jy;
+            if (dx < -I.cf_mx || dx >= I.cf_mx || dy < -I.cf_my || dy >= I.cf_my) continue;
+            rowsum += I.cf[(size_t)(dy + I.cf_my) * (2 * I.cf_mx) + (dx + I.cf_mx)];
+        }
+        acc[0] = rowsum;
+    }
+    write_partials<1>(acc, 0, partials, 1);
+}
+
+// ------------------------------------------------------------------------------------------
+// TangCG element steps (src/m_solvpt.f90:1948-2384)
+// ------------------------------------------------------------------------------------------
+
+__device__ inline void nt_from_p(double px, double py, double& nx, double& ny)
+{
+    double th = atan2(py, px);
+    nx = cos(th);
+    ny = sin(th);
+}
+
+__global__ void __launch_bounds__(ELEM_THREADS) k_tcg_nt(const TcgItem* items, const int* gstart,
+                                                         int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const TcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        double nx, ny;
+        nt_from_p(I.px[i], I.py[i], nx, ny);
+        I.nx[i] = nx;
+        I.ny[i] = ny;
+    }
+}
+
+// r = -s, projected on t in the slip area
+__global__ void __launch_bounds__(ELEM_THREADS) k_tcg_rset(const TcgItem* items, const int* gstart,
+                                                           int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const TcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        double rx = -I.sx[i], ry = -I.sy[i];
+        if (I.el[i] == SLIP) {
+            double tx = -I.ny[i], ty = I.nx[i];
+            double perp = tx * rx + ty * ry;
+            rx = perp * tx;
+            ry = perp * ty;
+        }
+        I.rx[i] = rx;
+        I.ry[i] = ry;
+    }
+}
+
+// Jacobian diagonal scaling and slip projection of z; returns (z.q, v.q) over all elements
+__global__ void __launch_bounds__(ELEM_THREADS) k_tcg_zprep(const TcgItem* items, const int* gstart,
+                                                            int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const TcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[2] = {0.0, 0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        double zx = I.zx[i], zy = I.zy[i];
+        if (I.el[i] == SLIP) {
+            double g = I.mu * I.pn[i];
+            double snrm = -I.sx[i] * I.nx[i] - I.sy[i] * I.ny[i];
+            zx = zx / (I.c11 + I.ga * snrm / g);
+            zy = zy / (I.c22 + I.ga * snrm / g);
+            double tx = -I.ny[i], ty = I.nx[i];
+            double perp = tx * zx + ty * zy;
+            zx = perp * tx;
+            zy = perp * ty;
+        } else {
+            zx = zx / I.c11;
+            zy = zy / I.c22;
+        }
+        I.zx[i] = zx;
+        I.zy[i] = zy;
+        acc[0] += zx * I.qx[i] + zy * I.qy[i];
+        acc[1] += I.vx[i] * I.qx[i] + I.vy[i] * I.qy[i];
+    }
+    write_partials<2>(acc, 0, partials, 2);
+}
+
+// q: slip projection and curvature term; returns (r.v, v.q)
+__global__ void __launch_bounds__(ELEM_THREADS) k_tcg_qpost(const TcgItem* items, const int* gstart,
+                                                            int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const TcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[2] = {0.0, 0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        double qx = I.qx[i], qy = I.qy[i];
+        if (I.el[i] == SLIP) {
+            double tx = -I.ny[i], ty = I.nx[i];
+            double perp = tx * qx + ty * qy;
+            qx = perp * tx;
+            qy = perp * ty;
+            double g = I.mu * I.pn[i];
+            double snrm = (I.nx[i] * I.sx[i] + I.ny[i] * I.sy[i]) / g;
+            qx -= snrm * I.vx[i];
+            qy -= snrm * I.vy[i];
+        }
+        I.qx[i] = qx;
+        I.qy[i] = qy;
+        acc[0] += I.rx[i] * I.vx[i] + I.ry[i] * I.vy[i];
+        acc[1] += I.vx[i] * qx + I.vy[i] * qy;
+    }
+    write_partials<2>(acc, 0, partials, 2);
+}
+
+// pold = p; p += alpha v (interior); slip clipped to the bound; returns (sum v^2, sum p^2)
+__global__ void __launch_bounds__(ELEM_THREADS) k_tcg_pupd(const TcgItem* items, const int* gstart,
+                                                           int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const TcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[2] = {0.0, 0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        double px = I.px[i], py = I.py[i];
+        I.poldx[i] = px;
+        I.poldy[i] = py;
+        int8_t s = I.el[i];
+        if (s >= ADHES) {
+            px += I.alpha * I.vx[i];
+            py += I.alpha * I.vy[i];
+        }
+        if (s == SLIP) {
+            double g = I.mu * I.pn[i];
+            double ptabs = sqrt(px * px + py * py);
+            px = px * g / ptabs;
+            py = py * g / ptabs;
+        }
+        I.px[i] = px;
+        I.py[i] = py;
+        acc[0] += I.vx[i] * I.vx[i] + I.vy[i] * I.vy[i];
+        acc[1] += px * px + py * py;
+    }
+    write_partials<2>(acc, 0, partials, 2);
+}
+
+// adhesion elements above the traction bound move to slip; returns #moved
+__global__ void __launch_bounds__(ELEM_THREADS) k_tcg_as(const TcgItem* items, const int* gstart,
+                                                         int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const TcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[1] = {0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        if (I.el[i] == ADHES) {
+            double g = I.mu * I.pn[i];
+            double px = I.px[i], py = I.py[i];
+            double ptabs = sqrt(px * px + py * py);
+            if (ptabs > g) {
+                I.el[i] = SLIP;
+                I.px[i] = px * g / ptabs;
+                I.py[i] = py * g / ptabs;
+                acc[0] += 1.0;
+            }
+        }
+    }
+    write_partials<1>(acc, 0, partials, 1);
+}
+
+// slip elements with s.p > 0 move to adhesion; pold -= p; returns (#moved, sum (pold-p)^2)
+__global__ void __launch_bounds__(ELEM_THREADS) k_tcg_sa(const TcgItem* items, const int* gstart,
+                                                         int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const TcgItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[2] = {0.0, 0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        if (I.el[i] == SLIP) {
+            double snrm = I.sx[i] * I.px[i] + I.sy[i] * I.py[i];
+            if (snrm > 0.0) {
+                I.el[i] = ADHES;
+                acc[0] += 1.0;
+            }
+        }
+        double dx = I.poldx[i] - I.px[i], dy = I.poldy[i] - I.py[i];
+        I.poldx[i] = dx;
+        I.poldy[i] = dy;
+        acc[1] += dx * dx + dy * dy;
+    }
+    write_partials<2>(acc, 0, partials, 2);
+}
+
+// ------------------------------------------------------------------------------------------
+// stang outer loop moves (src/m_stang.f90:433-508)
+// ------------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(ELEM_THREADS) k_st_as(const StItem* items, const int* gstart,
+                                                        int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const StItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[1] = {0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        if (I.el[i] == ADHES) {
+            double px = I.px[i], py = I.py[i];
+            double pabs = sqrt(px * px + py * py);
+            double g = I.mu * I.pn[i];
+            if (pabs >= g + I.tol) {
+                I.el[i] = SLIP;
+                I.px[i] = px * g / pabs;
+                I.py[i] = py * g / pabs;
+                acc[0] += 1.0;
+            }
+        }
+    }
+    write_partials<1>(acc, 0, partials, 1);
+}
+
+__global__ void __launch_bounds__(ELEM_THREADS) k_st_sa(const StItem* items, const int* gstart,
+                                                        int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const StItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[1] = {0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        if (I.el[i] == SLIP) {
+            double ww = I.sx[i] * I.px[i] + I.sy[i] * I.py[i];
+            double tl = I.tol1 * fabs(I.px[i]) + I.tol2 * fabs(I.py[i]) +
+                        I.errpt * (fabs(I.sx[i]) + fabs(I.sy[i]));
+            if (ww > tl) {
+                I.el[i] = ADHES;
+                acc[0] += 1.0;
+            }
+        }
+    }
+    write_partials<1>(acc, 0, partials, 1);
+}
+
+// ------------------------------------------------------------------------------------------
+// GDsteady (src/gdsteady.f90)
+// ------------------------------------------------------------------------------------------
+
+// compute_dp, src/gdsteady.f90:610-635 (one thread per row)
+__global__ void __launch_bounds__(ROW_THREADS) k_gd_dp(const GdItem* items, const int* gstart,
+                                                       int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GdItem& I = items[it];
+    int iy = (blockIdx.x - gstart[it]) * ROW_THREADS + threadIdx.x;
+    if (iy >= I.my) return;
+    int mx = I.mx;
+    size_t r0 = (size_t)iy * mx;
+    double nxp = I.px[r0 + mx - 1], nyp = I.py[r0 + mx - 1];
+    I.dpx[r0 + mx - 1] = nxp;
+    I.dpy[r0 + mx - 1] = nyp;
+    for (int ix = mx - 2; ix >= 0; ix--) {
+        double px = I.px[r0 + ix], py = I.py[r0 + ix];
+        I.dpx[r0 + ix] = px - nxp;
+        I.dpy[r0 + ix] = py - nyp;
+        nxp = px;
+        nyp = py;
+    }
+}
+
+// project_searchdir, src/gdsteady.f90:639-803: dv from the residual r, with v // t in slip
+__global__ void __launch_bounds__(ROW_THREADS) k_gd_project(const GdItem* items, const int* gstart,
+                                                            int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GdItem& I = items[it];
+    int iy = (blockIdx.x - gstart[it]) * ROW_THREADS + threadIdx.x;
+    if (iy >= I.my) return;
+    const int mx = I.mx;
+    const size_t r0 = (size_t)iy * mx;
+    const int imeth = I.imeth, kdown = I.kdown;
+    double vxn = 0.0, vyn = 0.0;   // v at ix+1
+    I.vx[r0 + mx - 1] = 0.0;
+    I.vy[r0 + mx - 1] = 0.0;
+    I.dvx[r0 + mx - 1] = I.rx[r0 + mx - 1];
+    I.dvy[r0 + mx - 1] = I.ry[r0 + mx - 1];
+    int last_se = 1 << 30;   // smallest processed (1-based) ix in slip/exterior, E_down
+    for (int ix = mx - 2; ix >= 0; ix--) {
+        size_t ii = r0 + ix;
+        int8_t s = I.el[ii];
+        double dvx = I.rx[ii], dvy = I.ry[ii], vx, vy;
+        if (s == ADHES) {
+            if (imeth == 1) {
+                vx = vxn + dvx;
+                vy = vyn + dvy;
+            } else if (imeth == 3) {
+                vx = I.fdecay * vxn + dvx;
+                vy = I.fdecay * vyn + dvy;
+                dvx = vx - vxn;
+                dvy = vy - vyn;
+            } else {
+                int ixf = ix + 1;
+                if (ixf + kdown <= mx) {
+                    double inx = I.rx[r0 + ix + kdown], iny = I.ry[r0 + ix + kdown];
+                    if (last_se <= ixf + kdown - 1) { inx = 0.0; iny = 0.0; }
+                    vx = vxn + dvx - inx;
+                    vy = vyn + dvy - iny;
+                } else {
+                    vx = vxn + dvx;
+                    vy = vyn + dvy;
+                }
+                dvx = vx - vxn;
+                dvy = vy - vyn;
+            }
+        } else if (s == SLIP) {
+            double g = I.mu * I.pn[ii];
+            double tx = -I.ny[ii], ty = I.nx[ii];
+            double vt = tx * dvx + ty * dvy;
+            vt = copysign(1.0, vt) * fmin(fabs(vt), I.fac_v * g);
+            vx = tx * vt;
+            vy = ty * vt;
+            dvx = vx - vxn;
+            dvy = vy - vyn;
+            last_se = ix + 1;
+        } else {
+            vx = 0.0;
+            vy = 0.0;
+            dvx = -vxn;
+            dvy = -vyn;
+            last_se = ix + 1;
+        }
+        I.vx[ii] = vx;
+        I.vy[ii] = vy;
+        I.dvx[ii] = dvx;
+        I.dvy[ii] = dvy;
+        vxn = vx;
+        vyn = vy;
+    }
+}
+
+// line-search probe at step alpha (src/gdsteady.f90:1090-1158): apply_trcbnd on
+// dp + alpha dv, then the residual terms of the new orientation.  Returns
+// (rho, c0adh, c1adh, c0slp, c1slp, c2slp) and, when I.first, also
+// (sum r^2, sum r.q dscl, sum_int (dscl q)^2, sum_int r^2).
+__global__ void __launch_bounds__(ROW_THREADS) k_gd_probe(const GdItem* items, const int* gstart,
+                                                          int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GdItem& I = items[it];
+    int iy = (blockIdx.x - gstart[it]) * ROW_THREADS + threadIdx.x;
+    double acc[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double tiny = 1e-12;
+    const double a = I.alpha;
+    if (iy < I.my) {
+        const int mx = I.mx;
+        const size_t r0 = (size_t)iy * mx;
+        double pxn = 0.0, pyn = 0.0;   // psnew at ix+1
+        for (int ix = mx - 1; ix >= 0; ix--) {
+            size_t ii = r0 + ix;
+            int8_t s = I.el[ii];
+            double g = I.mu * I.pn[ii];
+            double px, py;
+            if (ix == mx - 1 || s <= EXTER) {
+                px = 0.0;
+                py = 0.0;
+            } else {
+                double dpx = I.dpx[ii] + a * I.dvx[ii], dpy = I.dpy[ii] + a * I.dvy[ii];
+                px = pxn + dpx;
+                py = pyn + dpy;
+                double ptabs = sqrt(px * px + py * py);
+                if (s == SLIP || ptabs >= g) {
+                    px = px * g / ptabs;
+                    py = py * g / ptabs;
+                }
+            }
+            pxn = px;
+            pyn = py;
+            double dscl = I.dscl[ii];
+            double sx = I.sx[ii], sy = I.sy[ii], qx = I.qx[ii], qy = I.qy[ii];
+            if (I.first) {
+                double rx = I.rx[ii], ry = I.ry[ii];
+                acc[6] += rx * rx + ry * ry;
+                acc[7] += (rx * qx + ry * qy) * dscl;
+                if (s >= ADHES) {
+                    acc[8] += dscl * dscl * (qx * qx + qy * qy);
+                    acc[9] += rx * rx + ry * ry;
+                }
+            }
+            if (s != ADHES && s != SLIP) continue;
+            double txo = -I.ny[ii], tyo = I.nx[ii];
+            double vt = I.vx[ii] * txo + I.vy[ii] * tyo;
+            double dth_da = (g * vt) / (fmax(tiny, g * g) + a * a * vt * vt);
+            double nxn, nyn;
+            nt_from_p(px, py, nxn, nyn);
+            double txn = -nyn, tyn = nxn;
+            double sn = sx * nxn + sy * nyn;
+            int8_t elnew = s;
+            if (elnew == SLIP && sn > 0.0) elnew = ADHES;
+            double st = sx * txn + sy * tyn, rt = -st;
+            double qn = qx * nxn + qy * nyn, qt = qx * txn + qy * tyn;
+            double d2 = dscl * dscl;
+            if (elnew == ADHES) {
+                acc[1] += 2.0 * d2 * (sx * qx + sy * qy);
+                acc[2] += 2.0 * d2 * (qx * qx + qy * qy);
+                double ex = sx + a * qx, ey = sy + a * qy;
+                acc[0] += d2 * (ex * ex + ey * ey);
+            } else {
+                acc[3] -= 2.0 * d2 * (rt * qt - rt * sn * dth_da);
+                acc[4] -= 2.0 * d2 * (-qt * qt + qt * sn * dth_da - rt * qn * dth_da);
+                acc[5] -= 2.0 * d2 * (qt * qn * dth_da);
+                double ex = st + a * qt;
+                acc[0] += d2 * ex * ex;
+            }
+        }
+    }
+    write_partials<10>(acc, 0, partials, 10);
+}
+
+// pold = p; dp += alpha dv on the interior; apply_trcbnd (src/gdsteady.f90:807-847); n from p
+__global__ void __launch_bounds__(ROW_THREADS) k_gd_update(const GdItem* items, const int* gstart,
+                                                           int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GdItem& I = items[it];
+    int iy = (blockIdx.x - gstart[it]) * ROW_THREADS + threadIdx.x;
+    if (iy >= I.my) return;
+    const int mx = I.mx;
+    const size_t r0 = (size_t)iy * mx;
+    double pxn = 0.0, pyn = 0.0;
+    for (int ix = mx - 1; ix >= 0; ix--) {
+        size_t ii = r0 + ix;
+        int8_t s = I.el[ii];
+        I.poldx[ii] = I.px[ii];
+        I.poldy[ii] = I.py[ii];
+        double dpx = I.dpx[ii], dpy = I.dpy[ii];
+        if (s >= ADHES) {
+            dpx += I.alpha * I.dvx[ii];
+            dpy += I.alpha * I.dvy[ii];
+        }
+        double px, py;
+        if (ix == mx - 1) {
+            px = 0.0;
+            py = 0.0;
+            I.dpx[ii] = dpx;   // not recomputed at the leading-edge column
+            I.dpy[ii] = dpy;
+        } else {
+            if (s <= EXTER) {
+                px = 0.0;
+                py = 0.0;
+            } else {
+                double g = I.mu * I.pn[ii];
+                px = pxn + dpx;
+                py = pyn + dpy;
+                double ptabs = sqrt(px * px + py * py);
+                if (s == SLIP || ptabs >= g) {
+                    px = px * g / ptabs;
+                    py = py * g / ptabs;
+                }
+            }
+            I.dpx[ii] = px - pxn;
+            I.dpy[ii] = py - pyn;
+        }
+        I.px[ii] = px;
+        I.py[ii] = py;
+        double nx, ny;
+        nt_from_p(px, py, nx, ny);
+        I.nx[ii] = nx;
+        I.ny[ii] = ny;
+        pxn = px;
+        pyn = py;
+    }
+}
+
+// plstrc without plasticity (elmtrc), src/m_solvpt.f90:3258-3807; the plastic branch only
+// reports a violated constraint when tau_c0 = 1e20 and k_tau = 0
+__device__ void elmtrc(int8_t& el, const double c11, const double c12, const double c21,
+                       const double c22, double eps, double pr[3], double mus, double s[2])
+{
+    const int maxnr = 10;
+    const double prc = 0.0001, taucs = 1e20, PI = 3.14159265358979323846;
+    double pold1 = pr[0], pold2 = pr[1];
+    double si0x = s[0] - c11 * pr[0] - c12 * pr[1];
+    double si0y = s[1] - c21 * pr[0] - c22 * pr[1];
+    double six = 0.0, siy = 0.0;
+    int itry = 0, violat = 1;
+    while (itry <= 3 && violat == 1) {
+        itry++;
+        if (el == ADHES) {
+            double trc_bound = mus * pr[2];
+            double detinv = 1.0 / (c11 * c22 - c12 * c21);
+            double dp0 = (-c22 * s[0] + c12 * s[1]) * detinv;
+            double dp1 = (c21 * s[0] - c11 * s[1]) * detinv;
+            pr[0] = pold1 + dp0;
+            pr[1] = pold2 + dp1;
+            double pabs2 = sqrt(pr[0] * pr[0] + pr[1] * pr[1]);
+            double bnd = fmin(trc_bound, taucs);
+            if (pabs2 <= bnd) {
+                violat = 0;
+            } else {
+                double pabs1 = sqrt((pold1 + dp0) * (pold1 + dp0) + (pold2 + dp1) * (pold2 + dp1));
+                if (pabs1 <= bnd) {
+                    pr[0] = pr[0] * bnd / pabs2;
+                    pr[1] = pr[1] * bnd / pabs2;
+                    violat = 0;
+                } else {
+                    violat = 1;
+                }
+            }
+            six = si0x + c11 * pr[0] + c12 * pr[1];
+            siy = si0y + c21 * pr[0] + c22 * pr[1];
+        } else if (el == SLIP) {
+            violat = 0;
+            double trc_bound = mus * pr[2];
+            double sabs = sqrt(si0x * si0x + si0y * si0y);
+            pr[0] = -trc_bound * si0x / sabs;
+            pr[1] = -trc_bound * si0y / sabs;
+            six = si0x + c11 * pr[0] + c12 * pr[1];
+            siy = si0y + c21 * pr[0] + c22 * pr[1];
+            double fk0 = pr[1] * six - pr[0] * siy;
+            double fk1 = pr[0] * pr[0] + pr[1] * pr[1] - trc_bound * trc_bound;
+            int itnr = 0;
+            while (itnr == 0 || (itnr < maxnr && fabs(fk0) + fabs(fk1) >= prc * eps * trc_bound)) {
+                itnr++;
+                double g00 = -2.0 * c21 * pr[0] - si0y + (c11 - c22) * pr[1];
+                double g01 = 2.0 * c12 * pr[1] + si0x + (c11 - c22) * pr[0];
+                double g10 = 2.0 * pr[0];
+                double g11 = 2.0 * pr[1];
+                double det = g00 * g11 - g01 * g10;
+                if (det == 0.0) {
+                    itnr = maxnr;
+                } else {
+                    double d0 = -(g11 * fk0 - g01 * fk1) / det;
+                    double d1 = -(-g10 * fk0 + g00 * fk1) / det;
+                    pr[0] += d0;
+                    pr[1] += d1;
+                }
+                six = si0x + c11 * pr[0] + c12 * pr[1];
+                siy = si0y + c21 * pr[0] + c22 * pr[1];
+                fk0 = pr[1] * six - pr[0] * siy;
+                fk1 = pr[0] * pr[0] + pr[1] * pr[1] - trc_bound * trc_bound;
+            }
+            double alph0 = atan2(pold2, pold1), alph1 = atan2(pr[1], pr[0]);
+            double dalph = alph1 - alph0;
+            if (dalph < -PI) dalph += 2.0 * PI;
+            if (dalph > PI) dalph -= 2.0 * PI;
+            alph1 = alph0 + dalph;
+            pr[0] = trc_bound * cos(alph1);
+            pr[1] = trc_bound * sin(alph1);
+            if (fabs(pr[0]) > fabs(pr[1])) {
+                if (pr[0] * six > 0.0) violat = 1;
+            } else {
+                if (pr[1] * siy > 0.0) violat = 1;
+            }
+        } else {
+            violat = 1;
+        }
+        if (itry <= 3 && violat == 1) {
+            if (el == ADHES) el = SLIP;
+            else if (el == SLIP) el = PLAST;
+            else el = ADHES;
+        }
+    }
+    s[0] = six;
+    s[1] = siy;
+}
+
+// element-state moves after a GD step (src/gdsteady.f90:375-418); returns #changes
+__global__ void __launch_bounds__(ELEM_THREADS) k_gd_moves(const GdItem* items, const int* gstart,
+                                                           int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GdItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[1] = {0.0};
+    const double tiny = 1e-12;
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        int8_t s = I.el[i];
+        double g = I.mu * I.pn[i];
+        if (s == ADHES) {
+            double ptabs = sqrt(I.px[i] * I.px[i] + I.py[i] * I.py[i]);
+            if (ptabs >= g - tiny) {
+                I.el[i] = SLIP;
+                acc[0] += 1.0;
+            }
+        } else if (s == SLIP) {
+            double pr[3] = {I.px[i], I.py[i], I.pn[i]};
+            double si[2] = {I.sx[i], I.sy[i]};
+            int8_t elopt = s;
+            elmtrc(elopt, I.c11, I.c12, I.c21, I.c22, 1e-6, pr, I.mu, si);
+            double snrm = I.nx[i] * I.sx[i] + I.ny[i] * I.sy[i];
+            double cosdth = (I.nx[i] * pr[0] + I.ny[i] * pr[1]) / g;
+            if ((snrm > 0.0 && (I.dscl[i] >= 0.001 || elopt == ADHES)) || cosdth <= -0.5) {
+                I.el[i] = ADHES;
+                acc[0] += 1.0;
+            }
+        }
+    }
+    write_partials<1>(acc, 0, partials, 1);
+}
+
+// angle-split of a slip element (compute_diagscaling, src/gdsteady.f90:927-980); returns
+// false when the angle error stays below tiny_err (elnew/fac_s then keep earlier values)
+__device__ inline bool gd_slip_split(const GdItem& I, int ii, int8_t& elnew, double& fac_s)
+{
+    const double tiny_err = 1e-6, PI = 3.14159265358979323846;
+    double pr[3] = {I.px[ii], I.py[ii], I.pn[ii]};
+    double si[2] = {I.sx[ii], I.sy[ii]};
+    double th_p0 = atan2(pr[1], pr[0]);
+    double th_s0 = atan2(-si[1], -si[0]);
+    double err_0 = th_s0 - th_p0;
+    if (fabs(err_0) >= PI) err_0 = err_0 - rint(err_0 / (2.0 * PI)) * 2.0 * PI;
+    if (fabs(err_0) <= tiny_err) {
+        pr[0] = I.mu * pr[2] * cos(th_p0 + 0.1);
+        pr[1] = I.mu * pr[2] * sin(th_p0 + 0.1);
+        si[0] = I.sx[ii] + I.c11 * (pr[0] - I.px[ii]) + I.c12 * (pr[1] - I.py[ii]);
+        si[1] = I.sy[ii] + I.c21 * (pr[0] - I.px[ii]) + I.c22 * (pr[1] - I.py[ii]);
+        th_p0 = atan2(pr[1], pr[0]);
+        th_s0 = atan2(-si[1], -si[0]);
+        err_0 = th_s0 - th_p0;
+        if (fabs(err_0) >= PI) err_0 = err_0 - rint(err_0 / (2.0 * PI)) * 2.0 * PI;
+    }
+    if (fabs(err_0) <= tiny_err) return false;
+    int8_t en = SLIP;
+    elmtrc(en, I.c11, I.c12, I.c21, I.c22, 1e-6, pr, I.mu, si);
+    elnew = en;
+    if (en == SLIP) {
+        double th_s1 = atan2(-si[1], -si[0]);
+        double upd_s = th_s1 - th_s0;
+        if (fabs(upd_s) >= PI) upd_s = upd_s - rint(upd_s / (2.0 * PI)) * 2.0 * PI;
+        fac_s = fabs(upd_s) / fabs(err_0);
+    }
+    return true;
+}
+
+// compute_diagscaling and the scaled residual (src/gdsteady.f90:901-1042, 429-439); when
+// not I.first also pold -= p on the interior.  Returns (sum (pold)^2, sum p^2, sum_int r^2).
+__global__ void __launch_bounds__(ELEM_THREADS) k_gd_diagres(const GdItem* items, const int* gstart,
+                                                             int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GdItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        int8_t s = I.el[i];
+        int8_t elnew = EXTER;
+        double fac_s = 0.0;
+        bool have = false;
+        if (s == SLIP) have = gd_slip_split(I, i, elnew, fac_s);
+        if (s == SLIP && !have) {
+            // the reference keeps elnew/fac_s of the last slip element it split
+            for (int j = i - 1; j >= 0; j--) {
+                if (I.el[j] != SLIP) continue;
+                if (gd_slip_split(I, j, elnew, fac_s)) break;
+            }
+        }
+        double dscl = I.dscl[i];
+        if (s == SLIP && elnew == SLIP) {
+            dscl = I.d_slp * pow(fac_s, I.pow_s);
+        } else if (s == ADHES || (s == SLIP && elnew == ADHES)) {
+            int ix = i % I.mx + 1;
+            int jx = ix;
+            while (jx > 1 && I.el[i - ix + jx - 1] == ADHES) jx--;
+            double dnew;
+            if (I.d_cns < I.d_ifc)
+                dnew = fmax(I.d_cns, I.d_ifc + (ix - jx - 1) * fmin(0.0, I.d_lin));
+            else
+                dnew = fmin(I.d_cns, I.d_ifc + (ix - jx - 1) * fmax(0.0, I.d_lin));
+            dscl = fmin(50.0 * dscl, dnew);
+        } else if (s != ADHES) {
+            dscl = 0.0;
+        }
+        I.dscl[i] = dscl;
+        double rx = 0.0, ry = 0.0;
+        if (s == ADHES) {
+            rx = -I.sx[i] * dscl;
+            ry = -I.sy[i] * dscl;
+        } else if (s == SLIP) {
+            double tx = -I.ny[i], ty = I.nx[i];
+            double st = tx * I.sx[i] + ty * I.sy[i];
+            rx = -st * tx * dscl;
+            ry = -st * ty * dscl;
+        }
+        I.rx[i] = rx;
+        I.ry[i] = ry;
+        double px = I.px[i], py = I.py[i];
+        if (!I.first) {
+            double ox = I.poldx[i], oy = I.poldy[i];
+            if (s >= ADHES) {
+                ox -= px;
+                oy -= py;
+                I.poldx[i] = ox;
+                I.poldy[i] = oy;
+            }
+            acc[0] += ox * ox + oy * oy;
+        }
+        acc[1] += px * px + py * py;
+        if (s >= ADHES) acc[2] += rx * rx + ry * ry;
+    }
+    write_partials<3>(acc, 0, partials, 3);
+}
+
+// ------------------------------------------------------------------------------------------
+// FFT products (src/m_aijpj.f90:712-1015)
+// ------------------------------------------------------------------------------------------
+
+template <typename R>
+struct LdsSync {
+    __device__ void operator()() const { __syncthreads(); }
+};
+
+// rows per work-group for a complex row length M, bounded by the LDS budget
+__host__ __device__ inline int rows_per_group(int M) { int r = 2048 / M; return r < 1 ? 1 : (r > 32 ? 32 : r); }
+// columns per work-group for a column length L and nc components
+__host__ __device__ inline int cols_per_group(int L, int nc) { int r = 2048 / (L * nc); return r < 1 ? 1 : (r > 16 ? 16 : r); }
+
+template <typename R>
+__device__ inline R to_r(double v) { return (R)v; }
+
+// forward real row transforms: rows y < ny_rows of `in` (row length nxr, zero padded to Lx)
+// -> S[c][y*H + kx]
+template <typename R>
+__global__ void __launch_bounds__(FFT_THREADS) k_rowfwd(const ConvItem* items, const int* gstart,
+                                                        int nitems)
+{
+    extern __shared__ char lds_raw[];
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const ConvItem& I = items[it];
+    const int Lx = I.Lx, M = Lx / 2, H = M + 1;
+    // mode 0: inputs are grid functions (rows my, length mx); modes 1/2: dense Lx*Ly arrays
+    const int nrow = I.mode == 0 ? I.my : I.Ly;
+    const int nxr = I.mode == 0 ? I.mx : I.Lx;
+    const int rpg = rows_per_group(M);
+    const int ngy = (nrow + rpg - 1) / rpg;
+    int t = blockIdx.x - gstart[it];
+    int c = t / ngy;
+    int y0 = (t - c * ngy) * rpg;
+    int nr = min(rpg, nrow - y0);
+    cpx<R>* a = (cpx<R>*)lds_raw;
+    cpx<R>* b = a + rpg * M;
+    const double* src = I.in[c];
+    for (int idx = threadIdx.x; idx < nr * M; idx += blockDim.x) {
+        int r = idx / M, n = idx - r * M;
+        int x0 = 2 * n;
+        const double* row = src + (size_t)(y0 + r) * nxr;
+        R re = x0 < nxr ? to_r<R>(row[x0]) : (R)0;
+        R im = x0 + 1 < nxr ? to_r<R>(row[x0 + 1]) : (R)0;
+        a[idx] = {re, im};
+    }
+    __syncthreads();
+    FftPlan plan = make_plan(M);
+    cpx<R>* res = fft_run<R>(a, b, plan, nr, (R)-1, (const cpx<R>*)I.twM, threadIdx.x, blockDim.x,
+                             LdsSync<R>());
+    cpx<R>* S = (cpx<R>*)I.wsS + (size_t)c * nrow * H;
+    const cpx<R>* twL = (const cpx<R>*)I.twLx;
+    for (int idx = threadIdx.x; idx < nr * H; idx += blockDim.x) {
+        int r = idx / H, k = idx - r * H;
+        cpx<R> Z = res[r * M + (k % M)];
+        cpx<R> Zc = cconj(res[r * M + ((M - k) % M)]);
+        cpx<R> E = {(R)0.5 * (Z.x + Zc.x), (R)0.5 * (Z.y + Zc.y)};
+        cpx<R> D = {(R)0.5 * (Z.x - Zc.x), (R)0.5 * (Z.y - Zc.y)};
+        // X = E - i W^k D
+        cpx<R> WD = cmul(twL[k % Lx], D);
+        cpx<R> X = {E.x + WD.y, E.y - WD.x};
+        S[(size_t)(y0 + r) * H + k] = X;
+    }
+}
+
+// column stage: forward column FFT of each input, spectral product (mode 0), spectrum store
+// (mode 1) or reciprocal (mode 2), inverse column FFT -> T[c][y*H + kx]
+template <typename R>
+__global__ void __launch_bounds__(FFT_THREADS) k_colmul(const ConvItem* items, const int* gstart,
+                                                        int nitems)
+{
+    extern __shared__ char lds_raw[];
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const ConvItem& I = items[it];
+    const int Lx = I.Lx, M = Lx / 2, H = M + 1, Ly = I.Ly;
+    const int nrow = I.mode == 0 ? I.my : I.Ly;
+    const int nrow_out = I.mode == 0 ? I.my : I.Ly;
+    const int nin = I.nin, nout = I.mode == 0 ? I.nout : nin;
+    const int nc = nin > nout ? nin : nout;
+    const int cpg = cols_per_group(Ly, nc);
+    int kx0 = (blockIdx.x - gstart[it]) * cpg;
+    int ncol = min(cpg, H - kx0);
+    cpx<R>* a = (cpx<R>*)lds_raw;
+    cpx<R>* b = a + (size_t)nc * cpg * Ly;
+    const cpx<R>* S = (const cpx<R>*)I.wsS;
+    for (int idx = threadIdx.x; idx < nin * ncol * Ly; idx += blockDim.x) {
+        int tr = idx / Ly, y = idx - tr * Ly;
+        int j = tr / ncol, kk = tr - j * ncol;
+        a[idx] = y < nrow ? S[((size_t)j * nrow + y) * H + kx0 + kk] : cpx<R>{(R)0, (R)0};
+    }
+    __syncthreads();
+    FftPlan plan = make_plan(Ly);
+    const cpx<R>* twy = (const cpx<R>*)I.twLy;
+    cpx<R>* f = fft_run<R>(a, b, plan, nin * ncol, (R)-1, twy, threadIdx.x, blockDim.x, LdsSync<R>());
+    cpx<R>* o = f == a ? b : a;
+    if (I.mode == 1) {
+        // spectrum store: spec[c] layout [kx*Ly + ky], scaled
+        for (int idx = threadIdx.x; idx < nin * ncol * Ly; idx += blockDim.x) {
+            int tr = idx / Ly, ky = idx - tr * Ly;
+            int j = tr / ncol, kk = tr - j * ncol;
+            cpx<R> v = f[idx];
+            v.x *= (R)I.scale;
+            v.y *= (R)I.scale;
+            ((cpx<R>*)I.out[j])[(size_t)(kx0 + kk) * Ly + ky] = v;
+        }
+        return;
+    }
+    for (int idx = threadIdx.x; idx < ncol * Ly; idx += blockDim.x) {
+        int kk = idx / Ly, ky = idx - kk * Ly;
+        int kx = kx0 + kk;
+        cpx<R> xin[3];
+        for (int j = 0; j < nin; j++) xin[j] = f[(j * ncol + kk) * Ly + ky];
+        if (I.mode == 2) {
+            for (int j = 0; j < nin; j++) {
+                double re = (double)xin[j].x, im = (double)xin[j].y;
+                double d = re * re + im * im;
+                o[(j * ncol + kk) * Ly + ky] = {(R)(re / d), (R)(-im / d)};
+            }
+        } else {
+            for (int i = 0; i < nout; i++) {
+                cpx<R> acc = {(R)0, (R)0};
+                for (int j = 0; j < nin; j++) {
+                    const cpx<R>* sp = (const cpx<R>*)I.spec[i][j];
+                    if (!sp) continue;
+                    acc = cadd(acc, cmul(sp[(size_t)kx * Ly + ky], xin[j]));
+                }
+                o[(i * ncol + kk) * Ly + ky] = acc;
+            }
+        }
+    }
+    __syncthreads();
+    cpx<R>* g = fft_run<R>(o, f, plan, nout * ncol, (R)1, twy, threadIdx.x, blockDim.x, LdsSync<R>());
+    cpx<R>* T = (cpx<R>*)I.wsT;
+    for (int idx = threadIdx.x; idx < nout * ncol * nrow_out; idx += blockDim.x) {
+        int tr = idx / nrow_out, y = idx - tr * nrow_out;
+        int i = tr / ncol, kk = tr - i * ncol;
+        T[((size_t)i * nrow_out + y) * H + kx0 + kk] = g[(i * ncol + kk) * Ly + y];
+    }
+}
+
+// inverse real row transforms T[c] -> out[c] rows, masked write (mode 0) or dense (mode 2)
+template <typename R>
+__global__ void __launch_bounds__(FFT_THREADS) k_rowinv(const ConvItem* items, const int* gstart,
+                                                        int nitems)
+{
+    extern __shared__ char lds_raw[];
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const ConvItem& I = items[it];
+    const int Lx = I.Lx, M = Lx / 2, H = M + 1;
+    const int nrow = I.mode == 0 ? I.my : I.Ly;
+    const int nxr = I.mode == 0 ? I.mx : I.Lx;
+    const int rpg = rows_per_group(M);
+    const int ngy = (nrow + rpg - 1) / rpg;
+    int t = blockIdx.x - gstart[it];
+    int c = t / ngy;
+    int y0 = (t - c * ngy) * rpg;
+    int nr = min(rpg, nrow - y0);
+    cpx<R>* a = (cpx<R>*)lds_raw;
+    cpx<R>* b = a + rpg * M;
+    const cpx<R>* T = (const cpx<R>*)I.wsT + (size_t)c * nrow * H;
+    const cpx<R>* twL = (const cpx<R>*)I.twLx;
+    for (int idx = threadIdx.x; idx < nr * M; idx += blockDim.x) {
+        int r = idx / M, k = idx - r * M;
+        const cpx<R>* row = T + (size_t)(y0 + r) * H;
+        cpx<R> X = row[k];
+        cpx<R> Xc = cconj(row[M - k]);
+        cpx<R> E = {(R)0.5 * (X.x + Xc.x), (R)0.5 * (X.y + Xc.y)};
+        cpx<R> D = {(R)0.5 * (X.x - Xc.x), (R)0.5 * (X.y - Xc.y)};
+        cpx<R> O = cmul(D, cconj(twL[k]));
+        a[idx] = {E.x - O.y, E.y + O.x};   // E + i O
+    }
+    __syncthreads();
+    FftPlan plan = make_plan(M);
+    cpx<R>* z = fft_run<R>(a, b, plan, nr, (R)1, (const cpx<R>*)I.twM, threadIdx.x, blockDim.x,
+                           LdsSync<R>());
+    double* dst = I.out[c];
+    const double* add = I.addend[c];
+    for (int idx = threadIdx.x; idx < nr * nxr; idx += blockDim.x) {
+        int r = idx / nxr, x = idx - r * nxr;
+        size_t ii = (size_t)(y0 + r) * nxr + x;
+        if (I.mode == 0 && I.mask != M_ALL && !in_mask(I.mask, I.el[ii])) continue;
+        cpx<R> zz = z[r * M + (x >> 1)];
+        double u = (double)((x & 1) ? zz.y : zz.x);
+        if (I.mode == 2) u *= I.scale;
+        if (add) u += add[ii];
+        dst[ii] = u;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// influence coefficients (elascf_pcwcns, src/m_visc.f90:431-604) and their spectra
+// ------------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(ELEM_THREADS) k_elascf(const CoefItem* items, const int* gstart, int nitems)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const CoefItem& I = items[it];
+    const int mx = I.mx, my = I.my;
+    const double PI = 3.14159265358979323846;
+    const double e1 = (1.0 - I.nuv) / PI, e2 = 1.0 / PI, e3 = I.nuv / PI;
+    const double tolx = I.dx * 1e-10, tolx2 = I.dx * 1e-20;
+    const size_t nb = (size_t)(2 * mx) * (2 * my);
+    int iy0 = my == 1 ? -my + 1 : -my;
+    size_t k = (size_t)(blockIdx.x - gstart[it]) * blockDim.x + threadIdx.x;
+    if (k < nb) {
+        int ix = (int)(k % (2 * mx)) - mx, iy = (int)(k / (2 * mx)) - my;
+        double c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (iy >= iy0) {
+            auto X = [&](int jx) { return (double)(float)jx * I.dx + I.xshft - 0.5 * I.dx; };
+            auto Y = [&](int jy) { return (double)(float)jy * I.dy + I.yshft - 0.5 * I.dy; };
+            auto Rr = [&](int jx, int jy) { double x = X(jx), y = Y(jy); return sqrt(x * x + y * y); };
+            // cf21 = nu/pi * (-r00 + r01 + r10 - r11)
+            double c21 = 0.0;
+            c21 = c21 - e3 * Rr(ix, iy);
+            c21 = c21 + e3 * Rr(ix, iy + 1);
+            c21 = c21 + e3 * Rr(ix + 1, iy);
+            c21 = c21 - e3 * Rr(ix + 1, iy + 1);
+            auto xly = [&](int jx, int jy) {   // x log((y1+r1)/(y0+r0)) over [jy, jy+1]
+                double x = X(jx), y0 = Y(jy), y1 = Y(jy + 1);
+                double r0 = sqrt(x * x + y0 * y0), r1 = sqrt(x * x + y1 * y1);
+                if (fmin(fabs(y0 + r0), fabs(y1 + r1)) < tolx) return 0.0;
+                return x * log((y1 + r1) / (y0 + r0));
+            };
+            auto ylx = [&](int jx, int jy) {
+                double y = Y(jy), x0 = X(jx), x1 = X(jx + 1);
+                double r0 = sqrt(x0 * x0 + y * y), r1 = sqrt(x1 * x1 + y * y);
+                if (fmin(fabs(x0 + r0), fabs(x1 + r1)) < tolx) return 0.0;
+                return y * log((x1 + r1) / (x0 + r0));
+            };
+            double xa = xly(ix, iy), xb = xly(ix + 1, iy), ya = ylx(ix, iy), yb = ylx(ix, iy + 1);
+            double c33 = 0.0, c11 = 0.0, c22 = 0.0;
+            c33 = c33 - e1 * (xa + ya);
+            c33 = c33 + e1 * (xb + yb);
+            c11 = c11 - (e1 * xa + e2 * ya);
+            c11 = c11 + (e1 * xb + e2 * yb);
+            c22 = c22 - (e1 * ya + e2 * xa);
+            c22 = c22 + (e1 * yb + e2 * xb);
+            double c13 = 0.0, c23 = 0.0;
+            if (fabs(I.akv) >= 1e-6) {
+                auto t13 = [&](int jx, int jy) {
+                    double x = X(jx), y = Y(jy), r = sqrt(x * x + y * y);
+                    return y * log(r) + x * atan(y / (x + tolx2));
+                };
+                auto t23 = [&](int jx, int jy) {
+                    double x = X(jx), y = Y(jy), r = sqrt(x * x + y * y);
+                    return x * log(r) + y * atan(x / (y + tolx2));
+                };
+                double ek = e2 * I.akv;
+                c13 = c13 - ek * t13(ix, iy);
+                c13 = c13 + ek * t13(ix, iy + 1);
+                c13 = c13 + ek * t13(ix + 1, iy);
+                c13 = c13 - ek * t13(ix + 1, iy + 1);
+                c23 = c23 - ek * t23(ix, iy);
+                c23 = c23 + ek * t23(ix, iy + 1);
+                c23 = c23 + ek * t23(ix + 1, iy);
+                c23 = c23 - ek * t23(ix + 1, iy + 1);
+            }
+            c[0] = c11; c[1] = c21; c[2] = c13;
+            c[3] = c21; c[4] = c22; c[5] = c23;
+            c[6] = -c13; c[7] = -c23; c[8] = c33;
+        }
+        for (int b = 0; b < 9; b++) I.cf[(size_t)b * nb + k] = c[b];
+    }
+}
+
+// rolling: cv keeps the normal row of cs; csv = cs - cv, cv = cs - csv on the tangential rows
+// (src/m_visc.f90:319-358 with facdq = 1)
+__global__ void k_rollcoef(int n, const double* cs, double* cv)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < (size_t)n; k += (size_t)gridDim.x * blockDim.x) {
+        for (int jk = 0; jk < 3; jk++) {
+            cv[(size_t)(6 + jk) * n + k] = cs[(size_t)(6 + jk) * n + k];
+            for (int ik = 0; ik < 2; ik++) {
+                size_t b = (size_t)(ik * 3 + jk) * n + k;
+                double csv = cs[b] - cv[b];
+                cv[b] = cs[b] - csv;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(ELEM_THREADS) k_embed(const EmbedItem* items, const int* gstart, int nitems)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const EmbedItem& I = items[it];
+    size_t n = (size_t)I.Lx * I.Ly;
+    size_t k = (size_t)(blockIdx.x - gstart[it]) * blockDim.x + threadIdx.x;
+    if (k < n) {
+        int x = (int)(k % I.Lx), y = (int)(k / I.Lx);
+        int dx = x < I.Lx / 2 ? x : x - I.Lx;
+        int dy = y < I.Ly / 2 ? y : y - I.Ly;
+        double v = 0.0;
+        if (dx >= -I.rx && dx < I.rx && dy >= -I.ry && dy < I.ry && dx >= -I.cf_mx && dx < I.cf_mx &&
+            dy >= -I.cf_my && dy < I.cf_my)
+            v = I.scale * I.cf[(size_t)(dy + I.cf_my) * (2 * I.cf_mx) + (dx + I.cf_mx)];
+        I.out[k] = v;
+    }
+}
+
+__global__ void __launch_bounds__(ELEM_THREADS) k_extract(const ExtractItem* items, const int* gstart, int nitems)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const ExtractItem& I = items[it];
+    size_t n = (size_t)(2 * I.cf_mx) * (2 * I.cf_my);
+    size_t k = (size_t)(blockIdx.x - gstart[it]) * blockDim.x + threadIdx.x;
+    if (k < n) {
+        int dx = (int)(k % (2 * I.cf_mx)) - I.cf_mx, dy = (int)(k / (2 * I.cf_mx)) - I.cf_my;
+        int x = (dx + I.Lx) % I.Lx, y = (dy + I.Ly) % I.Ly;
+        I.cf[k] = I.scale * I.in[(size_t)y * I.Lx + x];
+    }
+}
+
+__global__ void k_to_float(size_t n, const double* in, float* out)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        out[k] = (float)in[k];
+}
+
+// ------------------------------------------------------------------------------------------
+// per-case fields and force sums
+// ------------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(ELEM_THREADS) k_setup(const SetupItem* items, const int* gstart,
+                                                        int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const SetupItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[2] = {-INFINITY, -INFINITY};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        int ix = i % I.mx, iy = i / I.mx;
+        double x = I.xc1 + ix * I.dx, y = I.yc1 + iy * I.dy;
+        // previous-time tractions (P digit)
+        if (I.prev) {
+            I.pvn[i] = I.pn[i];
+            I.pvx[i] = I.px[i];
+            I.pvy[i] = I.py[i];
+        } else {
+            I.pvn[i] = 0.0;
+            I.pvx[i] = 0.0;
+            I.pvy[i] = 0.0;
+        }
+        double h;
+        if (I.ibase == 1) {
+            h = I.prm[0] * x * x + I.prm[1] * x * y + I.prm[2] * y * y + I.prm[3] * x + I.prm[4] * y + I.prm[5];
+            I.h[i] = h;
+        } else {
+            h = I.h[i];
+        }
+        // rigid slip w = (xi - phi (y + yofs), eta + phi (x + xofs)), times -dq
+        double vx = 1.0 * -(y + I.yofs - 0.0) * I.cphi + 0.0;
+        double vy = 1.0 * (x + I.xofs - 0.0) * I.cphi + 0.0;
+        vx += I.cksi;
+        vy += I.ceta;
+        I.hx[i] = vx * -I.dq;
+        I.hy[i] = vy * -I.dq;
+        // initial estimate
+        if (I.zero_pn) I.pn[i] = 0.0;
+        if (I.zero_pt) {
+            I.px[i] = 0.0;
+            I.py[i] = 0.0;
+        }
+        if (I.hertz) {
+            double ax = x / I.hz_aa, by = y / I.hz_bb;
+            double f = 1.0 - ax * ax - by * by;
+            double pn = I.pnmax * sqrt(fmax(0.0, f));
+            I.pn[i] = pn;
+            I.el[i] = pn > (double)1e-20f ? ADHES : EXTER;
+        }
+        if (h > (double)1e29f) I.el[i] = EXTER;
+        acc[0] = fmax(acc[0], h);
+        acc[1] = fmax(acc[1], -h);
+    }
+    write_partials<2>(acc, 1, partials, 2);
+}
+
+__global__ void __launch_bounds__(ELEM_THREADS) k_forces(const SetupItem* items, const int* gstart,
+                                                         int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const SetupItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        int ix = i % I.mx, iy = i / I.mx;
+        double x = I.xc1 + ix * I.dx, y = I.yc1 + iy * I.dy;
+        acc[0] += I.pn[i];
+        acc[1] += I.px[i];
+        acc[2] += I.py[i];
+        acc[3] += I.px[i] * y;
+        acc[4] += I.py[i] * x;
+    }
+    write_partials<5>(acc, 0, partials, 5);
+}
+
+// ------------------------------------------------------------------------------------------
+// host-side launchers
+// ------------------------------------------------------------------------------------------
+
+static int ngroups_elem(int n) { return (n + ELEM_THREADS * ELEM_PER_THREAD - 1) / (ELEM_THREADS * ELEM_PER_THREAD); }
+static int ngroups_rows(int my) { return (my + ROW_THREADS - 1) / ROW_THREADS; }
+
+int op_nred(int kind, const void* item)
+{
+    switch (kind) {
+    case OP_VEC: return 2;
+    case OP_NCG_NEGPN: return 3;
+    case OP_NCG_DD: return 2;
+    case OP_NCG_EXPAND: return 1;
+    case OP_NCG_HMIN: return 1;
+    case OP_SN_CONTRACT: return 2;
+    case OP_SN_EXPAND: return 2;
+    case OP_SN_FINAL: return 1;
+    case OP_AIJ_ONES: return 1;
+    case OP_TCG_ZPREP: return 2;
+    case OP_TCG_QPOST: return 2;
+    case OP_TCG_PUPD: return 2;
+    case OP_TCG_AS: return 1;
+    case OP_TCG_SA: return 2;
+    case OP_ST_AS: return 1;
+    case OP_ST_SA: return 1;
+    case OP_GD_PROBE: return 10;
+    case OP_GD_MOVES: return 1;
+    case OP_GD_DIAGRES: return 3;
+    case OP_SETUP: return 2;
+    case OP_FORCES: return 5;
+    default: return 0;
+    }
+    (void)item;
+}
+
+int op_combine_mode(int kind, const void* item)
+{
+    if (kind == OP_VEC) {
+        int sub = ((const VecItem*)item)->sub;
+        if (sub == V_MAXABS || sub == V_MAX) return 1;
+        if (sub == V_MIN) return 2;
+    }
+    if (kind == OP_SETUP) return 1;
+    return 0;
+}
+
+int op_groups(int kind, const void* item, int conv_stage)
+{
+    switch (kind) {
+    case OP_VEC: return ngroups_elem(((const VecItem*)item)->n);
+    case OP_NCG_NEGPN: case OP_NCG_DD: case OP_NCG_EXPAND: case OP_NCG_HMIN:
+    case OP_SN_CONTRACT: case OP_SN_EXPAND: case OP_SN_FINAL:
+        return ngroups_elem(((const NcgItem*)item)->n);
+    case OP_AIJ_ONES: return ngroups_rows(((const AijItem*)item)->my);
+    case OP_TCG_NT: case OP_TCG_RSET: case OP_TCG_ZPREP: case OP_TCG_QPOST: case OP_TCG_PUPD:
+    case OP_TCG_AS: case OP_TCG_SA:
+        return ngroups_elem(((const TcgItem*)item)->n);
+    case OP_ST_AS: case OP_ST_SA: return ngroups_elem(((const StItem*)item)->n);
+    case OP_GD_DP: case OP_GD_PROJECT: case OP_GD_PROBE: case OP_GD_UPDATE:
+        return ngroups_rows(((const GdItem*)item)->my);
+    case OP_GD_MOVES: case OP_GD_DIAGRES: return ngroups_elem(((const GdItem*)item)->n);
+    case OP_SETUP: case OP_FORCES: return ngroups_elem(((const SetupItem*)item)->n);
+    case OP_ELASCF: { const CoefItem* c = (const CoefItem*)item; return (int)(((size_t)(2 * c->mx) * (2 * c->my) + ELEM_THREADS - 1) / ELEM_THREADS); }
+    case OP_EMBED: { const EmbedItem* e = (const EmbedItem*)item; return (int)(((size_t)e->Lx * e->Ly + ELEM_THREADS - 1) / ELEM_THREADS); }
+    case OP_EXTRACT: { const ExtractItem* e = (const ExtractItem*)item; return (int)(((size_t)(2 * e->cf_mx) * (2 * e->cf_my) + ELEM_THREADS - 1) / ELEM_THREADS); }
+    case OP_CONV: case OP_CONV32: {
+        const ConvItem* c = (const ConvItem*)item;
+        int M = c->Lx / 2, H = M + 1;
+        int nrow = c->mode == 0 ? c->my : c->Ly;
+        int nout = c->mode == 0 ? c->nout : c->nin;
+        if (conv_stage == 0) return c->nin * ((nrow + rows_per_group(M) - 1) / rows_per_group(M));
+        if (conv_stage == 1) {
+            int nc = c->nin > nout ? c->nin : nout;
+            int cpg = cols_per_group(c->Ly, nc);
+            return (H + cpg - 1) / cpg;
+        }
+        if (c->mode == 1) return 0;
+        return nout * ((nrow + rows_per_group(M) - 1) / rows_per_group(M));
+    }
+    }
+    return 0;
+}
+
+static size_t conv_lds(int kind, const ConvItem* c, int stage)
+{
+    size_t es = kind == OP_CONV32 ? 8 : 16;
+    int M = c->Lx / 2;
+    if (stage == 0 || stage == 2) return 2 * (size_t)rows_per_group(M) * M * es;
+    int nout = c->mode == 0 ? c->nout : c->nin;
+    int nc = c->nin > nout ? c->nin : nout;
+    return 2 * (size_t)nc * cols_per_group(c->Ly, nc) * c->Ly * es;
+}
+
+void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, int ngroups,
+               double* d_partials, int conv_stage, size_t lds_bytes, hipStream_t s)
+{
+    if (ngroups <= 0) return;
+    dim3 ge(ngroups), te(ELEM_THREADS), tr(ROW_THREADS), tf(FFT_THREADS);
+    switch (kind) {
+    case OP_VEC: hipLaunchKernelGGL(k_vec, ge, te, 0, s, (const VecItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_NCG_NEGPN: hipLaunchKernelGGL(k_ncg_negpn, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_NCG_DD: hipLaunchKernelGGL(k_ncg_dd, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_NCG_EXPAND: hipLaunchKernelGGL(k_ncg_expand, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_NCG_HMIN: hipLaunchKernelGGL(k_ncg_hmin, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_SN_CONTRACT: hipLaunchKernelGGL(k_sn_contract, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_SN_EXPAND: hipLaunchKernelGGL(k_sn_expand, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_SN_FINAL: hipLaunchKernelGGL(k_sn_final, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_AIJ_ONES: hipLaunchKernelGGL(k_aij_ones, ge, tr, 0, s, (const AijItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_TCG_NT: hipLaunchKernelGGL(k_tcg_nt, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_TCG_RSET: hipLaunchKernelGGL(k_tcg_rset, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_TCG_ZPREP: hipLaunchKernelGGL(k_tcg_zprep, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_TCG_QPOST: hipLaunchKernelGGL(k_tcg_qpost, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_TCG_PUPD: hipLaunchKernelGGL(k_tcg_pupd, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_TCG_AS: hipLaunchKernelGGL(k_tcg_as, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_TCG_SA: hipLaunchKernelGGL(k_tcg_sa, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_ST_AS: hipLaunchKernelGGL(k_st_as, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_ST_SA: hipLaunchKernelGGL(k_st_sa, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_GD_DP: hipLaunchKernelGGL(k_gd_dp, ge, tr, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_GD_PROJECT: hipLaunchKernelGGL(k_gd_project, ge, tr, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_GD_PROBE: hipLaunchKernelGGL(k_gd_probe, ge, tr, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_GD_UPDATE: hipLaunchKernelGGL(k_gd_update, ge, tr, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_GD_MOVES: hipLaunchKernelGGL(k_gd_moves, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_GD_DIAGRES: hipLaunchKernelGGL(k_gd_diagres, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_SETUP: hipLaunchKernelGGL(k_setup, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_FORCES: hipLaunchKernelGGL(k_forces, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_ELASCF: hipLaunchKernelGGL(k_elascf, ge, te, 0, s, (const CoefItem*)d_items, d_gstart, nitems); break;
+    case OP_EMBED: hipLaunchKernelGGL(k_embed, ge, te, 0, s, (const EmbedItem*)d_items, d_gstart, nitems); break;
+    case OP_EXTRACT: hipLaunchKernelGGL(k_extract, ge, te, 0, s, (const ExtractItem*)d_items, d_gstart, nitems); break;
+    case OP_CONV:
+        if (conv_stage == 0) hipLaunchKernelGGL(k_rowfwd<double>, ge, tf, lds_bytes, s, (const ConvItem*)d_items, d_gstart, nitems);
+        else if (conv_stage == 1) hipLaunchKernelGGL(k_colmul<double>, ge, tf, lds_bytes, s, (const ConvItem*)d_items, d_gstart, nitems);
+        else hipLaunchKernelGGL(k_rowinv<double>, ge, tf, lds_bytes, s, (const ConvItem*)d_items, d_gstart, nitems);
+        break;
+    case OP_CONV32:
+        if (conv_stage == 0) hipLaunchKernelGGL(k_rowfwd<float>, ge, tf, lds_bytes, s, (const ConvItem*)d_items, d_gstart, nitems);
+        else if (conv_stage == 1) hipLaunchKernelGGL(k_colmul<float>, ge, tf, lds_bytes, s, (const ConvItem*)d_items, d_gstart, nitems);
+        else hipLaunchKernelGGL(k_rowinv<float>, ge, tf, lds_bytes, s, (const ConvItem*)d_items, d_gstart, nitems);
+        break;
+    }
+}
+
+// Algorithmic HBM bytes of one item of `kind` in `stage`: every array the kernel must read or
+// write, each element once (FFT intermediates of the three product stages included as the
+// stages' inputs/outputs).  Used for the achieved-bandwidth report only.
+double op_bytes(int kind, const void* item, int stage)
+{
+    switch (kind) {
+    case OP_VEC: {
+        const VecItem* v = (const VecItem*)item;
+        static const int arrays[V_NSUB] = {1, 2, 2, 3, 3, 2, 2, 1, 1, 1, 1, 1, 0};
+        return (double)v->n * (1.0 + 8.0 * v->ncomp * arrays[v->sub]);
+    }
+    case OP_NCG_NEGPN: case OP_NCG_HMIN: case OP_SN_CONTRACT: case OP_SN_FINAL:
+        return ((const NcgItem*)item)->n * 18.0;
+    case OP_NCG_DD: return ((const NcgItem*)item)->n * 25.0;
+    case OP_NCG_EXPAND: return ((const NcgItem*)item)->n * 34.0;
+    case OP_SN_EXPAND: return ((const NcgItem*)item)->n * 17.0;
+    case OP_TCG_NT: return ((const TcgItem*)item)->n * 32.0;
+    case OP_TCG_RSET: return ((const TcgItem*)item)->n * 49.0;
+    case OP_TCG_ZPREP: return ((const TcgItem*)item)->n * 105.0;
+    case OP_TCG_QPOST: return ((const TcgItem*)item)->n * 121.0;
+    case OP_TCG_PUPD: return ((const TcgItem*)item)->n * 89.0;
+    case OP_TCG_AS: return ((const TcgItem*)item)->n * 26.0;
+    case OP_TCG_SA: return ((const TcgItem*)item)->n * 82.0;
+    case OP_ST_AS: case OP_ST_SA: return ((const StItem*)item)->n * 42.0;
+    case OP_GD_DP: { const GdItem* g = (const GdItem*)item; return (double)g->n * 32.0; }
+    case OP_GD_PROJECT: { const GdItem* g = (const GdItem*)item; return (double)g->n * 73.0; }
+    case OP_GD_PROBE: { const GdItem* g = (const GdItem*)item; return (double)g->n * (129.0 + (g->first ? 16.0 : 0.0)); }
+    case OP_GD_UPDATE: { const GdItem* g = (const GdItem*)item; return (double)g->n * 121.0; }
+    case OP_GD_MOVES: { const GdItem* g = (const GdItem*)item; return (double)g->n * 66.0; }
+    case OP_GD_DIAGRES: { const GdItem* g = (const GdItem*)item; return (double)g->n * 121.0; }
+    case OP_SETUP: { const SetupItem* s = (const SetupItem*)item; return (double)s->n * 97.0; }
+    case OP_FORCES: { const SetupItem* s = (const SetupItem*)item; return (double)s->n * 24.0; }
+    case OP_CONV: case OP_CONV32: {
+        const ConvItem* c = (const ConvItem*)item;
+        double es = kind == OP_CONV32 ? 8.0 : 16.0;
+        int M = c->Lx / 2, H = M + 1;
+        int nrow = c->mode == 0 ? c->my : c->Ly;
+        int nxr = c->mode == 0 ? c->mx : c->Lx;
+        int nout = c->mode == 0 ? c->nout : c->nin;
+        double npt = (double)nrow * nxr, nsp = (double)nrow * H;
+        if (stage == 0) return c->nin * (npt * 8.0 + nsp * es);
+        if (stage == 1) {
+            int nspec = 0;
+            if (c->mode == 0)
+                for (int i = 0; i < c->nout; i++)
+                    for (int j = 0; j < c->nin; j++) nspec += c->spec[i][j] != nullptr;
+            double out = c->mode == 1 ? c->nin * (double)H * c->Ly * es : nout * nsp * es;
+            return c->nin * nsp * es + nspec * (double)H * c->Ly * es + out;
+        }
+        if (c->mode == 1) return 0.0;
+        return nout * (nsp * es + npt * (8.0 + (c->mode == 0 ? 1.0 : 0.0)));
+    }
+    case OP_ELASCF: { const CoefItem* c = (const CoefItem*)item; return 9.0 * 8.0 * (2 * c->mx) * (2 * c->my); }
+    case OP_EMBED: { const EmbedItem* e = (const EmbedItem*)item; return 8.0 * e->Lx * e->Ly; }
+    case OP_EXTRACT: { const ExtractItem* e = (const ExtractItem*)item; return 16.0 * (2 * e->cf_mx) * (2 * e->cf_my); }
+    }
+    return 0.0;
+}
+
+size_t op_lds(int kind, const void* item, int stage)
+{
+    if (kind == OP_CONV || kind == OP_CONV32) return conv_lds(kind, (const ConvItem*)item, stage);
+    return 0;
+}
+
+void launch_finalize(const int* d_gstart, const int* d_modes, int nitems, int K,
+                     const double* d_partials, double* d_results, hipStream_t s)
+{
+    int n = nitems * K;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_finalize, dim3((n + 255) / 256), dim3(256), 0, s, d_gstart, d_modes, nitems, K,
+                       d_partials, d_results);
+}
+
+
+void launch_rollcoef(int n, const double* cs, double* cv, hipStream_t s)
+{
+    int g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_rollcoef, dim3(g), dim3(256), 0, s, n, cs, cv);
+}
+
+
+
+void launch_to_float(size_t n, const double* in, float* out, hipStream_t s)
+{
+    int g = (int)((n + 255) / 256);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_to_float, dim3(g), dim3(256), 0, s, n, in, out);
+}
+
+}  // namespace cntc

@@ -1,0 +1,32 @@
+// kernels.hpp -- host-side entry points of the HIP kernels in kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "ops.hpp"
+
+namespace cntc {
+
+// number of reduction values an item of `kind` produces (0: none)
+int op_nred(int kind, const void* item);
+// combine mode of the reduction: 0 sum, 1 max, 2 min
+int op_combine_mode(int kind, const void* item);
+// work-groups an item needs (conv_stage 0/1/2 selects the stage of an FFT product)
+int op_groups(int kind, const void* item, int conv_stage);
+// algorithmic HBM bytes of one item in one stage (for the bandwidth report)
+double op_bytes(int kind, const void* item, int stage);
+// dynamic LDS bytes of one work-group for this item and stage
+size_t op_lds(int kind, const void* item, int stage);
+// launch one batched kernel over `nitems` items whose work-groups are given by d_gstart
+void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, int ngroups,
+               double* d_partials, int conv_stage, size_t lds_bytes, hipStream_t s);
+// per-item reduction of the group partials
+void launch_finalize(const int* d_gstart, const int* d_modes, int nitems, int K,
+                     const double* d_partials, double* d_results, hipStream_t s);
+
+void launch_rollcoef(int n, const double* cs, double* cv, hipStream_t s);
+void launch_to_float(size_t n, const double* in, float* out, hipStream_t s);
+
+}  // namespace cntc

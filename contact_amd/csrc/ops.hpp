@@ -1,0 +1,210 @@
+// ops.hpp -- per-patch work items of the batched device operations.
+//
+// Every device operation of the engine is launched once for all patches that asked for it in
+// the same scheduler round.  Each participating patch contributes one item; the work-groups of
+// a launch are split between items by a prefix table (item i owns groups [gstart[i],
+// gstart[i+1])).  Reductions are deterministic: every group writes its partial sums, and a
+// finalize pass adds the partials of each item in group order.
+#pragma once
+
+#include <cstdint>
+
+namespace cntc {
+
+// element-state codes, src/m_gridfunc.f90:30-37
+enum : int8_t { EXTER = 0, ADHES = 1, SLIP = 2, PLAST = 3 };
+
+// element selections, src/m_gridfunc.f90:25-33 (AllElm, AllInt) plus single states
+enum Mask : int { M_ALL = 0, M_INT = 1, M_EXT = 2, M_ADH = 3, M_SLP = 4 };
+
+enum OpKind : int {
+    OP_VEC = 0,
+    OP_NCG_NEGPN,
+    OP_NCG_DD,
+    OP_NCG_EXPAND,
+    OP_NCG_HMIN,
+    OP_SN_CONTRACT,
+    OP_SN_EXPAND,
+    OP_SN_FINAL,
+    OP_AIJ_ONES,
+    OP_TCG_NT,
+    OP_TCG_RSET,
+    OP_TCG_ZPREP,
+    OP_TCG_QPOST,
+    OP_TCG_PUPD,
+    OP_TCG_AS,
+    OP_TCG_SA,
+    OP_ST_AS,
+    OP_ST_SA,
+    OP_GD_DP,
+    OP_GD_PROJECT,
+    OP_GD_PROBE,
+    OP_GD_UPDATE,
+    OP_GD_MOVES,
+    OP_GD_DIAGRES,
+    OP_CONV,
+    OP_CONV32,
+    OP_ELASCF,
+    OP_EMBED,
+    OP_EXTRACT,
+    OP_SETUP,
+    OP_FORCES,
+    OP_COUNT_KINDS
+};
+
+// sub-operations of OP_VEC (masked BLAS-1 over one or two components)
+enum VecSub : int {
+    V_SET = 0,      // x = a
+    V_COPY,         // y = x
+    V_SCAL,         // x *= a
+    V_AXPY,         // y += a x
+    V_AXPBY,        // y = a x + b y
+    V_ADDC,         // x += a
+    V_DOT,          // sum x*y, count
+    V_SUM,          // sum x, count
+    V_SUMSQ,        // sum x*x, count
+    V_MAXABS,       // max |x|
+    V_MIN,          // min x
+    V_MAX,          // max x
+    V_COUNT,        // count of selected elements
+    V_NSUB
+};
+
+struct VecItem {
+    int n, sub, mask, ncomp;
+    const int8_t* el;
+    double* x[2];
+    double* y[2];
+    double a, b;
+};
+
+// NormCG / snorm element steps (src/m_solvpn.f90:298-393, src/m_snorm.f90:195-325)
+struct NcgItem {
+    int n, mode;     // k_ncg_hmin: 1 = enter where h < hmin (p kept), 2 = h <= hmin (p = 0)
+    int8_t* el;
+    double* p;       // pn
+    double* dd;      // A ps - rhs
+    const double* rhs;
+    double* res;
+    double* v;
+    const double* h; // hstot
+    const double* unn;
+    double avg;      // mean subtracted from dd on the interior (proj_avg)
+    double davg;     // penetration estimate (N=1) or 0
+    double errpn, pen, tol, hmin;
+};
+
+// sum over the row extents of one element's influence row, with unit tractions
+// (gf3_AijPj with p = 1, src/m_aijpj.f90:158-185)
+struct AijItem {
+    int mx, my, ix, iy;   // 1-based element
+    int cf_mx, cf_my, ik, jk;
+    const int8_t* el;
+    const double* cf;     // coefficient block (ik,jk), layout [(iy+cf_my)*2cf_mx + ix+cf_mx]
+    double ga_inv;
+};
+
+// TangCG steps (src/m_solvpt.f90:1841-2442)
+struct TcgItem {
+    int n, pad;
+    int8_t* el;
+    double* px; double* py; const double* pn;
+    double* sx; double* sy;
+    double* nx; double* ny;
+    double* rx; double* ry;
+    double* zx; double* zy;
+    double* vx; double* vy;
+    double* qx; double* qy;
+    double* poldx; double* poldy;
+    double mu, ga, c11, c22;
+    double alpha;
+};
+
+// stang outer-loop moves (src/m_stang.f90:433-508)
+struct StItem {
+    int n, pad;
+    int8_t* el;
+    double* px; double* py; const double* pn;
+    const double* sx; const double* sy;
+    double mu, tol, tol1, tol2, errpt;
+};
+
+// GDsteady (src/gdsteady.f90); row kernels use n = my rows, element kernels n = mx*my
+struct GdItem {
+    int n, mx, my, imeth;
+    int kdown, first;
+    int8_t* el;
+    double* px; double* py; const double* pn;
+    double* dpx; double* dpy;
+    double* sx; double* sy;
+    const double* wsx; const double* wsy;
+    double* nx; double* ny;
+    double* rx; double* ry;
+    double* dvx; double* dvy;
+    double* vx; double* vy;
+    double* qx; double* qy;
+    double* dscl;
+    double* poldx; double* poldy;
+    double mu, alpha, fdecay, fac_v;
+    double c11, c12, c21, c22;
+    double d_ifc, d_lin, d_cns, d_slp, pow_s;
+};
+
+// one FFT product (gf3_VecAijPj via fft_VecAijPj, src/m_aijpj.f90:258-451, 712-1015)
+struct ConvItem {
+    int mx, my, Lx, Ly;        // grid and padded real sizes; complex row length M = Lx/2
+    int nin, nout, mask, mode; // mode 0: product, 1: forward transform only, 2: reciprocal
+    const double* in[3];
+    double* out[3];
+    const double* addend[3];   // added to the product where written (may be null)
+    const void* spec[3][3];    // [out][in] spectra, layout [kx*Ly + ky], null = zero block
+    const int8_t* el;
+    void* wsS;                 // row spectra [c][y*H + kx]
+    void* wsT;
+    const void* twM;           // exp(-2 pi i e/M), e < M
+    const void* twLx;          // exp(-2 pi i e/Lx), e < Lx
+    const void* twLy;          // exp(-2 pi i e/Ly), e < Ly
+    double scale;              // mode 1/2: factor applied to the spectrum
+};
+
+// per-case fields from the case parameters (set_prev_data, set_norm_rhs, set_tang_rhs and
+// init_curr_data, src/m_sdis.f90:208-768); returns (max h, max -h).  Forces: returns
+// (sum pn, sum px, sum py, sum px*y, sum py*x).
+struct SetupItem {
+    int n, mx, my, ibase;          // ibase 1: quadratic gap from prm; otherwise h is given
+    int prev, zero_pn, zero_pt, hertz;
+    int8_t* el;
+    double* h; double* hx; double* hy;
+    double* pn; double* px; double* py;
+    double* pvn; double* pvx; double* pvy;
+    double xc1, yc1, dx, dy;
+    double prm[6];
+    double cksi, ceta, cphi, dq, xofs, yofs;
+    double hz_aa, hz_bb, pnmax;
+};
+
+// generation of the influence coefficients (elascf_pcwcns, src/m_visc.f90:431-604)
+struct CoefItem {
+    int mx, my, pad0, pad1;
+    double dx, dy, xshft, yshft, akv, nuv;
+    double* cf;   // 9 blocks of (2mx)(2my)
+};
+
+// embedding of a coefficient block into the padded real array of a transform
+struct EmbedItem {
+    int Lx, Ly, rx, ry;        // padded sizes; offsets d kept for -rx <= d < rx (and y)
+    int cf_mx, cf_my, pad0, pad1;
+    const double* cf;          // block in [(iy+cf_my)*2cf_mx + ix+cf_mx] layout
+    double* out;               // Lx*Ly real, x fastest
+    double scale;
+};
+
+// extraction of a circulant result back into a coefficient block
+struct ExtractItem {
+    int Lx, Ly, cf_mx, cf_my;
+    const double* in;          // Lx*Ly
+    double* cf;                // block
+    double scale;
+};
+
+}  // namespace cntc

@@ -1,0 +1,172 @@
+/*
+ * cntc_amd.h -- C ABI of the MI355X contact-solve library (libcntc_amd.so).
+ *
+ * Module-3 subset of CONTACT's cntc_* library interface: same symbol names (lower case, no
+ * trailing underscore), same argument lists (every argument by pointer), same units and error
+ * codes.  A caller of the reference library (the Fortran ISO_C_BINDING interface
+ * src/contact_addon.ifc, the C prototypes matlab_intfc/contact_addon.h:7-148 or the ctypes
+ * wrapper python_intfc/__init__.py) links this library instead.  Each declaration cites the
+ * reference routine it replaces.
+ *
+ * Scope of the solver behind cntc_calculate: module 3 (basic contact), elastic half-space with
+ * quasi-identical materials (K = 0), Coulomb friction (L = 0), uniform grids, creepages
+ * prescribed (F = 0), T = 0..3 with chi = 0, normal problem by NormCG, steady rolling by
+ * GDsteady, shift and transient rolling (dq = dx) by TangCG.  Other settings make
+ * cntc_calculate return CNTC_err_other (-99) instead of solving.
+ */
+#ifndef CNTC_AMD_H
+#define CNTC_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- initialisation -------------------------------------------------------------------- */
+
+/* src/contact_addon.f90:237 cntc_initializeFirst */
+void cntc_initializefirst(int *ifcver, int *ierror, int *ioutput, const char *c_wrkdir,
+                          const char *c_outdir, const char *c_expnam, int *len_wrkdir,
+                          int *len_outdir, int *len_expnam);
+/* src/contact_addon.f90:476-576 cntc_initialize (imodul must be 3) */
+void cntc_initialize(int *ire, int *imodul, int *ifcver, int *ierror, const char *c_outdir,
+                     int *len_outdir);
+/* src/contact_addon.f90:580 cntc_setGlobalFlags */
+void cntc_setglobalflags(int *lenflg, int *params, int *values);
+
+/* ---- case set-up ----------------------------------------------------------------------- */
+
+/* src/contact_addon.f90:780-973 cntc_setFlags */
+void cntc_setflags(int *ire, int *icp, int *lenflg, int *params, int *values);
+/* src/contact_addon.f90:1128-1283 cntc_setSolverFlags */
+void cntc_setsolverflags(int *ire, int *icp, int *gdigit, int *nints, int *iparam, int *nreals,
+                         double *rparam);
+/* src/contact_addon.f90:1287-1473 cntc_setMaterialParameters (M=0) */
+void cntc_setmaterialparameters(int *ire, int *icp, int *mdigit, int *nparam, double *rparam);
+/* src/contact_addon.f90:1554 cntc_setTimestep */
+void cntc_settimestep(int *ire, int *icp, double *dt);
+/* src/contact_addon.f90:1587 cntc_setReferenceVelocity */
+void cntc_setreferencevelocity(int *ire, int *icp, double *veloc);
+/* src/contact_addon.f90:1619-1667 cntc_setRollingStepsize */
+void cntc_setrollingstepsize(int *ire, int *icp, double *chi, double *dq);
+/* src/contact_addon.f90:1671 cntc_setFrictionMethod (L=0) */
+void cntc_setfrictionmethod(int *ire, int *icp, int *imeth, int *nparam, double *params);
+/* src/contact_addon.f90:1972-2093 cntc_setHertzContact (ipotcn -1, -3) */
+void cntc_sethertzcontact(int *ire, int *icp, int *ipotcn, int *nparam, double *rparam);
+/* src/contact_addon.f90:2097-2257 cntc_setPotContact (ipotcn 1..4) */
+void cntc_setpotcontact(int *ire, int *icp, int *ipotcn, int *nparam, double *rparam);
+/* src/contact_addon.f90:2294-2323 cntc_setPenetration */
+void cntc_setpenetration(int *ire, int *icp, double *pen);
+/* src/contact_addon.f90:2327-2357 cntc_setNormalForce */
+void cntc_setnormalforce(int *ire, int *icp, double *fn);
+/* src/contact_addon.f90:2361-2487 cntc_setUndeformedDistc (ibase 1, 9) */
+void cntc_setundeformeddistc(int *ire, int *icp, int *ibase, int *nparam, double *prmudf);
+/* src/contact_addon.f90:2491-2530 cntc_setCreepages */
+void cntc_setcreepages(int *ire, int *icp, double *vx, double *vy, double *phi);
+
+/* ---- calculation ----------------------------------------------------------------------- */
+
+/* src/contact_addon.f90:3519-3552, 3754-3900 cntc_calculate / cntc_calculate3 */
+void cntc_calculate(int *ire, int *icp, int *ierror);
+
+/* No counterpart in the reference (additive entry point): solve the cases of `nprob`
+   (ire, icp) problems, each set up with the calls above, in one batched run on the GPU.
+   Replaces the reference's pattern of one cntc_calculate per OpenMP thread
+   (src/test_table.f90:196-292); ierror[k] is what cntc_calculate would return. */
+void cntc_calculatebatch(int *nprob, const int *ire, const int *icp, int *ierror);
+
+/* ---- results --------------------------------------------------------------------------- */
+
+/* src/contact_addon.f90:5021-5066 cntc_getHertzContact */
+void cntc_gethertzcontact(int *ire, int *icp, int *lenarr, double *values);
+/* src/contact_addon.f90:5070 cntc_getNumElements */
+void cntc_getnumelements(int *ire, int *icp, int *mx, int *my);
+/* src/contact_addon.f90 cntc_getGridDiscretization */
+void cntc_getgriddiscretization(int *ire, int *icp, double *dx, double *dy);
+/* src/contact_addon.f90 cntc_getPotContact (ipotcn=1 form: mx, my, xc1, yc1, dx, dy) */
+void cntc_getpotcontact(int *ire, int *icp, int *lenarr, double *values);
+/* src/contact_addon.f90 cntc_getPenetration */
+void cntc_getpenetration(int *ire, int *icp, double *pen);
+/* src/contact_addon.f90 cntc_getCreepages */
+void cntc_getcreepages(int *ire, int *icp, double *vx, double *vy, double *phi);
+/* src/contact_addon.f90:5275-5318 cntc_getContactForces */
+void cntc_getcontactforces(int *ire, int *icp, double *fn, double *tx, double *ty, double *mz);
+/* src/contact_addon.f90:5460-5497 cntc_getContactPatchAreas */
+void cntc_getcontactpatchareas(int *ire, int *icp, double *carea, double *harea, double *sarea);
+/* src/contact_addon.f90:5501-5552 cntc_getElementDivision */
+void cntc_getelementdivision(int *ire, int *icp, int *lenarr, int *eldiv);
+/* src/contact_addon.f90:5556 cntc_getMaximumPressure */
+void cntc_getmaximumpressure(int *ire, int *icp, double *pnmax);
+/* src/contact_addon.f90:5591 cntc_getMaximumTraction */
+void cntc_getmaximumtraction(int *ire, int *icp, double *ptmax);
+/* src/contact_addon.f90:5672+ cntc_getFieldData (h, mu, px, py, pn, sx, sy, wx, wy) */
+void cntc_getfielddata(int *ire, int *icp, int *ifld, int *lenarr, double *fld);
+/* src/contact_addon.f90:5827-5855 cntc_getTractions */
+void cntc_gettractions(int *ire, int *icp, int *lenarr, double *pn, double *px, double *py);
+/* src/contact_addon.f90 cntc_getMicroslip */
+void cntc_getmicroslip(int *ire, int *icp, int *lenarr, double *sx, double *sy);
+/* src/contact_addon.f90:6011-6040 cntc_getCalculationTime */
+void cntc_getcalculationtime(int *ire, int *icp, double *tcpu, double *twall);
+/* src/contact_addon.f90:6044-6067 cntc_resetCalculationTime */
+void cntc_resetcalculationtime(int *ire, int *icp);
+
+/* ---- kernel timing (no counterpart in the reference) ------------------------------------ */
+
+/* For benchmark and profiling callers: mode 1 records HIP events around every kernel launch
+   of subsequent calculations (resetting earlier records), mode 0 stops recording. */
+void cntc_amdprofile(int *mode);
+/* Per kernel (kinds[k] = operation*4 + stage): seconds, algorithmic bytes and launches
+   accumulated since recording started; *nkinds receives the number of kernels recorded. */
+void cntc_amdprofileget(int *lenarr, int *nkinds, int *kinds, double *seconds, double *bytes,
+                        double *launches);
+
+/* ---- finalisation ---------------------------------------------------------------------- */
+
+/* src/contact_addon.f90:6191-6243 cntc_finalize */
+void cntc_finalize(int *ire);
+/* src/contact_addon.f90:6294 cntc_finalizeLast */
+void cntc_finalizelast(void);
+
+/* ---- flag codes (values of src/caddon_flags.inc:11-178) -------------------------------- */
+#define CNTC_if_units   1933
+#define CNTC_un_cntc    1934
+#define CNTC_un_spck    1935
+#define CNTC_un_si      1936
+#define CNTC_ic_pvtime  1970
+#define CNTC_ic_bound   1971
+#define CNTC_ic_tang    1972
+#define CNTC_ic_norm    1973
+#define CNTC_ic_force   1974
+#define CNTC_ic_discns  1977
+#define CNTC_ic_inflcf  1978
+#define CNTC_ic_iestim  1983
+#define CNTC_ic_output  1984
+#define CNTC_ic_flow    1985
+#define CNTC_ic_matfil  1987
+#define CNTC_ic_sens    1988
+#define CNTC_if_idebug  2000
+#define CNTC_if_wrtinp  2002
+#define CNTC_if_openmp  2003
+/* No counterpart in the reference: 0 = FP64 products (default), 1 = FP32 FFT products inside
+   the FP64 refinement (residual, projections and forces stay FP64). */
+#define CNTC_if_precision 2009
+
+#define CNTC_fld_h    1
+#define CNTC_fld_mu   2
+#define CNTC_fld_px   3
+#define CNTC_fld_py   4
+#define CNTC_fld_pn   5
+#define CNTC_fld_sx   15
+#define CNTC_fld_sy   16
+#define CNTC_fld_wx   22
+#define CNTC_fld_wy   23
+
+#define CNTC_err_allow  (-12)
+#define CNTC_err_norm   (-27)
+#define CNTC_err_tang   (-28)
+#define CNTC_err_icp    (-31)
+#define CNTC_err_other  (-99)
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -204,3 +204,65 @@ def fft(x, sign=-1):
     ro, io = np.zeros(len(x)), np.zeros(len(x))
     lib().orc_fft(len(x), _dptr(re), _dptr(im), _dptr(ro), _dptr(io), sign)
     return ro + 1j * io
+
+
+# GDsteady parameters of the reference's steady-rolling inputs (perfc_test/tang_problm_1c.inp:12)
+GD_PARAMS = dict(fdecay=1.0, betath=0.05, kdowfb=1, d_ifc=2.0, d_lin=-1.0, d_cns=1.0, d_slp=2.6,
+                 pow_s=1.0)
+
+
+def case_input(c):
+    """Oracle input record (and element-wise gap or None) of a case dict as issued by
+    contact_amd.workloads.apply_case."""
+    inp = default_input()
+    inp.tang = c["tang"]
+    inp.norm = c["norm"]
+    inp.fn = c.get("fn", 0.0)
+    inp.pen = c.get("pen", 0.0)
+    inp.poiss1 = inp.poiss2 = c["nu"]
+    inp.gg1 = inp.gg2 = c["g"]
+    inp.fstat = c["fstat"]
+    inp.veloc = c.get("veloc", 10000.0)
+    inp.maxgs = c["maxgs"]
+    inp.maxin = c["maxin"]
+    inp.maxnr = 30
+    inp.maxout = 1
+    inp.eps = c["eps"]
+    g = GD_PARAMS
+    inp.gausei = 5
+    inp.fdecay, inp.betath, inp.kdowfb = g["fdecay"], g["betath"], g["kdowfb"]
+    inp.d_ifc, inp.d_lin, inp.d_cns, inp.d_slp, inp.pow_s = (g["d_ifc"], g["d_lin"], g["d_cns"],
+                                                             g["d_slp"], g["pow_s"])
+    if c.get("pvtime") is not None:
+        inp.pvtime = c["pvtime"]
+        inp.iestim = c.get("iestim", 0)
+    h = None
+    if c["ipotcn"] == -3:
+        inp.ipotcn = -3
+        inp.mx, inp.my = c["mx"], c["my"]
+        inp.hz_aa, inp.hz_bb, inp.hz_scale = c["aa"], c["bb"], c["scale"]
+    else:
+        inp.ipotcn = 3
+        inp.mx, inp.my = c["mx"], c["my"]
+        inp.dx, inp.dy = c["dx"], c["dy"]
+        inp.xc1 = c["xl"] + 0.5 * c["dx"]
+        inp.yc1 = c["yl"] + 0.5 * c["dy"]
+        if c.get("h") is not None:
+            inp.ibase = 9
+            h = np.asarray(c["h"], dtype=np.float64)
+        else:
+            inp.ibase = 1
+            for k in range(6):
+                inp.prmudf[k] = c["prmudf"][k]
+    if c.get("dq") is not None:
+        inp.dq = c["dq"]
+    inp.cksi, inp.ceta, inp.cphi = c["cksi"], c["ceta"], c["cphi"]
+    return inp, h
+
+
+def solve_case(c):
+    """Solve a case dict with the CPU restatement; returns (rc, forces, problem)."""
+    P = Problem()
+    inp, h = case_input(c)
+    rc = P.calculate(inp, h)
+    return rc, P.forces(), P

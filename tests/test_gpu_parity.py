@@ -1,0 +1,107 @@
+"""Parity of the HIP engine (through the cntc_* C ABI) with the CPU restatement.
+
+Tolerances: element states identical; tractions and forces to 1e-8 relative of the force
+scale when both solves run to eps=1e-10 or tighter; with the reference's own solver
+settings (eps=1e-6 .. 1e-8), forces within 1e-6 relative and states identical except at
+elements whose slip/traction margin lies within the solver tolerance.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests import cases as C
+
+pytestmark = pytest.mark.gpu
+
+
+def run_gpu(cntc, c, ire=1, icp=1):
+    C.apply_cntc(cntc, ire, icp, c)
+    rc = cntc.calculate(ire, icp)
+    fn, fx, fy, mz = cntc.getcontactforces(ire, icp)
+    pn, px, py = cntc.gettractions(ire, icp)
+    el = cntc.getelementdivision(ire, icp)
+    return rc, (fn, fx, fy, mz), (pn.ravel(), px.ravel(), py.ravel()), el.ravel()
+
+
+def run_oracle(O, c):
+    P = O.Problem()
+    inp, h = C.to_oracle(O, c)
+    rc = P.calculate(inp, h)
+    return rc, P.forces(), P.tractions(), P.eldiv(), P
+
+
+def compare(res_g, res_o, ftol, ptol, name, max_el_diff=0):
+    rc_g, f_g, p_g, el_g = res_g
+    rc_o, f_o, p_o, el_o, _ = res_o
+    print(name, "gpu rc", rc_g, "forces", f_g, "| oracle rc", rc_o, "forces", f_o, flush=True)
+    assert rc_g == rc_o == 0
+    fscale = max(abs(f_o[0]), 1e-30)
+    for a, b in zip(f_g[:3], f_o[:3]):
+        assert abs(a - b) <= ftol * fscale, (name, f_g, f_o)
+    pscale = max(np.abs(p_o[0]).max(), 1e-30)
+    for a, b in zip(p_g, p_o):
+        assert np.abs(a - b).max() <= ptol * pscale, (name, np.abs(a - b).max() / pscale)
+    ndiff = int((el_g != el_o).sum())
+    assert ndiff <= max_el_diff, (name, ndiff)
+
+
+def test_cattaneo_tangcg(cntc, oracle):
+    """testbank.inp case 1 (T=1, TangCG with FFT preconditioner)."""
+    c = C.cattaneo_case()
+    g = run_gpu(cntc, c)
+    o = run_oracle(oracle, c)
+    compare(g, o, 1e-8, 1e-7, "cattaneo")
+    # golden, testbank/testbank.ref_out:44-47: FN 1720., FX -451.5, C=177, A=45, S=132
+    fn, fx = g[1][0], g[1][1]
+    assert abs(fn - 1720.) < 0.5 and abs(fx - (-451.5)) < 0.05
+    assert (g[3] > 0).sum() == 177 and (g[3] == 1).sum() == 45 and (g[3] == 2).sum() == 132
+
+
+def test_table_first_cases(cntc, oracle):
+    """First rows of test_table (mx=11, steady rolling, GDsteady) against the oracle."""
+    rows, _ = C.table_cases(11, 3220)
+    P = oracle.Problem()
+    for r in rows[:40:3] + rows[1150:1160:3]:
+        base = C.hertz_case(r["aa"], r["bb"], 11, 11, 82000.0, fstat=0.3, eps=1e-6, maxgs=299, maxin=1)
+        rc = P.calculate(C.to_oracle(oracle, base)[0])
+        _, _, _, _, rho, cp = P.hertz()
+        cksi = r["cmgn"] * 0.3 * math.cos(r["cang"]) * cp / rho
+        ceta = r["cmgn"] * 0.3 * math.sin(r["cang"]) * cp / rho
+        cphi = 0.3 * r["spin"] / rho
+        c = C.hertz_case(r["aa"], r["bb"], 11, 11, 82000.0, cksi=cksi, ceta=ceta, cphi=cphi,
+                         eps=1e-6, maxgs=299, maxin=1)
+        g = run_gpu(cntc, c)
+        o = run_oracle(oracle, c)
+        compare(g, o, 1e-6, 1e-5, "table case %d" % r["icase"], max_el_diff=2)
+
+
+def test_steady_rolling_40(cntc, oracle):
+    """40x40 steady rolling Hertz patch at tight tolerance (GDsteady)."""
+    c = C.hertz_case(7.193, 5.411, 40, 40, 1e5, cksi=-0.001, eps=1e-10, maxgs=2000, maxin=20,
+                     scale=1.3)
+    g = run_gpu(cntc, c)
+    o = run_oracle(oracle, c)
+    compare(g, o, 1e-8, 1e-6, "hertz40", max_el_diff=0)
+
+
+def test_batch_equals_single(cntc, oracle):
+    """Cases solved in one batched launch give the same results as solved one by one."""
+    cs = [C.hertz_case(4.0 + k, 6.0 + 0.5 * k, 12 + 3 * k, 10 + 2 * k, 5e4 + 1e4 * k,
+                       cksi=-0.0005 * (k + 1), ceta=0.0003 * k, eps=1e-8, maxgs=999, maxin=20)
+          for k in range(5)]
+    single = []
+    for k, c in enumerate(cs):
+        single.append(run_gpu(cntc, c, ire=10 + k))
+        assert single[-1][0] == 0
+        compare(single[-1], run_oracle(oracle, c), 1e-6, 1e-5, "batch case %d" % k, max_el_diff=2)
+    for k, c in enumerate(cs):
+        C.apply_cntc(cntc, 20 + k, 1, c)
+    err = cntc.calculatebatch([20 + k for k in range(5)], [1] * 5)
+    assert (err == 0).all()
+    for k in range(5):
+        f = cntc.getcontactforces(20 + k, 1)
+        pn, px, py = cntc.gettractions(20 + k, 1)
+        assert f == single[k][1]
+        assert np.array_equal(pn.ravel(), single[k][2][0])
+        assert np.array_equal(px.ravel(), single[k][2][1])
