@@ -1,0 +1,281 @@
+#!/usr/bin/env python3
+"""Throughput of the batched contact solve on one MI355X.
+
+Workload (one step): a new case for each of 2048 Hertzian steady-rolling contact patches --
+grid sides drawn log-uniform in [20, 256], Hertz semi-axis ratio log-uniform in [0.2, 5],
+Fn uniform 20-150 kN, creepages uniform +-0.005, spin uniform +-0.5 rad/m, friction uniform
+0.2-0.4, seed 2024 -- set through the cntc_* calls and solved by one cntc_calculatebatch
+(NormCG + GDsteady, eps 1e-8) with FP32 FFT products inside the FP64 refinement.
+
+Reported beside the throughput:
+* the force error of every patch against the FP64 solve of the same batch;
+* the plain FP64 path (one cntc_calculate per patch) on a systematic subset;
+* per-patch latency (batch submit to result) p50/p99;
+* the dominant kernel's achieved bandwidth from HIP events over the timed steps;
+* the CPU restatement timed on the host cores (a baseline, not the target).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor, FIRST_COMPLETED, wait
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+IRE_MIXED, IRE_FP64 = 1, 301
+
+
+def ids(k, base):
+    return base + k // 9, 1 + k % 9
+
+
+def kernel_name(op):
+    conv = {"conv32_fwd": "k_rowfwd<float>", "conv32_mul": "k_colmul<float>",
+            "conv32_inv": "k_rowinv<float>", "conv_fwd": "k_rowfwd<double>",
+            "conv_mul": "k_colmul<double>", "conv_inv": "k_rowinv<double>"}
+    return conv.get(op, "k_" + op)
+
+
+# ---- CPU baseline: the oracle on the host cores ------------------------------------------
+
+def cpu_leg(cases, seconds, cores, max_elems=3000, seed=7):
+    """Solve seeded-random patches of the batch (N <= max_elems) with the CPU restatement on
+    `cores` threads for about `seconds`; returns rates and the forces of the solved patches."""
+    from oracle import pyoracle as O
+    O.lib()
+    rng = np.random.default_rng(seed)
+    order = [int(k) for k in rng.permutation(len(cases))
+             if cases[k]["mx"] * cases[k]["my"] <= max_elems]
+
+    def one(k):
+        c = cases[k]
+        P = O.Problem()
+        inp, h = O.case_input(c)
+        t0 = time.perf_counter()
+        rc = P.calculate(inp, h)
+        dt = time.perf_counter() - t0
+        itnorm, ittang, itcg, itgs = P.iters()
+        return k, rc, dt, c["mx"] * c["my"] * (itcg + itgs), P.forces()
+
+    done = []
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        pending, pos = set(), 0
+        while pos < len(order) and len(pending) < cores:
+            pending.add(ex.submit(one, order[pos]))
+            pos += 1
+        while pending:
+            fin, pending = wait(pending, return_when=FIRST_COMPLETED)
+            done.extend(f.result() for f in fin)
+            while (pos < len(order) and len(pending) < cores
+                   and time.perf_counter() - t0 < seconds):
+                pending.add(ex.submit(one, order[pos]))
+                pos += 1
+    wall = time.perf_counter() - t0
+    ok = [d for d in done if d[1] == 0]
+    return dict(wall=wall, n=len(ok), cpu_seconds=sum(d[2] for d in ok),
+                elem_iters=float(sum(d[3] for d in ok)),
+                forces={d[0]: d[4] for d in ok}, max_elems=max_elems)
+
+
+# ---- GPU legs -----------------------------------------------------------------------------
+
+def new_case(cntc, ire, icp, c):
+    # the first set-call after a calculate starts a new case (src/test_caddon.f90:28-34)
+    cntc.setnormalforce(ire, icp, c["fn"])
+    cntc.setcreepages(ire, icp, c["cksi"], c["ceta"], c["cphi"])
+
+
+def device_sync():
+    hip = ctypes.CDLL("libamdhip64.so")
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError("hipDeviceSynchronize failed: %d" % rc)
+
+
+def forces_of(cntc, idl):
+    return np.array([cntc.getcontactforces(i, j) for i, j in idl])
+
+
+def rel_err(a, b):
+    """Per-component relative error |a - b| / |b| (Fn, Fx, Fy)."""
+    return np.abs(a[:, :3] - b[:, :3]) / np.maximum(np.abs(b[:, :3]), 1e-300)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--patches", type=int, default=2048)
+    ap.add_argument("--plain", type=int, default=32, help="patches in the one-at-a-time subset")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: patches are independent, so a multi-GPU run would "
+              "split the batch by patch with no exchange; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from contact_amd import workloads as W
+    cases_fp64 = W.hertz_batch(n=args.patches, seed=2024, eps=1e-8, precision=0)
+    cases = W.hertz_batch(n=args.patches, seed=2024, eps=1e-8, precision=1)
+    npatch = len(cases)
+    n_elem = np.array([c["mx"] * c["my"] for c in cases], dtype=np.float64)
+
+    # CPU leg first: it runs threads only and must not share the process with an active GPU run
+    cpu = None
+    if not args.no_cpu:
+        print("# cpu baseline: %d threads, %.0f s budget" % (args.cpu_cores, args.cpu_seconds),
+              file=sys.stderr, flush=True)
+        cpu = cpu_leg(cases_fp64, args.cpu_seconds, args.cpu_cores)
+        print("# cpu baseline: %d patches in %.1f s" % (cpu["n"], cpu["wall"]), file=sys.stderr,
+              flush=True)
+
+    from contact_amd import cntc
+    cntc.lib()
+    mixed = [ids(k, IRE_MIXED) for k in range(npatch)]
+    fp64 = [ids(k, IRE_FP64) for k in range(npatch)]
+    for k in range(npatch):
+        W.apply_case(cntc, *mixed[k], cases[k])
+        W.apply_case(cntc, *fp64[k], cases_fp64[k])
+
+    # FP64 solve of the whole batch: the accuracy reference of the mixed-precision result
+    print("# fp64 batch", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    err = cntc.calculatebatch([i for i, _ in fp64], [j for _, j in fp64])
+    t_fp64 = time.perf_counter() - t0
+    if np.any(err != 0):
+        raise SystemExit("fp64 batch: %d patches failed (codes %s)" % (np.sum(err != 0), sorted(set(err))))
+    F64 = forces_of(cntc, fp64)
+
+    def step():
+        for k in range(npatch):
+            new_case(cntc, *mixed[k], cases[k])
+        e = cntc.calculatebatch([i for i, _ in mixed], [j for _, j in mixed])
+        if np.any(e != 0):
+            raise SystemExit("mixed batch: %d patches failed (codes %s)" % (np.sum(e != 0), sorted(set(e))))
+
+    for w in range(args.warmup):
+        print("# warmup %d" % w, file=sys.stderr, flush=True)
+        step()
+
+    cntc.amdprofile(True)
+    device_sync()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        if s == args.steps - 1:
+            for i, j in mixed:
+                cntc.resetcalculationtime(i, j)
+        step()
+        print("# step %d done at %.2f s" % (s, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    device_sync()
+    elapsed = time.perf_counter() - t0
+    prof = cntc.amdprofileget()
+    cntc.amdprofile(False)
+
+    FM = forces_of(cntc, mixed)
+    e_mixed = rel_err(FM, F64)
+    its = np.array([cntc.getiterations(i, j) for i, j in mixed])
+    elem_iters = float(np.sum(n_elem * (its[:, 1] + its[:, 3])))
+    lat = np.array([cntc.getcalculationtime(i, j)[1] for i, j in mixed])
+
+    # plain FP64 path: one cntc_calculate per patch on a systematic subset
+    sub = list(range(0, npatch, max(1, npatch // max(1, args.plain))))[:args.plain]
+    print("# plain fp64 subset: %d patches" % len(sub), file=sys.stderr, flush=True)
+    t_plain = []
+    FP = []
+    for k in sub:
+        new_case(cntc, *fp64[k], cases_fp64[k])
+        t1 = time.perf_counter()
+        rc = cntc.calculate(*fp64[k])
+        t_plain.append(time.perf_counter() - t1)
+        if rc != 0:
+            raise SystemExit("plain solve of patch %d failed: %d" % (k, rc))
+        FP.append(cntc.getcontactforces(*fp64[k]))
+    t_plain = np.array(t_plain)
+    e_plain = rel_err(np.array(FP), F64[sub])
+
+    value = npatch * args.steps / elapsed
+    plain_rate = len(sub) / float(np.sum(t_plain))
+
+    # dominant kernel over the timed steps
+    name, (sec, byt, nl) = max(prof.items(), key=lambda kv: kv[1][0])
+    achieved = byt / sec / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": kernel_name(name), "avg_launch_us": round(sec / nl * 1e6, 2),
+                "bytes_per_launch": round(byt / nl), "launches": nl,
+                "share_of_kernel_time": round(sec / sum(v[0] for v in prof.values()), 3)}
+    kernel_sec = sum(v[0] for v in prof.values())
+
+    cpu_baseline = None
+    if cpu is not None and cpu["n"] > 0:
+        per_patch = elem_iters / npatch            # element-iterations per patch of this mix
+        rate = cpu["elem_iters"] / cpu["cpu_seconds"] * args.cpu_cores
+        cpu_baseline = {
+            "value": round(rate / per_patch, 4), "unit": "patches/s", "cores": args.cpu_cores,
+            "kind": "port",
+            "sample": ("CPU restatement (oracle/cntc_oracle.c, FP64) on %d threads for %.1f s: %d "
+                       "seeded-random patches of the batch with N<=%d, %.3g element-iterations "
+                       "in %.1f thread-seconds; patches/s of the full mix = that element-iteration "
+                       "rate / the mix's %.3g element-iterations per patch (GPU count)"
+                       % (args.cpu_cores, cpu["wall"], cpu["n"], cpu["max_elems"],
+                          cpu["elem_iters"], cpu["cpu_seconds"], per_patch)),
+        }
+        ks = sorted(cpu["forces"])
+        FO = np.array([cpu["forces"][k] for k in ks])
+        cpu_baseline["mixed_vs_oracle_force_err_max"] = float(np.max(rel_err(FM[ks], FO)))
+        cpu_baseline["fp64_vs_oracle_force_err_max"] = float(np.max(rel_err(F64[ks], FO)))
+
+    out = {
+        "metric": "contact patches solved/s at 1e-6 force err vs FP64 plain; p50/p99 latency",
+        "value": round(value, 2),
+        "unit": "patches/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "hertz_batch_2048_mixed_precision" if npatch == 2048 else "hertz_batch_%d_mixed_precision" % npatch,
+            "patches": npatch, "elements": int(n_elem.sum()), "grid_sides": [20, 256],
+            "solver": "NormCG + GDsteady, eps 1e-8", "precision": "FP32 FFT products in FP64 refinement",
+            "seed": 2024,
+        },
+        "force_err_max": float(np.max(e_mixed)),
+        "force_err_p99": float(np.percentile(np.max(e_mixed, axis=1), 99)),
+        "force_err_ok": bool(np.max(e_mixed) <= 1e-6),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                       "what": "batch submit to patch result, last timed step"},
+        "element_iterations_per_s": round(elem_iters * args.steps / elapsed, 0),
+        "plain_fp64": {"patches_per_s": round(plain_rate, 3), "patches": len(sub),
+                       "latency_ms_p50": round(float(np.percentile(t_plain, 50)) * 1e3, 2),
+                       "latency_ms_p99": round(float(np.percentile(t_plain, 99)) * 1e3, 2),
+                       "force_err_vs_batch_max": float(np.max(e_plain))},
+        "speedup_vs_plain_fp64": round(value / plain_rate, 2),
+        "fp64_batch_first_call_patches_per_s": round(npatch / t_fp64, 2),
+        "kernel_time_fraction": round(kernel_sec / elapsed, 3),
+        "roofline": roofline,
+        "cpu_baseline": cpu_baseline,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

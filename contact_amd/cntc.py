@@ -55,8 +55,11 @@ _PROTOS = {
     "cntc_getfielddata": [_pi, _pi, _pi, _pi, _pd],
     "cntc_gettractions": [_pi, _pi, _pi, _pd, _pd, _pd],
     "cntc_getmicroslip": [_pi, _pi, _pi, _pd, _pd],
+    "cntc_getiterations": [_pi, _pi, _pi, _pi],
     "cntc_getcalculationtime": [_pi, _pi, _pd, _pd],
     "cntc_resetcalculationtime": [_pi, _pi],
+    "cntc_amdprofile": [_pi],
+    "cntc_amdprofileget": [_pi, _pi, _pi, _pd, _pd, _pd],
     "cntc_finalize": [_pi],
     "cntc_finalizelast": [],
 }
@@ -255,6 +258,13 @@ def getmicroslip(ire=1, icp=1):
     return sx.reshape(my, mx), sy.reshape(my, mx)
 
 
+def getiterations(ire=1, icp=1):
+    """(itnorm, itcg, ittang, itgs) of the last case of (ire, icp)."""
+    v = np.zeros(4, dtype=np.int32)
+    lib().cntc_getiterations(_i(ire), _i(icp), _i(4), v.ctypes.data_as(_pi))
+    return tuple(int(x) for x in v)
+
+
 def getcalculationtime(ire=1, icp=1):
     c, w = c_double(0), c_double(0)
     lib().cntc_getcalculationtime(_i(ire), _i(icp), ctypes.byref(c), ctypes.byref(w))
@@ -263,6 +273,36 @@ def getcalculationtime(ire=1, icp=1):
 
 def resetcalculationtime(ire=1, icp=1):
     lib().cntc_resetcalculationtime(_i(ire), _i(icp))
+
+
+# operation names of the profile keys (key = operation * 4 + stage), in the order of OpKind
+OP_NAMES = ["vec", "ncg_negpn", "ncg_dd", "ncg_expand", "ncg_hmin", "sn_contract", "sn_expand",
+            "sn_final", "aij_ones", "tcg_nt", "tcg_rset", "tcg_zprep", "tcg_qpost", "tcg_pupd",
+            "tcg_as", "tcg_sa", "st_as", "st_sa", "gd_dp", "gd_project", "gd_probe", "gd_update",
+            "gd_moves", "gd_diagres", "conv", "conv32", "elascf", "embed", "extract", "setup",
+            "forces"]
+
+
+def amdprofile(on):
+    """Start (True; clears earlier records) or stop per-kernel HIP-event timing."""
+    lib().cntc_amdprofile(_i(1 if on else 0))
+
+
+def amdprofileget(lenarr=256):
+    """Per-kernel records since amdprofile(True): {name: (seconds, algorithmic bytes, launches)}."""
+    kinds = np.zeros(lenarr, dtype=np.int32)
+    sec, byt, nl = (np.zeros(lenarr) for _ in range(3))
+    n = c_int(0)
+    lib().cntc_amdprofileget(_i(lenarr), ctypes.byref(n), _iarr(kinds)[1], _darr(sec)[1],
+                             _darr(byt)[1], _darr(nl)[1])
+    out = {}
+    for k in range(min(n.value, lenarr)):
+        op, stage = divmod(int(kinds[k]), 4)
+        name = OP_NAMES[op] if op < len(OP_NAMES) else "op%d" % op
+        if op in (24, 25):
+            name += ("_fwd", "_mul", "_inv")[stage]
+        out[name] = (float(sec[k]), float(byt[k]), int(nl[k]))
+    return out
 
 
 def finalize(ire):

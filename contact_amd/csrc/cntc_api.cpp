@@ -365,7 +365,7 @@ static void calculate_many(int n, const int* ire, const int* icp, int* ierror)
         if (!P || j >= pr.size() || P != pr[j]) continue;
         ierror[k] = P->patch.out.ierror;
         P->calculated = true;
-        P->twall += tw / pr.size();
+        P->twall += P->patch.out.t_done > 0 ? P->patch.out.t_done : tw;
         P->tcpu += tc / pr.size();
         j++;
     }
@@ -564,6 +564,18 @@ void cntc_getmicroslip(int* ire, int* icp, int* lenarr, double* sx, double* sy)
     int f;
     f = CNTC_fld_sx; cntc_getfielddata(ire, icp, &f, lenarr, sx);
     f = CNTC_fld_sy; cntc_getfielddata(ire, icp, &f, lenarr, sy);
+}
+
+// No counterpart in the reference (the iteration counts only appear in its output file):
+// values = {itnorm, itcg, ittang, itgs} of the last case.
+void cntc_getiterations(int* ire, int* icp, int* lenarr, int* values)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    const auto& o = P->patch.out;
+    int v[4] = {o.itnorm, o.itcg, o.ittang, o.itgs};
+    for (int k = 0; k < std::min(*lenarr, 4); k++) values[k] = v[k];
 }
 
 void cntc_getcalculationtime(int* ire, int* icp, double* tcpu, double* twall)

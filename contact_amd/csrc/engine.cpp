@@ -1750,6 +1750,7 @@ static Task calculate(Ctx C)
     if (std::fabs(R.mz) < 0.5 * P.in.eps * (P.muscal * P.fntrue + TINY)) R.mz = 0.0;
     R.pen = P.pen;
     P.have_solution = true;
+    R.t_done = std::chrono::duration<double>(std::chrono::steady_clock::now() - C.E->t_solve0).count();
     if (rc) R.ierror = rc;
     else if (R.itnorm < 0) R.ierror = ERR_NORM;
     else if (R.ittang < 0) R.ierror = ERR_TANG;
@@ -1759,6 +1760,7 @@ static Task calculate(Ctx C)
 int Engine::solve(const std::vector<Patch*>& patches)
 {
     last_error.clear();
+    t_solve0 = std::chrono::steady_clock::now();
     std::vector<Patch*> ok;
     for (Patch* p : patches)
         if (prepare(*p)) ok.push_back(p);

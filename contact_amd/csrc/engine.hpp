@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -41,6 +42,7 @@ struct CaseResult {
     double pen = 0;
     int itnorm = 0, ittang = 0, itcg = 0, itgs = 0;
     int gd_stagnated = 0;
+    double t_done = 0;               // seconds from the start of the solve call to this result
     double hz_a1 = 0, hz_b1 = 0, hz_aa = 0, hz_bb = 0, hz_rho = 0, hz_cp = 0;
 };
 
@@ -105,6 +107,7 @@ public:
     // twiddle tables exp(-2 pi i e / L), e < L, cached per length and precision
     const void* twiddles(int L, bool fp32);
     std::string last_error;
+    std::chrono::steady_clock::time_point t_solve0;   // start of the current solve call
     // per-kernel timing (HIP events around every launch); key = kind*4 + stage
     void set_profile(bool on);
     int get_profile(int len, int* kinds, double* seconds, double* bytes, long* launches) const;

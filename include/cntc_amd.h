@@ -104,7 +104,12 @@ void cntc_getfielddata(int *ire, int *icp, int *ifld, int *lenarr, double *fld);
 void cntc_gettractions(int *ire, int *icp, int *lenarr, double *pn, double *px, double *py);
 /* src/contact_addon.f90 cntc_getMicroslip */
 void cntc_getmicroslip(int *ire, int *icp, int *lenarr, double *sx, double *sy);
-/* src/contact_addon.f90:6011-6040 cntc_getCalculationTime */
+/* No counterpart in the reference, which prints these counts in its flow output
+   (src/m_snorm.f90:329-331, src/m_stang.f90:57): values = {NORM outer iterations, NormCG iterations, TANG outer
+   iterations, tangential solver iterations} of the last case. */
+void cntc_getiterations(int *ire, int *icp, int *lenarr, int *values);
+/* src/contact_addon.f90:6011-6040 cntc_getCalculationTime (for cntc_calculatebatch, twall is
+   the time from the start of the batch to this problem's result) */
 void cntc_getcalculationtime(int *ire, int *icp, double *tcpu, double *twall);
 /* src/contact_addon.f90:6044-6067 cntc_resetCalculationTime */
 void cntc_resetcalculationtime(int *ire, int *icp);

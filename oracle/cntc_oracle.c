@@ -58,8 +58,8 @@ static cpx cmul(cpx a, cpx b) { cpx c = {a.re * b.re - a.im * b.im, a.re * b.im 
 
 typedef struct { int n; cpx *w; } twtab;    /* w[k] = exp(-2 pi i k / n) */
 
-static twtab tw_cache[16];
-static int tw_ncache = 0;
+static _Thread_local twtab tw_cache[16];   /* per thread: the solver may run on several threads */
+static _Thread_local int tw_ncache = 0;
 
 static const cpx *twiddles(int n)
 {
@@ -344,6 +344,7 @@ struct orc_problem {
     int itnorm, ittang, itcg, itgs;
     int ierror;
     int gd_stagnated;
+    int used_stdygs;
 };
 
 static void eldiv_alloc(eldiv *e, int npot, int my)
@@ -802,6 +803,8 @@ void orc_default_input(orc_input *inp)
     inp->pvtime = 2;
     inp->iestim = 0;
     inp->gausei = 0;
+    inp->omegah = (double)0.90f;    /* src/m_hierarch_data.f90:1736-1737 */
+    inp->omegas = 1.0;
     inp->poiss1 = inp->poiss2 = 0.28;
     inp->gg1 = inp->gg2 = 1.0;
     inp->fstat = 0.30;
@@ -1718,6 +1721,98 @@ static int gdsteady(orc_problem *P, const gdpar *G, int maxgd, double eps, doubl
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* SteadyGS, src/m_solvpt.f90:2825-3254 (chi = 0, no plasticity, no third body)                 */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Gauss-Seidel sweeps over the rows, each row from the trailing edge (ix = 1) to the leading
+   edge (ix = mx).  Per interior element: the slip of the other elements by direct summation
+   over the differences dp (gf3_AijPj), a local 2x2 problem (plstrc) with the coefficients of
+   the element relative to the first upstream non-adhesion element, then the tractions of the
+   upstream part of the row are rebuilt from dp with the traction bound applied. */
+static int stdygs(orc_problem *P, double **ws, const double *mus, int k, double eps, int maxgs,
+                  double omegah, double omegas, int *itgs_out, double *err_out)
+{
+    int mx = P->mx, my = P->my, npot = P->npot, *el = P->igs.el;
+    double *px = P->ps[0], *py = P->ps[1], *pn = P->ps[2];
+    double *sx = P->ss[0], *sy = P->ss[1];
+    double *dpx = dalloc(npot), *dpy = dalloc(npot);
+    double *DP[3] = {dpx, dpy, NULL};
+    const inflcf *cs = &P->cs;
+    /* dp = backward differences along x; the leading-edge column keeps p (:2893-2908) */
+    for (int iy = 0; iy < my; iy++)
+        for (int ix = 1; ix <= mx; ix++) {
+            int ii = ix - 1 + iy * mx;
+            if (ix != mx) { dpx[ii] = px[ii] - px[ii + 1]; dpy[ii] = py[ii] - py[ii + 1]; }
+            else { dpx[ii] = px[ii]; dpy[ii] = py[ii]; }
+        }
+    int nadh, nslip, nplast, nexter;
+    count_el(el, npot, &nadh, &nslip, &nplast, &nexter);
+    double facnel = (double)sqrtf((float)npot / (float)(nadh + nslip + nplast));
+    int itgs = 0;
+    double dif = 2.0, difid = 1.0, dif1 = 0.0;
+    while (dif >= difid && itgs < maxgs) {
+        itgs++;
+        dif = 0.0;
+        for (int iy = 1; iy <= my; iy++) {
+            for (int ix = 1; ix <= mx; ix++) {
+                int ii = ix + (iy - 1) * mx;          /* 1-based */
+                if (el[ii - 1] < ORC_ADHES) continue;
+                /* first upstream element that is not in adhesion (:2972-2977) */
+                int jx = ix - 1;
+                while (jx >= 1 && el[jx + (iy - 1) * mx - 1] == ORC_ADHES && jx != 1) jx--;
+                double coef[2][2];
+                for (int a = 0; a < 2; a++)
+                    for (int b = 0; b < 2; b++)
+                        coef[a][b] = cs->ga_inv * (cf_at(cs, a, b, 0, 0) - cf_at(cs, a, b, jx - ix, 0));
+                double s[2];
+                s[0] = ws[0][ii - 1] + aijpj(P, ii, 1, DP, -2, cs, &P->igs);
+                s[1] = ws[1][ii - 1] + aijpj(P, ii, 2, DP, -2, cs, &P->igs);
+                double pr[3] = {px[ii - 1], py[ii - 1], pn[ii - 1]};
+                elmtrc(&el[ii - 1], coef, eps, omegah, omegas, pr, mus[ii - 1], s);
+                dif += sqr(pr[0] - px[ii - 1]) + sqr(pr[1] - py[ii - 1]);
+                dpx[ii - 1] += pr[0] - px[ii - 1];
+                dpy[ii - 1] += pr[1] - py[ii - 1];
+                px[ii - 1] = pr[0];
+                py[ii - 1] = pr[1];
+                sx[ii - 1] = s[0];
+                sy[ii - 1] = s[1];
+                /* rebuild the upstream part of the row (:3086-3127) */
+                for (int kx = ix - 1; kx >= 1; kx--) {
+                    int jj = kx + (iy - 1) * mx - 1, jn = jj + 1;
+                    if (el[jj] == ORC_ADHES) {
+                        px[jj] = px[jn] + dpx[jj];
+                        py[jj] = py[jn] + dpy[jj];
+                        double ptabs = sqrt(px[jj] * px[jj] + py[jj] * py[jj]);
+                        double ptbnd = mus[jj] * pn[jj];
+                        if (ptabs > ptbnd) {
+                            px[jj] = px[jj] * ptbnd / ptabs;
+                            py[jj] = py[jj] * ptbnd / ptabs;
+                            dpx[jj] = px[jj] - px[jn];
+                            dpy[jj] = py[jj] - py[jn];
+                        }
+                    } else if (el[jj] == ORC_SLIP || el[jj] == ORC_PLAST) {
+                        dpx[jj] = px[jj] - px[jn];
+                        dpy[jj] = py[jj] - py[jn];
+                    } else if (el[jn] >= ORC_ADHES) {
+                        dpx[jj] = -px[jn];
+                        dpy[jj] = -py[jn];
+                    }
+                }
+            }
+        }
+        dif = sqrt(dif / (2.0 * k));
+        difid = eps * dmax(1e-6, facnel * gf_rms(ALLELM, el, npot, px, py));
+        if (itgs == 1) dif1 = dif;
+    }
+    (void)dif1;
+    *itgs_out = itgs;
+    *err_out = dif;
+    free(dpx);
+    free(dpy);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* TangCG, src/m_solvpt.f90:1841-2442                                                           */
 /* ------------------------------------------------------------------------------------------ */
 
@@ -1932,6 +2027,23 @@ static int stang(orc_problem *P, const orc_input *inp, int *ittang_out)
     if (k <= 0) { free(mus); return CNTC_ERR_OTHER; }
     int rc = set_facdt(P, inp->tang);
     if (rc) { free(mus); return rc; }
+    /* steady rolling needs exterior elements at the trailing edge; otherwise the reference
+       switches to ConvexGS (src/m_stang.f90:131-143, 185-190), which is not restated here */
+    if (inp->tang == 3) {
+        for (int iy = 0; iy < my; iy++)
+            if (el[iy * mx] >= ORC_ADHES) { free(mus); return CNTC_ERR_OTHER; }
+        if (inp->gausei == 2) { free(mus); return CNTC_ERR_OTHER; }
+    }
+    /* relaxation parameters of SteadyGS, src/m_stang.f90:192-223 */
+    double omegah = inp->omegah, omegas = inp->omegas;
+    if (inp->gausei == 0 || inp->gausei == 4 || inp->gausei == 5) {
+        if (k <= 25) { omegah = 1.0; omegas = 1.0; }
+        else if (inp->tang == 3) {
+            if (P->dx / P->dy <= 5.0) { omegah = 0.9; omegas = 1.0; }
+            else if (P->dx / P->dy <= 15.0) { omegah = 0.8; omegas = 0.8; }
+            else { omegah = 0.8; omegas = 0.6; }
+        } else { omegah = 0.5; omegas = 0.5; }
+    }
 
     /* stang_rhs, src/m_stang.f90:749-951 */
     double *ws[2] = {dalloc(npot), dalloc(npot)};
@@ -1966,10 +2078,20 @@ static int stang(orc_problem *P, const orc_input *inp, int *ittang_out)
         double errpt = 0.0;
         /* solvpt with F=0: wstot = wsfix1 (src/m_solvpt.f90:106-125) and one tang_solver call */
         areas(&P->igs, mx, my);
-        if (inp->tang == 3) {
+        if (inp->tang == 3 && inp->gausei == 5) {
+            /* GDsteady; SteadyGS restarts from its state when it stagnates
+               (src/m_solvpt.f90:466-485) */
             int lstagn = 0;
             rc = gdsteady(P, &G, inp->maxgs, inp->eps, ws, mus, &it, &errpt, &lstagn);
             P->gd_stagnated = lstagn;
+            if (!rc && lstagn) {
+                itgs += abs(it);
+                rc = stdygs(P, ws, mus, k, inp->eps, inp->maxgs, omegah, omegas, &it, &errpt);
+                P->used_stdygs = 1;
+            }
+        } else if (inp->tang == 3) {
+            rc = stdygs(P, ws, mus, k, inp->eps, inp->maxgs, omegah, omegas, &it, &errpt);
+            P->used_stdygs = 1;
         } else {
             rc = tangcg(P, inp->maxgs, inp->eps, ws, mus, &it, &errpt);
         }
@@ -2130,6 +2252,7 @@ int orc_calculate(orc_problem *P, const orc_input *inp)
     P->ierror = 0;
     P->itnorm = P->ittang = P->itcg = P->itgs = 0;
     P->gd_stagnated = 0;
+    P->used_stdygs = 0;
     if (inp->tang < 0 || inp->tang > 3) return P->ierror = CNTC_ERR_OTHER;
     /* combin_mater, src/m_hierarch_data.f90:1543-1553 */
     P->ga = 2.0 / (1.0 / inp->gg1 + 1.0 / inp->gg2);
@@ -2355,3 +2478,4 @@ void orc_get_iters(const orc_problem *p, int *itnorm, int *ittang, int *itcg, in
 {
     *itnorm = p->itnorm; *ittang = p->ittang; *itcg = p->itcg; *itgs = p->itgs;
 }
+int orc_get_stagnated(const orc_problem *p) { return p->gd_stagnated + 2 * p->used_stdygs; }

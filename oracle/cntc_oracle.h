@@ -54,6 +54,8 @@ typedef struct orc_input {
     double eps;
     double fdecay, betath, d_ifc, d_lin, d_cns, d_slp, pow_s;
     int kdowfb;
+    /* relaxation of SteadyGS for G=3 (G=0, 4, 5 use the reference's defaults) */
+    double omegah, omegas;
 } orc_input;
 
 /* fills an input record with the reference's module-3 defaults */
@@ -82,6 +84,9 @@ double orc_get_pen(const orc_problem *p);
 void orc_get_hertz(const orc_problem *p, double *a1, double *b1, double *aa, double *bb,
                    double *rho, double *cp);
 void orc_get_iters(const orc_problem *p, int *itnorm, int *ittang, int *itcg, int *itgs);
+/* bit 0: the last GDsteady call ended in stagnation (src/gdsteady.f90:560-585);
+   bit 1: SteadyGS ran in the last case */
+int  orc_get_stagnated(const orc_problem *p);
 
 /* stand-alone building blocks, exposed for known-answer tests */
 int  orc_opt_fft_size(int n);

@@ -37,6 +37,7 @@ class OrcInput(ctypes.Structure):
         ("fdecay", ctypes.c_double), ("betath", ctypes.c_double), ("d_ifc", ctypes.c_double),
         ("d_lin", ctypes.c_double), ("d_cns", ctypes.c_double), ("d_slp", ctypes.c_double),
         ("pow_s", ctypes.c_double), ("kdowfb", ctypes.c_int),
+        ("omegah", ctypes.c_double), ("omegas", ctypes.c_double),
     ]
 
 
@@ -71,6 +72,8 @@ def lib():
         L.orc_get_pen.restype = ctypes.c_double
         L.orc_get_hertz.argtypes = [vp, dp, dp, dp, dp, dp, dp]
         L.orc_get_iters.argtypes = [vp, ip, ip, ip, ip]
+        L.orc_get_stagnated.argtypes = [vp]
+        L.orc_get_stagnated.restype = ctypes.c_int
         L.orc_opt_fft_size.argtypes = [ctypes.c_int]
         L.orc_opt_fft_size.restype = ctypes.c_int
         L.orc_elascf_pcwcns.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int,
@@ -158,6 +161,9 @@ class Problem:
         lib().orc_get_hertz(self._p, *[ctypes.byref(x) for x in v])
         return tuple(x.value for x in v)
 
+    def stagnated(self):
+        return lib().orc_get_stagnated(self._p)
+
     def iters(self):
         v = [ctypes.c_int() for _ in range(4)]
         lib().orc_get_iters(self._p, *[ctypes.byref(x) for x in v])
@@ -229,7 +235,7 @@ def case_input(c):
     inp.maxout = 1
     inp.eps = c["eps"]
     g = GD_PARAMS
-    inp.gausei = 5
+    inp.gausei = c.get("gausei", 5)
     inp.fdecay, inp.betath, inp.kdowfb = g["fdecay"], g["betath"], g["kdowfb"]
     inp.d_ifc, inp.d_lin, inp.d_cns, inp.d_slp, inp.pow_s = (g["d_ifc"], g["d_lin"], g["d_cns"],
                                                              g["d_slp"], g["pow_s"])
