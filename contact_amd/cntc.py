@@ -259,9 +259,10 @@ def getmicroslip(ire=1, icp=1):
 
 
 def getiterations(ire=1, icp=1):
-    """(itnorm, itcg, ittang, itgs) of the last case of (ire, icp)."""
-    v = np.zeros(4, dtype=np.int32)
-    lib().cntc_getiterations(_i(ire), _i(icp), _i(4), v.ctypes.data_as(_pi))
+    """(itnorm, itcg, ittang, itgs, flags) of the last case of (ire, icp); flags bit 0:
+    GDsteady stagnated, bit 1: SteadyGS ran."""
+    v = np.zeros(5, dtype=np.int32)
+    lib().cntc_getiterations(_i(ire), _i(icp), _i(5), v.ctypes.data_as(_pi))
     return tuple(int(x) for x in v)
 
 

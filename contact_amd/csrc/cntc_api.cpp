@@ -162,6 +162,10 @@ void cntc_setsolverflags(int* ire, int* icp, int* gdigit, int* nints, int* ipara
         in.maxin = std::max(1, iparam[1]);
         in.maxnr = std::max(1, iparam[2]);
         in.maxout = std::max(1, iparam[3]);
+        if (gd == 2 || gd == 3) {
+            in.omegah = std::max(1e-20, rparam[1]);
+            in.omegas = std::max(1e-20, rparam[2]);
+        }
         if (gd == 5) {
             in.fdecay = rparam[1];
             in.betath = rparam[2];
@@ -574,8 +578,8 @@ void cntc_getiterations(int* ire, int* icp, int* lenarr, int* values)
     Problem* P = solved(*ire, *icp);
     if (!P) return;
     const auto& o = P->patch.out;
-    int v[4] = {o.itnorm, o.itcg, o.ittang, o.itgs};
-    for (int k = 0; k < std::min(*lenarr, 4); k++) values[k] = v[k];
+    int v[5] = {o.itnorm, o.itcg, o.ittang, o.itgs, o.gd_stagnated + 2 * o.used_stdygs};
+    for (int k = 0; k < std::min(*lenarr, 5); k++) values[k] = v[k];
 }
 
 void cntc_getcalculationtime(int* ire, int* icp, double* tcpu, double* twall)

@@ -138,6 +138,7 @@ union AnyItem {
     EmbedItem emb;
     ExtractItem ext;
     SetupItem setup;
+    GsItem gs;
     AnyItem() { std::memset(this, 0, sizeof(*this)); }
 };
 
@@ -158,6 +159,7 @@ static size_t item_size(int kind)
     case OP_EMBED: return sizeof(EmbedItem);
     case OP_EXTRACT: return sizeof(ExtractItem);
     case OP_SETUP: case OP_FORCES: return sizeof(SetupItem);
+    case OP_STDYGS: case OP_GUARD: return sizeof(GsItem);
     }
     return 0;
 }
@@ -1629,12 +1631,71 @@ static Task tangcg(Ctx C, int maxcg, double eps, int* itcg_out, double* err_out)
 // stang, src/m_stang.f90:28-951 (Coulomb friction, creepages prescribed)
 // ------------------------------------------------------------------------------------------
 
+static GsItem gs_item(Patch& P, int k, double omegah, double omegas)
+{
+    GsItem g{};
+    g.n = P.n; g.mx = P.mx; g.my = P.my; g.k = k;
+    g.maxgs = P.in.maxgs;
+    g.eps = P.in.eps; g.omegah = omegah; g.omegas = omegas;
+    g.ga_inv = 1.0 / P.ga;
+    g.mu = P.in.fstat;
+    g.facnel = (double)std::sqrt((float)P.n / (float)std::max(1, k));
+    g.el = P.el;
+    g.px = P.px; g.py = P.py; g.pn = P.pn;
+    g.sx = P.sx; g.sy = P.sy;
+    g.wsx = P.wsx; g.wsy = P.wsy;
+    g.dpx = P.w[0]; g.dpy = P.w[1];
+    g.list = (int*)P.w[2];
+    size_t blk = (size_t)(2 * P.mx) * (2 * P.my);
+    g.cf[0][0] = P.cf;
+    g.cf[0][1] = P.cf + blk;
+    g.cf[1][0] = P.cf + 3 * blk;
+    g.cf[1][1] = P.cf + 4 * blk;
+    return g;
+}
+
+// SteadyGS, src/m_solvpt.f90:2825-3254: all sweeps in one launch per patch
+static Task stdygs(Ctx C, int k, double omegah, double omegas, int* itgs_out, double* err_out)
+{
+    Patch& P = *C.P;
+    AnyItem it;
+    it.gs = gs_item(P, k, omegah, omegas);
+    if (op_lds(OP_STDYGS, &it.gs, 0) > 65536) co_return ERR_OTHER;
+    Res r = co_await C.opr(OP_STDYGS, it);
+    *itgs_out = (int)r.v[0];
+    *err_out = r.v[1];
+    P.out.used_stdygs = 1;
+    co_return 0;
+}
+
 static Task stang(Ctx C, int* ittang_out)
 {
     Patch& P = *C.P;
     Res r;
     r = co_await C.count(M_INT);
     if (r.v[0] <= 0) co_return ERR_OTHER;
+    const int k = (int)r.v[0];
+    if (P.in.tang == 3) {
+        // steady rolling needs exterior elements at the trailing edge, else the reference
+        // switches to ConvexGS (src/m_stang.f90:131-143, 185-190); ConvexGS (G=2) is not
+        // provided
+        if (P.in.gausei == 2) co_return ERR_OTHER;
+        AnyItem gi;
+        gi.gs = gs_item(P, k, 1.0, 1.0);
+        Res g = co_await C.opr(OP_GUARD, gi);
+        if (g.v[0] > 0) co_return ERR_OTHER;
+    }
+    // relaxation of SteadyGS (src/m_stang.f90:192-223)
+    double omegah = P.in.omegah, omegas = P.in.omegas;
+    if (P.in.gausei == 0 || P.in.gausei == 4 || P.in.gausei == 5) {
+        if (k <= 25) { omegah = 1.0; omegas = 1.0; }
+        else if (P.in.tang == 3) {
+            double ar = P.dx / P.dy;
+            if (ar <= 5.0) { omegah = 0.9; omegas = 1.0; }
+            else if (ar <= 15.0) { omegah = 0.8; omegas = 0.8; }
+            else { omegah = 0.8; omegas = 0.6; }
+        } else { omegah = 0.5; omegas = 0.5; }
+    }
     // stang_rhs: ws = -facdt h_t (- A'_tt(cv) pv_t for T=2) on the interior; facdt = 1 for
     // interior elements when dq = dx (src/m_leadedge.f90:290-318)
     C.set(M_ALL, P.wsx, 0.0, P.wsy);
@@ -1652,10 +1713,18 @@ static Task stang(Ctx C, int* ittang_out)
         int it = 0;
         double errpt = 0.0;
         int rc;
-        if (P.in.tang == 3) {
+        if (P.in.tang == 3 && P.in.gausei == 5) {
+            // GDsteady; SteadyGS restarts from its state when it stagnates
+            // (src/m_solvpt.f90:466-485)
             int lstagn = 0;
             rc = co_await gdsteady(C, P.in.maxgs, P.in.eps, &it, &errpt, &lstagn);
             P.out.gd_stagnated = lstagn;
+            if (!rc && lstagn) {
+                itgs += std::abs(it);
+                rc = co_await stdygs(C, k, omegah, omegas, &it, &errpt);
+            }
+        } else if (P.in.tang == 3) {
+            rc = co_await stdygs(C, k, omegah, omegas, &it, &errpt);
         } else {
             rc = co_await tangcg(C, P.in.maxgs, P.in.eps, &it, &errpt);
         }

@@ -32,6 +32,7 @@ struct CaseInput {
     double fdecay = (double)0.90f, betath = (double)0.10f, d_ifc = 1.0, d_lin = 1.0, d_cns = 1.0,
            d_slp = 1.0, pow_s = (double)0.90f;
     int kdowfb = 1;
+    double omegah = (double)0.90f, omegas = 1.0;   // SteadyGS relaxation for G=2, 3
     int precision = 0;              // 0: FP64 products; 1: FP32 products in FP64 refinement
 };
 
@@ -42,6 +43,7 @@ struct CaseResult {
     double pen = 0;
     int itnorm = 0, ittang = 0, itcg = 0, itgs = 0;
     int gd_stagnated = 0;
+    int used_stdygs = 0;
     double t_done = 0;               // seconds from the start of the solve call to this result
     double hz_a1 = 0, hz_b1 = 0, hz_aa = 0, hz_bb = 0, hz_rho = 0, hz_cp = 0;
 };

@@ -826,7 +826,8 @@ __global__ void __launch_bounds__(ROW_THREADS) k_gd_update(const GdItem* items, 
 // plstrc without plasticity (elmtrc), src/m_solvpt.f90:3258-3807; the plastic branch only
 // reports a violated constraint when tau_c0 = 1e20 and k_tau = 0
 __device__ void elmtrc(int8_t& el, const double c11, const double c12, const double c21,
-                       const double c22, double eps, double pr[3], double mus, double s[2])
+                       const double c22, double eps, double pr[3], double mus, double s[2],
+                       double omegah = 1.0, double omegas = 1.0)
 {
     const int maxnr = 10;
     const double prc = 0.0001, taucs = 1e20, PI = 3.14159265358979323846;
@@ -842,8 +843,8 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
             double detinv = 1.0 / (c11 * c22 - c12 * c21);
             double dp0 = (-c22 * s[0] + c12 * s[1]) * detinv;
             double dp1 = (c21 * s[0] - c11 * s[1]) * detinv;
-            pr[0] = pold1 + dp0;
-            pr[1] = pold2 + dp1;
+            pr[0] = pold1 + omegah * dp0;
+            pr[1] = pold2 + omegah * dp1;
             double pabs2 = sqrt(pr[0] * pr[0] + pr[1] * pr[1]);
             double bnd = fmin(trc_bound, taucs);
             if (pabs2 <= bnd) {
@@ -895,7 +896,7 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
             double dalph = alph1 - alph0;
             if (dalph < -PI) dalph += 2.0 * PI;
             if (dalph > PI) dalph -= 2.0 * PI;
-            alph1 = alph0 + dalph;
+            alph1 = alph0 + omegas * dalph;
             pr[0] = trc_bound * cos(alph1);
             pr[1] = trc_bound * sin(alph1);
             if (fabs(pr[0]) > fabs(pr[1])) {
@@ -1456,6 +1457,216 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_forces(const SetupItem* items,
     write_partials<5>(acc, 0, partials, 5);
 }
 
+
+// ------------------------------------------------------------------------------------------
+// SteadyGS, src/m_solvpt.f90:2825-3254 (chi = 0, no plasticity, no third body)
+// ------------------------------------------------------------------------------------------
+
+constexpr int GS_THREADS = 1024;
+
+// Rows with an interior element in the trailing-edge column ix = 1: steady rolling then
+// needs ConvexGS (src/m_stang.f90:131-143, 185-190).  One thread per row.
+__global__ void __launch_bounds__(ELEM_THREADS) k_guard(const GsItem* items, const int* gstart,
+                                                        int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GsItem& I = items[it];
+    int iy = (blockIdx.x - gstart[it]) * ELEM_THREADS + threadIdx.x;
+    double acc[1] = {0.0};
+    if (iy < I.my && I.el[(size_t)iy * I.mx] >= ADHES) acc[0] = 1.0;
+    write_partials<1>(acc, 0, partials, 1);
+}
+
+// One work-group per patch runs all sweeps.  Per interior element (rows in order, each row
+// from the trailing edge ix = 1 to the leading edge): the slip due to all differences dp by
+// direct summation over the row extents of the contact area (gf3_AijPj, all threads), the
+// local 2x2 problem with the coefficients relative to the first upstream non-adhesion
+// element (thread 0), then the upstream part of the row is rebuilt from dp with the traction
+// bound applied (:3086-3127).  The rebuild stops at the first upstream element that is not
+// in adhesion: the elements beyond it depend on its unchanged traction only.  The current
+// row lives in LDS; dp is written through to global memory for the summation.
+// Returns (itgs, dif).
+__global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, const int* gstart,
+                                                       int nitems, double* partials)
+{
+    extern __shared__ double gs_lds[];
+    __shared__ double red[16 * MAXK];
+    __shared__ double s_ctl[4];
+    __shared__ int s_len;
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GsItem& I = items[it];
+    const int mx = I.mx, my = I.my, n = I.n, tid = threadIdx.x, T = blockDim.x;
+    double* rpx = gs_lds;
+    double* rpy = rpx + mx;
+    double* rdpx = rpy + mx;
+    double* rdpy = rdpx + mx;
+    double* rg = rdpy + mx;
+    int* roff = (int*)(rg + mx);          // my + 1 row offsets into the summation list
+    int* rj0 = roff + my + 1;             // first jx (0-based) of each row's range
+    int8_t* rel = (int8_t*)(rj0 + my);
+    double* dpx = I.dpx;
+    double* dpy = I.dpy;
+    const double* c00 = I.cf[0][0];
+    const double* c01 = I.cf[0][1];
+    const double* c10 = I.cf[1][0];
+    const double* c11 = I.cf[1][1];
+
+    // dp = backward differences along x; the leading-edge column keeps p (:2893-2908)
+    for (int i = tid; i < n; i += T) {
+        int ix = i % mx;
+        if (ix != mx - 1) {
+            dpx[i] = I.px[i] - I.px[i + 1];
+            dpy[i] = I.py[i] - I.py[i + 1];
+        } else {
+            dpx[i] = I.px[i];
+            dpy[i] = I.py[i];
+        }
+    }
+    // summation ranges jx in [row1st-1, rowlst+1] per row (src/m_aijpj.f90:173-185)
+    for (int jy = tid; jy < my; jy += T) {
+        const int8_t* row = I.el + (size_t)jy * mx;
+        int first = 0;
+        while (first < mx - 1 && row[first] <= EXTER) first++;
+        int last = -1;
+        for (int ix = first; ix < mx; ix++)
+            if (row[ix] >= ADHES) last = ix;
+        int j0 = first - 1 < 0 ? 0 : first - 1;
+        int j1 = last + 1 > mx - 1 ? mx - 1 : last + 1;
+        rj0[jy] = j0;
+        roff[jy + 1] = j1 >= j0 ? j1 - j0 + 1 : 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        roff[0] = 0;
+        for (int jy = 0; jy < my; jy++) roff[jy + 1] += roff[jy];
+        s_len = roff[my];
+    }
+    __syncthreads();
+    const int len = s_len;
+    for (int jy = tid; jy < my; jy += T) {
+        int j0 = rj0[jy], cnt = roff[jy + 1] - roff[jy];
+        for (int q = 0; q < cnt; q++) {
+            int jx = j0 + q;
+            I.list[2 * (roff[jy] + q)] = jy * mx + jx;
+            I.list[2 * (roff[jy] + q) + 1] = jy * 2 * mx + jx;
+        }
+    }
+    __syncthreads();
+
+    int itgs = 0;
+    double dif = 2.0, difid = 1.0;
+    while (dif >= difid && itgs < I.maxgs) {
+        itgs++;
+        double difacc = 0.0;   // thread 0
+        for (int iy = 0; iy < my; iy++) {
+            const size_t r0 = (size_t)iy * mx;
+            for (int ix = tid; ix < mx; ix += T) {
+                rpx[ix] = I.px[r0 + ix];
+                rpy[ix] = I.py[r0 + ix];
+                rdpx[ix] = dpx[r0 + ix];
+                rdpy[ix] = dpy[r0 + ix];
+                rg[ix] = I.mu * I.pn[r0 + ix];
+                rel[ix] = I.el[r0 + ix];
+            }
+            __syncthreads();
+            for (int ix = 0; ix < mx; ix++) {
+                if (rel[ix] < ADHES) continue;
+                // s = ws + A dp at (ix, iy)
+                const int base = (iy + my) * 2 * mx + ix + mx;
+                double acc[2] = {0.0, 0.0};
+                for (int e = tid; e < len; e += T) {
+                    int j = I.list[2 * e], c = base - I.list[2 * e + 1];
+                    double qx = dpx[j], qy = dpy[j];
+                    acc[0] += c00[c] * qx + c01[c] * qy;
+                    acc[1] += c10[c] * qx + c11[c] * qy;
+                }
+                group_reduce<2>(acc, 0, red);
+                if (tid == 0) {
+                    const size_t ii = r0 + ix;
+                    int jx = ix - 1;
+                    while (jx > 0 && rel[jx] == ADHES) jx--;
+                    const int b0 = my * 2 * mx + mx, bj = b0 + (jx - ix);
+                    double a11 = I.ga_inv * (c00[b0] - c00[bj]);
+                    double a12 = I.ga_inv * (c01[b0] - c01[bj]);
+                    double a21 = I.ga_inv * (c10[b0] - c10[bj]);
+                    double a22 = I.ga_inv * (c11[b0] - c11[bj]);
+                    double sv[2] = {I.wsx[ii] + I.ga_inv * acc[0], I.wsy[ii] + I.ga_inv * acc[1]};
+                    double pr[3] = {rpx[ix], rpy[ix], I.pn[ii]};
+                    int8_t e8 = rel[ix];
+                    elmtrc(e8, a11, a12, a21, a22, I.eps, pr, I.mu, sv, I.omegah, I.omegas);
+                    double ddx = pr[0] - rpx[ix], ddy = pr[1] - rpy[ix];
+                    difacc += ddx * ddx + ddy * ddy;
+                    rdpx[ix] += ddx;
+                    rdpy[ix] += ddy;
+                    dpx[ii] = rdpx[ix];
+                    dpy[ii] = rdpy[ix];
+                    rpx[ix] = pr[0];
+                    rpy[ix] = pr[1];
+                    rel[ix] = e8;
+                    I.sx[ii] = sv[0];
+                    I.sy[ii] = sv[1];
+                    for (int kx = ix - 1; kx >= 0; kx--) {
+                        int8_t ek = rel[kx];
+                        if (ek == ADHES) {
+                            double qx = rpx[kx + 1] + rdpx[kx], qy = rpy[kx + 1] + rdpy[kx];
+                            double ptabs = sqrt(qx * qx + qy * qy), ptbnd = rg[kx];
+                            if (ptabs > ptbnd) {
+                                qx = qx * ptbnd / ptabs;
+                                qy = qy * ptbnd / ptabs;
+                                rdpx[kx] = qx - rpx[kx + 1];
+                                rdpy[kx] = qy - rpy[kx + 1];
+                                dpx[r0 + kx] = rdpx[kx];
+                                dpy[r0 + kx] = rdpy[kx];
+                            }
+                            rpx[kx] = qx;
+                            rpy[kx] = qy;
+                        } else if (ek == SLIP || ek == PLAST) {
+                            rdpx[kx] = rpx[kx] - rpx[kx + 1];
+                            rdpy[kx] = rpy[kx] - rpy[kx + 1];
+                            dpx[r0 + kx] = rdpx[kx];
+                            dpy[r0 + kx] = rdpy[kx];
+                            break;
+                        } else {
+                            if (rel[kx + 1] >= ADHES) {
+                                rdpx[kx] = -rpx[kx + 1];
+                                rdpy[kx] = -rpy[kx + 1];
+                                dpx[r0 + kx] = rdpx[kx];
+                                dpy[r0 + kx] = rdpy[kx];
+                            }
+                            break;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            for (int ix = tid; ix < mx; ix += T) {
+                I.px[r0 + ix] = rpx[ix];
+                I.py[r0 + ix] = rpy[ix];
+                I.el[r0 + ix] = rel[ix];
+            }
+            __syncthreads();
+        }
+        // dif = sqrt(sum |dp_k|^2 / 2k); difid from the rms of p over all elements
+        double sq[1] = {0.0};
+        for (int i = tid; i < n; i += T) sq[0] += I.px[i] * I.px[i] + I.py[i] * I.py[i];
+        group_reduce<1>(sq, 0, red);
+        if (tid == 0) {
+            double rms = sqrt(sq[0] / (double)(2 * n > 1 ? 2 * n : 1));
+            s_ctl[0] = sqrt(difacc / (2.0 * I.k));
+            s_ctl[1] = I.eps * fmax(1e-6, I.facnel * rms);
+        }
+        __syncthreads();
+        dif = s_ctl[0];
+        difid = s_ctl[1];
+        __syncthreads();
+    }
+    double out[2] = {(double)itgs, dif};
+    if (tid == 0) {
+        partials[(size_t)blockIdx.x * 2] = out[0];
+        partials[(size_t)blockIdx.x * 2 + 1] = out[1];
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // host-side launchers
 // ------------------------------------------------------------------------------------------
@@ -1487,6 +1698,8 @@ int op_nred(int kind, const void* item)
     case OP_GD_DIAGRES: return 3;
     case OP_SETUP: return 2;
     case OP_FORCES: return 5;
+    case OP_STDYGS: return 2;
+    case OP_GUARD: return 1;
     default: return 0;
     }
     (void)item;
@@ -1519,6 +1732,8 @@ int op_groups(int kind, const void* item, int conv_stage)
         return ngroups_rows(((const GdItem*)item)->my);
     case OP_GD_MOVES: case OP_GD_DIAGRES: return ngroups_elem(((const GdItem*)item)->n);
     case OP_SETUP: case OP_FORCES: return ngroups_elem(((const SetupItem*)item)->n);
+    case OP_STDYGS: return 1;
+    case OP_GUARD: return (((const GsItem*)item)->my + ELEM_THREADS - 1) / ELEM_THREADS;
     case OP_ELASCF: { const CoefItem* c = (const CoefItem*)item; return (int)(((size_t)(2 * c->mx) * (2 * c->my) + ELEM_THREADS - 1) / ELEM_THREADS); }
     case OP_EMBED: { const EmbedItem* e = (const EmbedItem*)item; return (int)(((size_t)e->Lx * e->Ly + ELEM_THREADS - 1) / ELEM_THREADS); }
     case OP_EXTRACT: { const ExtractItem* e = (const ExtractItem*)item; return (int)(((size_t)(2 * e->cf_mx) * (2 * e->cf_my) + ELEM_THREADS - 1) / ELEM_THREADS); }
@@ -1582,6 +1797,8 @@ void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, i
     case OP_GD_DIAGRES: hipLaunchKernelGGL(k_gd_diagres, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_SETUP: hipLaunchKernelGGL(k_setup, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_FORCES: hipLaunchKernelGGL(k_forces, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_STDYGS: hipLaunchKernelGGL(k_stdygs, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_GUARD: hipLaunchKernelGGL(k_guard, ge, te, 0, s, (const GsItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_ELASCF: hipLaunchKernelGGL(k_elascf, ge, te, 0, s, (const CoefItem*)d_items, d_gstart, nitems); break;
     case OP_EMBED: hipLaunchKernelGGL(k_embed, ge, te, 0, s, (const EmbedItem*)d_items, d_gstart, nitems); break;
     case OP_EXTRACT: hipLaunchKernelGGL(k_extract, ge, te, 0, s, (const ExtractItem*)d_items, d_gstart, nitems); break;
@@ -1651,6 +1868,8 @@ double op_bytes(int kind, const void* item, int stage)
         return nout * (nsp * es + npt * (8.0 + (c->mode == 0 ? 1.0 : 0.0)));
     }
     case OP_ELASCF: { const CoefItem* c = (const CoefItem*)item; return 9.0 * 8.0 * (2 * c->mx) * (2 * c->my); }
+    case OP_STDYGS: { const GsItem* g = (const GsItem*)item; return (double)g->n * 73.0; }
+    case OP_GUARD: { const GsItem* g = (const GsItem*)item; return (double)g->my; }
     case OP_EMBED: { const EmbedItem* e = (const EmbedItem*)item; return 8.0 * e->Lx * e->Ly; }
     case OP_EXTRACT: { const ExtractItem* e = (const ExtractItem*)item; return 16.0 * (2 * e->cf_mx) * (2 * e->cf_my); }
     }
@@ -1660,6 +1879,10 @@ double op_bytes(int kind, const void* item, int stage)
 size_t op_lds(int kind, const void* item, int stage)
 {
     if (kind == OP_CONV || kind == OP_CONV32) return conv_lds(kind, (const ConvItem*)item, stage);
+    if (kind == OP_STDYGS) {
+        const GsItem* g = (const GsItem*)item;
+        return (size_t)g->mx * (5 * sizeof(double) + 1) + (size_t)(2 * g->my + 1) * sizeof(int) + 16;
+    }
     return 0;
 }
 

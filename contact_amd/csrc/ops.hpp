@@ -49,6 +49,8 @@ enum OpKind : int {
     OP_EXTRACT,
     OP_SETUP,
     OP_FORCES,
+    OP_STDYGS,
+    OP_GUARD,
     OP_COUNT_KINDS
 };
 
@@ -148,6 +150,20 @@ struct GdItem {
     double mu, alpha, fdecay, fac_v;
     double c11, c12, c21, c22;
     double d_ifc, d_lin, d_cns, d_slp, pow_s;
+};
+
+// SteadyGS (src/m_solvpt.f90:2825-3254): one work-group runs every sweep of one patch
+struct GsItem {
+    int n, mx, my, k;          // k = number of interior elements
+    int maxgs, pad;
+    double eps, omegah, omegas, ga_inv, mu, facnel;
+    int8_t* el;
+    double* px; double* py; const double* pn;
+    double* sx; double* sy;
+    const double* wsx; const double* wsy;
+    double* dpx; double* dpy;  // work: differences along x
+    int* list;                 // work: 2n ints, (element, jy*2mx + jx) of the summation range
+    const double* cf[2][2];    // tangential blocks of cs, layout [(iy+my)*2mx + ix+mx]
 };
 
 // one FFT product (gf3_VecAijPj via fft_VecAijPj, src/m_aijpj.f90:258-451, 712-1015)

@@ -129,10 +129,14 @@ def apply_case(cntc, ire, icp, c):
     cntc.setmaterialparameters(ire, icp, 0, [c["nu"], c["nu"], c["g"], c["g"]])
     cntc.setfrictionmethod(ire, icp, 0, [c["fstat"], c["fstat"]])
     cntc.setreferencevelocity(ire, icp, c.get("veloc", 10000.0))
-    g = GD_PARAMS
-    cntc.setsolverflags(ire, icp, 5, [c["maxgs"], c["maxin"], 30, 1, g["kdowfb"]],
-                        [c["eps"], g["fdecay"], g["betath"], g["d_ifc"], g["d_lin"], g["d_cns"],
-                         g["d_slp"], g["pow_s"]])
+    if c.get("gausei", 5) == 5:
+        g = GD_PARAMS
+        cntc.setsolverflags(ire, icp, 5, [c["maxgs"], c["maxin"], 30, 1, g["kdowfb"]],
+                            [c["eps"], g["fdecay"], g["betath"], g["d_ifc"], g["d_lin"],
+                             g["d_cns"], g["d_slp"], g["pow_s"]])
+    else:
+        # G=0: the default solvers (SteadyGS for T=3), src/test_table.f90:151-160
+        cntc.setsolverflags(ire, icp, 0, [c["maxgs"], c["maxin"], 30, 1], [c["eps"]])
     if c["ipotcn"] == -3:
         cntc.sethertzcontact(ire, icp, -3, [c["mx"], c["my"], c["aa"], c["bb"], c["scale"]])
     else:

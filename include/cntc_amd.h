@@ -11,8 +11,10 @@
  * Scope of the solver behind cntc_calculate: module 3 (basic contact), elastic half-space with
  * quasi-identical materials (K = 0), Coulomb friction (L = 0), uniform grids, creepages
  * prescribed (F = 0), T = 0..3 with chi = 0, normal problem by NormCG, steady rolling by
- * GDsteady, shift and transient rolling (dq = dx) by TangCG.  Other settings make
- * cntc_calculate return CNTC_err_other (-99) instead of solving.
+ * GDsteady (G=5, with the reference's SteadyGS fall-back) or SteadyGS (G=0, 3, 4), shift and
+ * transient rolling (dq = dx) by TangCG.  Other settings (ConvexGS: G=2, or steady rolling
+ * without exterior elements at the trailing edge) make cntc_calculate return
+ * CNTC_err_other (-99) instead of solving.
  */
 #ifndef CNTC_AMD_H
 #define CNTC_AMD_H
@@ -105,8 +107,9 @@ void cntc_gettractions(int *ire, int *icp, int *lenarr, double *pn, double *px, 
 /* src/contact_addon.f90 cntc_getMicroslip */
 void cntc_getmicroslip(int *ire, int *icp, int *lenarr, double *sx, double *sy);
 /* No counterpart in the reference, which prints these counts in its flow output
-   (src/m_snorm.f90:329-331, src/m_stang.f90:57): values = {NORM outer iterations, NormCG iterations, TANG outer
-   iterations, tangential solver iterations} of the last case. */
+   (src/m_snorm.f90:329-331, src/m_stang.f90:57): values = {NORM outer iterations, NormCG
+   iterations, TANG outer iterations, tangential solver iterations, solver flags} of the last
+   case; flags bit 0: GDsteady stagnated, bit 1: SteadyGS ran. */
 void cntc_getiterations(int *ire, int *icp, int *lenarr, int *values);
 /* src/contact_addon.f90:6011-6040 cntc_getCalculationTime (for cntc_calculatebatch, twall is
    the time from the start of the batch to this problem's result) */

@@ -8,13 +8,14 @@ import math
 import numpy as np
 
 def hertz_case(aa, bb, mx, my, fn, cksi=0.0, ceta=0.0, cphi=0.0, fstat=0.3, eps=1e-6,
-               maxgs=299, maxin=1, scale=None, tang=3, g=82000.0, nu=0.28, precision=0):
+               maxgs=299, maxin=1, scale=None, tang=3, g=82000.0, nu=0.28, precision=0,
+               gausei=5):
     """Hertzian patch with prescribed semi-axes (cntc_setHertzContact ipotcn=-3)."""
     if scale is None:
         scale = float(np.float32(mx) / np.float32(max(1, mx - 4)))
     return dict(tang=tang, norm=1, fn=fn, ipotcn=-3, mx=mx, my=my, aa=aa, bb=bb, scale=scale,
                 cksi=cksi, ceta=ceta, cphi=cphi, fstat=fstat, g=g, nu=nu, eps=eps, maxgs=maxgs,
-                maxin=maxin, veloc=10000.0, precision=precision)
+                maxin=maxin, veloc=10000.0, precision=precision, gausei=gausei)
 
 
 def grid_case(mx, my, xl, yl, dx, dy, prmudf=None, h=None, norm=0, pen=0.0, fn=0.0, tang=1,
@@ -60,6 +61,30 @@ def table_cases(mx=11, ncase=3220):
                         cmgn=vecmagn[ivmgn - 1], cang=vecangl[ivang - 1], spin=spin[ispin - 1],
                         fstat=fstat, mx=mx))
     return out, rc
+
+
+def table_golden():
+    """Rows of testbank/test_table_mx11.ref_fx (tests/golden/test_table_mx11_fx.csv):
+    Fx/(mu Fn), Fy/(mu Fn), Mz/(mu Fn cp), % slip per case."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "test_table_mx11_fx.csv")
+    return np.loadtxt(path, delimiter=",", comments="#")
+
+
+def table_case_inputs(O, r, gausei):
+    """The cntc settings of table row r (src/test_table.f90:196-292): creepages scaled by
+    mu cp / rho of the row's Hertz ellipse (rho, cp from the oracle's Hertz solution)."""
+    base = hertz_case(r["aa"], r["bb"], r["mx"], r["mx"], 82000.0, fstat=0.3, eps=1e-6,
+                      maxgs=299, maxin=1, gausei=gausei)
+    P = O.Problem()
+    P.calculate(to_oracle(O, base)[0])
+    _, _, _, _, rho, cp = P.hertz()
+    cksi = r["cmgn"] * 0.3 * math.cos(r["cang"]) * cp / rho
+    ceta = r["cmgn"] * 0.3 * math.sin(r["cang"]) * cp / rho
+    cphi = 0.3 * r["spin"] / rho
+    c = hertz_case(r["aa"], r["bb"], r["mx"], r["mx"], 82000.0, cksi=cksi, ceta=ceta, cphi=cphi,
+                   eps=1e-6, maxgs=299, maxin=1, gausei=gausei)
+    return c, cp
 
 
 def apply_cntc(cntc, ire, icp, c):

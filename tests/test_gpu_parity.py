@@ -105,3 +105,42 @@ def test_batch_equals_single(cntc, oracle):
         assert f == single[k][1]
         assert np.array_equal(pn.ravel(), single[k][2][0])
         assert np.array_equal(px.ravel(), single[k][2][1])
+
+
+def test_steadygs_table_golden(cntc, oracle):
+    """test_table rows with the reference's default solver for T=3 (G=0: SteadyGS) against
+    the golden rows of testbank/test_table_mx11.ref_fx and against the oracle."""
+    rows, _ = C.table_cases(11, 3220)
+    gold = C.table_golden()
+    worst = 0.0
+    for r in rows[::23]:
+        c, cp = C.table_case_inputs(oracle, r, gausei=0)
+        g = run_gpu(cntc, c)
+        o = run_oracle(oracle, c)
+        compare(g, o, 1e-6, 1e-5, "table G=0 case %d" % r["icase"], max_el_diff=2)
+        fn, fx, fy, mz = g[1]
+        el = g[3]
+        vals = [fx / (0.3 * fn), fy / (0.3 * fn), mz / (0.3 * fn * cp)]
+        ref = gold[r["icase"] - 1]
+        d = max(abs(a - b) for a, b in zip(vals, ref[:3]))
+        worst = max(worst, d)
+        assert d <= 1.5e-6, (r["icase"], vals, ref)
+        pct = 100.0 * (el == 2).sum() / (el > 0).sum()
+        assert abs(pct - ref[3]) <= 0.051, (r["icase"], pct, ref[3])
+    print("table G=0: worst golden difference %.2e" % worst)
+
+
+def test_gdsteady_stagnation_fallback(cntc, oracle):
+    """Hertzian patches on which GDsteady stagnates: both solvers fall back to SteadyGS
+    (src/m_solvpt.f90:466-485) and must agree to the solver tolerance."""
+    from contact_amd import workloads as W
+    cases = W.hertz_batch(n=2048)
+    for k in (381, 1505):
+        c = cases[k]
+        rc_g = run_gpu(cntc, c)
+        its = cntc.getiterations(1, 1)
+        o = run_oracle(oracle, c)
+        print("patch", k, "gpu iterations", its, "oracle flags", o[4].stagnated())
+        assert its[4] & 2, "SteadyGS did not run on the GPU"
+        assert o[4].stagnated() & 2, "SteadyGS did not run in the oracle"
+        compare(rc_g, o, 1e-6, 1e-5, "fallback patch %d" % k, max_el_diff=4)
