@@ -25,6 +25,7 @@ _pi, _pd = POINTER(c_int), POINTER(c_double)
 _PROTOS = {
     "cntc_initializefirst": [_pi, _pi, _pi, c_char_p, c_char_p, c_char_p, _pi, _pi, _pi],
     "cntc_initialize": [_pi, _pi, _pi, _pi, c_char_p, _pi],
+    "cntc_getmaxnumthreads": [_pi],
     "cntc_setglobalflags": [_pi, _pi, _pi],
     "cntc_setflags": [_pi, _pi, _pi, _pi, _pi],
     "cntc_setsolverflags": [_pi, _pi, _pi, _pi, _pi, _pi, _pd],
@@ -103,6 +104,12 @@ def initialize(ire, imodul=3, outdir=" ", idebug=1):
     s = outdir.encode()
     lib().cntc_initialize(_i(ire), _i(imodul), ctypes.byref(ifcver), ctypes.byref(ierr), s, _i(len(s)))
     return [ifcver.value, ierr.value]
+
+
+def getmaxnumthreads():
+    v = c_int(0)
+    lib().cntc_getmaxnumthreads(ctypes.byref(v))
+    return v.value
 
 
 def setglobalflags(params, values):

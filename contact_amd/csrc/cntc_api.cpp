@@ -114,6 +114,13 @@ void cntc_initialize(int* ire, int* imodul, int* ifcver, int* ierror, const char
     if (ierror) *ierror = 0;
 }
 
+// src/m_caddon_data.f90:310-332: calls are serialised by one lock, so one thread at a time;
+// a caller solves many problems concurrently through cntc_calculatebatch instead
+void cntc_getmaxnumthreads(int* mxthrd)
+{
+    if (mxthrd) *mxthrd = 1;
+}
+
 void cntc_setglobalflags(int* lenflg, int* params, int* values)
 {
     std::lock_guard<std::mutex> g(store().mtx);

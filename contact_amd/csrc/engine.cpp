@@ -1263,30 +1263,28 @@ static GdItem gd_item(Patch& P)
     return I;
 }
 
-static Task gdsteady(Ctx C, int maxgd, double eps, int* itgd_out, double* err_out, int* lstagn)
+// cf00: the tangential coefficients of element (0,0), unscaled (cs blocks xx, xy, yx, yy)
+static Task gdsteady(Ctx C, const double cf00[4], int maxgd, double eps, int* itgd_out,
+                     double* err_out, int* lstagn)
 {
     Patch& P = *C.P;
     const double tiny = 1e-12;
     const int npot = P.n;
     const bool fp32 = P.in.precision == 1;
     GdItem G = gd_item(P);
-    {
-        // coefficients of element (0,0) times 1/G: symmetric blocks of cs
-        std::vector<double> h(4);
-        int mx = P.mx, my = P.my;
-        size_t nb = (size_t)(2 * mx) * (2 * my), k = (size_t)my * (2 * mx) + mx;
-        HIPCHK(hipMemcpyAsync(&h[0], P.cf + 0 * nb + k, 8, hipMemcpyDeviceToHost, C.E->stream()));
-        HIPCHK(hipMemcpyAsync(&h[1], P.cf + 1 * nb + k, 8, hipMemcpyDeviceToHost, C.E->stream()));
-        HIPCHK(hipMemcpyAsync(&h[2], P.cf + 3 * nb + k, 8, hipMemcpyDeviceToHost, C.E->stream()));
-        HIPCHK(hipMemcpyAsync(&h[3], P.cf + 4 * nb + k, 8, hipMemcpyDeviceToHost, C.E->stream()));
-        HIPCHK(hipStreamSynchronize(C.E->stream()));
-        G.c11 = h[0] / P.ga; G.c12 = h[1] / P.ga; G.c21 = h[2] / P.ga; G.c22 = h[3] / P.ga;
-    }
+    G.c11 = cf00[0] / P.ga; G.c12 = cf00[1] / P.ga; G.c21 = cf00[2] / P.ga; G.c22 = cf00[3] / P.ga;
     // gd_meth / kdown (src/contact_addon.f90:1240-1247)
     int gd_meth = 1, kdown = 3;
     if (P.in.fdecay >= 0.999) gd_meth = 1;
     else if (P.in.fdecay <= 0.001) { gd_meth = 2; kdown = std::max(1, (int)std::lround(-P.in.fdecay)); }
     else gd_meth = 3;
+    {
+        // the row tiles of the GD kernels must fit the LDS of one work-group
+        AnyItem t;
+        t.gd = G;
+        if (op_lds(OP_GD_PROJECT, &t.gd, 0) > 65536 || op_lds(OP_GD_PROBE, &t.gd, 0) > 65536)
+            co_return ERR_OTHER;
+    }
     Res r;
     r = co_await C.count(M_INT);
     double ncon = r.v[0];
@@ -1675,6 +1673,7 @@ static Task stang(Ctx C, int* ittang_out)
     r = co_await C.count(M_INT);
     if (r.v[0] <= 0) co_return ERR_OTHER;
     const int k = (int)r.v[0];
+    double cf00[4] = {0.0, 0.0, 0.0, 0.0};
     if (P.in.tang == 3) {
         // steady rolling needs exterior elements at the trailing edge, else the reference
         // switches to ConvexGS (src/m_stang.f90:131-143, 185-190); ConvexGS (G=2) is not
@@ -1684,6 +1683,7 @@ static Task stang(Ctx C, int* ittang_out)
         gi.gs = gs_item(P, k, 1.0, 1.0);
         Res g = co_await C.opr(OP_GUARD, gi);
         if (g.v[0] > 0) co_return ERR_OTHER;
+        for (int q = 0; q < 4; q++) cf00[q] = g.v[1 + q];
     }
     // relaxation of SteadyGS (src/m_stang.f90:192-223)
     double omegah = P.in.omegah, omegas = P.in.omegas;
@@ -1717,7 +1717,7 @@ static Task stang(Ctx C, int* ittang_out)
             // GDsteady; SteadyGS restarts from its state when it stagnates
             // (src/m_solvpt.f90:466-485)
             int lstagn = 0;
-            rc = co_await gdsteady(C, P.in.maxgs, P.in.eps, &it, &errpt, &lstagn);
+            rc = co_await gdsteady(C, cf00, P.in.maxgs, P.in.eps, &it, &errpt, &lstagn);
             P.out.gd_stagnated = lstagn;
             if (!rc && lstagn) {
                 itgs += std::abs(it);

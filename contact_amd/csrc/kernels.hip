@@ -598,95 +598,148 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_st_sa(const StItem* items, con
 // GDsteady (src/gdsteady.f90)
 // ------------------------------------------------------------------------------------------
 
-// compute_dp, src/gdsteady.f90:610-635 (one thread per row)
-__global__ void __launch_bounds__(ROW_THREADS) k_gd_dp(const GdItem* items, const int* gstart,
-                                                       int nitems, double* partials)
+// The GD row recurrences (project_searchdir, apply_trcbnd) run from the leading edge
+// (ix = mx) to the trailing edge along each row.  A work-group takes a tile of whole rows:
+// it loads the recurrence inputs with coalesced reads into LDS, one thread per row runs the
+// recurrence in LDS, and the element-wise rest is done by all threads with coalesced access.
+constexpr int GD_THREADS = 256;
+constexpr int GD_TILE = 2048;          // elements per tile (probe, update)
+constexpr int GD_TILE_PROJ = 1024;     // elements per tile (project)
+
+__host__ __device__ inline int gd_rows_per_tile(int mx, int my, int tile)
+{
+    int r = tile / mx;
+    if (r < 1) r = 1;
+    if (r > GD_THREADS) r = GD_THREADS;
+    return r < my ? r : my;
+}
+__host__ __device__ inline int gd_row_stride(int mx) { return mx | 1; }
+
+// compute_dp, src/gdsteady.f90:610-635: dp = backward differences along x (element-wise)
+__global__ void __launch_bounds__(ELEM_THREADS) k_gd_dp(const GdItem* items, const int* gstart,
+                                                        int nitems, double* partials)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const GdItem& I = items[it];
-    int iy = (blockIdx.x - gstart[it]) * ROW_THREADS + threadIdx.x;
-    if (iy >= I.my) return;
-    int mx = I.mx;
-    size_t r0 = (size_t)iy * mx;
-    double nxp = I.px[r0 + mx - 1], nyp = I.py[r0 + mx - 1];
-    I.dpx[r0 + mx - 1] = nxp;
-    I.dpy[r0 + mx - 1] = nyp;
-    for (int ix = mx - 2; ix >= 0; ix--) {
-        double px = I.px[r0 + ix], py = I.py[r0 + ix];
-        I.dpx[r0 + ix] = px - nxp;
-        I.dpy[r0 + ix] = py - nyp;
-        nxp = px;
-        nyp = py;
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        int ix = i % I.mx;
+        if (ix == I.mx - 1) {
+            I.dpx[i] = I.px[i];
+            I.dpy[i] = I.py[i];
+        } else {
+            I.dpx[i] = I.px[i] - I.px[i + 1];
+            I.dpy[i] = I.py[i] - I.py[i + 1];
+        }
     }
 }
 
 // project_searchdir, src/gdsteady.f90:639-803: dv from the residual r, with v // t in slip
-__global__ void __launch_bounds__(ROW_THREADS) k_gd_project(const GdItem* items, const int* gstart,
-                                                            int nitems, double* partials)
+__global__ void __launch_bounds__(GD_THREADS) k_gd_project(const GdItem* items, const int* gstart,
+                                                           int nitems, double* partials)
 {
+    extern __shared__ double gd_lds[];
     int it = find_item(gstart, nitems, blockIdx.x);
     const GdItem& I = items[it];
-    int iy = (blockIdx.x - gstart[it]) * ROW_THREADS + threadIdx.x;
-    if (iy >= I.my) return;
-    const int mx = I.mx;
-    const size_t r0 = (size_t)iy * mx;
-    const int imeth = I.imeth, kdown = I.kdown;
-    double vxn = 0.0, vyn = 0.0;   // v at ix+1
-    I.vx[r0 + mx - 1] = 0.0;
-    I.vy[r0 + mx - 1] = 0.0;
-    I.dvx[r0 + mx - 1] = I.rx[r0 + mx - 1];
-    I.dvy[r0 + mx - 1] = I.ry[r0 + mx - 1];
-    int last_se = 1 << 30;   // smallest processed (1-based) ix in slip/exterior, E_down
-    for (int ix = mx - 2; ix >= 0; ix--) {
-        size_t ii = r0 + ix;
-        int8_t s = I.el[ii];
-        double dvx = I.rx[ii], dvy = I.ry[ii], vx, vy;
-        if (s == ADHES) {
-            if (imeth == 1) {
-                vx = vxn + dvx;
-                vy = vyn + dvy;
-            } else if (imeth == 3) {
-                vx = I.fdecay * vxn + dvx;
-                vy = I.fdecay * vyn + dvy;
-                dvx = vx - vxn;
-                dvy = vy - vyn;
-            } else {
-                int ixf = ix + 1;
-                if (ixf + kdown <= mx) {
-                    double inx = I.rx[r0 + ix + kdown], iny = I.ry[r0 + ix + kdown];
-                    if (last_se <= ixf + kdown - 1) { inx = 0.0; iny = 0.0; }
-                    vx = vxn + dvx - inx;
-                    vy = vyn + dvy - iny;
-                } else {
+    const int mx = I.mx, S = gd_row_stride(mx);
+    const int R = gd_rows_per_tile(mx, I.my, GD_TILE_PROJ);
+    const int row0 = (blockIdx.x - gstart[it]) * R;
+    const int nrow = min(R, I.my - row0);
+    const int ne = nrow * mx;
+    const size_t g0 = (size_t)row0 * mx;
+    double* lrx = gd_lds;
+    double* lry = lrx + R * S;
+    double* lg = lry + R * S;
+    double* lnx = lg + R * S;        // n, then v
+    double* lny = lnx + R * S;
+    double* ldx = lny + R * S;       // dv
+    double* ldy = ldx + R * S;
+    int8_t* lel = (int8_t*)(ldy + R * S);
+    for (int e = threadIdx.x; e < ne; e += GD_THREADS) {
+        int r = e / mx, l = r * S + (e - r * mx);
+        size_t i = g0 + e;
+        int8_t st = I.el[i];
+        lel[l] = st;
+        lrx[l] = I.rx[i];
+        lry[l] = I.ry[i];
+        if (st == SLIP) {
+            lg[l] = I.mu * I.pn[i];
+            lnx[l] = I.nx[i];
+            lny[l] = I.ny[i];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nrow) {
+        const int l0 = threadIdx.x * S;
+        const int imeth = I.imeth, kdown = I.kdown;
+        double vxn = 0.0, vyn = 0.0;   // v at ix+1
+        lnx[l0 + mx - 1] = 0.0;
+        lny[l0 + mx - 1] = 0.0;
+        ldx[l0 + mx - 1] = lrx[l0 + mx - 1];
+        ldy[l0 + mx - 1] = lry[l0 + mx - 1];
+        int last_se = 1 << 30;   // smallest processed (1-based) ix in slip/exterior, E_down
+        for (int ix = mx - 2; ix >= 0; ix--) {
+            const int l = l0 + ix;
+            int8_t st = lel[l];
+            double dvx = lrx[l], dvy = lry[l], vx, vy;
+            if (st == ADHES) {
+                if (imeth == 1) {
                     vx = vxn + dvx;
                     vy = vyn + dvy;
+                } else if (imeth == 3) {
+                    vx = I.fdecay * vxn + dvx;
+                    vy = I.fdecay * vyn + dvy;
+                    dvx = vx - vxn;
+                    dvy = vy - vyn;
+                } else {
+                    int ixf = ix + 1;
+                    if (ixf + kdown <= mx) {
+                        double inx = lrx[l + kdown], iny = lry[l + kdown];
+                        if (last_se <= ixf + kdown - 1) { inx = 0.0; iny = 0.0; }
+                        vx = vxn + dvx - inx;
+                        vy = vyn + dvy - iny;
+                    } else {
+                        vx = vxn + dvx;
+                        vy = vyn + dvy;
+                    }
+                    dvx = vx - vxn;
+                    dvy = vy - vyn;
                 }
+            } else if (st == SLIP) {
+                double g = lg[l];
+                double tx = -lny[l], ty = lnx[l];
+                double vt = tx * dvx + ty * dvy;
+                vt = copysign(1.0, vt) * fmin(fabs(vt), I.fac_v * g);
+                vx = tx * vt;
+                vy = ty * vt;
                 dvx = vx - vxn;
                 dvy = vy - vyn;
+                last_se = ix + 1;
+            } else {
+                vx = 0.0;
+                vy = 0.0;
+                dvx = -vxn;
+                dvy = -vyn;
+                last_se = ix + 1;
             }
-        } else if (s == SLIP) {
-            double g = I.mu * I.pn[ii];
-            double tx = -I.ny[ii], ty = I.nx[ii];
-            double vt = tx * dvx + ty * dvy;
-            vt = copysign(1.0, vt) * fmin(fabs(vt), I.fac_v * g);
-            vx = tx * vt;
-            vy = ty * vt;
-            dvx = vx - vxn;
-            dvy = vy - vyn;
-            last_se = ix + 1;
-        } else {
-            vx = 0.0;
-            vy = 0.0;
-            dvx = -vxn;
-            dvy = -vyn;
-            last_se = ix + 1;
+            lnx[l] = vx;
+            lny[l] = vy;
+            ldx[l] = dvx;
+            ldy[l] = dvy;
+            vxn = vx;
+            vyn = vy;
         }
-        I.vx[ii] = vx;
-        I.vy[ii] = vy;
-        I.dvx[ii] = dvx;
-        I.dvy[ii] = dvy;
-        vxn = vx;
-        vyn = vy;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ne; e += GD_THREADS) {
+        int r = e / mx, l = r * S + (e - r * mx);
+        size_t i = g0 + e;
+        I.vx[i] = lnx[l];
+        I.vy[i] = lny[l];
+        I.dvx[i] = ldx[l];
+        I.dvy[i] = ldy[l];
     }
 }
 
@@ -694,123 +747,182 @@ __global__ void __launch_bounds__(ROW_THREADS) k_gd_project(const GdItem* items,
 // dp + alpha dv, then the residual terms of the new orientation.  Returns
 // (rho, c0adh, c1adh, c0slp, c1slp, c2slp) and, when I.first, also
 // (sum r^2, sum r.q dscl, sum_int (dscl q)^2, sum_int r^2).
-__global__ void __launch_bounds__(ROW_THREADS) k_gd_probe(const GdItem* items, const int* gstart,
-                                                          int nitems, double* partials)
+__global__ void __launch_bounds__(GD_THREADS) k_gd_probe(const GdItem* items, const int* gstart,
+                                                         int nitems, double* partials)
 {
+    extern __shared__ double gd_lds[];
     int it = find_item(gstart, nitems, blockIdx.x);
     const GdItem& I = items[it];
-    int iy = (blockIdx.x - gstart[it]) * ROW_THREADS + threadIdx.x;
-    double acc[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const double tiny = 1e-12;
+    const int mx = I.mx, S = gd_row_stride(mx);
+    const int R = gd_rows_per_tile(mx, I.my, GD_TILE);
+    const int row0 = (blockIdx.x - gstart[it]) * R;
+    const int nrow = min(R, I.my - row0);
+    const int ne = nrow * mx;
+    const size_t g0 = (size_t)row0 * mx;
     const double a = I.alpha;
-    if (iy < I.my) {
-        const int mx = I.mx;
-        const size_t r0 = (size_t)iy * mx;
-        double pxn = 0.0, pyn = 0.0;   // psnew at ix+1
+    double* lpx = gd_lds;            // dp + alpha dv, then the new p
+    double* lpy = lpx + R * S;
+    double* lg = lpy + R * S;
+    int8_t* lel = (int8_t*)(lg + R * S);
+    for (int e = threadIdx.x; e < ne; e += GD_THREADS) {
+        int r = e / mx, l = r * S + (e - r * mx);
+        size_t i = g0 + e;
+        int8_t st = I.el[i];
+        lel[l] = st;
+        if (st >= ADHES) {
+            lg[l] = I.mu * I.pn[i];
+            lpx[l] = I.dpx[i] + a * I.dvx[i];
+            lpy[l] = I.dpy[i] + a * I.dvy[i];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nrow) {
+        const int l0 = threadIdx.x * S;
+        double pxn = 0.0, pyn = 0.0;   // new p at ix+1
         for (int ix = mx - 1; ix >= 0; ix--) {
-            size_t ii = r0 + ix;
-            int8_t s = I.el[ii];
-            double g = I.mu * I.pn[ii];
+            const int l = l0 + ix;
+            int8_t st = lel[l];
             double px, py;
-            if (ix == mx - 1 || s <= EXTER) {
+            if (ix == mx - 1 || st <= EXTER) {
                 px = 0.0;
                 py = 0.0;
             } else {
-                double dpx = I.dpx[ii] + a * I.dvx[ii], dpy = I.dpy[ii] + a * I.dvy[ii];
-                px = pxn + dpx;
-                py = pyn + dpy;
+                double g = lg[l];
+                px = pxn + lpx[l];
+                py = pyn + lpy[l];
                 double ptabs = sqrt(px * px + py * py);
-                if (s == SLIP || ptabs >= g) {
+                if (st == SLIP || ptabs >= g) {
                     px = px * g / ptabs;
                     py = py * g / ptabs;
                 }
             }
+            lpx[l] = px;
+            lpy[l] = py;
             pxn = px;
             pyn = py;
-            double dscl = I.dscl[ii];
-            double sx = I.sx[ii], sy = I.sy[ii], qx = I.qx[ii], qy = I.qy[ii];
-            if (I.first) {
-                double rx = I.rx[ii], ry = I.ry[ii];
-                acc[6] += rx * rx + ry * ry;
-                acc[7] += (rx * qx + ry * qy) * dscl;
-                if (s >= ADHES) {
-                    acc[8] += dscl * dscl * (qx * qx + qy * qy);
-                    acc[9] += rx * rx + ry * ry;
-                }
+        }
+    }
+    __syncthreads();
+    double acc[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double tiny = 1e-12;
+    for (int e = threadIdx.x; e < ne; e += GD_THREADS) {
+        int r = e / mx, l = r * S + (e - r * mx);
+        size_t ii = g0 + e;
+        int8_t st = lel[l];
+        double dscl = I.dscl[ii];
+        double sx = I.sx[ii], sy = I.sy[ii], qx = I.qx[ii], qy = I.qy[ii];
+        if (I.first) {
+            double rx = I.rx[ii], ry = I.ry[ii];
+            acc[6] += rx * rx + ry * ry;
+            acc[7] += (rx * qx + ry * qy) * dscl;
+            if (st >= ADHES) {
+                acc[8] += dscl * dscl * (qx * qx + qy * qy);
+                acc[9] += rx * rx + ry * ry;
             }
-            if (s != ADHES && s != SLIP) continue;
-            double txo = -I.ny[ii], tyo = I.nx[ii];
-            double vt = I.vx[ii] * txo + I.vy[ii] * tyo;
-            double dth_da = (g * vt) / (fmax(tiny, g * g) + a * a * vt * vt);
-            double nxn, nyn;
-            nt_from_p(px, py, nxn, nyn);
-            double txn = -nyn, tyn = nxn;
-            double sn = sx * nxn + sy * nyn;
-            int8_t elnew = s;
-            if (elnew == SLIP && sn > 0.0) elnew = ADHES;
-            double st = sx * txn + sy * tyn, rt = -st;
-            double qn = qx * nxn + qy * nyn, qt = qx * txn + qy * tyn;
-            double d2 = dscl * dscl;
-            if (elnew == ADHES) {
-                acc[1] += 2.0 * d2 * (sx * qx + sy * qy);
-                acc[2] += 2.0 * d2 * (qx * qx + qy * qy);
-                double ex = sx + a * qx, ey = sy + a * qy;
-                acc[0] += d2 * (ex * ex + ey * ey);
-            } else {
-                acc[3] -= 2.0 * d2 * (rt * qt - rt * sn * dth_da);
-                acc[4] -= 2.0 * d2 * (-qt * qt + qt * sn * dth_da - rt * qn * dth_da);
-                acc[5] -= 2.0 * d2 * (qt * qn * dth_da);
-                double ex = st + a * qt;
-                acc[0] += d2 * ex * ex;
-            }
+        }
+        if (st != ADHES && st != SLIP) continue;
+        double px = lpx[l], py = lpy[l], g = lg[l];
+        double txo = -I.ny[ii], tyo = I.nx[ii];
+        double vt = I.vx[ii] * txo + I.vy[ii] * tyo;
+        double dth_da = (g * vt) / (fmax(tiny, g * g) + a * a * vt * vt);
+        double nxn, nyn;
+        nt_from_p(px, py, nxn, nyn);
+        double txn = -nyn, tyn = nxn;
+        double sn = sx * nxn + sy * nyn;
+        int8_t elnew = st;
+        if (elnew == SLIP && sn > 0.0) elnew = ADHES;
+        double stt = sx * txn + sy * tyn, rt = -stt;
+        double qn = qx * nxn + qy * nyn, qt = qx * txn + qy * tyn;
+        double d2 = dscl * dscl;
+        if (elnew == ADHES) {
+            acc[1] += 2.0 * d2 * (sx * qx + sy * qy);
+            acc[2] += 2.0 * d2 * (qx * qx + qy * qy);
+            double ex = sx + a * qx, ey = sy + a * qy;
+            acc[0] += d2 * (ex * ex + ey * ey);
+        } else {
+            acc[3] -= 2.0 * d2 * (rt * qt - rt * sn * dth_da);
+            acc[4] -= 2.0 * d2 * (-qt * qt + qt * sn * dth_da - rt * qn * dth_da);
+            acc[5] -= 2.0 * d2 * (qt * qn * dth_da);
+            double ex = stt + a * qt;
+            acc[0] += d2 * ex * ex;
         }
     }
     write_partials<10>(acc, 0, partials, 10);
 }
 
 // pold = p; dp += alpha dv on the interior; apply_trcbnd (src/gdsteady.f90:807-847); n from p
-__global__ void __launch_bounds__(ROW_THREADS) k_gd_update(const GdItem* items, const int* gstart,
-                                                           int nitems, double* partials)
+__global__ void __launch_bounds__(GD_THREADS) k_gd_update(const GdItem* items, const int* gstart,
+                                                          int nitems, double* partials)
 {
+    extern __shared__ double gd_lds[];
     int it = find_item(gstart, nitems, blockIdx.x);
     const GdItem& I = items[it];
-    int iy = (blockIdx.x - gstart[it]) * ROW_THREADS + threadIdx.x;
-    if (iy >= I.my) return;
-    const int mx = I.mx;
-    const size_t r0 = (size_t)iy * mx;
-    double pxn = 0.0, pyn = 0.0;
-    for (int ix = mx - 1; ix >= 0; ix--) {
-        size_t ii = r0 + ix;
-        int8_t s = I.el[ii];
-        I.poldx[ii] = I.px[ii];
-        I.poldy[ii] = I.py[ii];
-        double dpx = I.dpx[ii], dpy = I.dpy[ii];
-        if (s >= ADHES) {
-            dpx += I.alpha * I.dvx[ii];
-            dpy += I.alpha * I.dvy[ii];
+    const int mx = I.mx, S = gd_row_stride(mx);
+    const int R = gd_rows_per_tile(mx, I.my, GD_TILE);
+    const int row0 = (blockIdx.x - gstart[it]) * R;
+    const int nrow = min(R, I.my - row0);
+    const int ne = nrow * mx;
+    const size_t g0 = (size_t)row0 * mx;
+    double* lpx = gd_lds;            // dp + alpha dv, then the new p
+    double* lpy = lpx + R * S;
+    double* lg = lpy + R * S;
+    int8_t* lel = (int8_t*)(lg + R * S);
+    for (int e = threadIdx.x; e < ne; e += GD_THREADS) {
+        int r = e / mx, l = r * S + (e - r * mx);
+        size_t i = g0 + e;
+        int8_t st = I.el[i];
+        lel[l] = st;
+        lg[l] = I.mu * I.pn[i];
+        double dpx = I.dpx[i], dpy = I.dpy[i];
+        if (st >= ADHES) {
+            dpx += I.alpha * I.dvx[i];
+            dpy += I.alpha * I.dvy[i];
         }
-        double px, py;
-        if (ix == mx - 1) {
-            px = 0.0;
-            py = 0.0;
-            I.dpx[ii] = dpx;   // not recomputed at the leading-edge column
-            I.dpy[ii] = dpy;
-        } else {
-            if (s <= EXTER) {
+        lpx[l] = dpx;
+        lpy[l] = dpy;
+        I.poldx[i] = I.px[i];
+        I.poldy[i] = I.py[i];
+        // the leading-edge column keeps dp (not recomputed from p)
+        if (e - r * mx == mx - 1) {
+            I.dpx[i] = dpx;
+            I.dpy[i] = dpy;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nrow) {
+        const int l0 = threadIdx.x * S;
+        double pxn = 0.0, pyn = 0.0;
+        for (int ix = mx - 1; ix >= 0; ix--) {
+            const int l = l0 + ix;
+            int8_t st = lel[l];
+            double px, py;
+            if (ix == mx - 1 || st <= EXTER) {
                 px = 0.0;
                 py = 0.0;
             } else {
-                double g = I.mu * I.pn[ii];
-                px = pxn + dpx;
-                py = pyn + dpy;
+                double g = lg[l];
+                px = pxn + lpx[l];
+                py = pyn + lpy[l];
                 double ptabs = sqrt(px * px + py * py);
-                if (s == SLIP || ptabs >= g) {
+                if (st == SLIP || ptabs >= g) {
                     px = px * g / ptabs;
                     py = py * g / ptabs;
                 }
             }
-            I.dpx[ii] = px - pxn;
-            I.dpy[ii] = py - pyn;
+            lpx[l] = px;
+            lpy[l] = py;
+            pxn = px;
+            pyn = py;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ne; e += GD_THREADS) {
+        int r = e / mx, ix = e - r * mx, l = r * S + ix;
+        size_t ii = g0 + e;
+        double px = lpx[l], py = lpy[l];
+        if (ix != mx - 1) {
+            I.dpx[ii] = px - lpx[l + 1];
+            I.dpy[ii] = py - lpy[l + 1];
         }
         I.px[ii] = px;
         I.py[ii] = py;
@@ -818,8 +930,6 @@ __global__ void __launch_bounds__(ROW_THREADS) k_gd_update(const GdItem* items, 
         nt_from_p(px, py, nx, ny);
         I.nx[ii] = nx;
         I.ny[ii] = ny;
-        pxn = px;
-        pyn = py;
     }
 }
 
@@ -1465,16 +1575,24 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_forces(const SetupItem* items,
 constexpr int GS_THREADS = 1024;
 
 // Rows with an interior element in the trailing-edge column ix = 1: steady rolling then
-// needs ConvexGS (src/m_stang.f90:131-143, 185-190).  One thread per row.
+// needs ConvexGS (src/m_stang.f90:131-143, 185-190).  One thread per row.  Also returns the
+// tangential coefficients of element (0,0) (blocks xx, xy, yx, yy, unscaled).
 __global__ void __launch_bounds__(ELEM_THREADS) k_guard(const GsItem* items, const int* gstart,
                                                         int nitems, double* partials)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const GsItem& I = items[it];
     int iy = (blockIdx.x - gstart[it]) * ELEM_THREADS + threadIdx.x;
-    double acc[1] = {0.0};
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (iy < I.my && I.el[(size_t)iy * I.mx] >= ADHES) acc[0] = 1.0;
-    write_partials<1>(acc, 0, partials, 1);
+    if (iy == 0) {
+        const size_t k0 = (size_t)I.my * 2 * I.mx + I.mx;
+        acc[1] = I.cf[0][0][k0];
+        acc[2] = I.cf[0][1][k0];
+        acc[3] = I.cf[1][0][k0];
+        acc[4] = I.cf[1][1][k0];
+    }
+    write_partials<5>(acc, 0, partials, 5);
 }
 
 // One work-group per patch runs all sweeps.  Per interior element (rows in order, each row
@@ -1699,7 +1817,7 @@ int op_nred(int kind, const void* item)
     case OP_SETUP: return 2;
     case OP_FORCES: return 5;
     case OP_STDYGS: return 2;
-    case OP_GUARD: return 1;
+    case OP_GUARD: return 5;
     default: return 0;
     }
     (void)item;
@@ -1728,8 +1846,12 @@ int op_groups(int kind, const void* item, int conv_stage)
     case OP_TCG_AS: case OP_TCG_SA:
         return ngroups_elem(((const TcgItem*)item)->n);
     case OP_ST_AS: case OP_ST_SA: return ngroups_elem(((const StItem*)item)->n);
-    case OP_GD_DP: case OP_GD_PROJECT: case OP_GD_PROBE: case OP_GD_UPDATE:
-        return ngroups_rows(((const GdItem*)item)->my);
+    case OP_GD_DP: return ngroups_elem(((const GdItem*)item)->n);
+    case OP_GD_PROJECT: case OP_GD_PROBE: case OP_GD_UPDATE: {
+        const GdItem* g = (const GdItem*)item;
+        int R = gd_rows_per_tile(g->mx, g->my, kind == OP_GD_PROJECT ? GD_TILE_PROJ : GD_TILE);
+        return (g->my + R - 1) / R;
+    }
     case OP_GD_MOVES: case OP_GD_DIAGRES: return ngroups_elem(((const GdItem*)item)->n);
     case OP_SETUP: case OP_FORCES: return ngroups_elem(((const SetupItem*)item)->n);
     case OP_STDYGS: return 1;
@@ -1789,10 +1911,10 @@ void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, i
     case OP_TCG_SA: hipLaunchKernelGGL(k_tcg_sa, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_ST_AS: hipLaunchKernelGGL(k_st_as, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_ST_SA: hipLaunchKernelGGL(k_st_sa, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_GD_DP: hipLaunchKernelGGL(k_gd_dp, ge, tr, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_GD_PROJECT: hipLaunchKernelGGL(k_gd_project, ge, tr, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_GD_PROBE: hipLaunchKernelGGL(k_gd_probe, ge, tr, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_GD_UPDATE: hipLaunchKernelGGL(k_gd_update, ge, tr, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_GD_DP: hipLaunchKernelGGL(k_gd_dp, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_GD_PROJECT: hipLaunchKernelGGL(k_gd_project, ge, dim3(GD_THREADS), lds_bytes, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_GD_PROBE: hipLaunchKernelGGL(k_gd_probe, ge, dim3(GD_THREADS), lds_bytes, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_GD_UPDATE: hipLaunchKernelGGL(k_gd_update, ge, dim3(GD_THREADS), lds_bytes, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_GD_MOVES: hipLaunchKernelGGL(k_gd_moves, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_GD_DIAGRES: hipLaunchKernelGGL(k_gd_diagres, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_SETUP: hipLaunchKernelGGL(k_setup, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, d_partials); break;
@@ -1879,6 +2001,12 @@ double op_bytes(int kind, const void* item, int stage)
 size_t op_lds(int kind, const void* item, int stage)
 {
     if (kind == OP_CONV || kind == OP_CONV32) return conv_lds(kind, (const ConvItem*)item, stage);
+    if (kind == OP_GD_PROJECT || kind == OP_GD_PROBE || kind == OP_GD_UPDATE) {
+        const GdItem* g = (const GdItem*)item;
+        int R = gd_rows_per_tile(g->mx, g->my, kind == OP_GD_PROJECT ? GD_TILE_PROJ : GD_TILE);
+        size_t cells = (size_t)R * gd_row_stride(g->mx);
+        return cells * ((kind == OP_GD_PROJECT ? 7 : 3) * sizeof(double) + 1) + 16;
+    }
     if (kind == OP_STDYGS) {
         const GsItem* g = (const GsItem*)item;
         return (size_t)g->mx * (5 * sizeof(double) + 1) + (size_t)(2 * g->my + 1) * sizeof(int) + 16;

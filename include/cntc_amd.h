@@ -32,6 +32,9 @@ void cntc_initializefirst(int *ifcver, int *ierror, int *ioutput, const char *c_
 /* src/contact_addon.f90:476-576 cntc_initialize (imodul must be 3) */
 void cntc_initialize(int *ire, int *imodul, int *ifcver, int *ierror, const char *c_outdir,
                      int *len_outdir);
+/* src/m_caddon_data.f90:310-332 cntc_getMaxNumThreads (1: calls are serialised; batches of
+   problems go through cntc_calculatebatch) */
+void cntc_getmaxnumthreads(int *mxthrd);
 /* src/contact_addon.f90:580 cntc_setGlobalFlags */
 void cntc_setglobalflags(int *lenflg, int *params, int *values);
 
