@@ -42,6 +42,11 @@ _PROTOS = {
     "cntc_setcreepages": [_pi, _pi, _pd, _pd, _pd],
     "cntc_calculate": [_pi, _pi, _pi],
     "cntc_calculatebatch": [_pi, _pi, _pi, _pi],
+    "cntc_servestart": [_pd, _pi],
+    "cntc_submit": [_pi, _pi, _pi],
+    "cntc_poll": [_pi, _pi, _pi, _pd],
+    "cntc_wait": [_pi, _pi, _pd],
+    "cntc_servestop": [],
     "cntc_gethertzcontact": [_pi, _pi, _pi, _pd],
     "cntc_getnumelements": [_pi, _pi, _pi, _pi],
     "cntc_getgriddiscretization": [_pi, _pi, _pd, _pd],
@@ -192,6 +197,38 @@ def calculatebatch(ire_list, icp_list):
     err = np.zeros(len(ire), dtype=np.int32)
     lib().cntc_calculatebatch(_i(len(ire)), irp, icpp, err.ctypes.data_as(_pi))
     return err
+
+
+def servestart(budget_bytes=0.0):
+    """Start the continuous-batching loop (device-memory budget of active patches)."""
+    err = c_int(0)
+    lib().cntc_servestart(_d(budget_bytes), ctypes.byref(err))
+    return err.value
+
+
+def submit(ire, icp):
+    """Queue the current case of (ire, icp); returns a ticket (> 0) or an error code."""
+    t = c_int(0)
+    lib().cntc_submit(_i(ire), _i(icp), ctypes.byref(t))
+    return t.value
+
+
+def poll(ticket):
+    """(done, ierror, latency_s): done 1 when finished, 0 in flight, -1 unknown ticket."""
+    done, err, lat = c_int(0), c_int(0), c_double(0.0)
+    lib().cntc_poll(_i(ticket), ctypes.byref(done), ctypes.byref(err), ctypes.byref(lat))
+    return done.value, err.value, lat.value
+
+
+def wait(ticket):
+    """Block until the request finishes; returns (ierror, latency_s)."""
+    err, lat = c_int(0), c_double(0.0)
+    lib().cntc_wait(_i(ticket), ctypes.byref(err), ctypes.byref(lat))
+    return err.value, lat.value
+
+
+def servestop():
+    lib().cntc_servestop()
 
 
 def gethertzcontact(ire=1, icp=1, lenarr=7):

@@ -42,12 +42,15 @@ struct Problem {
     Scaling scl;
     double tcpu = 0, twall = 0;
     bool calculated = false;
+    int ticket = 0;          // request of the serving loop in flight for this problem
+    int last_rc = 0;
 };
 
 struct Store {
     std::mutex mtx;
     std::unique_ptr<Engine> engine;
     std::map<std::pair<int, int>, std::unique_ptr<Problem>> probs;
+    std::map<int, Problem*> tickets;
     std::map<int, int> ire_module;
     int precision = 0;
     int idebug = 0;
@@ -61,7 +64,26 @@ Store& store()
 
 const int MAX_REID = 999, MAX_CPID = 9;
 
-// cntc_activate, src/m_caddon_data.f90:89-225: auto-initialise, check ids
+// wait for the in-flight request of P (serving loop) and record its outcome
+void settle(Problem* P)
+{
+    Store& S = store();
+    if (!P || P->ticket <= 0 || !S.engine) return;
+    int rc = 0;
+    double lat = 0.0;
+    if (S.engine->poll(P->ticket, &rc, &lat, true) == 1) {
+        P->last_rc = rc;
+        P->twall += lat;
+        P->calculated = true;
+        S.engine->forget(P->ticket);
+    }
+    S.tickets.erase(P->ticket);
+    P->ticket = 0;
+}
+
+// cntc_activate, src/m_caddon_data.f90:89-225: auto-initialise, check ids.  A problem whose
+// solve is in flight in the serving loop is settled first: the first set-call after a
+// calculate starts a new case.
 Problem* activate(int ire, int icp)
 {
     Store& S = store();
@@ -76,6 +98,7 @@ Problem* activate(int ire, int icp)
         p->patch.in.precision = S.precision;
         it = S.probs.emplace(key, std::move(p)).first;
     }
+    settle(it->second.get());
     return it->second.get();
 }
 
@@ -360,6 +383,21 @@ static void calculate_many(int n, const int* ire, const int* icp, int* ierror)
         pr.push_back(P);
     }
     if (!E || ps.empty()) return;
+    if (E->serving()) {
+        // the serving loop owns the stream: submit every problem and wait for each
+        for (Problem* P : pr) {
+            P->ticket = E->submit(&P->patch);
+            S.tickets[P->ticket] = P;
+        }
+        size_t j = 0;
+        for (int k = 0; k < n; k++) {
+            Problem* P = activate(ire[k], icp[k]);   // settles the request
+            if (!P || j >= pr.size() || P != pr[j]) continue;
+            ierror[k] = P->last_rc;
+            j++;
+        }
+        return;
+    }
     auto t0 = std::chrono::steady_clock::now();
     std::clock_t c0 = std::clock();
     try {
@@ -404,6 +442,7 @@ static Problem* solved(int ire, int icp)
 {
     auto it = store().probs.find({ire, icp});
     if (it == store().probs.end()) return nullptr;
+    settle(it->second.get());
     return it->second.get();
 }
 
@@ -606,6 +645,84 @@ void cntc_resetcalculationtime(int* ire, int* icp)
     P->tcpu = P->twall = 0.0;
 }
 
+// ---- continuous batching (no counterpart in the reference) --------------------------------
+
+void cntc_servestart(double* budget_bytes, int* ierror)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Engine* E = engine();
+    if (!E) { if (ierror) *ierror = CNTC_err_other; return; }
+    double b = budget_bytes && *budget_bytes > 0 ? *budget_bytes : 200e9;
+    E->serve_start((size_t)b);
+    if (ierror) *ierror = 0;
+}
+
+void cntc_servestop(void)
+{
+    Engine* E;
+    {
+        std::lock_guard<std::mutex> g(store().mtx);
+        E = store().engine.get();
+    }
+    if (E) E->serve_stop();
+}
+
+void cntc_submit(int* ire, int* icp, int* ticket)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Store& S = store();
+    Problem* P = activate(*ire, *icp);
+    if (!P) { *ticket = CNTC_err_icp; return; }
+    Engine* E = engine();
+    if (!E) { *ticket = CNTC_err_other; return; }
+    if (!E->serving()) E->serve_start((size_t)200e9);
+    int id = E->submit(&P->patch);
+    if (id <= 0) { *ticket = CNTC_err_other; return; }
+    P->ticket = id;
+    S.tickets[id] = P;
+    *ticket = id;
+}
+
+void cntc_poll(int* ticket, int* done, int* ierror, double* latency)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Store& S = store();
+    auto it = S.tickets.find(*ticket);
+    if (it == S.tickets.end() || !S.engine) { *done = -1; return; }
+    int rc = 0;
+    double lat = 0.0;
+    int st = S.engine->poll(*ticket, &rc, &lat, false);
+    *done = st == 1 ? 1 : 0;
+    if (st == 1) {
+        if (ierror) *ierror = rc;
+        if (latency) *latency = lat;
+        settle(it->second);
+    }
+}
+
+void cntc_wait(int* ticket, int* ierror, double* latency)
+{
+    Engine* E;
+    {
+        std::lock_guard<std::mutex> g(store().mtx);
+        Store& S = store();
+        if (S.tickets.find(*ticket) == S.tickets.end() || !S.engine) {
+            if (ierror) *ierror = CNTC_err_other;
+            return;
+        }
+        E = S.engine.get();
+    }
+    int rc = 0;
+    double lat = 0.0;
+    E->poll(*ticket, &rc, &lat, true);
+    std::lock_guard<std::mutex> g(store().mtx);
+    Store& S = store();
+    auto it = S.tickets.find(*ticket);
+    if (it != S.tickets.end()) settle(it->second);
+    if (ierror) *ierror = rc;
+    if (latency) *latency = lat;
+}
+
 // ---- profiling (no counterpart in the reference) -----------------------------------------
 
 void cntc_amdprofile(int* mode)
@@ -631,8 +748,12 @@ void cntc_finalize(int* ire)
     std::lock_guard<std::mutex> g(store().mtx);
     Store& S = store();
     for (auto it = S.probs.begin(); it != S.probs.end();) {
-        if (it->first.first == *ire) it = S.probs.erase(it);
-        else ++it;
+        if (it->first.first == *ire) {
+            settle(it->second.get());
+            it = S.probs.erase(it);
+        } else {
+            ++it;
+        }
     }
     S.ire_module.erase(*ire);
 }
@@ -641,6 +762,8 @@ void cntc_finalizelast(void)
 {
     std::lock_guard<std::mutex> g(store().mtx);
     Store& S = store();
+    for (auto& kv : S.probs) settle(kv.second.get());
+    if (S.engine) S.engine->serve_stop();
     S.probs.clear();
     S.ire_module.clear();
     S.engine.reset();

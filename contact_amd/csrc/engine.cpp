@@ -108,6 +108,15 @@ struct Task {
     std::coroutine_handle<promise_type> h;
     explicit Task(std::coroutine_handle<promise_type> hh) : h(hh) {}
     Task(Task&& o) noexcept : h(o.h) { o.h = nullptr; }
+    Task& operator=(Task&& o) noexcept
+    {
+        if (this != &o) {
+            if (h) h.destroy();
+            h = o.h;
+            o.h = nullptr;
+        }
+        return *this;
+    }
     Task(const Task&) = delete;
     ~Task()
     {
@@ -213,6 +222,25 @@ public:
     std::map<int, ProfEntry> prof;   // key kind*4 + stage
     std::vector<hipEvent_t> ev_pool;
     void set_waiter(int slot, std::coroutine_handle<> h) { waiter_[slot] = h; }
+
+    // one flush of every pending request and resumption of the waiting tasks of `slots`;
+    // false when nothing was pending
+    bool step(const std::vector<int>& slots)
+    {
+        bool any = false;
+        for (int sl : slots)
+            if (!q_[sl].empty()) any = true;
+        if (!any) return false;
+        flush();
+        for (int sl : slots)
+            if (waiter_[sl]) {
+                auto h = waiter_[sl];
+                waiter_[sl] = nullptr;
+                h.resume();
+            }
+        return true;
+    }
+    int nslots() const { return (int)q_.size(); }
 
     // run all tasks to completion
     void run(std::vector<Task>& tasks, const std::vector<int>& slots)
@@ -536,6 +564,7 @@ Engine::Engine()
 
 Engine::~Engine()
 {
+    serve_stop();
     sched_.reset();
     for (auto& t : tw_) (void)hipFree(t.second);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -671,6 +700,22 @@ static bool hertz3d(double e_star, int ipotcn, int norm, double& a1, double& b1,
 }
 
 // Host part of contac's set-up (src/m_scontc.f90:37-176): checks, Hertz, grid, h and w.
+// device bytes of one patch's arena (grid functions, coefficients, spectra, workspaces)
+size_t patch_bytes(int mx, int my)
+{
+    int Lx = 2 * opt_fft_size(mx), Ly = 2 * opt_fft_size(my);
+    int M = Lx / 2, H = M + 1;
+    size_t n = (size_t)mx * my;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t nb = (size_t)(2 * mx) * (2 * my);
+    size_t bsz = (size_t)H * Ly;
+    size_t pH = (size_t)mx + 1, pL = 2 * (size_t)my;
+    size_t ws = std::max((size_t)3 * Ly * H, pH * pL) * 16;
+    size_t emb = std::max((size_t)Lx * Ly, nb) * 8;
+    return al(n) + al(37 * n * 8) + al(9 * nb * 8) * 2 + al(3 * nb * 8) + al(9 * bsz * 16) * 2 +
+           al(3 * bsz * 16) + al(9 * bsz * 8) + al(3 * bsz * 8) + al(3 * emb) + 2 * al(ws) + 4096;
+}
+
 bool Engine::prepare(Patch& P)
 {
     CaseInput& in = P.in;
@@ -783,8 +828,7 @@ bool Engine::prepare(Patch& P)
     size_t pH = (size_t)mx + 1, pL = 2 * (size_t)my;   // preconditioner transform (2mx x 2my)
     size_t ws = std::max((size_t)3 * P.Ly * H, pH * pL) * 16;
     size_t emb = std::max((size_t)P.Lx * P.Ly, (size_t)(2 * mx) * (2 * my)) * 8;
-    size_t need = al(n) + al(37 * n * 8) + al(9 * nb * 8) * 2 + al(3 * nb * 8) + al(9 * bsz * 16) * 2 +
-                  al(3 * bsz * 16) + al(9 * bsz * 8) + al(3 * bsz * 8) + al(3 * emb) + 2 * al(ws) + 4096;
+    size_t need = patch_bytes(mx, my);
     P.mem.ensure(need);
     char* b = P.mem.base;
     auto take = [&](size_t bytes) { char* r = b; b += al(bytes); return r; };
@@ -1506,24 +1550,15 @@ static Task gdsteady(Ctx C, const double cf00[4], int maxgd, double eps, int* it
 // TangCG, src/m_solvpt.f90:1841-2442
 // ------------------------------------------------------------------------------------------
 
-static Task tangcg(Ctx C, int maxcg, double eps, int* itcg_out, double* err_out)
+// cf00: the tangential coefficients of element (0,0), unscaled (cs blocks xx, xy, yx, yy)
+static Task tangcg(Ctx C, const double cf00[4], int maxcg, double eps, int* itcg_out, double* err_out)
 {
     Patch& P = *C.P;
     const int num_inn = 4;
     const double small = 1e-6;
     const int npot = P.n;
     const bool fp32 = P.in.precision == 1;
-    double c11, c22;
-    {
-        int mx = P.mx, my = P.my;
-        size_t nb = (size_t)(2 * mx) * (2 * my), k = (size_t)my * (2 * mx) + mx;
-        double h[2];
-        HIPCHK(hipMemcpyAsync(&h[0], P.cf + 0 * nb + k, 8, hipMemcpyDeviceToHost, C.E->stream()));
-        HIPCHK(hipMemcpyAsync(&h[1], P.cf + 4 * nb + k, 8, hipMemcpyDeviceToHost, C.E->stream()));
-        HIPCHK(hipStreamSynchronize(C.E->stream()));
-        c11 = h[0];
-        c22 = h[1];
-    }
+    const double c11 = cf00[0], c22 = cf00[3];
     co_await setup_prec(C, 0);
     co_await setup_prec(C, 1);
     TcgItem T{};
@@ -1674,15 +1709,15 @@ static Task stang(Ctx C, int* ittang_out)
     if (r.v[0] <= 0) co_return ERR_OTHER;
     const int k = (int)r.v[0];
     double cf00[4] = {0.0, 0.0, 0.0, 0.0};
-    if (P.in.tang == 3) {
+    {
         // steady rolling needs exterior elements at the trailing edge, else the reference
         // switches to ConvexGS (src/m_stang.f90:131-143, 185-190); ConvexGS (G=2) is not
-        // provided
-        if (P.in.gausei == 2) co_return ERR_OTHER;
+        // provided.  The same launch returns the coefficients of element (0,0).
+        if (P.in.tang == 3 && P.in.gausei == 2) co_return ERR_OTHER;
         AnyItem gi;
         gi.gs = gs_item(P, k, 1.0, 1.0);
         Res g = co_await C.opr(OP_GUARD, gi);
-        if (g.v[0] > 0) co_return ERR_OTHER;
+        if (P.in.tang == 3 && g.v[0] > 0) co_return ERR_OTHER;
         for (int q = 0; q < 4; q++) cf00[q] = g.v[1 + q];
     }
     // relaxation of SteadyGS (src/m_stang.f90:192-223)
@@ -1726,7 +1761,7 @@ static Task stang(Ctx C, int* ittang_out)
         } else if (P.in.tang == 3) {
             rc = co_await stdygs(C, k, omegah, omegas, &it, &errpt);
         } else {
-            rc = co_await tangcg(C, P.in.maxgs, P.in.eps, &it, &errpt);
+            rc = co_await tangcg(C, cf00, P.in.maxgs, P.in.eps, &it, &errpt);
         }
         itgs += std::abs(it);
         if (rc) { ret = rc; break; }
@@ -1845,6 +1880,164 @@ int Engine::solve(const std::vector<Patch*>& patches)
     sched_->run(tasks, slots);
     HIPCHK(hipStreamSynchronize(stream_));
     return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// continuous batching: requests are admitted between flushes while others are in flight
+// ------------------------------------------------------------------------------------------
+
+static double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+bool Engine::serve_start(size_t budget_bytes)
+{
+    std::lock_guard<std::mutex> lk(smtx_);
+    if (serving_) return true;
+    budget_ = budget_bytes;
+    in_use_ = 0;
+    stop_ = false;
+    serving_ = true;
+    server_ = std::thread([this] { serve_loop(); });
+    return true;
+}
+
+void Engine::serve_stop()
+{
+    {
+        std::lock_guard<std::mutex> lk(smtx_);
+        if (!serving_) return;
+        stop_ = true;
+    }
+    scv_.notify_all();
+    server_.join();
+    std::lock_guard<std::mutex> lk(smtx_);
+    serving_ = false;
+}
+
+int Engine::submit(Patch* p)
+{
+    std::lock_guard<std::mutex> lk(smtx_);
+    if (!serving_ || stop_) return -1;
+    int id = next_ticket_++;
+    Ticket& t = tickets_[id];
+    t.p = p;
+    t.t_submit = now_s();
+    queue_.push_back(id);
+    scv_.notify_all();
+    return id;
+}
+
+int Engine::poll(int ticket, int* rc, double* latency, bool wait)
+{
+    std::unique_lock<std::mutex> lk(smtx_);
+    auto it = tickets_.find(ticket);
+    if (it == tickets_.end()) return -1;
+    if (wait) dcv_.wait(lk, [&] { return tickets_[ticket].state == 2; });
+    Ticket& t = tickets_[ticket];
+    if (t.state != 2) return 0;
+    if (rc) *rc = t.rc;
+    if (latency) *latency = t.t_done - t.t_submit;
+    return 1;
+}
+
+void Engine::forget(int ticket)
+{
+    std::lock_guard<std::mutex> lk(smtx_);
+    auto it = tickets_.find(ticket);
+    if (it != tickets_.end() && it->second.state == 2) tickets_.erase(it);
+}
+
+void Engine::serve_loop()
+{
+    struct Act {
+        int ticket, slot;
+        size_t bytes;
+        Task task;
+    };
+    const int cap = 16384;
+    sched_->resize(cap);
+    std::vector<int> free_slots;
+    for (int k = cap - 1; k >= 0; k--) free_slots.push_back(k);
+    std::vector<Act> active;
+    auto finish = [&](int id, int rc, size_t bytes) {
+        std::lock_guard<std::mutex> lk(smtx_);
+        Ticket& t = tickets_[id];
+        t.rc = rc;
+        t.state = 2;
+        t.t_done = now_s();
+        in_use_ -= bytes;
+        dcv_.notify_all();
+    };
+    for (;;) {
+        std::vector<std::pair<int, size_t>> admit;
+        {
+            std::unique_lock<std::mutex> lk(smtx_);
+            if (active.empty()) scv_.wait(lk, [&] { return stop_ || !queue_.empty(); });
+            if (stop_ && queue_.empty() && active.empty()) break;
+            // admission by device-memory footprint of the active patches
+            while (!queue_.empty() && (int)admit.size() < (int)free_slots.size()) {
+                Ticket& t = tickets_[queue_.front()];
+                size_t b = patch_bytes(std::max(1, t.p->in.mx), std::max(1, t.p->in.my));
+                if ((!active.empty() || !admit.empty()) && in_use_ + b > budget_) break;
+                in_use_ += b;
+                t.state = 1;
+                admit.push_back({queue_.front(), b});
+                queue_.pop_front();
+            }
+        }
+        for (auto& ab : admit) {
+            Patch* p = tickets_at(ab.first);
+            bool ok = false;
+            try {
+                t_solve0 = std::chrono::steady_clock::now();
+                ok = prepare(*p);
+            } catch (const std::exception& ex) {
+                last_error = ex.what();
+                p->out.ierror = ERR_OTHER;
+            }
+            if (!ok) {
+                finish(ab.first, p->out.ierror ? p->out.ierror : ERR_OTHER, ab.second);
+                continue;
+            }
+            int slot = free_slots.back();
+            free_slots.pop_back();
+            p->slot = slot;
+            active.push_back(Act{ab.first, slot, ab.second, calculate(Ctx{sched_.get(), p, slot, this})});
+            active.back().task.h.resume();
+        }
+        if (active.empty()) continue;
+        std::vector<int> slots;
+        for (auto& a : active) slots.push_back(a.slot);
+        try {
+            sched_->step(slots);
+        } catch (const std::exception& ex) {
+            last_error = ex.what();
+            for (auto& a : active) finish(a.ticket, ERR_OTHER, a.bytes);
+            active.clear();
+            sched_->resize(cap);
+            free_slots.clear();
+            for (int k = cap - 1; k >= 0; k--) free_slots.push_back(k);
+            continue;
+        }
+        for (size_t k = 0; k < active.size();) {
+            if (active[k].task.h.done()) {
+                Patch* p = tickets_at(active[k].ticket);
+                finish(active[k].ticket, p->out.ierror, active[k].bytes);
+                free_slots.push_back(active[k].slot);
+                active.erase(active.begin() + k);
+            } else {
+                k++;
+            }
+        }
+    }
+}
+
+Patch* Engine::tickets_at(int id)
+{
+    std::lock_guard<std::mutex> lk(smtx_);
+    return tickets_[id].p;
 }
 
 void Engine::set_profile(bool on)

@@ -5,9 +5,14 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <condition_variable>
 #include <cstdint>
+#include <deque>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace cntc {
@@ -94,6 +99,17 @@ struct Patch {
 
 class Scheduler;
 
+// a request of the continuous-batching loop
+struct Ticket {
+    Patch* p = nullptr;
+    int state = 0;                   // 0 queued, 1 admitted, 2 done
+    int rc = 0;
+    double t_submit = 0, t_done = 0;
+};
+
+// device bytes of one patch's arena
+size_t patch_bytes(int mx, int my);
+
 // The engine owns the stream, the staging buffers and the twiddle-table cache.
 class Engine {
 public:
@@ -115,8 +131,30 @@ public:
     int get_profile(int len, int* kinds, double* seconds, double* bytes, long* launches) const;
     long rounds() const;
 
+    // Continuous batching.  A server thread owns the stream and the scheduler while it runs:
+    // it admits queued requests between flushes (up to `budget_bytes` of active patch
+    // arenas, at least one), runs all admitted solves through the same batched launches and
+    // completes each request as soon as its own solve ends.  solve() must not be called
+    // while serving.
+    bool serve_start(size_t budget_bytes);
+    void serve_stop();
+    bool serving() const { return serving_; }
+    int submit(Patch* p);                                   // ticket > 0, -1 when not serving
+    int poll(int ticket, int* rc, double* latency, bool wait);   // 1 done, 0 pending, -1 unknown
+    void forget(int ticket);                                // drop a completed ticket
+
 private:
     bool prepare(Patch& p);
+    void serve_loop();
+    Patch* tickets_at(int id);
+    std::thread server_;
+    std::mutex smtx_;
+    std::condition_variable scv_, dcv_;
+    std::deque<int> queue_;
+    std::map<int, Ticket> tickets_;
+    int next_ticket_ = 1;
+    bool serving_ = false, stop_ = false;
+    size_t budget_ = 0, in_use_ = 0;
     hipStream_t stream_ = nullptr;
     std::unique_ptr<Scheduler> sched_;
     std::vector<std::pair<long, void*>> tw_;

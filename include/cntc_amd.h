@@ -79,6 +79,24 @@ void cntc_calculate(int *ire, int *icp, int *ierror);
    (src/test_table.f90:196-292); ierror[k] is what cntc_calculate would return. */
 void cntc_calculatebatch(int *nprob, const int *ire, const int *icp, int *ierror);
 
+/* ---- continuous batching (no counterpart in the reference) ------------------------------ */
+
+/* For a multibody caller that issues patch solves as they arise.  cntc_servestart starts the
+   serving loop: queued requests are admitted between batched launches while others are in
+   flight, up to *budget_bytes of device memory of active patches (<= 0: 200 GB), and each
+   request completes as soon as its own solve ends.  cntc_submit queues the current case of
+   (ire, icp) and returns at once with *ticket > 0 (or an error code < 0); it starts the loop
+   if needed.  cntc_poll reports *done = 1 with the outcome, 0 while in flight, -1 for an
+   unknown ticket; cntc_wait blocks.  *latency is submit-to-result in seconds.  Any set-call
+   or getter on a problem in flight first waits for its result.  cntc_servestop finishes all
+   queued requests and stops the loop; cntc_calculate and cntc_calculatebatch go through the
+   loop while it runs. */
+void cntc_servestart(double *budget_bytes, int *ierror);
+void cntc_submit(int *ire, int *icp, int *ticket);
+void cntc_poll(int *ticket, int *done, int *ierror, double *latency);
+void cntc_wait(int *ticket, int *ierror, double *latency);
+void cntc_servestop(void);
+
 /* ---- results --------------------------------------------------------------------------- */
 
 /* src/contact_addon.f90:5021-5066 cntc_getHertzContact */
