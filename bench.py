@@ -101,6 +101,80 @@ def device_sync():
         raise RuntimeError("hipDeviceSynchronize failed: %d" % rc)
 
 
+def serve_leg(cntc, W, seconds, rate, nchan=64, seed=77, ire0=801):
+    """Serving: `nchan` wheel channels with fixed grids from the batch's size distribution,
+    each stepping T=2 transient rolling (dq = dx, warm start from its previous tractions),
+    requests as Poisson arrivals at `rate` per second on random channels with creepages as
+    seeded random walks.  A channel's request waits for the channel's previous one.
+    Latency = arrival to result."""
+    rng = np.random.default_rng(seed)
+    chans = W.hertz_batch(n=nchan, seed=seed)
+    ids = [(ire0 + k // 9, 1 + k % 9) for k in range(nchan)]
+    state = []
+    for (ire, icp), c in zip(ids, chans):
+        c = dict(c)
+        c.update(tang=2, eps=1e-8, maxgs=500, maxin=50, pvtime=2, iestim=0)
+        W.apply_case(cntc, ire, icp, c)
+        state.append(c)
+    # first step of every channel (initiation), not timed
+    err = cntc.calculatebatch([i for i, _ in ids], [j for _, j in ids])
+    if np.any(err != 0):
+        raise SystemExit("serving warm-up failed: %s" % sorted(set(err)))
+    for c in state:
+        c.update(pvtime=0, iestim=3)
+    cntc.servestart(200e9)
+    n_req = max(1, int(seconds * rate))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    chan_of = rng.integers(0, nchan, n_req)
+    walk = rng.normal(0.0, 2e-4, (n_req, 2))
+    waiting = [[] for _ in range(nchan)]
+    inflight = {}
+    lat, t0, nxt, done = [], time.perf_counter(), 0, 0
+
+    def start(ch, t_arr, r):
+        c = state[ch]
+        c["cksi"] = float(np.clip(c["cksi"] + walk[r, 0], -0.005, 0.005))
+        c["ceta"] = float(np.clip(c["ceta"] + walk[r, 1], -0.005, 0.005))
+        ire, icp = ids[ch]
+        cntc.setcreepages(ire, icp, c["cksi"], c["ceta"], c["cphi"])
+        t = cntc.submit(ire, icp)
+        if t <= 0:
+            raise SystemExit("serving submit failed: %d" % t)
+        inflight[t] = (ch, t_arr)
+
+    while done < n_req:
+        now = time.perf_counter() - t0
+        while nxt < n_req and arrivals[nxt] <= now:
+            ch = int(chan_of[nxt])
+            busy = any(v[0] == ch for v in inflight.values())
+            if busy:
+                waiting[ch].append((arrivals[nxt], nxt))
+            else:
+                start(ch, arrivals[nxt], nxt)
+            nxt += 1
+        for t in list(inflight):
+            d, rc, _ = cntc.poll(t)
+            if d != 1:
+                continue
+            ch, t_arr = inflight.pop(t)
+            if rc != 0:
+                raise SystemExit("serving request failed: %d" % rc)
+            lat.append(time.perf_counter() - t0 - t_arr)
+            done += 1
+            if waiting[ch]:
+                ta, r = waiting[ch].pop(0)
+                start(ch, ta, r)
+        time.sleep(0.0002)
+    wall = time.perf_counter() - t0
+    cntc.servestop()
+    lat = np.array(lat)
+    return {"workload": "%d channels, T=2 transient rolling, Poisson arrivals" % nchan,
+            "offered_rate": rate, "requests": n_req, "achieved_rate": round(n_req / wall, 1),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "elements_per_channel_mean": round(float(np.mean([c["mx"] * c["my"] for c in chans])), 1)}
+
+
 def forces_of(cntc, idl):
     return np.array([cntc.getcontactforces(i, j) for i, j in idl])
 
@@ -120,6 +194,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-rate", type=float, default=1000.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: patches are independent, so a multi-GPU run would "
@@ -206,6 +282,11 @@ def main():
     t_plain = np.array(t_plain)
     e_plain = rel_err(np.array(FP), F64[sub])
 
+    serving = None
+    if args.serve_seconds > 0:
+        print("# serving leg", file=sys.stderr, flush=True)
+        serving = serve_leg(cntc, W, args.serve_seconds, args.serve_rate)
+
     value = npatch * args.steps / elapsed
     plain_rate = len(sub) / float(np.sum(t_plain))
 
@@ -271,6 +352,7 @@ def main():
         "speedup_vs_plain_fp64": round(value / plain_rate, 2),
         "fp64_batch_first_call_patches_per_s": round(npatch / t_fp64, 2),
         "kernel_time_fraction": round(kernel_sec / elapsed, 3),
+        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
     }
