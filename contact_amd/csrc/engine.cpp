@@ -15,6 +15,7 @@
 #include <cmath>
 #include <coroutine>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <map>
@@ -175,14 +176,20 @@ static size_t item_size(int kind)
 
 struct Req {
     int kind;
+    int slot = -1;
     AnyItem item;
     double* res = nullptr;
 };
+
+// operations that run for a long time in one launch (SteadyGS: all sweeps of a patch) go to a
+// side stream, so that the batched launches of the other patches are not held behind them
+static bool is_async_kind(int kind) { return kind == OP_STDYGS; }
 
 class Scheduler {
 public:
     explicit Scheduler(hipStream_t s) : s_(s)
     {
+        for (auto& st : side_) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         cap_ = 64ull << 20;
         HIPCHK(hipHostMalloc((void**)&hstage_, cap_, hipHostMallocDefault));
         HIPCHK(hipMalloc((void**)&dstage_, cap_));
@@ -192,6 +199,9 @@ public:
     }
     ~Scheduler()
     {
+        drop_async();
+        for (auto& op : async_free_) free_async(op);
+        for (auto& st : side_) (void)hipStreamDestroy(st);
         for (auto ev : ev_pool) (void)hipEventDestroy(ev);
         (void)hipHostFree(hstage_);
         (void)hipFree(dstage_);
@@ -201,16 +211,26 @@ public:
     }
     void resize(int nslots)
     {
+        drop_async();
         q_.assign(nslots, {});
         waiter_.assign(nslots, nullptr);
+        async_wait_.assign(nslots, 0);
     }
     void enqueue(int slot, int kind, const AnyItem& it)
     {
         Req r;
         r.kind = kind;
+        r.slot = slot;
         r.item = it;
         q_[slot].push_back(r);
     }
+    bool any_queued() const
+    {
+        for (auto& q : q_)
+            if (!q.empty()) return true;
+        return false;
+    }
+    bool any_async() const { return !async_.empty(); }
     void set_result_dest(int slot, double* dst) { q_[slot].back().res = dst; }
 
     // per-kernel timing with HIP events on the scheduler's stream (enabled on request)
@@ -230,15 +250,13 @@ public:
         bool any = false;
         for (int sl : slots)
             if (!q_[sl].empty()) any = true;
-        if (!any) return false;
-        flush();
-        for (int sl : slots)
-            if (waiter_[sl]) {
-                auto h = waiter_[sl];
-                waiter_[sl] = nullptr;
-                h.resume();
-            }
-        return true;
+        if (any) {
+            flush();
+            resume_ready(slots);
+        }
+        launch_pending(!any_queued());
+        if (!async_.empty()) any = complete_async(false) > 0 || any;
+        return any;
     }
     int nslots() const { return (int)q_.size(); }
 
@@ -247,33 +265,179 @@ public:
     {
         for (size_t i = 0; i < tasks.size(); i++) tasks[i].h.resume();
         for (;;) {
-            bool any = false;
-            for (auto& q : q_)
-                if (!q.empty()) any = true;
-            if (!any) break;
-            flush();
-            for (size_t i = 0; i < slots.size(); i++) {
-                int sl = slots[i];
-                if (waiter_[sl]) {
-                    auto h = waiter_[sl];
-                    waiter_[sl] = nullptr;
-                    h.resume();
-                }
+            if (any_queued()) {
+                flush();
+                resume_ready(slots);
             }
+            launch_pending(!any_queued());
+            // side-stream operations: wait for one only when nothing else can proceed
+            if (!async_.empty()) complete_async(!any_queued() && pending_.empty());
+            if (!any_queued() && async_.empty() && pending_.empty()) break;
         }
         for (auto& t : tasks)
             if (!t.h.done()) throw std::runtime_error("scheduler: a patch solve did not finish");
     }
 
 private:
+    // resume the tasks of `slots` whose results arrived (not those waiting on a side stream)
+    void resume_ready(const std::vector<int>& slots)
+    {
+        for (int sl : slots)
+            if (waiter_[sl] && !async_wait_[sl]) {
+                auto h = waiter_[sl];
+                waiter_[sl] = nullptr;
+                h.resume();
+            }
+    }
+
+    // one launch of side-stream operations (all items of one flush)
+    struct AsyncOp {
+        hipStream_t stream = nullptr;
+        hipEvent_t ev_start = nullptr, ev_done = nullptr;
+        char* host = nullptr;        // pinned: items | group table | results
+        char* dev = nullptr;         // device: items | group table | partials
+        size_t cap = 0;
+        int nitems = 0, K = 0;
+        size_t res_off = 0;
+        std::vector<int> slots;
+        std::vector<double*> dest;
+    };
+    void free_async(AsyncOp& op)
+    {
+        if (op.ev_start) (void)hipEventDestroy(op.ev_start);
+        if (op.ev_done) (void)hipEventDestroy(op.ev_done);
+        if (op.host) (void)hipHostFree(op.host);
+        if (op.dev) (void)hipFree(op.dev);
+        op = AsyncOp();
+    }
+    AsyncOp take_async(size_t bytes)
+    {
+        for (size_t k = 0; k < async_free_.size(); k++)
+            if (async_free_[k].cap >= bytes) {
+                AsyncOp op = async_free_[k];
+                async_free_.erase(async_free_.begin() + k);
+                op.slots.clear();
+                op.dest.clear();
+                return op;
+            }
+        AsyncOp op;
+        op.cap = std::max(bytes, (size_t)1 << 16);
+        HIPCHK(hipHostMalloc((void**)&op.host, op.cap, hipHostMallocDefault));
+        HIPCHK(hipMalloc((void**)&op.dev, op.cap));
+        HIPCHK(hipEventCreateWithFlags(&op.ev_start, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&op.ev_done, hipEventDisableTiming));
+        return op;
+    }
+    // wait for every side-stream operation and forget it (its tasks are being discarded)
+    void drop_async()
+    {
+        pending_.clear();
+        for (auto& op : async_) {
+            (void)hipEventSynchronize(op.ev_done);
+            async_free_.push_back(op);
+        }
+        async_.clear();
+    }
+    // hand the results of finished side-stream operations to their tasks and resume them;
+    // with `block`, wait for the oldest one first
+    int complete_async(bool block)
+    {
+        int n = 0;
+        for (size_t k = 0; k < async_.size();) {
+            AsyncOp& op = async_[k];
+            if (block && k == 0) {
+                HIPCHK(hipEventSynchronize(op.ev_done));
+            } else {
+                hipError_t e = hipEventQuery(op.ev_done);
+                if (e == hipErrorNotReady) { k++; continue; }
+                HIPCHK(e);
+            }
+            const double* r = (const double*)(op.host + op.res_off);
+            std::vector<int> sl = op.slots;
+            for (int i = 0; i < op.nitems; i++) {
+                if (op.dest[i])
+                    for (int j = 0; j < op.K; j++) op.dest[i][j] = r[(size_t)i * op.K + j];
+                async_wait_[op.slots[i]] = 0;
+            }
+            async_free_.push_back(op);
+            async_.erase(async_.begin() + k);
+            for (int s2 : sl)
+                if (waiter_[s2]) {
+                    auto h = waiter_[s2];
+                    waiter_[s2] = nullptr;
+                    h.resume();
+                }
+            n++;
+            block = false;
+        }
+        return n;
+    }
+    // launch the pending side-stream operations: when the main stream has nothing queued, or
+    // when enough have gathered that one launch fills a good part of the GPU (each item is
+    // one work-group)
+    void launch_pending(bool main_idle)
+    {
+        if (pending_.empty() || (!main_idle && pending_.size() < 64)) return;
+        std::vector<Req*> v1, v2;
+        // CNTC_GS_KERNEL=1 selects the variant without row staging for every size (testing)
+        const char* forced = std::getenv("CNTC_GS_KERNEL");
+        bool force_v1 = forced && forced[0] == '1';
+        for (auto& r : pending_) {
+            bool small = !force_v1 && op_lds(r.kind, &r.item, 1) <= 65536;
+            (small ? v2 : v1).push_back(&r);
+        }
+        if (!v2.empty()) launch_async(v2, 1);
+        if (!v1.empty()) launch_async(v1, 0);
+        pending_.clear();
+    }
+    void launch_async(const std::vector<Req*>& reqs, int variant)
+    {
+        int kind = reqs[0]->kind, nitems = (int)reqs.size();
+        size_t isz = item_size(kind);
+        int K = op_nred(kind, &reqs[0]->item);
+        auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        size_t items_b = align(isz * nitems), gs_b = align(sizeof(int) * (nitems + 1));
+        size_t res_b = align(sizeof(double) * K * nitems);
+        AsyncOp op = take_async(items_b + gs_b + res_b);
+        op.stream = side_[next_side_++ % NSIDE];
+        op.nitems = nitems;
+        op.K = K;
+        op.res_off = items_b + gs_b;
+        size_t lds = 0;
+        int* gs = (int*)(op.host + items_b);
+        int tot = 0;
+        for (int i = 0; i < nitems; i++) {
+            std::memcpy(op.host + isz * i, &reqs[i]->item, isz);
+            gs[i] = tot;
+            tot += op_groups(kind, &reqs[i]->item, variant);
+            lds = std::max(lds, op_lds(kind, &reqs[i]->item, variant));
+            op.slots.push_back(reqs[i]->slot);
+            op.dest.push_back(reqs[i]->res);
+            async_wait_[reqs[i]->slot] = 1;
+        }
+        gs[nitems] = tot;
+        // start after everything issued so far on the main stream (the inputs of these items)
+        HIPCHK(hipEventRecord(op.ev_start, s_));
+        HIPCHK(hipStreamWaitEvent(op.stream, op.ev_start, 0));
+        HIPCHK(hipMemcpyAsync(op.dev, op.host, items_b + gs_b, hipMemcpyHostToDevice, op.stream));
+        launch_op(kind, op.dev, (const int*)(op.dev + items_b), nitems, tot, (double*)(op.dev + op.res_off),
+                  variant, lds, op.stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(op.host + op.res_off, op.dev + op.res_off, res_b, hipMemcpyDeviceToHost,
+                              op.stream));
+        HIPCHK(hipEventRecord(op.ev_done, op.stream));
+        async_.push_back(op);
+    }
+
     struct Group {
         int kind;
         std::vector<int> reqs;   // indices into flat list
     };
     void flush()
     {
-        // collect requests level by level
-        std::vector<Req*> flat;
+        // collect requests level by level; side-stream operations are set aside (a task
+        // waiting on one has nothing queued behind it)
+        std::vector<Req*> flat, async_reqs;
         std::vector<std::vector<Group>> levels;
         size_t maxlen = 0;
         for (auto& q : q_) maxlen = std::max(maxlen, q.size());
@@ -281,6 +445,10 @@ private:
             std::map<int, Group> byk;
             for (auto& q : q_)
                 if (lev < q.size()) {
+                    if (is_async_kind(q[lev].kind)) {
+                        async_reqs.push_back(&q[lev]);
+                        continue;
+                    }
                     int idx = (int)flat.size();
                     flat.push_back(&q[lev]);
                     auto& g = byk[q[lev].kind];
@@ -391,6 +559,10 @@ private:
                                 L.nitems, L.K, dpart_ + L.part_off, dres_ + L.res_off, s_);
         }
         HIPCHK(hipGetLastError());
+        for (Req* r : async_reqs) {
+            pending_.push_back(*r);
+            async_wait_[r->slot] = 1;
+        }
         if (res > 0) HIPCHK(hipMemcpyAsync(hres_, dres_, res * sizeof(double), hipMemcpyDeviceToHost, s_));
         HIPCHK(hipStreamSynchronize(s_));
         for (size_t k = 0; k < pkeys.size(); k++) {
@@ -411,6 +583,12 @@ private:
     }
 
     hipStream_t s_;
+    static constexpr int NSIDE = 3;
+    hipStream_t side_[NSIDE] = {};
+    unsigned next_side_ = 0;
+    std::vector<AsyncOp> async_, async_free_;
+    std::vector<Req> pending_;
+    std::vector<char> async_wait_;
     std::vector<std::vector<Req>> q_;
     std::vector<std::coroutine_handle<>> waiter_;
     char* hstage_ = nullptr;
@@ -1693,7 +1871,7 @@ static Task stdygs(Ctx C, int k, double omegah, double omegas, int* itgs_out, do
     Patch& P = *C.P;
     AnyItem it;
     it.gs = gs_item(P, k, omegah, omegas);
-    if (op_lds(OP_STDYGS, &it.gs, 0) > 65536) co_return ERR_OTHER;
+    if (op_lds(OP_STDYGS, &it.gs, 0) > 65536 && op_lds(OP_STDYGS, &it.gs, 1) > 65536) co_return ERR_OTHER;
     Res r = co_await C.opr(OP_STDYGS, it);
     *itgs_out = (int)r.v[0];
     *err_out = r.v[1];
@@ -2011,7 +2189,8 @@ void Engine::serve_loop()
         std::vector<int> slots;
         for (auto& a : active) slots.push_back(a.slot);
         try {
-            sched_->step(slots);
+            // nothing to flush while only side-stream work is pending: poll it gently
+            if (!sched_->step(slots)) std::this_thread::sleep_for(std::chrono::microseconds(50));
         } catch (const std::exception& ex) {
             last_error = ex.what();
             for (auto& a : active) finish(a.ticket, ERR_OTHER, a.bytes);

@@ -1785,6 +1785,238 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, cons
     }
 }
 
+
+// SteadyGS for rows of up to a few hundred elements: the same sweeps as k_stdygs, with the
+// influence of the other rows on the current row summed once at the start of the row (all
+// threads; coefficient rows and source-row differences staged through LDS), so that the
+// sequential part along the row only adds the current row's own influence and runs on one
+// wavefront without work-group barriers.  Summation order differs from gf3_AijPj; the sums
+// are the same.  Returns (itgs, dif).
+__global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, const int* gstart,
+                                                        int nitems, double* partials)
+{
+    extern __shared__ double gs_lds[];
+    __shared__ double red_s[16 * MAXK];
+    __shared__ double s_ctl[4];
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GsItem& I = items[it];
+    const int mx = I.mx, my = I.my, n = I.n, tid = threadIdx.x, T = blockDim.x;
+    const int M2 = 2 * mx;
+    double* rpx = gs_lds;
+    double* rpy = rpx + mx;
+    double* rdpx = rpy + mx;
+    double* rdpy = rdpx + mx;
+    double* rg = rdpy + mx;
+    double* ucx = rg + mx;                 // sums over the other rows, per column
+    double* ucy = ucx + mx;
+    double* stc = ucy + mx;                // 4 staged coefficient rows (xx, xy, yx, yy)
+    double* sdx = stc + 4 * M2;            // staged differences of one source row
+    double* sdy = sdx + mx;
+    double* red = sdy + mx;                // 2 partial sums per thread
+    int* rj0 = (int*)(red + 2 * GS_THREADS);   // summation range per row
+    int* rj1 = rj0 + my;
+    int* ri0 = rj1 + my;                   // interior range per row
+    int* ri1 = ri0 + my;
+    int8_t* rel = (int8_t*)(ri1 + my);
+    double* dpx = I.dpx;
+    double* dpy = I.dpy;
+    const double* cfb[4] = {I.cf[0][0], I.cf[0][1], I.cf[1][0], I.cf[1][1]};
+
+    for (int i = tid; i < n; i += T) {
+        int ix = i % mx;
+        if (ix != mx - 1) {
+            dpx[i] = I.px[i] - I.px[i + 1];
+            dpy[i] = I.py[i] - I.py[i + 1];
+        } else {
+            dpx[i] = I.px[i];
+            dpy[i] = I.py[i];
+        }
+    }
+    for (int jy = tid; jy < my; jy += T) {
+        const int8_t* row = I.el + (size_t)jy * mx;
+        int first = 0;
+        while (first < mx - 1 && row[first] <= EXTER) first++;
+        int last = -1, lo = mx;
+        for (int ix = 0; ix < mx; ix++)
+            if (row[ix] >= ADHES) {
+                last = ix;
+                if (ix < lo) lo = ix;
+            }
+        rj0[jy] = first - 1 < 0 ? 0 : first - 1;
+        rj1[jy] = last + 1 > mx - 1 ? mx - 1 : last + 1;
+        ri0[jy] = lo;
+        ri1[jy] = last;
+    }
+    __syncthreads();
+
+    int itgs = 0;
+    double dif = 2.0, difid = 1.0;
+    while (dif >= difid && itgs < I.maxgs) {
+        itgs++;
+        double difacc = 0.0;   // thread 0
+        for (int iy = 0; iy < my; iy++) {
+            if (ri1[iy] < ri0[iy]) continue;
+            const int t0 = ri0[iy], nt = ri1[iy] - t0 + 1;
+            const int P = T / nt > 0 ? T / nt : 1;
+            const int ti = tid % nt, part = tid / nt;
+            const bool active = part < P && ti < nt;
+            const size_t r0 = (size_t)iy * mx;
+            double ax = 0.0, ay = 0.0;
+            for (int jy = 0; jy < my; jy++) {
+                const int j0 = rj0[jy], j1 = rj1[jy];
+                if (jy == iy || j1 < j0) continue;
+                const int d = iy - jy, xlo = t0 - j1, w = t0 + nt - 1 - j0 - xlo + 1;
+                __syncthreads();
+                for (int e = tid; e < 4 * w; e += T) {
+                    int b = e / w, q = e - b * w;
+                    stc[b * M2 + q] = cfb[b][(size_t)(d + my) * M2 + (xlo + q + mx)];
+                }
+                for (int e = tid; e <= j1 - j0; e += T) {
+                    sdx[e] = dpx[(size_t)jy * mx + j0 + e];
+                    sdy[e] = dpy[(size_t)jy * mx + j0 + e];
+                }
+                __syncthreads();
+                if (active) {
+                    const int ix = t0 + ti;
+                    for (int q = part; q <= j1 - j0; q += P) {
+                        int o = ix - (j0 + q) - xlo;
+                        double qx = sdx[q], qy = sdy[q];
+                        ax += stc[o] * qx + stc[M2 + o] * qy;
+                        ay += stc[2 * M2 + o] * qx + stc[3 * M2 + o] * qy;
+                    }
+                }
+            }
+            red[2 * tid] = ax;
+            red[2 * tid + 1] = ay;
+            __syncthreads();
+            for (int c = tid; c < nt; c += T) {
+                double sx = 0.0, sy = 0.0;
+                for (int p = 0; p < P; p++) {
+                    sx += red[2 * (p * nt + c)];
+                    sy += red[2 * (p * nt + c) + 1];
+                }
+                ucx[t0 + c] = sx;
+                ucy[t0 + c] = sy;
+            }
+            for (int ix = tid; ix < mx; ix += T) {
+                rpx[ix] = I.px[r0 + ix];
+                rpy[ix] = I.py[r0 + ix];
+                rdpx[ix] = dpx[r0 + ix];
+                rdpy[ix] = dpy[r0 + ix];
+                rg[ix] = I.mu * I.pn[r0 + ix];
+                rel[ix] = I.el[r0 + ix];
+            }
+            for (int e = tid; e < 4 * M2; e += T) {
+                int b = e / M2, q = e - b * M2;
+                stc[e] = cfb[b][(size_t)my * M2 + q];      // offsets (q - mx, 0)
+            }
+            __syncthreads();
+            if (tid < 64) {
+                const int lane = tid, j0 = rj0[iy], j1 = rj1[iy];
+                for (int ix = t0; ix < t0 + nt; ix++) {
+                    if (rel[ix] < ADHES) continue;
+                    double qsx = 0.0, qsy = 0.0;
+                    for (int jx = j0 + lane; jx <= j1; jx += 64) {
+                        int o = ix - jx + mx;
+                        double qx = rdpx[jx], qy = rdpy[jx];
+                        qsx += stc[o] * qx + stc[M2 + o] * qy;
+                        qsy += stc[2 * M2 + o] * qx + stc[3 * M2 + o] * qy;
+                    }
+                    for (int off = 32; off > 0; off >>= 1) {
+                        qsx += __shfl_xor(qsx, off);
+                        qsy += __shfl_xor(qsy, off);
+                    }
+                    if (lane == 0) {
+                        const size_t ii = r0 + ix;
+                        int jx = ix - 1;
+                        while (jx > 0 && rel[jx] == ADHES) jx--;
+                        const int b0 = mx, bj = mx + (jx - ix);
+                        double a11 = I.ga_inv * (stc[b0] - stc[bj]);
+                        double a12 = I.ga_inv * (stc[M2 + b0] - stc[M2 + bj]);
+                        double a21 = I.ga_inv * (stc[2 * M2 + b0] - stc[2 * M2 + bj]);
+                        double a22 = I.ga_inv * (stc[3 * M2 + b0] - stc[3 * M2 + bj]);
+                        double sv[2] = {I.wsx[ii] + I.ga_inv * (ucx[ix] + qsx),
+                                        I.wsy[ii] + I.ga_inv * (ucy[ix] + qsy)};
+                        double pr[3] = {rpx[ix], rpy[ix], I.pn[ii]};
+                        int8_t e8 = rel[ix];
+                        elmtrc(e8, a11, a12, a21, a22, I.eps, pr, I.mu, sv, I.omegah, I.omegas);
+                        double ddx = pr[0] - rpx[ix], ddy = pr[1] - rpy[ix];
+                        difacc += ddx * ddx + ddy * ddy;
+                        rdpx[ix] += ddx;
+                        rdpy[ix] += ddy;
+                        rpx[ix] = pr[0];
+                        rpy[ix] = pr[1];
+                        rel[ix] = e8;
+                        I.sx[ii] = sv[0];
+                        I.sy[ii] = sv[1];
+                        for (int kx = ix - 1; kx >= 0; kx--) {
+                            int8_t ek = rel[kx];
+                            if (ek == ADHES) {
+                                double qx = rpx[kx + 1] + rdpx[kx], qy = rpy[kx + 1] + rdpy[kx];
+                                double ptabs = sqrt(qx * qx + qy * qy), ptbnd = rg[kx];
+                                if (ptabs > ptbnd) {
+                                    qx = qx * ptbnd / ptabs;
+                                    qy = qy * ptbnd / ptabs;
+                                    rdpx[kx] = qx - rpx[kx + 1];
+                                    rdpy[kx] = qy - rpy[kx + 1];
+                                }
+                                rpx[kx] = qx;
+                                rpy[kx] = qy;
+                            } else if (ek == SLIP || ek == PLAST) {
+                                rdpx[kx] = rpx[kx] - rpx[kx + 1];
+                                rdpy[kx] = rpy[kx] - rpy[kx + 1];
+                                break;
+                            } else {
+                                if (rel[kx + 1] >= ADHES) {
+                                    rdpx[kx] = -rpx[kx + 1];
+                                    rdpy[kx] = -rpy[kx + 1];
+                                }
+                                break;
+                            }
+                        }
+                    }
+                    // lane 0's LDS updates are read by all lanes in the next element
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+            }
+            __syncthreads();
+            for (int ix = tid; ix < mx; ix += T) {
+                I.px[r0 + ix] = rpx[ix];
+                I.py[r0 + ix] = rpy[ix];
+                dpx[r0 + ix] = rdpx[ix];
+                dpy[r0 + ix] = rdpy[ix];
+                I.el[r0 + ix] = rel[ix];
+            }
+            __syncthreads();
+        }
+        double sq[1] = {0.0};
+        for (int i = tid; i < n; i += T) sq[0] += I.px[i] * I.px[i] + I.py[i] * I.py[i];
+        group_reduce<1>(sq, 0, red_s);
+        if (tid == 0) {
+            double rms = sqrt(sq[0] / (double)(2 * n > 1 ? 2 * n : 1));
+            s_ctl[0] = sqrt(difacc / (2.0 * I.k));
+            s_ctl[1] = I.eps * fmax(1e-6, I.facnel * rms);
+        }
+        __syncthreads();
+        dif = s_ctl[0];
+        difid = s_ctl[1];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        partials[(size_t)blockIdx.x * 2] = (double)itgs;
+        partials[(size_t)blockIdx.x * 2 + 1] = dif;
+    }
+}
+
+// dynamic LDS of k_stdygs2 for a grid of mx x my
+__host__ __device__ inline size_t gs2_lds(int mx, int my)
+{
+    return (size_t)(17 * mx) * sizeof(double) + 2 * GS_THREADS * sizeof(double) +
+           (size_t)4 * my * sizeof(int) + mx + 16;
+}
+
 // ------------------------------------------------------------------------------------------
 // host-side launchers
 // ------------------------------------------------------------------------------------------
@@ -1919,7 +2151,10 @@ void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, i
     case OP_GD_DIAGRES: hipLaunchKernelGGL(k_gd_diagres, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_SETUP: hipLaunchKernelGGL(k_setup, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_FORCES: hipLaunchKernelGGL(k_forces, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_STDYGS: hipLaunchKernelGGL(k_stdygs, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_STDYGS:
+        if (conv_stage == 1) hipLaunchKernelGGL(k_stdygs2, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, d_partials);
+        else hipLaunchKernelGGL(k_stdygs, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, d_partials);
+        break;
     case OP_GUARD: hipLaunchKernelGGL(k_guard, ge, te, 0, s, (const GsItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_ELASCF: hipLaunchKernelGGL(k_elascf, ge, te, 0, s, (const CoefItem*)d_items, d_gstart, nitems); break;
     case OP_EMBED: hipLaunchKernelGGL(k_embed, ge, te, 0, s, (const EmbedItem*)d_items, d_gstart, nitems); break;
@@ -2009,6 +2244,7 @@ size_t op_lds(int kind, const void* item, int stage)
     }
     if (kind == OP_STDYGS) {
         const GsItem* g = (const GsItem*)item;
+        if (stage == 1) return gs2_lds(g->mx, g->my);
         return (size_t)g->mx * (5 * sizeof(double) + 1) + (size_t)(2 * g->my + 1) * sizeof(int) + 16;
     }
     return 0;

@@ -130,10 +130,14 @@ def test_steadygs_table_golden(cntc, oracle):
     print("table G=0: worst golden difference %.2e" % worst)
 
 
-def test_gdsteady_stagnation_fallback(cntc, oracle):
+@pytest.mark.parametrize("kernel", ["staged", "direct"])
+def test_gdsteady_stagnation_fallback(cntc, oracle, kernel, monkeypatch):
     """Hertzian patches on which GDsteady stagnates: both solvers fall back to SteadyGS
-    (src/m_solvpt.f90:466-485) and must agree to the solver tolerance."""
+    (src/m_solvpt.f90:466-485) and must agree to the solver tolerance.  Both SteadyGS kernels
+    (row-staged for rows up to ~480 elements, direct summation beyond) are checked."""
     from contact_amd import workloads as W
+    if kernel == "direct":
+        monkeypatch.setenv("CNTC_GS_KERNEL", "1")
     cases = W.hertz_batch(n=2048)
     for k in (381, 1505):
         c = cases[k]
