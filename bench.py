@@ -4,8 +4,10 @@
 Workload (one step): a new case for each of 2048 Hertzian steady-rolling contact patches --
 grid sides drawn log-uniform in [20, 256], Hertz semi-axis ratio log-uniform in [0.2, 5],
 Fn uniform 20-150 kN, creepages uniform +-0.005, spin uniform +-0.5 rad/m, friction uniform
-0.2-0.4, seed 2024 -- set through the cntc_* calls and solved by one cntc_calculatebatch
-(NormCG + GDsteady, eps 1e-8) with FP32 FFT products inside the FP64 refinement.
+0.2-0.4, seed 2024 -- set through the cntc_* calls and submitted at once to the
+continuous-batching loop (cntc_submit), which runs all of them through shared batched launches
+(NormCG + GDsteady with the SteadyGS fall-back, eps 1e-8) with FP32 FFT products inside the FP64
+refinement.
 
 Reported beside the throughput:
 * the force error of every patch against the FP64 solve of the same batch;
@@ -166,7 +168,6 @@ def serve_leg(cntc, W, seconds, rate, nchan=64, seed=77, ire0=801):
                 start(ch, ta, r)
         time.sleep(0.0002)
     wall = time.perf_counter() - t0
-    cntc.servestop()
     lat = np.array(lat)
     return {"workload": "%d channels, T=2 transient rolling, Poisson arrivals" % nchan,
             "offered_rate": rate, "requests": n_req, "achieved_rate": round(n_req / wall, 1),
@@ -179,6 +180,34 @@ def forces_of(cntc, idl):
     return np.array([cntc.getcontactforces(i, j) for i, j in idl])
 
 
+def solve_batch(cntc, idl, label):
+    """Submit every problem of `idl` to the serving loop at once and wait for all of them,
+    reporting progress as patches complete.  Returns (error codes, latencies in s)."""
+    t0 = time.perf_counter()
+    tick = {}
+    for k, (i, j) in enumerate(idl):
+        t = cntc.submit(i, j)
+        if t <= 0:
+            raise SystemExit("%s: submit failed (%d)" % (label, t))
+        tick[t] = k
+    err = np.zeros(len(idl), dtype=np.int64)
+    lat = np.zeros(len(idl))
+    last = t0
+    while tick:
+        for t in list(tick):
+            d, rc, l = cntc.poll(t)
+            if d == 1:
+                k = tick.pop(t)
+                err[k], lat[k] = rc, l
+        now = time.perf_counter()
+        if now - last >= 15.0 and tick:
+            last = now
+            print("# %s: %d/%d patches done at %.0f s" % (label, len(idl) - len(tick), len(idl), now - t0),
+                  file=sys.stderr, flush=True)
+        time.sleep(0.002)
+    return err, lat
+
+
 def rel_err(a, b):
     """Per-component relative error |a - b| / |b| (Fn, Fx, Fy)."""
     return np.abs(a[:, :3] - b[:, :3]) / np.maximum(np.abs(b[:, :3]), 1e-300)
@@ -187,7 +216,7 @@ def rel_err(a, b):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--patches", type=int, default=2048)
     ap.add_argument("--plain", type=int, default=32, help="patches in the one-at-a-time subset")
@@ -226,34 +255,32 @@ def main():
         W.apply_case(cntc, *mixed[k], cases[k])
         W.apply_case(cntc, *fp64[k], cases_fp64[k])
 
+    # all solves go through the continuous-batching loop: a batch is every patch submitted at once
+    cntc.servestart(200e9)
     # FP64 solve of the whole batch: the accuracy reference of the mixed-precision result
-    print("# fp64 batch", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    err = cntc.calculatebatch([i for i, _ in fp64], [j for _, j in fp64])
+    err, _ = solve_batch(cntc, fp64, "fp64 batch")
     t_fp64 = time.perf_counter() - t0
     if np.any(err != 0):
         raise SystemExit("fp64 batch: %d patches failed (codes %s)" % (np.sum(err != 0), sorted(set(err))))
     F64 = forces_of(cntc, fp64)
 
-    def step():
+    def step(label):
         for k in range(npatch):
             new_case(cntc, *mixed[k], cases[k])
-        e = cntc.calculatebatch([i for i, _ in mixed], [j for _, j in mixed])
+        e, l = solve_batch(cntc, mixed, label)
         if np.any(e != 0):
             raise SystemExit("mixed batch: %d patches failed (codes %s)" % (np.sum(e != 0), sorted(set(e))))
+        return l
 
     for w in range(args.warmup):
-        print("# warmup %d" % w, file=sys.stderr, flush=True)
-        step()
+        step("warmup %d" % w)
 
     cntc.amdprofile(True)
     device_sync()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        if s == args.steps - 1:
-            for i, j in mixed:
-                cntc.resetcalculationtime(i, j)
-        step()
+        lat = step("step %d" % s)
         print("# step %d done at %.2f s" % (s, time.perf_counter() - t0), file=sys.stderr, flush=True)
     device_sync()
     elapsed = time.perf_counter() - t0
@@ -264,7 +291,7 @@ def main():
     e_mixed = rel_err(FM, F64)
     its = np.array([cntc.getiterations(i, j) for i, j in mixed])
     elem_iters = float(np.sum(n_elem * (its[:, 1] + its[:, 3])))
-    lat = np.array([cntc.getcalculationtime(i, j)[1] for i, j in mixed])
+    flags = its[:, 4]
 
     # plain FP64 path: one cntc_calculate per patch on a systematic subset
     sub = list(range(0, npatch, max(1, npatch // max(1, args.plain))))[:args.plain]
@@ -286,6 +313,7 @@ def main():
     if args.serve_seconds > 0:
         print("# serving leg", file=sys.stderr, flush=True)
         serving = serve_leg(cntc, W, args.serve_seconds, args.serve_rate)
+    cntc.servestop()
 
     value = npatch * args.steps / elapsed
     plain_rate = len(sub) / float(np.sum(t_plain))
@@ -344,6 +372,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
                        "what": "batch submit to patch result, last timed step"},
+        "steadygs_fallbacks": int(np.sum(flags >= 2)),
         "element_iterations_per_s": round(elem_iters * args.steps / elapsed, 0),
         "plain_fp64": {"patches_per_s": round(plain_rate, 3), "patches": len(sub),
                        "latency_ms_p50": round(float(np.percentile(t_plain, 50)) * 1e3, 2),

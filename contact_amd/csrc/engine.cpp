@@ -375,9 +375,18 @@ private:
     // launch the pending side-stream operations: when the main stream has nothing queued, or
     // when enough have gathered that one launch fills a good part of the GPU (each item is
     // one work-group)
+    bool side_idle(hipStream_t st) const
+    {
+        for (auto& op : async_)
+            if (op.stream == st) return false;
+        return true;
+    }
     void launch_pending(bool main_idle)
     {
-        if (pending_.empty() || (!main_idle && pending_.size() < 64)) return;
+        if (pending_.empty()) return;
+        bool idle_side = false;
+        for (auto st : side_) idle_side = idle_side || side_idle(st);
+        if (!main_idle && pending_.size() < 256 && !(idle_side && pending_.size() >= 16)) return;
         std::vector<Req*> v1, v2;
         // CNTC_GS_KERNEL=1 selects the variant without row staging for every size (testing)
         const char* forced = std::getenv("CNTC_GS_KERNEL");
@@ -399,7 +408,10 @@ private:
         size_t items_b = align(isz * nitems), gs_b = align(sizeof(int) * (nitems + 1));
         size_t res_b = align(sizeof(double) * K * nitems);
         AsyncOp op = take_async(items_b + gs_b + res_b);
+        // an idle side stream if there is one (operations on one stream run one after another)
         op.stream = side_[next_side_++ % NSIDE];
+        for (auto st : side_)
+            if (side_idle(st)) { op.stream = st; break; }
         op.nitems = nitems;
         op.K = K;
         op.res_off = items_b + gs_b;
