@@ -392,7 +392,7 @@ private:
         const char* forced = std::getenv("CNTC_GS_KERNEL");
         bool force_v1 = forced && forced[0] == '1';
         for (auto& r : pending_) {
-            bool small = !force_v1 && op_lds(r.kind, &r.item, 1) <= 65536;
+            bool small = !force_v1 && op_lds(r.kind, &r.item, 1) <= MAX_LDS;
             (small ? v2 : v1).push_back(&r);
         }
         if (!v2.empty()) launch_async(v2, 1);
@@ -1009,6 +1009,11 @@ bool Engine::prepare(Patch& P)
     // padded sizes of the products
     P.Lx = 2 * opt_fft_size(mx);
     P.Ly = 2 * opt_fft_size(my);
+    if (P.Ly > 2048 || P.Lx > 4096) {   // column stage of the products must fit the LDS
+        R.ierror = ERR_OTHER;
+        last_error = "grid too large for the FFT products (my <= 1024, mx <= 2048)";
+        return false;
+    }
     // device memory
     int M = P.Lx / 2, H = M + 1;
     size_t n = P.n;
@@ -1516,7 +1521,7 @@ static Task gdsteady(Ctx C, const double cf00[4], int maxgd, double eps, int* it
         // the row tiles of the GD kernels must fit the LDS of one work-group
         AnyItem t;
         t.gd = G;
-        if (op_lds(OP_GD_PROJECT, &t.gd, 0) > 65536 || op_lds(OP_GD_PROBE, &t.gd, 0) > 65536)
+        if (op_lds(OP_GD_PROJECT, &t.gd, 0) > MAX_LDS || op_lds(OP_GD_PROBE, &t.gd, 0) > MAX_LDS)
             co_return ERR_OTHER;
     }
     Res r;
@@ -1883,7 +1888,7 @@ static Task stdygs(Ctx C, int k, double omegah, double omegas, int* itgs_out, do
     Patch& P = *C.P;
     AnyItem it;
     it.gs = gs_item(P, k, omegah, omegas);
-    if (op_lds(OP_STDYGS, &it.gs, 0) > 65536 && op_lds(OP_STDYGS, &it.gs, 1) > 65536) co_return ERR_OTHER;
+    if (op_lds(OP_STDYGS, &it.gs, 0) > MAX_LDS && op_lds(OP_STDYGS, &it.gs, 1) > MAX_LDS) co_return ERR_OTHER;
     Res r = co_await C.opr(OP_STDYGS, it);
     *itgs_out = (int)r.v[0];
     *err_out = r.v[1];

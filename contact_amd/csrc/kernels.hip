@@ -1926,6 +1926,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                         qsx += __shfl_xor(qsx, off);
                         qsy += __shfl_xor(qsy, off);
                     }
+                    double Dx = 0.0, Dy = 0.0;
                     if (lane == 0) {
                         const size_t ii = r0 + ix;
                         int jx = ix - 1;
@@ -1940,45 +1941,89 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                         double pr[3] = {rpx[ix], rpy[ix], I.pn[ii]};
                         int8_t e8 = rel[ix];
                         elmtrc(e8, a11, a12, a21, a22, I.eps, pr, I.mu, sv, I.omegah, I.omegas);
-                        double ddx = pr[0] - rpx[ix], ddy = pr[1] - rpy[ix];
-                        difacc += ddx * ddx + ddy * ddy;
-                        rdpx[ix] += ddx;
-                        rdpy[ix] += ddy;
+                        Dx = pr[0] - rpx[ix];
+                        Dy = pr[1] - rpy[ix];
+                        difacc += Dx * Dx + Dy * Dy;
+                        rdpx[ix] += Dx;
+                        rdpy[ix] += Dy;
                         rpx[ix] = pr[0];
                         rpy[ix] = pr[1];
                         rel[ix] = e8;
                         I.sx[ii] = sv[0];
                         I.sy[ii] = sv[1];
-                        for (int kx = ix - 1; kx >= 0; kx--) {
-                            int8_t ek = rel[kx];
-                            if (ek == ADHES) {
-                                double qx = rpx[kx + 1] + rdpx[kx], qy = rpy[kx + 1] + rdpy[kx];
-                                double ptabs = sqrt(qx * qx + qy * qy), ptbnd = rg[kx];
-                                if (ptabs > ptbnd) {
-                                    qx = qx * ptbnd / ptabs;
-                                    qy = qy * ptbnd / ptabs;
-                                    rdpx[kx] = qx - rpx[kx + 1];
-                                    rdpy[kx] = qy - rpy[kx + 1];
-                                }
-                                rpx[kx] = qx;
-                                rpy[kx] = qy;
-                            } else if (ek == SLIP || ek == PLAST) {
-                                rdpx[kx] = rpx[kx] - rpx[kx + 1];
-                                rdpy[kx] = rpy[kx] - rpy[kx + 1];
-                                break;
-                            } else {
-                                if (rel[kx + 1] >= ADHES) {
-                                    rdpx[kx] = -rpx[kx + 1];
-                                    rdpy[kx] = -rpy[kx + 1];
-                                }
-                                break;
-                            }
-                        }
                     }
-                    // lane 0's LDS updates are read by all lanes in the next element
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    // Rebuild the upstream part of the row (:3086-3127).  The adhesion elements
+                    // upstream satisfy p(k) = p(k+1) + dp(k), so they all move by the change D of
+                    // p(ix) until one exceeds its traction bound (clipped there, D becomes that
+                    // element's change) or a non-adhesion element ends the chain.  The wavefront
+                    // handles 64 upstream elements per step.
+                    Dx = __shfl(Dx, 0);
+                    Dy = __shfl(Dy, 0);
+                    int hi = ix - 1;
+                    while (hi >= 0) {
+                        const int kx = hi - lane;
+                        const bool valid = kx >= 0;
+                        const int8_t ek = valid ? rel[kx] : EXTER;
+                        const unsigned long long smask = __ballot(!valid || ek != ADHES);
+                        const int sl = smask ? __ffsll((long long)smask) - 1 : 64;
+                        double ox = 0.0, oy = 0.0, qx = 0.0, qy = 0.0;
+                        bool clip = false;
+                        if (lane < sl) {
+                            ox = rpx[kx];
+                            oy = rpy[kx];
+                            qx = ox + Dx;
+                            qy = oy + Dy;
+                            clip = sqrt(qx * qx + qy * qy) > rg[kx];
+                        }
+                        const unsigned long long cmask = __ballot(clip);
+                        const int cl = cmask ? __ffsll((long long)cmask) - 1 : 64;
+                        const int nmove = sl < cl ? sl : cl;
+                        if (lane < nmove) {
+                            rpx[kx] = qx;
+                            rpy[kx] = qy;
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        if (cl < sl) {
+                            // clip at lane cl and continue below it with that element's change
+                            double nDx = 0.0, nDy = 0.0;
+                            if (lane == cl) {
+                                double ptabs = sqrt(qx * qx + qy * qy), ptbnd = rg[kx];
+                                qx = qx * ptbnd / ptabs;
+                                qy = qy * ptbnd / ptabs;
+                                rdpx[kx] = qx - rpx[kx + 1];
+                                rdpy[kx] = qy - rpy[kx + 1];
+                                rpx[kx] = qx;
+                                rpy[kx] = qy;
+                                nDx = qx - ox;
+                                nDy = qy - oy;
+                            }
+                            Dx = __shfl(nDx, cl);
+                            Dy = __shfl(nDy, cl);
+                            hi -= cl + 1;
+                        } else if (sl < 64) {
+                            // the first non-adhesion element ends the chain
+                            if (lane == sl && valid) {
+                                if (ek == SLIP || ek == PLAST) {
+                                    rdpx[kx] = rpx[kx] - rpx[kx + 1];
+                                    rdpy[kx] = rpy[kx] - rpy[kx + 1];
+                                } else if (rel[kx + 1] >= ADHES) {
+                                    rdpx[kx] = -rpx[kx + 1];
+                                    rdpy[kx] = -rpy[kx + 1];
+                                }
+                            }
+                            hi = -1;
+                        } else {
+                            hi -= 64;
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    }
                 }
             }
             __syncthreads();

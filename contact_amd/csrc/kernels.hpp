@@ -9,6 +9,9 @@
 
 namespace cntc {
 
+// LDS one work-group may allocate on gfx950 (160 KB per CU, all of it to one work-group)
+constexpr size_t MAX_LDS = 160 * 1024;
+
 // number of reduction values an item of `kind` produces (0: none)
 int op_nred(int kind, const void* item);
 // combine mode of the reduction: 0 sum, 1 max, 2 min

@@ -148,3 +148,16 @@ def test_gdsteady_stagnation_fallback(cntc, oracle, kernel, monkeypatch):
         assert its[4] & 2, "SteadyGS did not run on the GPU"
         assert o[4].stagnated() & 2, "SteadyGS did not run in the oracle"
         compare(rc_g, o, 1e-6, 1e-5, "fallback patch %d" % k, max_el_diff=4)
+
+
+def test_steadygs_wide_rows(cntc, oracle):
+    """SteadyGS (G=0) on rows of 520 elements: the row-staged kernel then needs more than 64 KB
+    of LDS per work-group (gfx950 allows 160 KB)."""
+    c = C.hertz_case(12.0, 3.0, 520, 16, 82000.0, cksi=0.0008, ceta=-0.0004, cphi=0.0002,
+                     eps=1e-6, maxgs=299, maxin=5, gausei=0)
+    g = run_gpu(cntc, c)
+    its = cntc.getiterations(1, 1)
+    o = run_oracle(oracle, c)
+    print("wide rows: gpu iterations", its, "oracle", o[4].iters())
+    assert its[4] & 2
+    compare(g, o, 1e-6, 1e-5, "wide rows", max_el_diff=4)
