@@ -325,7 +325,7 @@ OP_NAMES = ["vec", "ncg_negpn", "ncg_dd", "ncg_expand", "ncg_hmin", "sn_contract
             "sn_final", "aij_ones", "tcg_nt", "tcg_rset", "tcg_zprep", "tcg_qpost", "tcg_pupd",
             "tcg_as", "tcg_sa", "st_as", "st_sa", "gd_dp", "gd_project", "gd_probe", "gd_update",
             "gd_moves", "gd_diagres", "conv", "conv32", "elascf", "embed", "extract", "setup",
-            "forces"]
+            "forces", "stdygs", "guard"]
 
 
 def amdprofile(on):
@@ -334,7 +334,8 @@ def amdprofile(on):
 
 
 def amdprofileget(lenarr=256):
-    """Per-kernel records since amdprofile(True): {name: (seconds, algorithmic bytes, launches)}."""
+    """Per-kernel records since amdprofile(True): {name: (seconds, algorithmic bytes, launches)};
+    for the SteadyGS kernels the second value is algorithmic flops."""
     kinds = np.zeros(lenarr, dtype=np.int32)
     sec, byt, nl = (np.zeros(lenarr) for _ in range(3))
     n = c_int(0)
@@ -346,6 +347,8 @@ def amdprofileget(lenarr=256):
         name = OP_NAMES[op] if op < len(OP_NAMES) else "op%d" % op
         if op in (24, 25):
             name += ("_fwd", "_mul", "_inv")[stage]
+        elif name == "stdygs" and stage == 1:
+            name = "stdygs_staged"
         out[name] = (float(sec[k]), float(byt[k]), int(nl[k]))
     return out
 

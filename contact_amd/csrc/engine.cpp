@@ -294,6 +294,9 @@ private:
     struct AsyncOp {
         hipStream_t stream = nullptr;
         hipEvent_t ev_start = nullptr, ev_done = nullptr;
+        hipEvent_t t0 = nullptr, t1 = nullptr;    // kernel timing (profiling)
+        int kind = 0, variant = 0;
+        bool timed = false;
         char* host = nullptr;        // pinned: items | group table | results
         char* dev = nullptr;         // device: items | group table | partials
         size_t cap = 0;
@@ -306,6 +309,8 @@ private:
     {
         if (op.ev_start) (void)hipEventDestroy(op.ev_start);
         if (op.ev_done) (void)hipEventDestroy(op.ev_done);
+        if (op.t0) (void)hipEventDestroy(op.t0);
+        if (op.t1) (void)hipEventDestroy(op.t1);
         if (op.host) (void)hipHostFree(op.host);
         if (op.dev) (void)hipFree(op.dev);
         op = AsyncOp();
@@ -326,6 +331,8 @@ private:
         HIPCHK(hipMalloc((void**)&op.dev, op.cap));
         HIPCHK(hipEventCreateWithFlags(&op.ev_start, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&op.ev_done, hipEventDisableTiming));
+        HIPCHK(hipEventCreate(&op.t0));
+        HIPCHK(hipEventCreate(&op.t1));
         return op;
     }
     // wait for every side-stream operation and forget it (its tasks are being discarded)
@@ -353,6 +360,14 @@ private:
                 HIPCHK(e);
             }
             const double* r = (const double*)(op.host + op.res_off);
+            if (op.timed) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, op.t0, op.t1));
+                auto& pe = prof[op.kind * 4 + op.variant];
+                pe.seconds += ms * 1e-3;
+                for (int i = 0; i < op.nitems; i++) pe.bytes += op.K >= 3 ? r[(size_t)i * op.K + 2] : 0.0;
+                pe.launches++;
+            }
             std::vector<int> sl = op.slots;
             for (int i = 0; i < op.nitems; i++) {
                 if (op.dest[i])
@@ -432,9 +447,14 @@ private:
         HIPCHK(hipEventRecord(op.ev_start, s_));
         HIPCHK(hipStreamWaitEvent(op.stream, op.ev_start, 0));
         HIPCHK(hipMemcpyAsync(op.dev, op.host, items_b + gs_b, hipMemcpyHostToDevice, op.stream));
+        op.kind = kind;
+        op.variant = variant;
+        op.timed = profile;
+        if (op.timed) HIPCHK(hipEventRecord(op.t0, op.stream));
         launch_op(kind, op.dev, (const int*)(op.dev + items_b), nitems, tot, (double*)(op.dev + op.res_off),
                   variant, lds, op.stream);
         HIPCHK(hipGetLastError());
+        if (op.timed) HIPCHK(hipEventRecord(op.t1, op.stream));
         HIPCHK(hipMemcpyAsync(op.host + op.res_off, op.dev + op.res_off, res_b, hipMemcpyDeviceToHost,
                               op.stream));
         HIPCHK(hipEventRecord(op.ev_done, op.stream));

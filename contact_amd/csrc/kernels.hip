@@ -1603,7 +1603,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_guard(const GsItem* items, con
 // bound applied (:3086-3127).  The rebuild stops at the first upstream element that is not
 // in adhesion: the elements beyond it depend on its unchanged traction only.  The current
 // row lives in LDS; dp is written through to global memory for the summation.
-// Returns (itgs, dif).
+// Returns (itgs, dif, algorithmic flops: 8 per coefficient-difference product of the sums).
 __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, const int* gstart,
                                                        int nitems, double* partials)
 {
@@ -1778,10 +1778,10 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, cons
         difid = s_ctl[1];
         __syncthreads();
     }
-    double out[2] = {(double)itgs, dif};
     if (tid == 0) {
-        partials[(size_t)blockIdx.x * 2] = out[0];
-        partials[(size_t)blockIdx.x * 2 + 1] = out[1];
+        partials[(size_t)blockIdx.x * 3] = (double)itgs;
+        partials[(size_t)blockIdx.x * 3 + 1] = dif;
+        partials[(size_t)blockIdx.x * 3 + 2] = 8.0 * itgs * (double)I.k * (double)len;
     }
 }
 
@@ -1791,7 +1791,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, cons
 // threads; coefficient rows and source-row differences staged through LDS), so that the
 // sequential part along the row only adds the current row's own influence and runs on one
 // wavefront without work-group barriers.  Summation order differs from gf3_AijPj; the sums
-// are the same.  Returns (itgs, dif).
+// are the same.  Returns (itgs, dif, algorithmic flops).
 __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, const int* gstart,
                                                         int nitems, double* partials)
 {
@@ -2050,8 +2050,11 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
         __syncthreads();
     }
     if (tid == 0) {
-        partials[(size_t)blockIdx.x * 2] = (double)itgs;
-        partials[(size_t)blockIdx.x * 2 + 1] = dif;
+        double len = 0.0;
+        for (int jy = 0; jy < my; jy++) len += rj1[jy] >= rj0[jy] ? rj1[jy] - rj0[jy] + 1 : 0;
+        partials[(size_t)blockIdx.x * 3] = (double)itgs;
+        partials[(size_t)blockIdx.x * 3 + 1] = dif;
+        partials[(size_t)blockIdx.x * 3 + 2] = 8.0 * itgs * (double)I.k * len;
     }
 }
 
@@ -2093,7 +2096,7 @@ int op_nred(int kind, const void* item)
     case OP_GD_DIAGRES: return 3;
     case OP_SETUP: return 2;
     case OP_FORCES: return 5;
-    case OP_STDYGS: return 2;
+    case OP_STDYGS: return 3;
     case OP_GUARD: return 5;
     default: return 0;
     }
