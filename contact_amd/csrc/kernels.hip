@@ -2246,7 +2246,9 @@ double op_bytes(int kind, const void* item, int stage)
     case OP_ST_AS: case OP_ST_SA: return ((const StItem*)item)->n * 42.0;
     case OP_GD_DP: { const GdItem* g = (const GdItem*)item; return (double)g->n * 32.0; }
     case OP_GD_PROJECT: { const GdItem* g = (const GdItem*)item; return (double)g->n * 73.0; }
-    case OP_GD_PROBE: { const GdItem* g = (const GdItem*)item; return (double)g->n * (129.0 + (g->first ? 16.0 : 0.0)); }
+    // six 2N FP64 vectors per probe (SURVEY-style count: dp, dv, s, q, n, v), two more (r) on
+    // the first probe of a line search
+    case OP_GD_PROBE: { const GdItem* g = (const GdItem*)item; return (double)g->n * (96.0 + (g->first ? 32.0 : 0.0)); }
     case OP_GD_UPDATE: { const GdItem* g = (const GdItem*)item; return (double)g->n * 121.0; }
     case OP_GD_MOVES: { const GdItem* g = (const GdItem*)item; return (double)g->n * 66.0; }
     case OP_GD_DIAGRES: { const GdItem* g = (const GdItem*)item; return (double)g->n * 121.0; }
@@ -2259,18 +2261,25 @@ double op_bytes(int kind, const void* item, int stage)
         int nrow = c->mode == 0 ? c->my : c->Ly;
         int nxr = c->mode == 0 ? c->mx : c->Lx;
         int nout = c->mode == 0 ? c->nout : c->nin;
-        double npt = (double)nrow * nxr, nsp = (double)nrow * H;
-        if (stage == 0) return c->nin * (npt * 8.0 + nsp * es);
+        // compulsory traffic of one product group: the inputs (8 B per point, row stage), the
+        // coefficient spectra (column stage) and the outputs (+ addend, + states; inverse row
+        // stage).  The half-spectrum round trips between the stages are not counted.
+        double npt = (double)nrow * nxr;
+        (void)nout;
+        if (stage == 0) return c->nin * npt * 8.0;
         if (stage == 1) {
+            if (c->mode == 1) return c->nin * (double)H * c->Ly * es;   // spectrum store
+            if (c->mode == 2) return 0.0;
             int nspec = 0;
-            if (c->mode == 0)
-                for (int i = 0; i < c->nout; i++)
-                    for (int j = 0; j < c->nin; j++) nspec += c->spec[i][j] != nullptr;
-            double out = c->mode == 1 ? c->nin * (double)H * c->Ly * es : nout * nsp * es;
-            return c->nin * nsp * es + nspec * (double)H * c->Ly * es + out;
+            for (int i = 0; i < c->nout; i++)
+                for (int j = 0; j < c->nin; j++) nspec += c->spec[i][j] != nullptr;
+            return nspec * (double)H * c->Ly * es;
         }
         if (c->mode == 1) return 0.0;
-        return nout * (nsp * es + npt * (8.0 + (c->mode == 0 ? 1.0 : 0.0)));
+        double outb = 8.0 + (c->mode == 0 ? 1.0 : 0.0);
+        for (int i = 0; i < c->nout && c->mode == 0; i++)
+            if (c->addend[i]) outb += 8.0 / c->nout;
+        return (c->mode == 0 ? c->nout : c->nin) * npt * outb;
     }
     case OP_ELASCF: { const CoefItem* c = (const CoefItem*)item; return 9.0 * 8.0 * (2 * c->mx) * (2 * c->my); }
     case OP_STDYGS: { const GsItem* g = (const GsItem*)item; return (double)g->n * 73.0; }
