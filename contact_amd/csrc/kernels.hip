@@ -1779,19 +1779,38 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, cons
         __syncthreads();
     }
     if (tid == 0) {
-        partials[(size_t)blockIdx.x * 3] = (double)itgs;
-        partials[(size_t)blockIdx.x * 3 + 1] = dif;
-        partials[(size_t)blockIdx.x * 3 + 2] = 8.0 * itgs * (double)I.k * (double)len;
+        partials[(size_t)blockIdx.x * 5] = (double)itgs;
+        partials[(size_t)blockIdx.x * 5 + 1] = dif;
+        partials[(size_t)blockIdx.x * 5 + 2] = 8.0 * itgs * (double)I.k * (double)len;
+        partials[(size_t)blockIdx.x * 5 + 3] = 0.0;
+        partials[(size_t)blockIdx.x * 5 + 4] = 0.0;
     }
 }
 
 
-// SteadyGS for rows of up to a few hundred elements: the same sweeps as k_stdygs, with the
-// influence of the other rows on the current row summed once at the start of the row (all
-// threads; coefficient rows and source-row differences staged through LDS), so that the
-// sequential part along the row only adds the current row's own influence and runs on one
-// wavefront without work-group barriers.  Summation order differs from gf3_AijPj; the sums
-// are the same.  Returns (itgs, dif, algorithmic flops).
+// SteadyGS for rows of up to ~1000 elements: the same sweeps as k_stdygs, with the influence
+// of the other rows on the current row summed once at the start of the row by all threads
+// (each source row's coefficient row and differences staged through LDS, alternating between
+// two buffers, one barrier per source row), so that the sequential part along the row only adds the
+// current row's own influence and runs on one wavefront without work-group barriers.
+// Summation order differs from gf3_AijPj; the sums are the same.  Returns (itgs, dif,
+// algorithmic flops, wall-clock ticks (100 MHz) spent in the row sums and in the sequential
+// parts).
+
+// LDS of k_stdygs2 without the staging area; the staging area holds `sr` source rows
+__host__ __device__ inline size_t gs2_base_lds(int mx, int my)
+{
+    return (size_t)(10 * mx) * sizeof(double) + 2 * GS_THREADS * sizeof(double) +
+           (size_t)4 * my * sizeof(int) + mx + 64;
+}
+__host__ __device__ inline size_t gs2_row_lds(int mx) { return (size_t)(10 * mx) * sizeof(double); }
+// two staging buffers: the next source row is written while others still read the current
+__host__ __device__ inline int gs2_stage_rows(int mx, int my) { (void)mx; (void)my; return 2; }
+__host__ __device__ inline size_t gs2_lds(int mx, int my)
+{
+    return gs2_base_lds(mx, my) + (size_t)gs2_stage_rows(mx, my) * gs2_row_lds(mx);
+}
+
 __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, const int* gstart,
                                                         int nitems, double* partials)
 {
@@ -1807,17 +1826,20 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
     double* rdpx = rpy + mx;
     double* rdpy = rdpx + mx;
     double* rg = rdpy + mx;
-    double* ucx = rg + mx;                 // sums over the other rows, per column
+    double* rpn = rg + mx;
+    double* rwx = rpn + mx;
+    double* rwy = rwx + mx;
+    double* ucx = rwy + mx;                // sums over the other rows, per column
     double* ucy = ucx + mx;
-    double* stc = ucy + mx;                // 4 staged coefficient rows (xx, xy, yx, yy)
-    double* sdx = stc + 4 * M2;            // staged differences of one source row
-    double* sdy = sdx + mx;
-    double* red = sdy + mx;                // 2 partial sums per thread
+    double* red = ucy + mx;                // 2 partial sums per thread
     int* rj0 = (int*)(red + 2 * GS_THREADS);   // summation range per row
     int* rj1 = rj0 + my;
     int* ri0 = rj1 + my;                   // interior range per row
     int* ri1 = ri0 + my;
     int8_t* rel = (int8_t*)(ri1 + my);
+    // staging area after the fixed part, 64-byte aligned (an offset from the LDS base keeps the
+    // compiler on LDS instructions)
+    double* stg = gs_lds + ((10 * mx + 2 * GS_THREADS) * 8 + 4 * my * 4 + mx + 63) / 64 * 8;
     double* dpx = I.dpx;
     double* dpy = I.dpy;
     const double* cfb[4] = {I.cf[0][0], I.cf[0][1], I.cf[1][0], I.cf[1][1]};
@@ -1851,41 +1873,47 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
 
     int itgs = 0;
     double dif = 2.0, difid = 1.0;
+    unsigned long long t_rows = 0, t_inrow = 0;   // thread 0: wall-clock ticks of the two phases
     while (dif >= difid && itgs < I.maxgs) {
         itgs++;
         double difacc = 0.0;   // thread 0
         for (int iy = 0; iy < my; iy++) {
             if (ri1[iy] < ri0[iy]) continue;
+            const unsigned long long tr0 = wall_clock64();
             const int t0 = ri0[iy], nt = ri1[iy] - t0 + 1;
             const int P = T / nt > 0 ? T / nt : 1;
             const int ti = tid % nt, part = tid / nt;
             const bool active = part < P && ti < nt;
             const size_t r0 = (size_t)iy * mx;
             double ax = 0.0, ay = 0.0;
+            int sbuf = 0;
             for (int jy = 0; jy < my; jy++) {
                 const int j0 = rj0[jy], j1 = rj1[jy];
                 if (jy == iy || j1 < j0) continue;
                 const int d = iy - jy, xlo = t0 - j1, w = t0 + nt - 1 - j0 - xlo + 1;
-                __syncthreads();
+                // two buffers used in turn: one barrier per source row suffices
+                double* sc = stg + (size_t)sbuf * 10 * mx;
+                sbuf ^= 1;
                 for (int e = tid; e < 4 * w; e += T) {
                     int b = e / w, q = e - b * w;
-                    stc[b * M2 + q] = cfb[b][(size_t)(d + my) * M2 + (xlo + q + mx)];
+                    sc[b * M2 + q] = cfb[b][(size_t)(d + my) * M2 + (xlo + q + mx)];
                 }
                 for (int e = tid; e <= j1 - j0; e += T) {
-                    sdx[e] = dpx[(size_t)jy * mx + j0 + e];
-                    sdy[e] = dpy[(size_t)jy * mx + j0 + e];
+                    sc[8 * mx + e] = dpx[(size_t)jy * mx + j0 + e];
+                    sc[9 * mx + e] = dpy[(size_t)jy * mx + j0 + e];
                 }
                 __syncthreads();
                 if (active) {
                     const int ix = t0 + ti;
                     for (int q = part; q <= j1 - j0; q += P) {
                         int o = ix - (j0 + q) - xlo;
-                        double qx = sdx[q], qy = sdy[q];
-                        ax += stc[o] * qx + stc[M2 + o] * qy;
-                        ay += stc[2 * M2 + o] * qx + stc[3 * M2 + o] * qy;
+                        double qx = sc[8 * mx + q], qy = sc[9 * mx + q];
+                        ax += sc[o] * qx + sc[M2 + o] * qy;
+                        ay += sc[2 * M2 + o] * qx + sc[3 * M2 + o] * qy;
                     }
                 }
             }
+            __syncthreads();
             red[2 * tid] = ax;
             red[2 * tid + 1] = ay;
             __syncthreads();
@@ -1903,14 +1931,20 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                 rpy[ix] = I.py[r0 + ix];
                 rdpx[ix] = dpx[r0 + ix];
                 rdpy[ix] = dpy[r0 + ix];
-                rg[ix] = I.mu * I.pn[r0 + ix];
+                double pn = I.pn[r0 + ix];
+                rpn[ix] = pn;
+                rg[ix] = I.mu * pn;
+                rwx[ix] = I.wsx[r0 + ix];
+                rwy[ix] = I.wsy[r0 + ix];
                 rel[ix] = I.el[r0 + ix];
             }
+            double* stc = stg;                     // coefficient row 0, offsets (q - mx, 0)
             for (int e = tid; e < 4 * M2; e += T) {
                 int b = e / M2, q = e - b * M2;
-                stc[e] = cfb[b][(size_t)my * M2 + q];      // offsets (q - mx, 0)
+                stc[e] = cfb[b][(size_t)my * M2 + q];
             }
             __syncthreads();
+            const unsigned long long tr1 = wall_clock64();
             if (tid < 64) {
                 const int lane = tid, j0 = rj0[iy], j1 = rj1[iy];
                 for (int ix = t0; ix < t0 + nt; ix++) {
@@ -1926,19 +1960,27 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                         qsx += __shfl_xor(qsx, off);
                         qsy += __shfl_xor(qsy, off);
                     }
+                    // first upstream element not in adhesion (:2972-2977): the largest jx in
+                    // [1, ix-1] with another state, else 0 (-1 at the trailing edge)
+                    int jx = ix >= 1 ? 0 : -1;
+                    for (int hi = ix - 1; hi > 0; hi -= 64) {
+                        const int kx = hi - lane;
+                        const unsigned long long m = __ballot(kx >= 1 && rel[kx] != ADHES);
+                        if (m) {
+                            jx = hi - (__ffsll((long long)m) - 1);
+                            break;
+                        }
+                    }
                     double Dx = 0.0, Dy = 0.0;
                     if (lane == 0) {
                         const size_t ii = r0 + ix;
-                        int jx = ix - 1;
-                        while (jx > 0 && rel[jx] == ADHES) jx--;
                         const int b0 = mx, bj = mx + (jx - ix);
                         double a11 = I.ga_inv * (stc[b0] - stc[bj]);
                         double a12 = I.ga_inv * (stc[M2 + b0] - stc[M2 + bj]);
                         double a21 = I.ga_inv * (stc[2 * M2 + b0] - stc[2 * M2 + bj]);
                         double a22 = I.ga_inv * (stc[3 * M2 + b0] - stc[3 * M2 + bj]);
-                        double sv[2] = {I.wsx[ii] + I.ga_inv * (ucx[ix] + qsx),
-                                        I.wsy[ii] + I.ga_inv * (ucy[ix] + qsy)};
-                        double pr[3] = {rpx[ix], rpy[ix], I.pn[ii]};
+                        double sv[2] = {rwx[ix] + I.ga_inv * (ucx[ix] + qsx), rwy[ix] + I.ga_inv * (ucy[ix] + qsy)};
+                        double pr[3] = {rpx[ix], rpy[ix], rpn[ix]};
                         int8_t e8 = rel[ix];
                         elmtrc(e8, a11, a12, a21, a22, I.eps, pr, I.mu, sv, I.omegah, I.omegas);
                         Dx = pr[0] - rpx[ix];
@@ -2027,6 +2069,11 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                 }
             }
             __syncthreads();
+            if (tid == 0) {
+                const unsigned long long tr2 = wall_clock64();
+                t_rows += tr1 - tr0;
+                t_inrow += tr2 - tr1;
+            }
             for (int ix = tid; ix < mx; ix += T) {
                 I.px[r0 + ix] = rpx[ix];
                 I.py[r0 + ix] = rpy[ix];
@@ -2052,17 +2099,12 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
     if (tid == 0) {
         double len = 0.0;
         for (int jy = 0; jy < my; jy++) len += rj1[jy] >= rj0[jy] ? rj1[jy] - rj0[jy] + 1 : 0;
-        partials[(size_t)blockIdx.x * 3] = (double)itgs;
-        partials[(size_t)blockIdx.x * 3 + 1] = dif;
-        partials[(size_t)blockIdx.x * 3 + 2] = 8.0 * itgs * (double)I.k * len;
+        partials[(size_t)blockIdx.x * 5] = (double)itgs;
+        partials[(size_t)blockIdx.x * 5 + 1] = dif;
+        partials[(size_t)blockIdx.x * 5 + 2] = 8.0 * itgs * (double)I.k * len;
+        partials[(size_t)blockIdx.x * 5 + 3] = (double)t_rows;
+        partials[(size_t)blockIdx.x * 5 + 4] = (double)t_inrow;
     }
-}
-
-// dynamic LDS of k_stdygs2 for a grid of mx x my
-__host__ __device__ inline size_t gs2_lds(int mx, int my)
-{
-    return (size_t)(17 * mx) * sizeof(double) + 2 * GS_THREADS * sizeof(double) +
-           (size_t)4 * my * sizeof(int) + mx + 16;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2096,7 +2138,7 @@ int op_nred(int kind, const void* item)
     case OP_GD_DIAGRES: return 3;
     case OP_SETUP: return 2;
     case OP_FORCES: return 5;
-    case OP_STDYGS: return 3;
+    case OP_STDYGS: return 5;
     case OP_GUARD: return 5;
     default: return 0;
     }
