@@ -589,6 +589,12 @@ private:
             }
             launch_op(L.kind, dstage_ + L.items_off, (const int*)(dstage_ + L.gstart_off), L.nitems,
                       L.ngroups, dpart_ + L.part_off, L.stage, L.lds, s_);
+            if (hipError_t le = hipGetLastError(); le != hipSuccess) {
+                char msg[256];
+                std::snprintf(msg, sizeof msg, "launch of operation %d stage %d failed (%s): %d groups, %zu B LDS",
+                              L.kind, L.stage, hipGetErrorString(le), L.ngroups, L.lds);
+                throw std::runtime_error(msg);
+            }
             if (profile && L.ngroups > 0) HIPCHK(hipEventRecord(ev_pool[nev++], s_));
             if (L.K > 0)
                 launch_finalize((const int*)(dstage_ + L.gstart_off), (const int*)(dstage_ + L.modes_off),
@@ -1156,8 +1162,11 @@ static Task setup_coefficients(Ctx C, bool need_cv)
     for (int b : bl) blocks.push_back({P.cf + (size_t)b * nb, (char*)P.spec + (size_t)b * bsz * 16});
     if (need_cv)
         for (int b : bl) blocks.push_back({P.cfv + (size_t)b * nb, (char*)P.specv + (size_t)b * bsz * 16});
-    for (size_t k0 = 0; k0 < blocks.size(); k0 += 3) {
-        int nin = (int)std::min<size_t>(3, blocks.size() - k0);
+    // three blocks per transform while the column stage fits the LDS (two at Ly = 2048)
+    const size_t per = 2 * (size_t)P.Ly * 16;   // LDS per block and column: two buffers, FP64
+    const size_t bper = per * 3 <= MAX_LDS ? 3 : (per * 2 <= MAX_LDS ? 2 : 1);
+    for (size_t k0 = 0; k0 < blocks.size(); k0 += bper) {
+        int nin = (int)std::min<size_t>(bper, blocks.size() - k0);
         AnyItem cv;
         ConvItem& c = cv.conv;
         c.mx = mx; c.my = my; c.Lx = P.Lx; c.Ly = P.Ly;
