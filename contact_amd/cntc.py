@@ -54,6 +54,7 @@ _PROTOS = {
     "cntc_getpenetration": [_pi, _pi, _pd],
     "cntc_getcreepages": [_pi, _pi, _pd, _pd, _pd],
     "cntc_getcontactforces": [_pi, _pi, _pd, _pd, _pd, _pd],
+    "cntc_getsensitivities": [_pi, _pi, _pi, _pi, _pd],
     "cntc_getcontactpatchareas": [_pi, _pi, _pd, _pd, _pd],
     "cntc_getelementdivision": [_pi, _pi, _pi, _pi],
     "cntc_getmaximumpressure": [_pi, _pi, _pd],
@@ -259,6 +260,14 @@ def getcontactforces(ire=1, icp=1):
     v = [c_double(0) for _ in range(4)]
     lib().cntc_getcontactforces(_i(ire), _i(icp), *[ctypes.byref(x) for x in v])
     return tuple(x.value for x in v)
+
+
+def getsensitivities(ire=1, icp=1, lenout=4, lenin=4):
+    """Sensitivities [iout, iin] of (Fn, Fx/(mu Fn), Fy/(mu Fn), Mz) to (pen, cksi, ceta, cphi),
+    python_intfc/cntc_getsensitivities.py."""
+    v = np.zeros(lenout * lenin)
+    lib().cntc_getsensitivities(_i(ire), _i(icp), _i(lenout), _i(lenin), v.ctypes.data_as(_pd))
+    return v.reshape(lenin, lenout).T.copy()
 
 
 def getcontactpatchareas(ire=1, icp=1):

@@ -169,6 +169,7 @@ void cntc_setflags(int* ire, int* icp, int* lenflg, int* params, int* values)
         case CNTC_ic_norm: in.norm = std::max(0, std::min(1, v)); break;
         case CNTC_ic_iestim: in.iestim = std::max(0, std::min(3, v)); break;
         case CNTC_if_precision: in.precision = v ? 1 : 0; break;
+        case CNTC_ic_sens: in.sens = std::max(0, std::min(3, v)); break;   // :844-846
         default: break;   // output, flow, file flags: no effect on the solution
         }
     }
@@ -185,6 +186,11 @@ void cntc_setsolverflags(int* ire, int* icp, int* gdigit, int* nints, int* ipara
     if (gd < 0 || gd > 6) return;
     if (*nints != nints_loc[gd] || *nreals != nreals_loc[gd]) return;
     CaseInput& in = P->patch.in;
+    if (gd == 6) {
+        // parameters of the sensitivity solves, src/contact_addon.f90:1266-1272
+        in.mxsens = std::max(1, iparam[0]);
+        in.epsens = std::max(1e-20, rparam[0]);
+    }
     if (gd != 1 && gd != 6) {
         in.gausei = gd;
         in.eps = std::max(1e-20, rparam[0]);
@@ -515,6 +521,21 @@ void cntc_getcontactforces(int* ire, int* icp, double* fn, double* tx, double* t
     *tx = P->scl.body * o.fx;
     *ty = P->scl.body * o.fy;
     *mz = P->scl.body * o.mz / P->scl.len;
+}
+
+// src/contact_addon.f90:5919-6007 (module 3: no mirroring)
+void cntc_getsensitivities(int* ire, int* icp, int* lenout, int* lenin, double* sens)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = solved(*ire, *icp);
+    if (!P) return;
+    const double L = P->scl.len, F = P->scl.forc;
+    double fac_in[4] = {L, 1.0, 1.0, 1.0 / L}, fac_out[4] = {F, 1.0, 1.0, F * L};
+    if (P->patch.in.tang == 1) { fac_in[1] = L; fac_in[2] = L; fac_in[3] = 1.0; }
+    for (int iin = 0; iin < *lenin; iin++)
+        for (int iout = 0; iout < *lenout; iout++)
+            if (iout < 4 && iin < 4)
+                sens[iout + iin * *lenout] = P->patch.sens[iout + 4 * iin] * fac_in[iin] / fac_out[iout];
 }
 
 static std::vector<int> eldiv_of(Problem* P)

@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <initializer_list>
 #include <map>
 #include <stdexcept>
 
@@ -149,6 +150,7 @@ union AnyItem {
     ExtractItem ext;
     SetupItem setup;
     GsItem gs;
+    SensItem sens;
     AnyItem() { std::memset(this, 0, sizeof(*this)); }
 };
 
@@ -170,6 +172,7 @@ static size_t item_size(int kind)
     case OP_EXTRACT: return sizeof(ExtractItem);
     case OP_SETUP: case OP_FORCES: return sizeof(SetupItem);
     case OP_STDYGS: case OP_GUARD: return sizeof(GsItem);
+    case OP_SENS: case OP_SENS_EXT: return sizeof(SensItem);
     }
     return 0;
 }
@@ -704,6 +707,7 @@ struct Ctx {
     ResAwait sum(int mask, double* x) { return opr(OP_VEC, vitem(V_SUM, mask, x, nullptr, nullptr, nullptr, 0, 0)); }
     ResAwait sumsq(int mask, double* x, double* x1 = nullptr) { return opr(OP_VEC, vitem(V_SUMSQ, mask, x, x1, nullptr, nullptr, 0, 0)); }
     ResAwait maxabs(int mask, double* x, double* x1 = nullptr) { return opr(OP_VEC, vitem(V_MAXABS, mask, x, x1, nullptr, nullptr, 0, 0)); }
+    ResAwait vmax(int mask, double* x) { return opr(OP_VEC, vitem(V_MAX, mask, x, nullptr, nullptr, nullptr, 0, 0)); }
     ResAwait count(int mask) { return opr(OP_VEC, vitem(V_COUNT, mask, P->pn, nullptr, nullptr, nullptr, 0, 0)); }
 
     // FFT product u[ik] (mask) = sum_jk C[ik][jk] * p[jk], src/m_aijpj.f90:258-451.
@@ -1290,10 +1294,12 @@ static NcgItem ncg_item(Patch& P)
     return I;
 }
 
-static Task normcg(Ctx C, int maxcg, double eps, double* pen, int* itcg_out, double* err_out)
+// ps0: initial estimate of a sensitivity solve (is_sens), whose stopping test measures the
+// change from it (src/m_solvpn.f90:94-97, 281-289); null otherwise
+static Task normcg(Ctx C, int norm, int maxcg, double eps, double* pen, int* itcg_out, double* err_out,
+                   double* ps0 = nullptr)
 {
     Patch& P = *C.P;
-    const int norm = P.in.norm;
     const bool fp32 = P.in.precision == 1;
     const int npot = P.n;
     double dxdy = P.dxdy, fntrue = P.fntrue;
@@ -1390,7 +1396,14 @@ static Task normcg(Ctx C, int maxcg, double eps, double* pen, int* itcg_out, dou
         rms_upd = std::fabs(alpha) * std::sqrt(r.v[0] / std::max(1.0, r.v[1]));
         if (itcg == 1) rms_upd1 = rms_upd;
         if (itcg <= 3 || itcg % 10 == 0) {
-            r = co_await C.sumsq(M_INT, ps);
+            if (ps0) {
+                double* tmp = P.w[8];
+                C.copy(M_ALL, ps, tmp);
+                C.axpy(M_ALL, -1.0, ps0, tmp);
+                r = co_await C.sumsq(M_INT, tmp);
+            } else {
+                r = co_await C.sumsq(M_INT, ps);
+            }
             rms_xk = std::sqrt(r.v[0] / std::max(1.0, r.v[1]));
         }
         C.copy(M_ALL, res, r_prv);
@@ -1452,6 +1465,56 @@ static Task normcg(Ctx C, int maxcg, double eps, double* pen, int* itcg_out, dou
     co_return ret;
 }
 
+// ------------------------------------------------------------------------------------------
+// sensitivities (src/m_solvpn.f90:531-650, src/m_solvpt.f90:497-796)
+// ------------------------------------------------------------------------------------------
+
+enum { IOUT_FN = 0, IOUT_FX = 1, IOUT_FY = 2, IOUT_MZ = 3 };
+enum { IIN_PEN = 0, IIN_KSI = 1, IIN_ETA = 2, IIN_PHI = 3 };
+static double& sens_at(Patch& P, int iout, int iin) { return P.sens[iout + 4 * iin]; }
+
+// copy of the element division and of up to 6 grid functions of a patch, as one launch
+static void snap(Ctx& C, const int8_t* el_src, int8_t* el_dst, std::initializer_list<double*> src,
+                 std::initializer_list<double*> dst)
+{
+    AnyItem it;
+    SensItem& q = it.sens;
+    q.n = C.P->n;
+    q.mx = C.P->mx;
+    q.mode = 0;
+    q.el_src = el_src;
+    q.el_dst = el_dst;
+    int k = 0;
+    for (double* a : src) q.src[k++] = a;
+    q.nvec = k;
+    k = 0;
+    for (double* a : dst) q.dst[k++] = a;
+    C.op(OP_SENS, it);
+}
+
+// sens_norm, src/m_solvpn.f90:531-650: dFn/dpen by one NormCG (N=0, maxit mxsens, eps
+// epsens) at pen + 0.0067 max(un), started from the converged pressures and element division,
+// which are restored afterwards.  unn holds A pn of the converged state.
+static Task sens_norm(Ctx C, double* unn)
+{
+    Patch& P = *C.P;
+    Res r = co_await C.vmax(M_INT, unn);
+    double pentru = std::max(r.v[0], -1e20);        // gf3_max starts at -1e20
+    double dpen = 0.0067 * pentru;
+    double penloc = P.pen + dpen;
+    int8_t* el0 = (int8_t*)P.w[20];
+    double* ps0 = P.w[15];
+    snap(C, P.el, el0, {P.pn}, {ps0});
+    int it = 0;
+    double err = 0.0;
+    int rc = co_await normcg(C, 0, P.in.mxsens, P.in.epsens, &penloc, &it, &err, ps0);
+    r = co_await C.sum(M_ALL, P.pn);
+    double fnloc = P.dxdy * r.v[0];
+    sens_at(P, IOUT_FN, IIN_PEN) = (fnloc - P.fntrue) / dpen;
+    snap(C, el0, P.el, {ps0}, {P.pn});
+    co_return rc;
+}
+
 // snorm, src/m_snorm.f90:31-378
 static Task snorm(Ctx C, int* itnorm_out)
 {
@@ -1460,7 +1523,6 @@ static Task snorm(Ctx C, int* itnorm_out)
     double* ps = P.pn;
     double* unn = P.w[7];
     bool use_vecfft = true;   // the product of the whole area serves the expansion test
-    (void)npot;
     co_await setup_prec(C, 2);
     int itnorm = 0, itcg = 0, ret = 0;
     bool zready;
@@ -1471,7 +1533,7 @@ static Task snorm(Ctx C, int* itnorm_out)
         C.set(M_EXT, ps, 0.0);
         int it = 0;
         double errpn = 0.0;
-        int rc = co_await normcg(C, P.in.maxgs, P.in.eps, &P.pen, &it, &errpn);
+        int rc = co_await normcg(C, P.in.norm, P.in.maxgs, P.in.eps, &P.pen, &it, &errpn);
         itcg += it;
         if (rc) { ret = rc; break; }
         AnyItem ic;
@@ -1500,6 +1562,14 @@ static Task snorm(Ctx C, int* itnorm_out)
     if (P.in.norm == 0) {
         r = co_await C.sum(M_ALL, ps);
         P.fntrue = P.dxdy * r.v[0];
+    }
+    // sensitivity dFn/dpen (src/m_snorm.f90:357-368); below 700 elements the reference forms
+    // unn after the final clean-up of the pressures
+    if (!ret && P.in.sens >= 2) {
+        if (npot < 700) C.conv_n(0, M_ALL, unn, ps);
+        ret = co_await sens_norm(C, unn);
+    } else {
+        sens_at(P, IOUT_FN, IIN_PEN) = 0.0;
     }
     *itnorm_out = itnorm;
     co_return ret;
@@ -1779,7 +1849,10 @@ static Task gdsteady(Ctx C, const double cf00[4], int maxgd, double eps, int* it
 // ------------------------------------------------------------------------------------------
 
 // cf00: the tangential coefficients of element (0,0), unscaled (cs blocks xx, xy, yx, yy)
-static Task tangcg(Ctx C, const double cf00[4], int maxcg, double eps, int* itcg_out, double* err_out)
+// p0x, p0y: initial estimate of a sensitivity solve (is_sens, src/m_solvpt.f90:1910-1914,
+// 2209-2216); null otherwise
+static Task tangcg(Ctx C, const double cf00[4], int maxcg, double eps, int* itcg_out, double* err_out,
+                   const double* p0x = nullptr, const double* p0y = nullptr)
 {
     Patch& P = *C.P;
     const int num_inn = 4;
@@ -1794,6 +1867,7 @@ static Task tangcg(Ctx C, const double cf00[4], int maxcg, double eps, int* itcg
     T.nx = P.w[0]; T.ny = P.w[1]; T.rx = P.w[2]; T.ry = P.w[3]; T.zx = P.w[4]; T.zy = P.w[5];
     T.vx = P.w[6]; T.vy = P.w[7]; T.qx = P.w[8]; T.qy = P.w[9]; T.poldx = P.w[10]; T.poldy = P.w[11];
     T.mu = P.in.fstat; T.ga = P.ga; T.c11 = c11; T.c22 = c22;
+    T.p0x = p0x; T.p0y = p0y;
     AnyItem t;
     t.tcg = T;
     Res r;
@@ -1892,12 +1966,12 @@ static Task tangcg(Ctx C, const double cf00[4], int maxcg, double eps, int* itcg
 // stang, src/m_stang.f90:28-951 (Coulomb friction, creepages prescribed)
 // ------------------------------------------------------------------------------------------
 
-static GsItem gs_item(Patch& P, int k, double omegah, double omegas)
+static GsItem gs_item(Patch& P, int k, double omegah, double omegas, int maxgs, double eps)
 {
     GsItem g{};
     g.n = P.n; g.mx = P.mx; g.my = P.my; g.k = k;
-    g.maxgs = P.in.maxgs;
-    g.eps = P.in.eps; g.omegah = omegah; g.omegas = omegas;
+    g.maxgs = maxgs;
+    g.eps = eps; g.omegah = omegah; g.omegas = omegas;
     g.ga_inv = 1.0 / P.ga;
     g.mu = P.in.fstat;
     g.facnel = (double)std::sqrt((float)P.n / (float)std::max(1, k));
@@ -1915,18 +1989,169 @@ static GsItem gs_item(Patch& P, int k, double omegah, double omegas)
     return g;
 }
 
-// SteadyGS, src/m_solvpt.f90:2825-3254: all sweeps in one launch per patch
-static Task stdygs(Ctx C, int k, double omegah, double omegas, int* itgs_out, double* err_out)
+// SteadyGS, src/m_solvpt.f90:2825-3254: all sweeps in one launch per patch.  p0x, p0y: initial
+// estimate of a sensitivity solve (is_sens, :2895-2899, 3138-3144); null otherwise.
+static Task stdygs(Ctx C, int k, double omegah, double omegas, int maxgs, double eps, int* itgs_out,
+                   double* err_out, const double* p0x = nullptr, const double* p0y = nullptr)
 {
     Patch& P = *C.P;
     AnyItem it;
-    it.gs = gs_item(P, k, omegah, omegas);
+    it.gs = gs_item(P, k, omegah, omegas, maxgs, eps);
+    it.gs.p0x = p0x;
+    it.gs.p0y = p0y;
     if (op_lds(OP_STDYGS, &it.gs, 0) > MAX_LDS && op_lds(OP_STDYGS, &it.gs, 1) > MAX_LDS) co_return ERR_OTHER;
     Res r = co_await C.opr(OP_STDYGS, it);
     *itgs_out = (int)r.v[0];
     *err_out = r.v[1];
     P.out.used_stdygs = 1;
     co_return 0;
+}
+
+static SetupItem setup_item(Patch& P)
+{
+    SetupItem S{};
+    S.n = P.n; S.mx = P.mx; S.my = P.my; S.ibase = P.ibase_eff;
+    S.el = P.el; S.h = P.h; S.hx = P.hx; S.hy = P.hy;
+    S.pn = P.pn; S.px = P.px; S.py = P.py; S.pvn = P.pvn; S.pvx = P.pvx; S.pvy = P.pvy;
+    S.xc1 = P.xc1; S.yc1 = P.yc1; S.dx = P.dx; S.dy = P.dy;
+    for (int k = 0; k < 6; k++) S.prm[k] = P.prm[k];
+    return S;
+}
+
+// Kalker's creep coefficients C11, C22, C23 (the table of src/m_hertz.f90:1158-1179, single-
+// precision values there) at g = 0.1, ..., 1.0 and poisson ratio 0, 1/4, 1/2; case 0 holds
+// g = a/b <= 1, case 1 holds g = b/a < 1.
+static const float kalker_c[2][10][3][3] = {   // [case][g row][C11, C22, C23][nu]
+    {{{2.51f, 3.31f, 4.85f}, {2.51f, 2.52f, 2.53f}, {.334f, .473f, .731f}},
+     {{2.59f, 3.37f, 4.81f}, {2.59f, 2.63f, 2.66f}, {.483f, .603f, .809f}},
+     {{2.68f, 3.44f, 4.80f}, {2.68f, 2.75f, 2.81f}, {.607f, .715f, .889f}},
+     {{2.78f, 3.53f, 4.82f}, {2.78f, 2.88f, 2.98f}, {.720f, .823f, .977f}},
+     {{2.88f, 3.62f, 4.83f}, {2.88f, 3.01f, 3.14f}, {.827f, .929f, 1.07f}},
+     {{2.98f, 3.72f, 4.91f}, {2.98f, 3.14f, 3.31f}, {.930f, 1.03f, 1.18f}},
+     {{3.09f, 3.81f, 4.97f}, {3.09f, 3.28f, 3.48f}, {1.03f, 1.14f, 1.29f}},
+     {{3.19f, 3.91f, 5.05f}, {3.19f, 3.41f, 3.65f}, {1.13f, 1.25f, 1.40f}},
+     {{3.29f, 4.01f, 5.12f}, {3.29f, 3.54f, 3.82f}, {1.23f, 1.36f, 1.51f}},
+     {{3.40f, 4.12f, 5.20f}, {3.40f, 3.67f, 3.98f}, {1.33f, 1.47f, 1.63f}}},
+    {{{10.7f, 11.7f, 12.9f}, {10.7f, 12.8f, 16.0f}, {12.2f, 14.6f, 18.0f}},
+     {{6.96f, 7.78f, 8.82f}, {6.96f, 8.14f, 9.79f}, {5.72f, 6.63f, 7.89f}},
+     {{5.57f, 6.34f, 7.34f}, {5.57f, 6.40f, 7.51f}, {3.79f, 4.32f, 5.01f}},
+     {{4.84f, 5.57f, 6.57f}, {4.84f, 5.48f, 6.31f}, {2.88f, 3.24f, 3.70f}},
+     {{4.37f, 5.10f, 6.11f}, {4.37f, 4.90f, 5.56f}, {2.35f, 2.62f, 2.96f}},
+     {{4.06f, 4.78f, 5.80f}, {4.06f, 4.50f, 5.04f}, {2.01f, 2.23f, 2.50f}},
+     {{3.82f, 4.54f, 5.58f}, {3.82f, 4.21f, 4.67f}, {1.76f, 1.95f, 2.18f}},
+     {{3.65f, 4.36f, 5.42f}, {3.65f, 3.99f, 4.39f}, {1.58f, 1.75f, 1.94f}},
+     {{3.51f, 4.22f, 5.30f}, {3.51f, 3.81f, 4.16f}, {1.44f, 1.59f, 1.77f}},
+     {{3.40f, 4.12f, 5.20f}, {3.40f, 3.67f, 3.98f}, {1.33f, 1.47f, 1.63f}}}};
+
+// linrol, src/m_hertz.f90:1147-1247: linear-theory creep coefficients at a/b and poisson ratio
+// nu: asymptotic forms for slender ellipses (g < 0.101), otherwise the table, linear in g and
+// quadratic (Lagrange) in nu for 1/C
+static void linrol(double nu, double aob, double& cxx, double& cyy, double& cyz)
+{
+    int icase = aob <= 1.0 ? 0 : 1;
+    double g = aob <= 1.0 ? aob : 1.0 / aob;
+    if (icase == 0 && g < 0.101) {
+        cxx = PI * PI / 4.0 / (1.0 - nu);
+        cyy = PI * PI / 4.0;
+        cyz = PI * std::sqrt(g) / 3.0 / (1.0 - nu) * (1.0 + nu * (std::log(16.0 / g) - 5.0));
+    } else if (icase == 1 && g < 0.101) {
+        double al = std::log(16.0 / (g * g));
+        double l4 = std::log(4.0);
+        cxx = 2.0 * PI / (al - 2.0 * nu) / g * (1.0 + (3.0 - l4) / (al - 2.0 * nu));
+        cyy = 2.0 * PI / g * (1.0 + (1.0 - nu) * (3.0 - l4) / ((1.0 - nu) * al + 2.0 * nu)) /
+              ((1.0 - nu) * al + 2.0 * nu);
+        cyz = 2.0 * PI / 3.0 / std::pow(g, 1.5) / ((1.0 - nu) * al - 2.0 + 4.0 * nu);
+    } else {
+        int irow = (int)(10.0 * g - 0.005);
+        double fac = 10.0 * g - (double)irow;
+        double c[3][3], cc[3];
+        for (int inu = 0; inu < 3; inu++)
+            for (int ic = 0; ic < 3; ic++)
+                c[inu][ic] = fac * (double)kalker_c[icase][irow][ic][inu] +
+                             (1.0 - fac) * (double)kalker_c[icase][irow - 1][ic][inu];
+        for (int ic = 0; ic < 3; ic++)
+            cc[ic] = (nu - 0.25) * (nu - 0.50) * 8.0 / c[0][ic] - (nu - 0.50) * nu * 16.0 / c[1][ic] +
+                     nu * (nu - 0.25) * 8.0 / c[2][ic];
+        cxx = 1.0 / cc[0];
+        cyy = 1.0 / cc[1];
+        cyz = 1.0 / cc[2];
+    }
+}
+
+// sens_tang, src/m_solvpt.f90:544-796: d(Fx, Fy)/(mu Fn) and dMz per creepage.  Each creepage
+// is raised by 1e-4 of its characteristic size (char_creep, :497-540) and one tangential solve
+// (tang_solver with is_sens: maxit mxsens, eps epsens, :382-493) runs from the converged
+// state on the fixed right-hand side plus that change; the state is restored after each.
+static Task sens_tang(Ctx C, int k, const double cf00[4], double omegah, double omegas)
+{
+    Patch& P = *C.P;
+    AnyItem fi;
+    fi.setup = setup_item(P);
+    Res s = co_await C.opr(OP_FORCES, fi);
+    double den = P.muscal * P.fntrue;
+    double fxrel = P.dxdy * s.v[1] / den, fyrel = P.dxdy * s.v[2] / den;
+    double mztrue = P.dxdy * (-s.v[3] + s.v[4]);
+    // char_creep: effective radius from the contact area, half-axes from its extents
+    Res cn = co_await C.count(M_INT);
+    AnyItem ei;
+    ei.sens.n = P.n;
+    ei.sens.mx = P.mx;
+    ei.sens.mode = 2;
+    ei.sens.el = P.el;
+    Res ex = co_await C.opr(OP_SENS_EXT, ei);
+    double ncon = cn.v[0];
+    double c = std::sqrt((double)(float)ncon * P.dx * P.dy / PI);
+    int ixmin = (int)-ex.v[0], ixmax = (int)ex.v[1], iymin = (int)-ex.v[2], iymax = (int)ex.v[3];
+    double a = (double)std::max(1, ixmax - ixmin + 1) * P.dx * 0.5;
+    double b = (double)std::max(1, iymax - iymin + 1) * P.dy * 0.5;
+    double c11, c22, c23;
+    linrol(P.nu, a / b, c11, c22, c23);
+    double ch[3];
+    ch[0] = P.in.fstat * P.fntrue / (c * c * P.ga * c11);
+    ch[1] = P.in.fstat * P.fntrue / (c * c * P.ga * c22);
+    ch[2] = P.in.fstat * P.fntrue / (c * c * c * P.ga * c23);
+    // saved state: element division, tractions and slip; the fixed right-hand side
+    int8_t* el0 = (int8_t*)P.w[20];
+    double *p0x = P.w[15], *p0y = P.w[16], *s0x = P.w[17], *s0y = P.w[18];
+    double *wbx = P.w[22], *wby = P.w[23];
+    snap(C, P.el, el0, {P.px, P.py, P.sx, P.sy, P.wsx, P.wsy}, {p0x, p0y, s0x, s0y, wbx, wby});
+    int stagn0 = P.out.gd_stagnated, used0 = P.out.used_stdygs, ret = 0;
+    for (int icrp = 0; icrp < 3 && !ret; icrp++) {
+        double dcrp = 0.0001 * ch[icrp];
+        AnyItem wi;
+        SensItem& q = wi.sens;
+        q.n = P.n; q.mx = P.mx; q.mode = 1; q.which = icrp;
+        q.el = P.el; q.wbx = wbx; q.wby = wby; q.wsx = P.wsx; q.wsy = P.wsy;
+        q.xc1 = P.xc1; q.yc1 = P.yc1; q.dx = P.dx; q.dy = P.dy; q.dq = P.dq; q.d = dcrp;
+        C.op(OP_SENS, wi);
+        int it = 0, rc;
+        double err = 0.0;
+        if (P.in.tang == 3 && P.in.gausei == 5) {
+            int lstagn = 0;
+            rc = co_await gdsteady(C, cf00, P.in.mxsens, P.in.epsens, &it, &err, &lstagn);
+            if (!rc && lstagn)
+                rc = co_await stdygs(C, k, omegah, omegas, P.in.mxsens, P.in.epsens, &it, &err, p0x, p0y);
+        } else if (P.in.tang == 3) {
+            rc = co_await stdygs(C, k, omegah, omegas, P.in.mxsens, P.in.epsens, &it, &err, p0x, p0y);
+        } else {
+            rc = co_await tangcg(C, cf00, P.in.mxsens, P.in.epsens, &it, &err, p0x, p0y);
+        }
+        if (rc) {
+            ret = rc;
+        } else {
+            Res f = co_await C.opr(OP_FORCES, fi);
+            double fxloc = P.dxdy * f.v[1] / den, fyloc = P.dxdy * f.v[2] / den;
+            double mzloc = P.dxdy * (-f.v[3] + f.v[4]);
+            sens_at(P, IOUT_FX, IIN_KSI + icrp) = (fxloc - fxrel) / dcrp;
+            sens_at(P, IOUT_FY, IIN_KSI + icrp) = (fyloc - fyrel) / dcrp;
+            sens_at(P, IOUT_MZ, IIN_KSI + icrp) = (mzloc - mztrue) / dcrp;
+        }
+        snap(C, el0, P.el, {p0x, p0y, s0x, s0y}, {P.px, P.py, P.sx, P.sy});
+    }
+    snap(C, nullptr, nullptr, {wbx, wby}, {P.wsx, P.wsy});
+    P.out.gd_stagnated = stagn0;
+    P.out.used_stdygs = used0;
+    co_return ret;
 }
 
 static Task stang(Ctx C, int* ittang_out)
@@ -1943,7 +2168,7 @@ static Task stang(Ctx C, int* ittang_out)
         // provided.  The same launch returns the coefficients of element (0,0).
         if (P.in.tang == 3 && P.in.gausei == 2) co_return ERR_OTHER;
         AnyItem gi;
-        gi.gs = gs_item(P, k, 1.0, 1.0);
+        gi.gs = gs_item(P, k, 1.0, 1.0, P.in.maxgs, P.in.eps);
         Res g = co_await C.opr(OP_GUARD, gi);
         if (P.in.tang == 3 && g.v[0] > 0) co_return ERR_OTHER;
         for (int q = 0; q < 4; q++) cf00[q] = g.v[1 + q];
@@ -1984,10 +2209,10 @@ static Task stang(Ctx C, int* ittang_out)
             P.out.gd_stagnated = lstagn;
             if (!rc && lstagn) {
                 itgs += std::abs(it);
-                rc = co_await stdygs(C, k, omegah, omegas, &it, &errpt);
+                rc = co_await stdygs(C, k, omegah, omegas, P.in.maxgs, P.in.eps, &it, &errpt);
             }
         } else if (P.in.tang == 3) {
-            rc = co_await stdygs(C, k, omegah, omegas, &it, &errpt);
+            rc = co_await stdygs(C, k, omegah, omegas, P.in.maxgs, P.in.eps, &it, &errpt);
         } else {
             rc = co_await tangcg(C, cf00, P.in.maxgs, P.in.eps, &it, &errpt);
         }
@@ -2014,6 +2239,8 @@ static Task stang(Ctx C, int* ittang_out)
     if (!zready && !ret) ittang = -1;
     P.out.itgs = itgs;
     *ittang_out = ittang;
+    // sensitivities to the creepages, src/m_stang.f90:724-731
+    if (!ret && P.in.sens >= 3) ret = co_await sens_tang(C, k, cf00, omegah, omegas);
     co_return ret;
 }
 
@@ -2022,16 +2249,6 @@ static Task stang(Ctx C, int* ittang_out)
 // (src/m_soutpt.f90:424-450)
 // ------------------------------------------------------------------------------------------
 
-static SetupItem setup_item(Patch& P)
-{
-    SetupItem S{};
-    S.n = P.n; S.mx = P.mx; S.my = P.my; S.ibase = P.ibase_eff;
-    S.el = P.el; S.h = P.h; S.hx = P.hx; S.hy = P.hy;
-    S.pn = P.pn; S.px = P.px; S.py = P.py; S.pvn = P.pvn; S.pvx = P.pvx; S.pvy = P.pvy;
-    S.xc1 = P.xc1; S.yc1 = P.yc1; S.dx = P.dx; S.dy = P.dy;
-    for (int k = 0; k < 6; k++) S.prm[k] = P.prm[k];
-    return S;
-}
 
 static Task calculate(Ctx C)
 {
@@ -2052,6 +2269,17 @@ static Task calculate(Ctx C)
         Res r = co_await C.opr(OP_SETUP, it);
         P.hmax = r.v[0];
         P.hmin = -r.v[1];
+    }
+    // initial sensitivities (init_curr_data, src/m_sdis.f90:753-764); eldiv0's estimate of
+    // dFn/dpen for N=1 is always replaced in snorm and is not formed
+    {
+        const int ihertz = P.in.ipotcn < 0 ? -P.in.ipotcn : 0;
+        if (P.in.iestim == 0)
+            for (int io = IOUT_FX; io <= IOUT_MZ; io++) sens_at(P, io, IIN_PEN) = 0.0;
+        if (P.in.iestim == 0 && (P.in.norm != 1 || ihertz != 0)) sens_at(P, IOUT_FN, IIN_PEN) = 0.0;
+        if (P.in.iestim == 0 || P.in.iestim == 2)
+            for (int ii = IIN_KSI; ii <= IIN_PHI; ii++)
+                for (int io = IOUT_FN; io <= IOUT_MZ; io++) sens_at(P, io, ii) = 0.0;
     }
     co_await setup_coefficients(C, P.in.tang == 2);
     int itnorm = 0, ittang = 0;

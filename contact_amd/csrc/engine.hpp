@@ -39,6 +39,10 @@ struct CaseInput {
     int kdowfb = 1;
     double omegah = (double)0.90f, omegas = 1.0;   // SteadyGS relaxation for G=2, 3
     int precision = 0;              // 0: FP64 products; 1: FP32 products in FP64 refinement
+    // sensitivities: S2 digit (2: dFn/dpen, 3: also the creepages) and their solver
+    // parameters (src/m_hierarch_data.f90:981, 1733-1735)
+    int sens = 1, mxsens = 30;
+    double epsens = 1e-4;
 };
 
 // Results of one case.
@@ -70,6 +74,9 @@ struct Patch {
     int Lx = 0, Ly = 0;
     bool have_solution = false;
     int slot = -1;                   // scheduler slot during a solve
+    // sensitivities sens(iout, iin) in internal units, iout + 4 iin: outputs fn, fx/(mu fn),
+    // fy/(mu fn), mz; inputs pen, cksi, ceta, cphi (src/m_hierarch_data.f90:122-124)
+    double sens[16] = {};
     // material and kinematics of the current case
     double ga = 1, nu = 0, ak = 0, muscal = 0.3, fntrue = 0, pen = 0, dq = 1, chi = 0;
     bool nt_cpl = false;

@@ -457,7 +457,8 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_qpost(const TcgItem* items
     write_partials<2>(acc, 0, partials, 2);
 }
 
-// pold = p; p += alpha v (interior); slip clipped to the bound; returns (sum v^2, sum p^2)
+// pold = p; p += alpha v (interior); slip clipped to the bound; returns (sum v^2, sum p^2), or
+// sum (p - p0)^2 in a sensitivity solve (src/m_solvpt.f90:2209-2216)
 __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_pupd(const TcgItem* items, const int* gstart,
                                                            int nitems, double* partials)
 {
@@ -485,6 +486,10 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_pupd(const TcgItem* items,
         I.px[i] = px;
         I.py[i] = py;
         acc[0] += I.vx[i] * I.vx[i] + I.vy[i] * I.vy[i];
+        if (I.p0x) {
+            px = px + -1.0 * I.p0x[i];
+            py = py + -1.0 * I.p0y[i];
+        }
         acc[1] += px * px + py * py;
     }
     write_partials<2>(acc, 0, partials, 2);
@@ -1569,6 +1574,50 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_forces(const SetupItem* items,
 
 
 // ------------------------------------------------------------------------------------------
+// sensitivity solves: state copies, perturbed right-hand sides, contact extents
+// (src/m_solvpn.f90:531-650, src/m_solvpt.f90:497-796)
+// ------------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(ELEM_THREADS) k_sens(const SensItem* items, const int* gstart,
+                                                       int nitems, double* partials)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const SensItem& I = items[it];
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    double acc[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        if (I.mode == 0) {
+            if (I.el_dst) I.el_dst[i] = I.el_src[i];
+            for (int v = 0; v < I.nvec; v++) I.dst[v][i] = I.src[v][i];
+        } else if (I.mode == 1) {
+            // wsloc = wstot + facdt dcrp dq (facdt = 1 on the interior), src/m_solvpt.f90:644-690
+            double wx = I.wbx[i], wy = I.wby[i];
+            if (I.el[i] >= ADHES) {
+                int ix = i % I.mx, iy = i / I.mx;
+                double x = I.xc1 + ix * I.dx, y = I.yc1 + iy * I.dy;
+                if (I.which == 0) wx = wx + I.d * I.dq;
+                else if (I.which == 1) wy = wy + I.d * I.dq;
+                else {
+                    wx = wx - I.d * y * I.dq;
+                    wy = wy + I.d * x * I.dq;
+                }
+            }
+            I.wsx[i] = wx;
+            I.wsy[i] = wy;
+        } else if (I.el[i] >= ADHES) {
+            double ix = (double)(i % I.mx + 1), iy = (double)(i / I.mx + 1);
+            acc[0] = fmax(acc[0], -ix);
+            acc[1] = fmax(acc[1], ix);
+            acc[2] = fmax(acc[2], -iy);
+            acc[3] = fmax(acc[3], iy);
+        }
+    }
+    if (I.mode == 2) write_partials<4>(acc, 1, partials, 4);
+}
+
+// ------------------------------------------------------------------------------------------
 // SteadyGS, src/m_solvpt.f90:2825-3254 (chi = 0, no plasticity, no third body)
 // ------------------------------------------------------------------------------------------
 
@@ -1766,7 +1815,15 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, cons
         }
         // dif = sqrt(sum |dp_k|^2 / 2k); difid from the rms of p over all elements
         double sq[1] = {0.0};
-        for (int i = tid; i < n; i += T) sq[0] += I.px[i] * I.px[i] + I.py[i] * I.py[i];
+        if (I.p0x) {
+            // sensitivity solve: rms of the change from the initial estimate (:3138-3144)
+            for (int i = tid; i < n; i += T) {
+                double ax = I.px[i] + -1.0 * I.p0x[i], ay = I.py[i] + -1.0 * I.p0y[i];
+                sq[0] += ax * ax + ay * ay;
+            }
+        } else {
+            for (int i = tid; i < n; i += T) sq[0] += I.px[i] * I.px[i] + I.py[i] * I.py[i];
+        }
         group_reduce<1>(sq, 0, red);
         if (tid == 0) {
             double rms = sqrt(sq[0] / (double)(2 * n > 1 ? 2 * n : 1));
@@ -2084,7 +2141,15 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
             __syncthreads();
         }
         double sq[1] = {0.0};
-        for (int i = tid; i < n; i += T) sq[0] += I.px[i] * I.px[i] + I.py[i] * I.py[i];
+        if (I.p0x) {
+            // sensitivity solve: rms of the change from the initial estimate (:3138-3144)
+            for (int i = tid; i < n; i += T) {
+                double ax = I.px[i] + -1.0 * I.p0x[i], ay = I.py[i] + -1.0 * I.p0y[i];
+                sq[0] += ax * ax + ay * ay;
+            }
+        } else {
+            for (int i = tid; i < n; i += T) sq[0] += I.px[i] * I.px[i] + I.py[i] * I.py[i];
+        }
         group_reduce<1>(sq, 0, red_s);
         if (tid == 0) {
             double rms = sqrt(sq[0] / (double)(2 * n > 1 ? 2 * n : 1));
@@ -2140,6 +2205,7 @@ int op_nred(int kind, const void* item)
     case OP_FORCES: return 5;
     case OP_STDYGS: return 5;
     case OP_GUARD: return 5;
+    case OP_SENS_EXT: return 4;
     default: return 0;
     }
     (void)item;
@@ -2152,7 +2218,7 @@ int op_combine_mode(int kind, const void* item)
         if (sub == V_MAXABS || sub == V_MAX) return 1;
         if (sub == V_MIN) return 2;
     }
-    if (kind == OP_SETUP) return 1;
+    if (kind == OP_SETUP || kind == OP_SENS_EXT) return 1;
     return 0;
 }
 
@@ -2177,6 +2243,7 @@ int op_groups(int kind, const void* item, int conv_stage)
     case OP_GD_MOVES: case OP_GD_DIAGRES: return ngroups_elem(((const GdItem*)item)->n);
     case OP_SETUP: case OP_FORCES: return ngroups_elem(((const SetupItem*)item)->n);
     case OP_STDYGS: return 1;
+    case OP_SENS: case OP_SENS_EXT: return ngroups_elem(((const SensItem*)item)->n);
     case OP_GUARD: return (((const GsItem*)item)->my + ELEM_THREADS - 1) / ELEM_THREADS;
     case OP_ELASCF: { const CoefItem* c = (const CoefItem*)item; return (int)(((size_t)(2 * c->mx) * (2 * c->my) + ELEM_THREADS - 1) / ELEM_THREADS); }
     case OP_EMBED: { const EmbedItem* e = (const EmbedItem*)item; return (int)(((size_t)e->Lx * e->Ly + ELEM_THREADS - 1) / ELEM_THREADS); }
@@ -2246,6 +2313,7 @@ void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, i
         else hipLaunchKernelGGL(k_stdygs, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, d_partials);
         break;
     case OP_GUARD: hipLaunchKernelGGL(k_guard, ge, te, 0, s, (const GsItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_SENS: case OP_SENS_EXT: hipLaunchKernelGGL(k_sens, ge, te, 0, s, (const SensItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_ELASCF: hipLaunchKernelGGL(k_elascf, ge, te, 0, s, (const CoefItem*)d_items, d_gstart, nitems); break;
     case OP_EMBED: hipLaunchKernelGGL(k_embed, ge, te, 0, s, (const EmbedItem*)d_items, d_gstart, nitems); break;
     case OP_EXTRACT: hipLaunchKernelGGL(k_extract, ge, te, 0, s, (const ExtractItem*)d_items, d_gstart, nitems); break;
@@ -2326,6 +2394,12 @@ double op_bytes(int kind, const void* item, int stage)
     case OP_ELASCF: { const CoefItem* c = (const CoefItem*)item; return 9.0 * 8.0 * (2 * c->mx) * (2 * c->my); }
     case OP_STDYGS: { const GsItem* g = (const GsItem*)item; return (double)g->n * 73.0; }
     case OP_GUARD: { const GsItem* g = (const GsItem*)item; return (double)g->my; }
+    case OP_SENS: {
+        const SensItem* q = (const SensItem*)item;
+        if (q->mode == 0) return (double)q->n * ((q->el_dst ? 2.0 : 0.0) + 16.0 * q->nvec);
+        return (double)q->n * 33.0;
+    }
+    case OP_SENS_EXT: return (double)((const SensItem*)item)->n;
     case OP_EMBED: { const EmbedItem* e = (const EmbedItem*)item; return 8.0 * e->Lx * e->Ly; }
     case OP_EXTRACT: { const ExtractItem* e = (const ExtractItem*)item; return 16.0 * (2 * e->cf_mx) * (2 * e->cf_my); }
     }

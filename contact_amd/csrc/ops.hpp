@@ -51,6 +51,8 @@ enum OpKind : int {
     OP_FORCES,
     OP_STDYGS,
     OP_GUARD,
+    OP_SENS,
+    OP_SENS_EXT,
     OP_COUNT_KINDS
 };
 
@@ -118,6 +120,7 @@ struct TcgItem {
     double* vx; double* vy;
     double* qx; double* qy;
     double* poldx; double* poldy;
+    const double* p0x; const double* p0y;   // sensitivity solve: stopping test on p - p0
     double mu, ga, c11, c22;
     double alpha;
 };
@@ -164,6 +167,23 @@ struct GsItem {
     double* dpx; double* dpy;  // work: differences along x
     int* list;                 // work: 2n ints, (element, jy*2mx + jx) of the summation range
     const double* cf[2][2];    // tangential blocks of cs, layout [(iy+my)*2mx + ix+mx]
+    const double* p0x; const double* p0y;   // sensitivity solve: stopping test on p - p0
+};
+
+// sensitivity solves (sens_norm, src/m_solvpn.f90:531-650; sens_tang, src/m_solvpt.f90:544-796)
+//   OP_SENS, mode 0: copy a patch state (element division and up to 6 grid functions)
+//   OP_SENS, mode 1: ws = wb plus the change of the rigid slip for creepage `which`
+//                    (0 cksi, 1 ceta, 2 cphi) by d, on the interior
+//   OP_SENS_EXT:     extents of the contact area, returns (-min ix, max ix, -min iy, max iy)
+struct SensItem {
+    int n, mx, mode, nvec;
+    int which, pad;
+    const int8_t* el;          // mode 1, extents: element division
+    const int8_t* el_src; int8_t* el_dst;
+    const double* src[6]; double* dst[6];
+    const double* wbx; const double* wby;
+    double* wsx; double* wsy;
+    double xc1, yc1, dx, dy, dq, d;
 };
 
 // one FFT product (gf3_VecAijPj via fft_VecAijPj, src/m_aijpj.f90:258-451, 712-1015)

@@ -152,3 +152,7 @@ def apply_case(cntc, ire, icp, c):
     else:
         cntc.setpenetration(ire, icp, c["pen"])
     cntc.setcreepages(ire, icp, c["cksi"], c["ceta"], c["cphi"])
+    if c.get("sens") is not None:
+        cntc.setflags(ire, icp, [1988], [c["sens"]])
+    if c.get("mxsens") is not None or c.get("epsens") is not None:
+        cntc.setsolverflags(ire, icp, 6, [c.get("mxsens", 30)], [c.get("epsens", 1e-4)])

@@ -113,6 +113,12 @@ void cntc_getpenetration(int *ire, int *icp, double *pen);
 void cntc_getcreepages(int *ire, int *icp, double *vx, double *vy, double *phi);
 /* src/contact_addon.f90:5275-5318 cntc_getContactForces */
 void cntc_getcontactforces(int *ire, int *icp, double *fn, double *tx, double *ty, double *mz);
+/* src/contact_addon.f90:5919-6007 cntc_getSensitivities: sens[iout + lenout*iin] =
+   d output / d input, outputs (Fn, Fx/(mu Fn), Fy/(mu Fn), Mz), inputs (pen, cksi, ceta, cphi),
+   in the caller's units; computed when CNTC_ic_sens is 2 (dFn/dpen) or 3 (also the
+   creepages), with the G=6 solver parameters of cntc_setsolverflags.  Entries beyond 4x4 are
+   left unchanged. */
+void cntc_getsensitivities(int *ire, int *icp, int *lenout, int *lenin, double *sens);
 /* src/contact_addon.f90:5460-5497 cntc_getContactPatchAreas */
 void cntc_getcontactpatchareas(int *ire, int *icp, double *carea, double *harea, double *sarea);
 /* src/contact_addon.f90:5501-5552 cntc_getElementDivision */

@@ -13,8 +13,8 @@
  * sizes are the reference's 2*opt_fft_size(n) for products and 2*cf_mx x 2*cf_my for the
  * preconditioner, so only rounding separates it from the MKL transform.
  *
- * Deviation from the reference (documented in DESIGN.md): steady rolling (T=3) is always
- * solved by GDsteady; the reference's SteadyGS/ConvexGS fallbacks are not restated.
+ * Not restated: ConvexGS (G=2, and the reference's switch to it when contact reaches the
+ * trailing-edge column in steady rolling); such cases return CNTC_err_other.
  * Fatal paths (abort_run) become error codes.
  */
 #include "cntc_oracle.h"
@@ -345,7 +345,16 @@ struct orc_problem {
     int ierror;
     int gd_stagnated;
     int used_stdygs;
+    /* sensitivities sens(iout, iin), column-major as in the reference (outpt%sens) */
+    double sens[16];
+    /* initial estimates of a sensitivity solve: the stopping tests measure the change from
+       them (is_sens, src/m_solvpn.f90:94-97, src/m_solvpt.f90:1910-1914, 2895-2899) */
+    const double *p0[3];
 };
+
+enum { IOUT_FN = 0, IOUT_FX = 1, IOUT_FY = 2, IOUT_MZ = 3 };
+enum { IIN_PEN = 0, IIN_KSI = 1, IIN_ETA = 2, IIN_PHI = 3 };
+#define SENS(P, io, ii) ((P)->sens[(io) + 4 * (ii)])
 
 static void eldiv_alloc(eldiv *e, int npot, int my)
 {
@@ -425,6 +434,19 @@ static double gf_rms(int iigs, const int *el, int n, const double *a, const doub
     if (b)
         for (int i = 0; i < n; i++)
             if (in_mask(iigs, el[i])) { s += b[i] * b[i]; cnt++; }
+    return sqrt(s / (double)(cnt > 1 ? cnt : 1));
+}
+/* rms of (a - a0, b - b0): gf3_copy + gf3_axpy(-1, ps0) + gf3_rms of the is_sens tests */
+static double gf_rms_diff(int iigs, const int *el, int n, const double *a, const double *a0,
+                          const double *b, const double *b0)
+{
+    double s = 0.0;
+    long cnt = 0;
+    for (int i = 0; i < n; i++)
+        if (in_mask(iigs, el[i])) { s += sqr(a[i] + -1.0 * a0[i]); cnt++; }
+    if (b)
+        for (int i = 0; i < n; i++)
+            if (in_mask(iigs, el[i])) { s += sqr(b[i] + -1.0 * b0[i]); cnt++; }
     return sqrt(s / (double)(cnt > 1 ? cnt : 1));
 }
 static void gf_proj_avg(const int *el, int n, double *a)     /* AllInt, src/m_gridfunc.f90:1325-1370 */
@@ -829,6 +851,9 @@ void orc_default_input(orc_input *inp)
     inp->d_cns = 1.0;
     inp->d_slp = 1.0;
     inp->pow_s = sp(0.90f);
+    inp->sens = 1;          /* src/m_hierarch_data.f90:981 */
+    inp->mxsens = 30;       /* src/m_hierarch_data.f90:1733-1735 */
+    inp->epsens = 1e-4;
 }
 
 /* gd_meth/kdown as derived by cntc_setsolverflags for G=5 (src/contact_addon.f90:1240-1247);
@@ -932,7 +957,9 @@ static int normcg(orc_problem *P, int ic_norm, int maxcg, double eps, const doub
             if (el[i] >= ORC_ADHES) ps[i] += alpha * v[i];
         rms_upd = fabs(alpha) * gf_rms(ALLINT, el, npot, v, NULL);
         if (itcg == 1) rms_upd1 = rms_upd;
-        if (itcg <= 3 || itcg % 10 == 0) rms_xk = gf_rms(ALLINT, el, npot, ps, NULL);
+        if (itcg <= 3 || itcg % 10 == 0)
+            rms_xk = P->p0[2] ? gf_rms_diff(ALLINT, el, npot, ps, P->p0[2], NULL, NULL)
+                              : gf_rms(ALLINT, el, npot, ps, NULL);
         memcpy(r_prv, res, sizeof(double) * npot);
 
         if (itinn < numinn && rms_upd >= eps * rms_xk) {
@@ -1005,6 +1032,37 @@ done:
     return ret;
 }
 
+/* sens_norm, src/m_solvpn.f90:531-650: dFn/dpen by one NormCG at a perturbed penetration
+   (N=0, maxit mxsens, eps epsens), started from the converged pressures and element
+   division, which are restored afterwards */
+static int sens_norm(orc_problem *P, const orc_input *inp, const double *hstot, const double *unn)
+{
+    int npot = P->npot, *el = P->igs.el;
+    double *ps = P->ps[2];
+    double pentru = -1e20;                          /* gf3_max(AllInt, unn), :571 */
+    for (int i = 0; i < npot; i++)
+        if (el[i] >= ORC_ADHES && unn[i] > pentru) pentru = unn[i];
+    double dpen = 0.0067 * pentru;
+    double penloc = P->pen + dpen;
+    int *el0 = ialloc(npot);
+    double *ps0 = dalloc(npot);
+    memcpy(el0, el, sizeof(int) * npot);
+    memcpy(ps0, ps, sizeof(double) * npot);
+    P->p0[2] = ps0;
+    int it = 0;
+    double err = 0.0;
+    int rc = normcg(P, 0, inp->mxsens, inp->epsens, hstot, &penloc, P->fntrue, &it, &err);
+    P->p0[2] = NULL;
+    double fnloc = P->dxdy * gf_sum(ALLELM, el, npot, ps);
+    SENS(P, IOUT_FN, IIN_PEN) = (fnloc - P->fntrue) / dpen;
+    memcpy(el, el0, sizeof(int) * npot);
+    memcpy(ps, ps0, sizeof(double) * npot);
+    areas(&P->igs, P->mx, P->my);
+    free(el0);
+    free(ps0);
+    return rc;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* snorm, src/m_snorm.f90:31-378                                                                */
 /* ------------------------------------------------------------------------------------------ */
@@ -1075,6 +1133,16 @@ static int snorm(orc_problem *P, const orc_input *inp, int *itnorm_out)
     P->itcg = itcg;
     areas(&P->igs, P->mx, P->my);
     if (inp->norm == 0) P->fntrue = P->dxdy * gf_sum(ALLELM, el, npot, ps);
+    /* sensitivity dFn/dpen (:357-368); below 700 elements unn is formed here (:359-361) */
+    if (!ret && inp->sens >= 2) {
+        if (!use_vecfft) {
+            double *U3[3] = {NULL, NULL, unn};
+            vecaijpj(P, el, ALLELM, U3, 3, P->ps, 3, &P->cs);
+        }
+        ret = sens_norm(P, inp, hstot, unn);
+    } else {
+        SENS(P, IOUT_FN, IIN_PEN) = 0.0;
+    }
     free(hstot); free(htang); free(tmp); free(unn);
     *itnorm_out = itnorm;
     return ret;
@@ -1801,7 +1869,9 @@ static int stdygs(orc_problem *P, double **ws, const double *mus, int k, double 
             }
         }
         dif = sqrt(dif / (2.0 * k));
-        difid = eps * dmax(1e-6, facnel * gf_rms(ALLELM, el, npot, px, py));
+        double rms = P->p0[0] ? gf_rms_diff(ALLELM, el, npot, px, P->p0[0], py, P->p0[1])
+                              : gf_rms(ALLELM, el, npot, px, py);
+        difid = eps * dmax(1e-6, facnel * rms);
         if (itgs == 1) dif1 = dif;
     }
     (void)dif1;
@@ -1930,7 +2000,8 @@ static int tangcg(orc_problem *P, int maxcg, double eps, double **ws, const doub
         dif = alpha * facnel * gf_rms(ALLELM, el, npot, vx, vy);
         difinn += dif;
         if (itcg <= 5 || itcg % 5 == 1) {
-            double ptang = facnel * gf_rms(ALLELM, el, npot, px, py);
+            double ptang = facnel * (P->p0[0] ? gf_rms_diff(ALLELM, el, npot, px, P->p0[0], py, P->p0[1])
+                                              : gf_rms(ALLELM, el, npot, px, py));
             difid = eps * dmax(1e-6, ptang);
             trsinn = sp(0.01f) * ptang;
         }
@@ -2014,6 +2085,164 @@ static int set_facdt(orc_problem *P, int tang)
         }
     }
     return 0;
+}
+
+/* Kalker's creep coefficients C11, C22, C23 (Kalker 1967 / 1990 table, as tabulated in
+   src/m_hertz.f90:1158-1179) for g = 0.1, 0.2, ..., 1.0 and poisson ratios 0, 1/4, 1/2.  Case 0
+   holds g = a/b <= 1, case 1 holds g = b/a < 1.  The reference stores them from single-precision
+   literals, kept here as floats. */
+static const float KALKER_C[2][10][3][3] = {   /* [case][g row][C11, C22, C23][nu] */
+    {{{2.51f, 3.31f, 4.85f}, {2.51f, 2.52f, 2.53f}, {.334f, .473f, .731f}},
+     {{2.59f, 3.37f, 4.81f}, {2.59f, 2.63f, 2.66f}, {.483f, .603f, .809f}},
+     {{2.68f, 3.44f, 4.80f}, {2.68f, 2.75f, 2.81f}, {.607f, .715f, .889f}},
+     {{2.78f, 3.53f, 4.82f}, {2.78f, 2.88f, 2.98f}, {.720f, .823f, .977f}},
+     {{2.88f, 3.62f, 4.83f}, {2.88f, 3.01f, 3.14f}, {.827f, .929f, 1.07f}},
+     {{2.98f, 3.72f, 4.91f}, {2.98f, 3.14f, 3.31f}, {.930f, 1.03f, 1.18f}},
+     {{3.09f, 3.81f, 4.97f}, {3.09f, 3.28f, 3.48f}, {1.03f, 1.14f, 1.29f}},
+     {{3.19f, 3.91f, 5.05f}, {3.19f, 3.41f, 3.65f}, {1.13f, 1.25f, 1.40f}},
+     {{3.29f, 4.01f, 5.12f}, {3.29f, 3.54f, 3.82f}, {1.23f, 1.36f, 1.51f}},
+     {{3.40f, 4.12f, 5.20f}, {3.40f, 3.67f, 3.98f}, {1.33f, 1.47f, 1.63f}}},
+    {{{10.7f, 11.7f, 12.9f}, {10.7f, 12.8f, 16.0f}, {12.2f, 14.6f, 18.0f}},
+     {{6.96f, 7.78f, 8.82f}, {6.96f, 8.14f, 9.79f}, {5.72f, 6.63f, 7.89f}},
+     {{5.57f, 6.34f, 7.34f}, {5.57f, 6.40f, 7.51f}, {3.79f, 4.32f, 5.01f}},
+     {{4.84f, 5.57f, 6.57f}, {4.84f, 5.48f, 6.31f}, {2.88f, 3.24f, 3.70f}},
+     {{4.37f, 5.10f, 6.11f}, {4.37f, 4.90f, 5.56f}, {2.35f, 2.62f, 2.96f}},
+     {{4.06f, 4.78f, 5.80f}, {4.06f, 4.50f, 5.04f}, {2.01f, 2.23f, 2.50f}},
+     {{3.82f, 4.54f, 5.58f}, {3.82f, 4.21f, 4.67f}, {1.76f, 1.95f, 2.18f}},
+     {{3.65f, 4.36f, 5.42f}, {3.65f, 3.99f, 4.39f}, {1.58f, 1.75f, 1.94f}},
+     {{3.51f, 4.22f, 5.30f}, {3.51f, 3.81f, 4.16f}, {1.44f, 1.59f, 1.77f}},
+     {{3.40f, 4.12f, 5.20f}, {3.40f, 3.67f, 3.98f}, {1.33f, 1.47f, 1.63f}}}};
+
+/* linrol, src/m_hertz.f90:1147-1247: asymptotic expressions for slender ellipses (g < 0.101),
+   otherwise linear interpolation in g between table rows and quadratic (Lagrange)
+   interpolation of 1/C in nu */
+void orc_linrol(double nu, double aob, double *cxx, double *cyy, double *cyz)
+{
+    int icase;
+    double g;
+    if (aob <= 1.0) { g = aob; icase = 0; }
+    else { g = 1.0 / aob; icase = 1; }
+    if (icase == 0 && g < 0.101) {
+        *cxx = PI * PI / 4.0 / (1.0 - nu);
+        *cyy = PI * PI / 4.0;
+        *cyz = PI * sqrt(g) / 3.0 / (1.0 - nu) * (1.0 + nu * (log(16.0 / g) - 5.0));
+    } else if (icase == 1 && g < 0.101) {
+        double al = log(16.0 / (g * g));
+        *cxx = 2.0 * PI / (al - 2.0 * nu) / g * (1.0 + (3.0 - log(4.0)) / (al - 2.0 * nu));
+        *cyy = 2.0 * PI / g * (1.0 + (1.0 - nu) * (3.0 - log(4.0)) / ((1.0 - nu) * al + 2.0 * nu)) /
+               ((1.0 - nu) * al + 2.0 * nu);
+        *cyz = 2.0 * PI / 3.0 / pow(g, 1.5) / ((1.0 - nu) * al - 2.0 + 4.0 * nu);
+    } else {
+        int irow = (int)(10.0 * g - 0.005);        /* 1-based row below g */
+        double fac = 10.0 * g - (double)irow;
+        double c[3][3], cc[3];
+        for (int inu = 0; inu < 3; inu++)
+            for (int ic = 0; ic < 3; ic++)
+                c[inu][ic] = fac * (double)KALKER_C[icase][irow][ic][inu] +
+                             (1.0 - fac) * (double)KALKER_C[icase][irow - 1][ic][inu];
+        for (int ic = 0; ic < 3; ic++)
+            cc[ic] = (nu - 0.25) * (nu - 0.50) * 8.0 / c[0][ic] - (nu - 0.50) * nu * 16.0 / c[1][ic] +
+                     nu * (nu - 0.25) * 8.0 / c[2][ic];
+        *cxx = 1.0 / cc[0];
+        *cyy = 1.0 / cc[1];
+        *cyz = 1.0 / cc[2];
+    }
+}
+
+/* char_creep, src/m_solvpt.f90:497-540: creepages that give a force of order mu*Fn in the
+   linear theory, for the effective radius and half-axes of the current contact area */
+static void char_creep(orc_problem *P, double *ch_ksi, double *ch_eta, double *ch_phi)
+{
+    int nadh, nslip, nplast, nexter;
+    count_el(P->igs.el, P->npot, &nadh, &nslip, &nplast, &nexter);
+    int ncon = nadh + nslip + nplast;
+    double c = sqrt((double)(float)ncon * P->dx * P->dy / PI);
+    areas(&P->igs, P->mx, P->my);
+    double a = (double)imax(1, P->igs.ixmax - P->igs.ixmin + 1) * P->dx * 0.5;
+    double b = (double)imax(1, P->igs.iymax - P->igs.iymin + 1) * P->dy * 0.5;
+    double c11, c22, c23;
+    orc_linrol(P->nu, a / b, &c11, &c22, &c23);
+    *ch_ksi = P->fstat * P->fntrue / (c * c * P->ga * c11);
+    *ch_eta = P->fstat * P->fntrue / (c * c * P->ga * c22);
+    *ch_phi = P->fstat * P->fntrue / (c * c * c * P->ga * c23);
+}
+
+/* sens_tang, src/m_solvpt.f90:544-796: d(Fx, Fy)/(mu Fn) and dMz per creepage by one
+   tang_solver call each (is_sens: maxit mxsens, eps epsens; src/m_solvpt.f90:382-493) on the
+   fixed right-hand side plus the creepage change, started from the converged solution, which
+   is restored after each */
+static int sens_tang(orc_problem *P, const orc_input *inp, double **ws, const double *mus, int k,
+                     double omegah, double omegas, const gdpar *G)
+{
+    int npot = P->npot, *el = P->igs.el;
+    double dxdy = P->dxdy;
+    double den = P->muscal * P->fntrue;
+    double fxrel = dxdy * gf_sum(ALLELM, el, npot, P->ps[0]) / den;
+    double fyrel = dxdy * gf_sum(ALLELM, el, npot, P->ps[1]) / den;
+    double sy = 0.0, sx = 0.0;
+    for (int i = 0; i < npot; i++) sy += P->ps[0][i] * P->y[i];
+    for (int i = 0; i < npot; i++) sx += P->ps[1][i] * P->x[i];
+    double mztrue = dxdy * (-sy + sx);
+    double ch[3];
+    char_creep(P, &ch[0], &ch[1], &ch[2]);
+    int *el0 = ialloc(npot);
+    double *save[4], *wsloc[2] = {dalloc(npot), dalloc(npot)};
+    double *st[4] = {P->ps[0], P->ps[1], P->ss[0], P->ss[1]};
+    for (int q = 0; q < 4; q++) { save[q] = dalloc(npot); memcpy(save[q], st[q], sizeof(double) * npot); }
+    memcpy(el0, el, sizeof(int) * npot);
+    int stagn0 = P->gd_stagnated, used0 = P->used_stdygs, ret = 0;
+    for (int icrp = 0; icrp < 3; icrp++) {
+        double dcrp = 0.0001 * ch[icrp];
+        for (int i = 0; i < npot; i++) { wsloc[0][i] = ws[0][i]; wsloc[1][i] = ws[1][i]; }
+        for (int i = 0; i < npot; i++) {
+            if (el[i] < ORC_ADHES) continue;
+            double f = P->facdt[i];
+            if (icrp == 0) wsloc[0][i] = wsloc[0][i] + f * dcrp * P->dq;
+            else if (icrp == 1) wsloc[1][i] = wsloc[1][i] + f * dcrp * P->dq;
+            else {
+                wsloc[0][i] = wsloc[0][i] - f * dcrp * P->y[i] * P->dq;
+                wsloc[1][i] = wsloc[1][i] + f * dcrp * P->x[i] * P->dq;
+            }
+        }
+        P->p0[0] = save[0];
+        P->p0[1] = save[1];
+        int it = 0, rc;
+        double err = 0.0;
+        areas(&P->igs, P->mx, P->my);
+        if (inp->tang == 3 && inp->gausei == 5) {
+            int lstagn = 0;
+            rc = gdsteady(P, G, inp->mxsens, inp->epsens, wsloc, mus, &it, &err, &lstagn);
+            if (!rc && lstagn) rc = stdygs(P, wsloc, mus, k, inp->epsens, inp->mxsens, omegah, omegas, &it, &err);
+        } else if (inp->tang == 3) {
+            rc = stdygs(P, wsloc, mus, k, inp->epsens, inp->mxsens, omegah, omegas, &it, &err);
+        } else {
+            rc = tangcg(P, inp->mxsens, inp->epsens, wsloc, mus, &it, &err);
+        }
+        P->p0[0] = P->p0[1] = NULL;
+        if (rc) { ret = rc; }
+        else {
+            double fxloc = dxdy * gf_sum(ALLELM, el, npot, P->ps[0]) / den;
+            double fyloc = dxdy * gf_sum(ALLELM, el, npot, P->ps[1]) / den;
+            double a = 0.0, b = 0.0;
+            for (int i = 0; i < npot; i++) a += P->ps[0][i] * P->y[i];
+            for (int i = 0; i < npot; i++) b += P->ps[1][i] * P->x[i];
+            double mzloc = dxdy * (-a + b);
+            SENS(P, IOUT_FX, IIN_KSI + icrp) = (fxloc - fxrel) / dcrp;
+            SENS(P, IOUT_FY, IIN_KSI + icrp) = (fyloc - fyrel) / dcrp;
+            SENS(P, IOUT_MZ, IIN_KSI + icrp) = (mzloc - mztrue) / dcrp;
+        }
+        memcpy(el, el0, sizeof(int) * npot);
+        for (int q = 0; q < 4; q++) memcpy(st[q], save[q], sizeof(double) * npot);
+        areas(&P->igs, P->mx, P->my);
+        if (ret) break;
+    }
+    P->gd_stagnated = stagn0;
+    P->used_stdygs = used0;
+    free(el0);
+    for (int q = 0; q < 4; q++) free(save[q]);
+    free(wsloc[0]);
+    free(wsloc[1]);
+    return ret;
 }
 
 static int stang(orc_problem *P, const orc_input *inp, int *ittang_out)
@@ -2130,6 +2359,8 @@ static int stang(orc_problem *P, const orc_input *inp, int *ittang_out)
     if (!zready && !ret) ittang = -1;
     P->itgs = itgs;
     *ittang_out = ittang;
+    /* sensitivities to the creepages, src/m_stang.f90:724-731 */
+    if (!ret && inp->sens >= 3) ret = sens_tang(P, inp, ws, mus, k, omegah, omegas, &G);
     free(mus); free(ws[0]); free(ws[1]); free(usn[0]); free(usn[1]); free(uvn[0]); free(uvn[1]);
     free(uvt[0]); free(uvt[1]); free(tmp[0]); free(tmp[1]);
     return ret;
@@ -2380,6 +2611,14 @@ int orc_calculate(orc_problem *P, const orc_input *inp)
     for (int i = 0; i < npot; i++)
         if (P->hs[2][i] > sp(1e29f)) P->igs.el[i] = ORC_EXTER;
     areas(&P->igs, mx, my);
+    /* initial sensitivities (:753-764); eldiv0's dFn/dpen estimate for N=1 is always replaced
+       in snorm (src/m_snorm.f90:357-368) and is not formed */
+    if (inp->iestim == 0)
+        for (int io = IOUT_FX; io <= IOUT_MZ; io++) SENS(P, io, IIN_PEN) = 0.0;
+    if (inp->iestim == 0 && (inp->norm != 1 || ihertz != 0)) SENS(P, IOUT_FN, IIN_PEN) = 0.0;
+    if (inp->iestim == 0 || inp->iestim == 2)
+        for (int ii = IIN_KSI; ii <= IIN_PHI; ii++)
+            for (int io = IOUT_FN; io <= IOUT_MZ; io++) SENS(P, io, ii) = 0.0;
 
     /* panprc, src/m_scontc.f90:356-553 */
     int maxout = fabs(P->ak) < 1e-4 ? 1 : 10;
@@ -2479,3 +2718,4 @@ void orc_get_iters(const orc_problem *p, int *itnorm, int *ittang, int *itcg, in
     *itnorm = p->itnorm; *ittang = p->ittang; *itcg = p->itcg; *itgs = p->itgs;
 }
 int orc_get_stagnated(const orc_problem *p) { return p->gd_stagnated + 2 * p->used_stdygs; }
+void orc_get_sens(const orc_problem *p, double *sens) { memcpy(sens, p->sens, sizeof p->sens); }

@@ -56,6 +56,10 @@ typedef struct orc_input {
     int kdowfb;
     /* relaxation of SteadyGS for G=3 (G=0, 4, 5 use the reference's defaults) */
     double omegah, omegas;
+    /* sensitivities: S2 digit (CNTC_ic_sens, src/contact_addon.f90:844-846) and the G=6
+       solver parameters (src/contact_addon.f90:1266-1272) */
+    int sens, mxsens;
+    double epsens;
 } orc_input;
 
 /* fills an input record with the reference's module-3 defaults */
@@ -87,6 +91,9 @@ void orc_get_iters(const orc_problem *p, int *itnorm, int *ittang, int *itcg, in
 /* bit 0: the last GDsteady call ended in stagnation (src/gdsteady.f90:560-585);
    bit 1: SteadyGS ran in the last case */
 int  orc_get_stagnated(const orc_problem *p);
+/* sensitivity matrix in internal units, sens[iout + 4*iin] with outputs (fn, fx/(mu fn),
+   fy/(mu fn), mz) and inputs (pen, cksi, ceta, cphi), src/m_hierarch_data.f90:122-124 */
+void orc_get_sens(const orc_problem *p, double *sens);
 
 /* stand-alone building blocks, exposed for known-answer tests */
 int  orc_opt_fft_size(int n);
@@ -101,6 +108,9 @@ void orc_conv_block(int mx, int my, const double *cfblock, const double *p, doub
 void orc_make_prec_block(int mx, int my, double ga, const double *cfblock, double *mblock);
 /* complete elliptic integrals B(m), D(m) with mc = 1-m (Carlson duplication) */
 void orc_ellip_bd(double mc, double *b, double *d);
+/* Kalker's linear-theory creep coefficients C11, C22, C23 at a/b and poisson ratio nu,
+   src/m_hertz.f90:1147-1247 */
+void orc_linrol(double nu, double aob, double *cxx, double *cyy, double *cyz);
 /* 1-D complex DFT of any length (forward sign -1), for checks against numpy */
 void orc_fft(int n, const double *re_in, const double *im_in, double *re_out, double *im_out,
              int sign);

@@ -38,6 +38,7 @@ class OrcInput(ctypes.Structure):
         ("d_lin", ctypes.c_double), ("d_cns", ctypes.c_double), ("d_slp", ctypes.c_double),
         ("pow_s", ctypes.c_double), ("kdowfb", ctypes.c_int),
         ("omegah", ctypes.c_double), ("omegas", ctypes.c_double),
+        ("sens", ctypes.c_int), ("mxsens", ctypes.c_int), ("epsens", ctypes.c_double),
     ]
 
 
@@ -74,6 +75,8 @@ def lib():
         L.orc_get_iters.argtypes = [vp, ip, ip, ip, ip]
         L.orc_get_stagnated.argtypes = [vp]
         L.orc_get_stagnated.restype = ctypes.c_int
+        L.orc_get_sens.argtypes = [vp, dp]
+        L.orc_linrol.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp, dp]
         L.orc_opt_fft_size.argtypes = [ctypes.c_int]
         L.orc_opt_fft_size.restype = ctypes.c_int
         L.orc_elascf_pcwcns.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int,
@@ -164,10 +167,23 @@ class Problem:
     def stagnated(self):
         return lib().orc_get_stagnated(self._p)
 
+    def sens(self):
+        """4x4 sensitivities in internal units, [iout, iin]: outputs (fn, fx/(mu fn),
+        fy/(mu fn), mz), inputs (pen, cksi, ceta, cphi)."""
+        s = np.zeros(16)
+        lib().orc_get_sens(self._p, _dptr(s))
+        return s.reshape(4, 4).T.copy()
+
     def iters(self):
         v = [ctypes.c_int() for _ in range(4)]
         lib().orc_get_iters(self._p, *[ctypes.byref(x) for x in v])
         return tuple(x.value for x in v)
+
+
+def linrol(nu, aob):
+    v = [ctypes.c_double() for _ in range(3)]
+    lib().orc_linrol(nu, aob, *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
 
 
 def opt_fft_size(n):
@@ -263,6 +279,9 @@ def case_input(c):
     if c.get("dq") is not None:
         inp.dq = c["dq"]
     inp.cksi, inp.ceta, inp.cphi = c["cksi"], c["ceta"], c["cphi"]
+    inp.sens = c.get("sens", 1)
+    inp.mxsens = c.get("mxsens", 30)
+    inp.epsens = c.get("epsens", 1e-4)
     return inp, h
 
 
