@@ -127,12 +127,13 @@ def device_sync():
         raise RuntimeError("hipDeviceSynchronize failed: %d" % rc)
 
 
-def serve_leg(cntc, W, seconds, rate, nchan=64, seed=77, ire0=801):
+def serve_leg(cntc, W, seconds, rate, nchan=64, seed=77, ire0=801, sens_frac=0.1):
     """Serving: `nchan` wheel channels with fixed grids from the batch's size distribution,
     each stepping T=2 transient rolling (dq = dx, warm start from its previous tractions),
     requests as Poisson arrivals at `rate` per second on random channels with creepages as
-    seeded random walks.  A channel's request waits for the channel's previous one.
-    Latency = arrival to result."""
+    seeded random walks.  A fraction `sens_frac` of the requests also asks the sensitivities
+    (CNTC_ic_sens = 3: one extra NormCG and three extra tangential solves).  A channel's
+    request waits for the channel's previous one.  Latency = arrival to result."""
     rng = np.random.default_rng(seed)
     chans = W.hertz_batch(n=nchan, seed=seed)
     ids = [(ire0 + k // 9, 1 + k % 9) for k in range(nchan)]
@@ -153,9 +154,10 @@ def serve_leg(cntc, W, seconds, rate, nchan=64, seed=77, ire0=801):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
     chan_of = rng.integers(0, nchan, n_req)
     walk = rng.normal(0.0, 2e-4, (n_req, 2))
+    want_sens = rng.random(n_req) < sens_frac
     waiting = [[] for _ in range(nchan)]
     inflight = {}
-    lat, t0, nxt, done = [], time.perf_counter(), 0, 0
+    lat, is_sens, t0, nxt, done = [], [], time.perf_counter(), 0, 0
 
     def start(ch, t_arr, r):
         c = state[ch]
@@ -163,10 +165,11 @@ def serve_leg(cntc, W, seconds, rate, nchan=64, seed=77, ire0=801):
         c["ceta"] = float(np.clip(c["ceta"] + walk[r, 1], -0.005, 0.005))
         ire, icp = ids[ch]
         cntc.setcreepages(ire, icp, c["cksi"], c["ceta"], c["cphi"])
+        cntc.setflags(ire, icp, [1988], [3 if want_sens[r] else 1])
         t = cntc.submit(ire, icp)
         if t <= 0:
             raise SystemExit("serving submit failed: %d" % t)
-        inflight[t] = (ch, t_arr)
+        inflight[t] = (ch, t_arr, r)
 
     while done < n_req:
         now = time.perf_counter() - t0
@@ -182,10 +185,15 @@ def serve_leg(cntc, W, seconds, rate, nchan=64, seed=77, ire0=801):
             d, rc, _ = cntc.poll(t)
             if d != 1:
                 continue
-            ch, t_arr = inflight.pop(t)
+            ch, t_arr, r = inflight.pop(t)
             if rc != 0:
                 raise SystemExit("serving request failed: %d" % rc)
             lat.append(time.perf_counter() - t0 - t_arr)
+            is_sens.append(bool(want_sens[r]))
+            if want_sens[r]:
+                S = cntc.getsensitivities(*ids[ch])
+                if not (np.all(np.isfinite(S)) and S[0, 0] > 0.0 and S[1, 1] < 0.0):
+                    raise SystemExit("serving sensitivities out of range: %s" % S)
             done += 1
             if waiting[ch]:
                 ta, r = waiting[ch].pop(0)
@@ -193,10 +201,14 @@ def serve_leg(cntc, W, seconds, rate, nchan=64, seed=77, ire0=801):
         time.sleep(0.0002)
     wall = time.perf_counter() - t0
     lat = np.array(lat)
-    return {"workload": "%d channels, T=2 transient rolling, Poisson arrivals" % nchan,
+    is_sens = np.array(is_sens)
+    pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 2) if len(a) else None
+    return {"workload": "%d channels, T=2 transient rolling, Poisson arrivals, %d%% with "
+                        "sensitivities" % (nchan, round(100 * sens_frac)),
             "offered_rate": rate, "requests": n_req, "achieved_rate": round(n_req / wall, 1),
-            "latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
-            "latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "latency_ms_p50": pct(lat, 50), "latency_ms_p99": pct(lat, 99),
+            "sens_requests": int(is_sens.sum()),
+            "sens_latency_ms_p50": pct(lat[is_sens], 50), "sens_latency_ms_p99": pct(lat[is_sens], 99),
             "elements_per_channel_mean": round(float(np.mean([c["mx"] * c["my"] for c in chans])), 1)}
 
 
