@@ -39,6 +39,7 @@ _PROTOS = {
     "cntc_setpenetration": [_pi, _pi, _pd],
     "cntc_setnormalforce": [_pi, _pi, _pd],
     "cntc_setundeformeddistc": [_pi, _pi, _pi, _pi, _pd],
+    "cntc_settangentialforces": [_pi, _pi, _pd, _pd],
     "cntc_setcreepages": [_pi, _pi, _pd, _pd, _pd],
     "cntc_calculate": [_pi, _pi, _pi],
     "cntc_calculatebatch": [_pi, _pi, _pi, _pi],
@@ -181,6 +182,11 @@ def setundeformeddistc(ire, icp, ibase, prmudf):
     lib().cntc_setundeformeddistc(_i(ire), _i(icp), _i(ibase), _i(len(ra)), rap)
 
 
+def settangentialforces(ire, icp, fx, fy):
+    """Fx, Fy relative to mu*Fn, for the F digit 1 or 2 (python_intfc/cntc_settangentialforces.py)."""
+    lib().cntc_settangentialforces(_i(ire), _i(icp), _d(fx), _d(fy))
+
+
 def setcreepages(ire, icp, vx, vy, phi):
     lib().cntc_setcreepages(_i(ire), _i(icp), _d(vx), _d(vy), _d(phi))
 
@@ -254,6 +260,13 @@ def getpenetration(ire=1, icp=1):
     v = c_double(0)
     lib().cntc_getpenetration(_i(ire), _i(icp), ctypes.byref(v))
     return v.value
+
+
+def getcreepages(ire=1, icp=1):
+    """(cksi, ceta, cphi) of the last case: with the F digit 1 or 2 the solved creepages."""
+    v = [c_double(0) for _ in range(3)]
+    lib().cntc_getcreepages(_i(ire), _i(icp), *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
 
 
 def getcontactforces(ire=1, icp=1):

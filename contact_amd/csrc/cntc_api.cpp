@@ -170,6 +170,7 @@ void cntc_setflags(int* ire, int* icp, int* lenflg, int* params, int* values)
         case CNTC_ic_iestim: in.iestim = std::max(0, std::min(3, v)); break;
         case CNTC_if_precision: in.precision = v ? 1 : 0; break;
         case CNTC_ic_sens: in.sens = std::max(0, std::min(3, v)); break;   // :844-846
+        case CNTC_ic_force: in.force3 = std::max(0, std::min(2, v)); break;  // :834-842, module 3
         default: break;   // output, flow, file flags: no effect on the solution
         }
     }
@@ -358,6 +359,23 @@ void cntc_setundeformeddistc(int* ire, int* icp, int* ibase, int* nparam, double
 }
 
 // src/contact_addon.f90:2491-2530
+// src/contact_addon.f90:2597-2650: Fx, Fy as fractions of mu*Fn, scaled to |(Fx, Fy)| <= 1; used
+// when the F digit is 1 (Fx) or 2 (Fx and Fy)
+void cntc_settangentialforces(int* ire, int* icp, double* fx, double* fy)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    CaseInput& in = P->patch.in;
+    in.fxrel = *fx;
+    in.fyrel = *fy;
+    double fabs = std::sqrt(*fx * *fx + *fy * *fy);
+    if (fabs > 1.0) {
+        in.fxrel = in.fxrel / fabs;
+        in.fyrel = in.fyrel / fabs;
+    }
+}
+
 void cntc_setcreepages(int* ire, int* icp, double* vx, double* vy, double* phi)
 {
     std::lock_guard<std::mutex> g(store().mtx);

@@ -1991,8 +1991,10 @@ static GsItem gs_item(Patch& P, int k, double omegah, double omegas, int maxgs, 
 
 // SteadyGS, src/m_solvpt.f90:2825-3254: all sweeps in one launch per patch.  p0x, p0y: initial
 // estimate of a sensitivity solve (is_sens, :2895-2899, 3138-3144); null otherwise.
+// info: 0, or 1 when maxgs was reached, 2 when also the mean rate of convergence exceeds 0.997,
+// 3 when it exceeds 1 (:3205-3234).
 static Task stdygs(Ctx C, int k, double omegah, double omegas, int maxgs, double eps, int* itgs_out,
-                   double* err_out, const double* p0x = nullptr, const double* p0y = nullptr)
+                   double* err_out, int* info, const double* p0x = nullptr, const double* p0y = nullptr)
 {
     Patch& P = *C.P;
     AnyItem it;
@@ -2001,10 +2003,60 @@ static Task stdygs(Ctx C, int k, double omegah, double omegas, int maxgs, double
     it.gs.p0y = p0y;
     if (op_lds(OP_STDYGS, &it.gs, 0) > MAX_LDS && op_lds(OP_STDYGS, &it.gs, 1) > MAX_LDS) co_return ERR_OTHER;
     Res r = co_await C.opr(OP_STDYGS, it);
-    *itgs_out = (int)r.v[0];
-    *err_out = r.v[1];
+    int itgs = (int)r.v[0];
+    double dif = r.v[1], dif1 = r.v[5];
+    double conv = 1.0;
+    if (dif * dif1 != 0.0 && itgs > 1) conv = std::exp(std::log(dif / dif1) / (itgs - 1));
+    *info = 0;
+    if (itgs >= maxgs) *info = 1;
+    if (itgs >= maxgs && conv > sp(0.997f)) *info = 2;
+    if (itgs >= maxgs && conv > 1.0) *info = 3;
+    *itgs_out = itgs;
+    *err_out = dif;
     P.out.used_stdygs = 1;
     co_return 0;
+}
+
+// tang_solver, src/m_solvpt.f90:382-493: GDsteady for T=3 with G=5, restarted by SteadyGS when it
+// stagnates (:466-485); SteadyGS for other G in steady rolling; TangCG for T=1, 2.  *it counts
+// the iterations of both solvers of a fall-back; *info is set by SteadyGS only.  p0x, p0y: the
+// initial estimate of a sensitivity solve (is_sens), null otherwise.
+static Task tang_solver(Ctx C, int k, const double cf00[4], double omegah, double omegas, int maxit,
+                        double eps, int* it_out, double* err_out, int* info,
+                        const double* p0x = nullptr, const double* p0y = nullptr)
+{
+    Patch& P = *C.P;
+    int it = 0, rc;
+    if (P.in.tang == 3 && P.in.gausei == 5) {
+        int lstagn = 0;
+        rc = co_await gdsteady(C, cf00, maxit, eps, &it, err_out, &lstagn);
+        P.out.gd_stagnated = lstagn;
+        *it_out = std::abs(it);
+        if (!rc && lstagn) {
+            rc = co_await stdygs(C, k, omegah, omegas, maxit, eps, &it, err_out, info, p0x, p0y);
+            *it_out += std::abs(it);
+        }
+    } else if (P.in.tang == 3) {
+        rc = co_await stdygs(C, k, omegah, omegas, maxit, eps, &it, err_out, info, p0x, p0y);
+        *it_out = std::abs(it);
+    } else {
+        rc = co_await tangcg(C, cf00, maxit, eps, &it, err_out, p0x, p0y);
+        *it_out = std::abs(it);
+    }
+    co_return rc;
+}
+
+// k_sens mode 1 on the interior: ws = wb + d dq along x (which 0) or y (which 1), or the spin
+// change -d y dq, +d x dq (which 2)
+static void add_rigid_slip(Ctx& C, double* wbx, double* wby, double* wsx, double* wsy, int which, double d)
+{
+    Patch& P = *C.P;
+    AnyItem wi;
+    SensItem& q = wi.sens;
+    q.n = P.n; q.mx = P.mx; q.mode = 1; q.which = which;
+    q.el = P.el; q.wbx = wbx; q.wby = wby; q.wsx = wsx; q.wsy = wsy;
+    q.xc1 = P.xc1; q.yc1 = P.yc1; q.dx = P.dx; q.dy = P.dy; q.dq = P.dq; q.d = d;
+    C.op(OP_SENS, wi);
 }
 
 static SetupItem setup_item(Patch& P)
@@ -2110,32 +2162,27 @@ static Task sens_tang(Ctx C, int k, const double cf00[4], double omegah, double 
     ch[0] = P.in.fstat * P.fntrue / (c * c * P.ga * c11);
     ch[1] = P.in.fstat * P.fntrue / (c * c * P.ga * c22);
     ch[2] = P.in.fstat * P.fntrue / (c * c * c * P.ga * c23);
-    // saved state: element division, tractions and slip; the fixed right-hand side
+    // saved state: element division, tractions and slip.  The right-hand side of the solves
+    // is wstot = wsfix1 plus, when forces are prescribed, the creepages solvpt adds (:600-614).
     int8_t* el0 = (int8_t*)P.w[20];
     double *p0x = P.w[15], *p0y = P.w[16], *s0x = P.w[17], *s0y = P.w[18];
-    double *wbx = P.w[22], *wby = P.w[23];
-    snap(C, P.el, el0, {P.px, P.py, P.sx, P.sy, P.wsx, P.wsy}, {p0x, p0y, s0x, s0y, wbx, wby});
+    double *wbx = P.w[19], *wby = P.w[21];
+    const int F = P.in.force3;
+    snap(C, P.el, el0, {P.px, P.py, P.sx, P.sy}, {p0x, p0y, s0x, s0y});
+    if (F == 0) {
+        snap(C, nullptr, nullptr, {P.wsx, P.wsy}, {wbx, wby});
+    } else {
+        add_rigid_slip(C, P.w[22], P.w[23], wbx, wby, 0, P.in.cksi);
+        if (F >= 2) add_rigid_slip(C, wbx, wby, wbx, wby, 1, P.in.ceta);
+    }
     int stagn0 = P.out.gd_stagnated, used0 = P.out.used_stdygs, ret = 0;
     for (int icrp = 0; icrp < 3 && !ret; icrp++) {
         double dcrp = 0.0001 * ch[icrp];
-        AnyItem wi;
-        SensItem& q = wi.sens;
-        q.n = P.n; q.mx = P.mx; q.mode = 1; q.which = icrp;
-        q.el = P.el; q.wbx = wbx; q.wby = wby; q.wsx = P.wsx; q.wsy = P.wsy;
-        q.xc1 = P.xc1; q.yc1 = P.yc1; q.dx = P.dx; q.dy = P.dy; q.dq = P.dq; q.d = dcrp;
-        C.op(OP_SENS, wi);
-        int it = 0, rc;
+        add_rigid_slip(C, wbx, wby, P.wsx, P.wsy, icrp, dcrp);
+        int it = 0, info = 0, rc;
         double err = 0.0;
-        if (P.in.tang == 3 && P.in.gausei == 5) {
-            int lstagn = 0;
-            rc = co_await gdsteady(C, cf00, P.in.mxsens, P.in.epsens, &it, &err, &lstagn);
-            if (!rc && lstagn)
-                rc = co_await stdygs(C, k, omegah, omegas, P.in.mxsens, P.in.epsens, &it, &err, p0x, p0y);
-        } else if (P.in.tang == 3) {
-            rc = co_await stdygs(C, k, omegah, omegas, P.in.mxsens, P.in.epsens, &it, &err, p0x, p0y);
-        } else {
-            rc = co_await tangcg(C, cf00, P.in.mxsens, P.in.epsens, &it, &err, p0x, p0y);
-        }
+        rc = co_await tang_solver(C, k, cf00, omegah, omegas, P.in.mxsens, P.in.epsens, &it, &err, &info,
+                                  p0x, p0y);
         if (rc) {
             ret = rc;
         } else {
@@ -2152,6 +2199,113 @@ static Task sens_tang(Ctx C, int k, const double cf00[4], double omegah, double 
     P.out.gd_stagnated = stagn0;
     P.out.used_stdygs = used0;
     co_return ret;
+}
+
+// solvpt, src/m_solvpt.f90:51-378.  F=0: one tang_solver call on wsfix1.  F=1 (Fx prescribed)
+// or F=2 (Fx and Fy): wstot = wsfix1 + facdt (cksi, ceta) dq, then Newton-Raphson on cksi (and
+// ceta) with the sensitivities d(Fx, Fy)/(mu Fn) per creepage, refreshed from each half step.
+// The creepages are the problem's kinematic state and keep their final values.
+static Task solvpt(Ctx C, int k, const double cf00[4], double omegah, double omegas, int* itgs_out,
+                   double* err_out)
+{
+    Patch& P = *C.P;
+    const int F = P.in.force3;
+    double* wfx = P.w[22];
+    double* wfy = P.w[23];
+    if (F >= 1) {
+        add_rigid_slip(C, wfx, wfy, P.wsx, P.wsy, 0, P.in.cksi);
+        if (F >= 2) add_rigid_slip(C, P.wsx, P.wsy, P.wsx, P.wsy, 1, P.in.ceta);
+    }
+    const double den = P.muscal * P.fntrue;
+    AnyItem fi;
+    fi.setup = setup_item(P);
+    int itgs = 0, info1 = 0, it = 0, rc;
+    double err = 0.0;
+    rc = co_await tang_solver(C, k, cf00, omegah, omegas, P.in.maxgs, P.in.eps, &it, &err, &info1);
+    itgs += it;
+    if (!rc && F >= 1 && info1 <= 1) {
+        Res f = co_await C.opr(OP_FORCES, fi);
+        double fxkp1 = P.dxdy * f.v[1] / den, fykp1 = P.dxdy * f.v[2] / den;
+        Res ca = co_await C.count(M_ADH);
+        Res ci = co_await C.count(M_INT);
+        double nadh = ca.v[0], ncon = ci.v[0];
+        int itnr = 0;
+        double df = std::fabs(P.in.fxrel - fxkp1);
+        if (F >= 2) df = df + std::fabs(P.in.fyrel - fykp1);
+        double& s_xk = sens_at(P, IOUT_FX, IIN_KSI);
+        double& s_yk = sens_at(P, IOUT_FY, IIN_KSI);
+        double& s_xe = sens_at(P, IOUT_FX, IIN_ETA);
+        double& s_ye = sens_at(P, IOUT_FY, IIN_ETA);
+        while (!rc && df > P.in.eps && itnr < P.in.maxnr && info1 <= 1) {
+            itnr++;
+            double dfx = P.in.fxrel - fxkp1, dfy = P.in.fyrel - fykp1, dcksi = 0.0, dceta = 0.0;
+            if (F == 1) {
+                dcksi = std::fabs(s_xk) > sp(1e-6f) ? dfx / s_xk : sp(0.00003f);
+            } else {
+                double det = s_xk * s_ye - s_yk * s_xe;
+                if (det > P.in.eps) {
+                    dcksi = (s_ye * dfx - s_yk * dfy) / det;
+                    dceta = (-s_xe * dfx + s_xk * dfy) / det;
+                } else {
+                    dcksi = 0.000003;
+                    dceta = 0.000003;
+                }
+            }
+            for (int ifxy = 1; ifxy <= F && !rc; ifxy++) {
+                double fxk = fxkp1, fyk = fykp1;
+                if (ifxy == 1) {
+                    P.in.cksi = P.in.cksi + dcksi;
+                    add_rigid_slip(C, P.wsx, P.wsy, P.wsx, P.wsy, 0, dcksi);
+                } else {
+                    P.in.ceta = P.in.ceta + dceta;
+                    add_rigid_slip(C, P.wsx, P.wsy, P.wsx, P.wsy, 1, dceta);
+                }
+                rc = co_await tang_solver(C, k, cf00, omegah, omegas, P.in.maxgs, P.in.eps, &it, &err, &info1);
+                f = co_await C.opr(OP_FORCES, fi);
+                fxkp1 = P.dxdy * f.v[1] / den;
+                fykp1 = P.dxdy * f.v[2] / den;
+                itgs += it;
+                ca = co_await C.count(M_ADH);
+                ci = co_await C.count(M_INT);
+                nadh = ca.v[0];
+                ncon = ci.v[0];
+                double dfxk = fxkp1 - fxk, dfyk = fykp1 - fyk;
+                double area = ncon * P.dxdy;
+                if (ifxy == 1) {
+                    if (nadh > 0) {
+                        double dpxavg = dfxk * P.muscal * P.fntrue / area;
+                        if (s_xk == 0.0 || std::fabs(dpxavg) > 10.0 * err) s_xk = dfxk / dcksi;
+                        double dpyavg = dfyk * P.muscal * P.fntrue / area;
+                        if (std::fabs(dpyavg) > 10.0 * err) s_yk = dfyk / dcksi;
+                    } else if (std::fabs(s_xk) < sp(1e-6f)) {
+                        s_xk = fxkp1 / P.in.cksi;
+                    }
+                } else {
+                    if (nadh > 0) {
+                        double dpxavg = dfxk * P.muscal * P.fntrue / area;
+                        if (std::fabs(dpxavg) > 10.0 * err) s_xe = dfxk / dceta;
+                        double dpyavg = dfyk * P.muscal * P.fntrue / area;
+                        if (s_ye == 0.0 || std::fabs(dpyavg) > 10.0 * err) s_ye = dfyk / dceta;
+                    } else if (std::fabs(s_ye) < sp(1e-6f)) {
+                        s_ye = fykp1 / P.in.ceta;
+                    }
+                }
+                df = std::fabs(P.in.fxrel - fxkp1);
+                if (F >= 2) df = df + std::fabs(P.in.fyrel - fykp1);
+            }
+        }
+        // the reference's error estimate adds the slip elements as the state code Slip = 2
+        // (`slip` in src/m_solvpt.f90:366 names that constant); no plastic elements here
+        err = err + 2.0 * df * P.muscal * P.fntrue / ((nadh + (double)SLIP) * P.dxdy);
+    } else {
+        sens_at(P, IOUT_FX, IIN_KSI) = 0.0;
+        sens_at(P, IOUT_FY, IIN_KSI) = 0.0;
+        sens_at(P, IOUT_FX, IIN_ETA) = 0.0;
+        sens_at(P, IOUT_FY, IIN_ETA) = 0.0;
+    }
+    *itgs_out = itgs;
+    *err_out = err;
+    co_return rc;
 }
 
 static Task stang(Ctx C, int* ittang_out)
@@ -2193,6 +2347,9 @@ static Task stang(Ctx C, int* ittang_out)
         C.conv_t(1, M_INT, ux, uy, P.pvx, P.pvy);
         C.axpy(M_INT, -1.0, ux, P.wsx, uy, P.wsy);
     }
+    // with forces prescribed, solvpt rebuilds its right-hand side from wsfix1 (kept in w[22],
+    // w[23]) and the current creepages at every call
+    if (P.in.force3 >= 1) snap(C, nullptr, nullptr, {P.wsx, P.wsy}, {P.w[22], P.w[23]});
     int ittang = 0, itgs = 0, ret = 0;
     bool zready = false;
     while (!zready && ittang < P.in.maxin) {
@@ -2200,23 +2357,8 @@ static Task stang(Ctx C, int* ittang_out)
         zready = true;
         int it = 0;
         double errpt = 0.0;
-        int rc;
-        if (P.in.tang == 3 && P.in.gausei == 5) {
-            // GDsteady; SteadyGS restarts from its state when it stagnates
-            // (src/m_solvpt.f90:466-485)
-            int lstagn = 0;
-            rc = co_await gdsteady(C, cf00, P.in.maxgs, P.in.eps, &it, &errpt, &lstagn);
-            P.out.gd_stagnated = lstagn;
-            if (!rc && lstagn) {
-                itgs += std::abs(it);
-                rc = co_await stdygs(C, k, omegah, omegas, P.in.maxgs, P.in.eps, &it, &errpt);
-            }
-        } else if (P.in.tang == 3) {
-            rc = co_await stdygs(C, k, omegah, omegas, P.in.maxgs, P.in.eps, &it, &errpt);
-        } else {
-            rc = co_await tangcg(C, cf00, P.in.maxgs, P.in.eps, &it, &errpt);
-        }
-        itgs += std::abs(it);
+        int rc = co_await solvpt(C, k, cf00, omegah, omegas, &it, &errpt);
+        itgs += it;
         if (rc) { ret = rc; break; }
         StItem S{};
         S.n = P.n; S.el = P.el; S.px = P.px; S.py = P.py; S.pn = P.pn; S.sx = P.sx; S.sy = P.sy;
@@ -2263,7 +2405,16 @@ static Task calculate(Ctx C)
         S.zero_pn = P.in.iestim == 0;
         S.zero_pt = P.in.iestim == 0 || P.in.tang == 0;
         S.hertz = P.hertz_init;
-        S.cksi = P.in.cksi; S.ceta = P.in.ceta; S.cphi = P.in.cphi; S.dq = P.dq;
+        // set_tang_rhs (src/m_sdis.f90:566-575): a creepage enters the rigid slip only when it
+        // is prescribed (F=0: both; F=1: ceta); init_curr_data starts the force-controlled ones
+        // from cksi = 1e-6, ceta = 0 (:759-763)
+        if (P.in.iestim == 0 || P.in.iestim == 2) {
+            if (P.in.force3 >= 1) P.in.cksi = 1e-6;
+            if (P.in.force3 == 2) P.in.ceta = 0.0;
+        }
+        S.cksi = P.in.force3 == 0 ? P.in.cksi : 0.0;
+        S.ceta = P.in.force3 <= 1 ? P.in.ceta : 0.0;
+        S.cphi = P.in.cphi; S.dq = P.dq;
         S.xofs = P.xofs; S.yofs = P.yofs;
         S.hz_aa = R.hz_aa; S.hz_bb = R.hz_bb; S.pnmax = P.hz_pnmax;
         Res r = co_await C.opr(OP_SETUP, it);

@@ -43,6 +43,10 @@ struct CaseInput {
     // parameters (src/m_hierarch_data.f90:981, 1733-1735)
     int sens = 1, mxsens = 30;
     double epsens = 1e-4;
+    // F digit: 0 creepages prescribed; 1 Fx, 2 Fx and Fy prescribed (fractions of mu Fn), the
+    // creepages cksi (and ceta) then being solved for and kept in cksi, ceta
+    int force3 = 0;
+    double fxrel = 0, fyrel = 0;
 };
 
 // Results of one case.

@@ -1652,7 +1652,8 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_guard(const GsItem* items, con
 // bound applied (:3086-3127).  The rebuild stops at the first upstream element that is not
 // in adhesion: the elements beyond it depend on its unchanged traction only.  The current
 // row lives in LDS; dp is written through to global memory for the summation.
-// Returns (itgs, dif, algorithmic flops: 8 per coefficient-difference product of the sums).
+// Returns (itgs, dif, algorithmic flops: 8 per coefficient-difference product of the sums,
+// 0, 0, dif of the first sweep).
 __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, const int* gstart,
                                                        int nitems, double* partials)
 {
@@ -1721,7 +1722,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, cons
     __syncthreads();
 
     int itgs = 0;
-    double dif = 2.0, difid = 1.0;
+    double dif = 2.0, difid = 1.0, dif1 = 0.0;
     while (dif >= difid && itgs < I.maxgs) {
         itgs++;
         double difacc = 0.0;   // thread 0
@@ -1833,14 +1834,16 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, cons
         __syncthreads();
         dif = s_ctl[0];
         difid = s_ctl[1];
+        if (itgs == 1) dif1 = dif;
         __syncthreads();
     }
     if (tid == 0) {
-        partials[(size_t)blockIdx.x * 5] = (double)itgs;
-        partials[(size_t)blockIdx.x * 5 + 1] = dif;
-        partials[(size_t)blockIdx.x * 5 + 2] = 8.0 * itgs * (double)I.k * (double)len;
-        partials[(size_t)blockIdx.x * 5 + 3] = 0.0;
-        partials[(size_t)blockIdx.x * 5 + 4] = 0.0;
+        partials[(size_t)blockIdx.x * 6] = (double)itgs;
+        partials[(size_t)blockIdx.x * 6 + 1] = dif;
+        partials[(size_t)blockIdx.x * 6 + 2] = 8.0 * itgs * (double)I.k * (double)len;
+        partials[(size_t)blockIdx.x * 6 + 3] = 0.0;
+        partials[(size_t)blockIdx.x * 6 + 4] = 0.0;
+        partials[(size_t)blockIdx.x * 6 + 5] = dif1;
     }
 }
 
@@ -1852,7 +1855,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, cons
 // current row's own influence and runs on one wavefront without work-group barriers.
 // Summation order differs from gf3_AijPj; the sums are the same.  Returns (itgs, dif,
 // algorithmic flops, wall-clock ticks (100 MHz) spent in the row sums and in the sequential
-// parts).
+// parts, dif of the first sweep).
 
 // LDS of k_stdygs2 without the staging area; the staging area holds `sr` source rows
 __host__ __device__ inline size_t gs2_base_lds(int mx, int my)
@@ -1929,7 +1932,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
     __syncthreads();
 
     int itgs = 0;
-    double dif = 2.0, difid = 1.0;
+    double dif = 2.0, difid = 1.0, dif1 = 0.0;
     unsigned long long t_rows = 0, t_inrow = 0;   // thread 0: wall-clock ticks of the two phases
     while (dif >= difid && itgs < I.maxgs) {
         itgs++;
@@ -2159,16 +2162,18 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
         __syncthreads();
         dif = s_ctl[0];
         difid = s_ctl[1];
+        if (itgs == 1) dif1 = dif;
         __syncthreads();
     }
     if (tid == 0) {
         double len = 0.0;
         for (int jy = 0; jy < my; jy++) len += rj1[jy] >= rj0[jy] ? rj1[jy] - rj0[jy] + 1 : 0;
-        partials[(size_t)blockIdx.x * 5] = (double)itgs;
-        partials[(size_t)blockIdx.x * 5 + 1] = dif;
-        partials[(size_t)blockIdx.x * 5 + 2] = 8.0 * itgs * (double)I.k * len;
-        partials[(size_t)blockIdx.x * 5 + 3] = (double)t_rows;
-        partials[(size_t)blockIdx.x * 5 + 4] = (double)t_inrow;
+        partials[(size_t)blockIdx.x * 6] = (double)itgs;
+        partials[(size_t)blockIdx.x * 6 + 1] = dif;
+        partials[(size_t)blockIdx.x * 6 + 2] = 8.0 * itgs * (double)I.k * len;
+        partials[(size_t)blockIdx.x * 6 + 3] = (double)t_rows;
+        partials[(size_t)blockIdx.x * 6 + 4] = (double)t_inrow;
+        partials[(size_t)blockIdx.x * 6 + 5] = dif1;
     }
 }
 
@@ -2203,7 +2208,7 @@ int op_nred(int kind, const void* item)
     case OP_GD_DIAGRES: return 3;
     case OP_SETUP: return 2;
     case OP_FORCES: return 5;
-    case OP_STDYGS: return 5;
+    case OP_STDYGS: return 6;
     case OP_GUARD: return 5;
     case OP_SENS_EXT: return 4;
     default: return 0;

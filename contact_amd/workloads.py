@@ -152,6 +152,9 @@ def apply_case(cntc, ire, icp, c):
     else:
         cntc.setpenetration(ire, icp, c["pen"])
     cntc.setcreepages(ire, icp, c["cksi"], c["ceta"], c["cphi"])
+    if c.get("force") is not None:
+        cntc.setflags(ire, icp, [1974], [c["force"]])
+        cntc.settangentialforces(ire, icp, c.get("fxrel", 0.0), c.get("fyrel", 0.0))
     if c.get("sens") is not None:
         cntc.setflags(ire, icp, [1988], [c["sens"]])
     if c.get("mxsens") is not None or c.get("epsens") is not None:

@@ -65,6 +65,9 @@ void cntc_setpenetration(int *ire, int *icp, double *pen);
 void cntc_setnormalforce(int *ire, int *icp, double *fn);
 /* src/contact_addon.f90:2361-2487 cntc_setUndeformedDistc (ibase 1, 9) */
 void cntc_setundeformeddistc(int *ire, int *icp, int *ibase, int *nparam, double *prmudf);
+/* src/contact_addon.f90:2597-2650 cntc_setTangentialForces: Fx, Fy relative to mu*Fn for the
+   F digit (CNTC_ic_force) 1 or 2; |(Fx, Fy)| > 1 is scaled to 1 */
+void cntc_settangentialforces(int *ire, int *icp, double *fx, double *fy);
 /* src/contact_addon.f90:2491-2530 cntc_setCreepages */
 void cntc_setcreepages(int *ire, int *icp, double *vx, double *vy, double *phi);
 

@@ -1797,8 +1797,10 @@ static int gdsteady(orc_problem *P, const gdpar *G, int maxgd, double eps, doubl
    over the differences dp (gf3_AijPj), a local 2x2 problem (plstrc) with the coefficients of
    the element relative to the first upstream non-adhesion element, then the tractions of the
    upstream part of the row are rebuilt from dp with the traction bound applied. */
+/* info: 0, or 1 when maxgs was reached, 2 when also the mean rate of convergence exceeds
+   0.997, 3 when it exceeds 1 (:3205-3234) */
 static int stdygs(orc_problem *P, double **ws, const double *mus, int k, double eps, int maxgs,
-                  double omegah, double omegas, int *itgs_out, double *err_out)
+                  double omegah, double omegas, int *itgs_out, double *err_out, int *info)
 {
     int mx = P->mx, my = P->my, npot = P->npot, *el = P->igs.el;
     double *px = P->ps[0], *py = P->ps[1], *pn = P->ps[2];
@@ -1874,7 +1876,12 @@ static int stdygs(orc_problem *P, double **ws, const double *mus, int k, double 
         difid = eps * dmax(1e-6, facnel * rms);
         if (itgs == 1) dif1 = dif;
     }
-    (void)dif1;
+    double conv = 1.0;
+    if (dif * dif1 != 0.0 && itgs > 1) conv = exp(log(dif / dif1) / (itgs - 1));
+    *info = 0;
+    if (itgs >= maxgs) *info = 1;
+    if (itgs >= maxgs && conv > sp(0.997f)) *info = 2;
+    if (itgs >= maxgs && conv > 1.0) *info = 3;
     *itgs_out = itgs;
     *err_out = dif;
     free(dpx);
@@ -2087,6 +2094,37 @@ static int set_facdt(orc_problem *P, int tang)
     return 0;
 }
 
+/* tang_solver, src/m_solvpt.f90:382-493: GDsteady for T=3 with G=5, restarted by SteadyGS when
+   it stagnates; SteadyGS for other G in steady rolling; TangCG for T=1, 2.  *it counts the
+   iterations of both solvers of a fall-back.  *info is set by SteadyGS only (TangCG and
+   GDsteady leave it). */
+static int tang_solver(orc_problem *P, const orc_input *inp, double **ws, const double *mus, int k,
+                       double omegah, double omegas, const gdpar *G, int maxit, double eps, int *it,
+                       double *err, int *info)
+{
+    int rc, it1 = 0;
+    areas(&P->igs, P->mx, P->my);
+    if (inp->tang == 3 && inp->gausei == 5) {
+        int lstagn = 0;
+        rc = gdsteady(P, G, maxit, eps, ws, mus, &it1, err, &lstagn);
+        P->gd_stagnated = lstagn;
+        *it = abs(it1);
+        if (!rc && lstagn) {
+            rc = stdygs(P, ws, mus, k, eps, maxit, omegah, omegas, &it1, err, info);
+            P->used_stdygs = 1;
+            *it += abs(it1);
+        }
+    } else if (inp->tang == 3) {
+        rc = stdygs(P, ws, mus, k, eps, maxit, omegah, omegas, &it1, err, info);
+        P->used_stdygs = 1;
+        *it = abs(it1);
+    } else {
+        rc = tangcg(P, maxit, eps, ws, mus, &it1, err);
+        *it = abs(it1);
+    }
+    return rc;
+}
+
 /* Kalker's creep coefficients C11, C22, C23 (Kalker 1967 / 1990 table, as tabulated in
    src/m_hertz.f90:1158-1179) for g = 0.1, 0.2, ..., 1.0 and poisson ratios 0, 1/4, 1/2.  Case 0
    holds g = a/b <= 1, case 1 holds g = b/a < 1.  The reference stores them from single-precision
@@ -2193,7 +2231,14 @@ static int sens_tang(orc_problem *P, const orc_input *inp, double **ws, const do
     int stagn0 = P->gd_stagnated, used0 = P->used_stdygs, ret = 0;
     for (int icrp = 0; icrp < 3; icrp++) {
         double dcrp = 0.0001 * ch[icrp];
+        /* wstot = wsfix1 plus the creepages that solvpt adds when forces are prescribed
+           (:600-614) */
         for (int i = 0; i < npot; i++) { wsloc[0][i] = ws[0][i]; wsloc[1][i] = ws[1][i]; }
+        for (int i = 0; i < npot; i++) {
+            if (el[i] < ORC_ADHES) continue;
+            if (inp->force3 >= 1) wsloc[0][i] = wsloc[0][i] + P->facdt[i] * P->cksi * P->dq;
+            if (inp->force3 >= 2) wsloc[1][i] = wsloc[1][i] + P->facdt[i] * P->ceta * P->dq;
+        }
         for (int i = 0; i < npot; i++) {
             if (el[i] < ORC_ADHES) continue;
             double f = P->facdt[i];
@@ -2206,18 +2251,9 @@ static int sens_tang(orc_problem *P, const orc_input *inp, double **ws, const do
         }
         P->p0[0] = save[0];
         P->p0[1] = save[1];
-        int it = 0, rc;
+        int it = 0, info = 0, rc;
         double err = 0.0;
-        areas(&P->igs, P->mx, P->my);
-        if (inp->tang == 3 && inp->gausei == 5) {
-            int lstagn = 0;
-            rc = gdsteady(P, G, inp->mxsens, inp->epsens, wsloc, mus, &it, &err, &lstagn);
-            if (!rc && lstagn) rc = stdygs(P, wsloc, mus, k, inp->epsens, inp->mxsens, omegah, omegas, &it, &err);
-        } else if (inp->tang == 3) {
-            rc = stdygs(P, wsloc, mus, k, inp->epsens, inp->mxsens, omegah, omegas, &it, &err);
-        } else {
-            rc = tangcg(P, inp->mxsens, inp->epsens, wsloc, mus, &it, &err);
-        }
+        rc = tang_solver(P, inp, wsloc, mus, k, omegah, omegas, G, inp->mxsens, inp->epsens, &it, &err, &info);
         P->p0[0] = P->p0[1] = NULL;
         if (rc) { ret = rc; }
         else {
@@ -2243,6 +2279,113 @@ static int sens_tang(orc_problem *P, const orc_input *inp, double **ws, const do
     free(wsloc[0]);
     free(wsloc[1]);
     return ret;
+}
+
+/* solvpt, src/m_solvpt.f90:51-378.  F=0: one tang_solver call on wsfix1.  F=1 (Fx prescribed)
+   or F=2 (Fx and Fy): wstot = wsfix1 + facdt (cksi, ceta) dq, then Newton-Raphson on cksi
+   (and ceta) with the sensitivities d(Fx, Fy)/(mu Fn) per creepage, which are refreshed
+   from each half step; *it sums the solver iterations. */
+static int solvpt(orc_problem *P, const orc_input *inp, double **wsfix, const double *mus, int k,
+                  double omegah, double omegas, const gdpar *G, int *itgs_out, double *err_out)
+{
+    int npot = P->npot, *el = P->igs.el, F = inp->force3;
+    double dxdy = P->dxdy, den = P->muscal * P->fntrue;
+    double *wstot[2] = {dalloc(npot), dalloc(npot)};
+    memcpy(wstot[0], wsfix[0], sizeof(double) * npot);
+    memcpy(wstot[1], wsfix[1], sizeof(double) * npot);
+    for (int i = 0; i < npot; i++) {
+        if (el[i] < ORC_ADHES) continue;
+        if (F >= 1) wstot[0][i] = wstot[0][i] + P->facdt[i] * P->cksi * P->dq;
+        if (F >= 2) wstot[1][i] = wstot[1][i] + P->facdt[i] * P->ceta * P->dq;
+    }
+    int itgs = 0, info1 = 0, it = 0, rc;
+    double err = 0.0;
+    rc = tang_solver(P, inp, wstot, mus, k, omegah, omegas, G, inp->maxgs, inp->eps, &it, &err, &info1);
+    double fxkp1 = dxdy * gf_sum(ALLELM, el, npot, P->ps[0]) / den;
+    double fykp1 = dxdy * gf_sum(ALLELM, el, npot, P->ps[1]) / den;
+    itgs += it;
+    int nadh, nslip, nplast, nexter;
+    count_el(el, npot, &nadh, &nslip, &nplast, &nexter);
+    if (!rc && F >= 1 && info1 <= 1) {
+        int itnr = 0;
+        double df = fabs(inp->fxrel - fxkp1);
+        if (F >= 2) df = df + fabs(inp->fyrel - fykp1);
+        while (!rc && df > inp->eps && itnr < inp->maxnr && info1 <= 1) {
+            itnr++;
+            double dfx = inp->fxrel - fxkp1, dfy = inp->fyrel - fykp1, dcksi = 0.0, dceta = 0.0;
+            if (F == 1) {
+                dceta = 0.0;
+                if (fabs(SENS(P, IOUT_FX, IIN_KSI)) > sp(1e-6f)) dcksi = dfx / SENS(P, IOUT_FX, IIN_KSI);
+                else dcksi = sp(0.00003f);
+            } else {
+                double det = SENS(P, IOUT_FX, IIN_KSI) * SENS(P, IOUT_FY, IIN_ETA) -
+                             SENS(P, IOUT_FY, IIN_KSI) * SENS(P, IOUT_FX, IIN_ETA);
+                if (det > inp->eps) {
+                    dcksi = (SENS(P, IOUT_FY, IIN_ETA) * dfx - SENS(P, IOUT_FY, IIN_KSI) * dfy) / det;
+                    dceta = (-SENS(P, IOUT_FX, IIN_ETA) * dfx + SENS(P, IOUT_FX, IIN_KSI) * dfy) / det;
+                } else {
+                    dcksi = 0.000003;
+                    dceta = 0.000003;
+                }
+            }
+            for (int ifxy = 1; ifxy <= F && !rc; ifxy++) {
+                double fxk = fxkp1, fyk = fykp1;
+                if (ifxy == 1) {
+                    P->cksi = P->cksi + dcksi;
+                    for (int i = 0; i < npot; i++)
+                        if (el[i] >= ORC_ADHES) wstot[0][i] = wstot[0][i] + P->facdt[i] * dcksi * P->dq;
+                } else {
+                    P->ceta = P->ceta + dceta;
+                    for (int i = 0; i < npot; i++)
+                        if (el[i] >= ORC_ADHES) wstot[1][i] = wstot[1][i] + P->facdt[i] * dceta * P->dq;
+                }
+                rc = tang_solver(P, inp, wstot, mus, k, omegah, omegas, G, inp->maxgs, inp->eps, &it,
+                                 &err, &info1);
+                fxkp1 = dxdy * gf_sum(ALLELM, el, npot, P->ps[0]) / den;
+                fykp1 = dxdy * gf_sum(ALLELM, el, npot, P->ps[1]) / den;
+                itgs += it;
+                count_el(el, npot, &nadh, &nslip, &nplast, &nexter);
+                double dfxk = fxkp1 - fxk, dfyk = fykp1 - fyk;
+                double area = (double)(nadh + nslip + nplast) * dxdy;
+                if (ifxy == 1) {
+                    if (nadh > 0) {
+                        double dpxavg = dfxk * P->muscal * P->fntrue / area;
+                        if (SENS(P, IOUT_FX, IIN_KSI) == 0.0 || fabs(dpxavg) > 10.0 * err)
+                            SENS(P, IOUT_FX, IIN_KSI) = dfxk / dcksi;
+                        double dpyavg = dfyk * P->muscal * P->fntrue / area;
+                        if (fabs(dpyavg) > 10.0 * err) SENS(P, IOUT_FY, IIN_KSI) = dfyk / dcksi;
+                    } else if (fabs(SENS(P, IOUT_FX, IIN_KSI)) < sp(1e-6f)) {
+                        SENS(P, IOUT_FX, IIN_KSI) = fxkp1 / P->cksi;
+                    }
+                } else {
+                    if (nadh > 0) {
+                        double dpxavg = dfxk * P->muscal * P->fntrue / area;
+                        if (fabs(dpxavg) > 10.0 * err) SENS(P, IOUT_FX, IIN_ETA) = dfxk / dceta;
+                        double dpyavg = dfyk * P->muscal * P->fntrue / area;
+                        if (SENS(P, IOUT_FY, IIN_ETA) == 0.0 || fabs(dpyavg) > 10.0 * err)
+                            SENS(P, IOUT_FY, IIN_ETA) = dfyk / dceta;
+                    } else if (fabs(SENS(P, IOUT_FY, IIN_ETA)) < sp(1e-6f)) {
+                        SENS(P, IOUT_FY, IIN_ETA) = fykp1 / P->ceta;
+                    }
+                }
+                df = fabs(inp->fxrel - fxkp1);
+                if (F >= 2) df = df + fabs(inp->fyrel - fykp1);
+            }
+        }
+        /* the reference's error estimate adds the slip elements as the state code Slip = 2
+           (`slip` in src/m_solvpt.f90:366 names that constant) */
+        err = err + 2.0 * df * P->muscal * P->fntrue / ((double)(nadh + ORC_SLIP + nplast) * dxdy);
+    } else {
+        SENS(P, IOUT_FX, IIN_KSI) = 0.0;
+        SENS(P, IOUT_FY, IIN_KSI) = 0.0;
+        SENS(P, IOUT_FX, IIN_ETA) = 0.0;
+        SENS(P, IOUT_FY, IIN_ETA) = 0.0;
+    }
+    free(wstot[0]);
+    free(wstot[1]);
+    *itgs_out = itgs;
+    *err_out = err;
+    return rc;
 }
 
 static int stang(orc_problem *P, const orc_input *inp, int *ittang_out)
@@ -2305,26 +2448,8 @@ static int stang(orc_problem *P, const orc_input *inp, int *ittang_out)
         zready = 1;
         int it = 0;
         double errpt = 0.0;
-        /* solvpt with F=0: wstot = wsfix1 (src/m_solvpt.f90:106-125) and one tang_solver call */
-        areas(&P->igs, mx, my);
-        if (inp->tang == 3 && inp->gausei == 5) {
-            /* GDsteady; SteadyGS restarts from its state when it stagnates
-               (src/m_solvpt.f90:466-485) */
-            int lstagn = 0;
-            rc = gdsteady(P, &G, inp->maxgs, inp->eps, ws, mus, &it, &errpt, &lstagn);
-            P->gd_stagnated = lstagn;
-            if (!rc && lstagn) {
-                itgs += abs(it);
-                rc = stdygs(P, ws, mus, k, inp->eps, inp->maxgs, omegah, omegas, &it, &errpt);
-                P->used_stdygs = 1;
-            }
-        } else if (inp->tang == 3) {
-            rc = stdygs(P, ws, mus, k, inp->eps, inp->maxgs, omegah, omegas, &it, &errpt);
-            P->used_stdygs = 1;
-        } else {
-            rc = tangcg(P, inp->maxgs, inp->eps, ws, mus, &it, &errpt);
-        }
-        itgs += abs(it);
+        rc = solvpt(P, inp, ws, mus, k, omegah, omegas, &G, &it, &errpt);
+        itgs += it;
         if (rc) { ret = rc; break; }
         int newins = 0;
         double tol = sqrt(2.0) * errpt;
@@ -2582,8 +2707,8 @@ int orc_calculate(orc_problem *P, const orc_input *inp)
         for (int i = 0; i < npot; i++) {
             P->hs[0][i] = 1.0 * -(P->y[i] + yofs - 0.0) * P->cphi + 0.0;
             P->hs[1][i] = 1.0 * (P->x[i] + xofs - 0.0) * P->cphi + 0.0;
-            P->hs[0][i] += P->cksi;
-            P->hs[1][i] += P->ceta;
+            if (inp->force3 == 0) P->hs[0][i] += P->cksi;   /* F=0: cksi prescribed */
+            if (inp->force3 <= 1) P->hs[1][i] += P->ceta;   /* F<=1: ceta prescribed */
             P->hs[0][i] *= -P->dq;
             P->hs[1][i] *= -P->dq;
         }
@@ -2616,6 +2741,10 @@ int orc_calculate(orc_problem *P, const orc_input *inp)
     if (inp->iestim == 0)
         for (int io = IOUT_FX; io <= IOUT_MZ; io++) SENS(P, io, IIN_PEN) = 0.0;
     if (inp->iestim == 0 && (inp->norm != 1 || ihertz != 0)) SENS(P, IOUT_FN, IIN_PEN) = 0.0;
+    if (inp->iestim == 0 || inp->iestim == 2) {
+        if (inp->force3 >= 1) P->cksi = 1e-6;
+        if (inp->force3 == 2) P->ceta = 0.0;
+    }
     if (inp->iestim == 0 || inp->iestim == 2)
         for (int ii = IIN_KSI; ii <= IIN_PHI; ii++)
             for (int io = IOUT_FN; io <= IOUT_MZ; io++) SENS(P, io, ii) = 0.0;
@@ -2719,3 +2848,9 @@ void orc_get_iters(const orc_problem *p, int *itnorm, int *ittang, int *itcg, in
 }
 int orc_get_stagnated(const orc_problem *p) { return p->gd_stagnated + 2 * p->used_stdygs; }
 void orc_get_sens(const orc_problem *p, double *sens) { memcpy(sens, p->sens, sizeof p->sens); }
+void orc_get_creepages(const orc_problem *p, double *cksi, double *ceta, double *cphi)
+{
+    *cksi = p->cksi;
+    *ceta = p->ceta;
+    *cphi = p->cphi;
+}

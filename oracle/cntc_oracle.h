@@ -60,6 +60,10 @@ typedef struct orc_input {
        solver parameters (src/contact_addon.f90:1266-1272) */
     int sens, mxsens;
     double epsens;
+    /* F digit (CNTC_ic_force, src/contact_addon.f90:834-842): 0 creepages prescribed, 1 Fx,
+       2 Fx and Fy prescribed as fractions of mu*Fn (cntc_setTangentialForces, :2597-2650) */
+    int force3;
+    double fxrel, fyrel;
 } orc_input;
 
 /* fills an input record with the reference's module-3 defaults */
@@ -94,6 +98,8 @@ int  orc_get_stagnated(const orc_problem *p);
 /* sensitivity matrix in internal units, sens[iout + 4*iin] with outputs (fn, fx/(mu fn),
    fy/(mu fn), mz) and inputs (pen, cksi, ceta, cphi), src/m_hierarch_data.f90:122-124 */
 void orc_get_sens(const orc_problem *p, double *sens);
+/* creepages of the last case: with F=1, 2 the Newton iteration's final cksi (and ceta) */
+void orc_get_creepages(const orc_problem *p, double *cksi, double *ceta, double *cphi);
 
 /* stand-alone building blocks, exposed for known-answer tests */
 int  orc_opt_fft_size(int n);

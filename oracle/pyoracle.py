@@ -39,6 +39,7 @@ class OrcInput(ctypes.Structure):
         ("pow_s", ctypes.c_double), ("kdowfb", ctypes.c_int),
         ("omegah", ctypes.c_double), ("omegas", ctypes.c_double),
         ("sens", ctypes.c_int), ("mxsens", ctypes.c_int), ("epsens", ctypes.c_double),
+        ("force3", ctypes.c_int), ("fxrel", ctypes.c_double), ("fyrel", ctypes.c_double),
     ]
 
 
@@ -76,6 +77,7 @@ def lib():
         L.orc_get_stagnated.argtypes = [vp]
         L.orc_get_stagnated.restype = ctypes.c_int
         L.orc_get_sens.argtypes = [vp, dp]
+        L.orc_get_creepages.argtypes = [vp, dp, dp, dp]
         L.orc_linrol.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp, dp]
         L.orc_opt_fft_size.argtypes = [ctypes.c_int]
         L.orc_opt_fft_size.restype = ctypes.c_int
@@ -173,6 +175,11 @@ class Problem:
         s = np.zeros(16)
         lib().orc_get_sens(self._p, _dptr(s))
         return s.reshape(4, 4).T.copy()
+
+    def creepages(self):
+        v = [ctypes.c_double() for _ in range(3)]
+        lib().orc_get_creepages(self._p, *[ctypes.byref(x) for x in v])
+        return tuple(x.value for x in v)
 
     def iters(self):
         v = [ctypes.c_int() for _ in range(4)]
@@ -279,6 +286,12 @@ def case_input(c):
     if c.get("dq") is not None:
         inp.dq = c["dq"]
     inp.cksi, inp.ceta, inp.cphi = c["cksi"], c["ceta"], c["cphi"]
+    inp.force3 = c.get("force", 0)
+    fx, fy = c.get("fxrel", 0.0), c.get("fyrel", 0.0)
+    fabs = (fx * fx + fy * fy) ** 0.5
+    if fabs > 1.0:      # cntc_setTangentialForces normalises |(fx, fy)| to 1
+        fx, fy = fx / fabs, fy / fabs
+    inp.fxrel, inp.fyrel = fx, fy
     inp.sens = c.get("sens", 1)
     inp.mxsens = c.get("mxsens", 30)
     inp.epsens = c.get("epsens", 1e-4)

@@ -35,6 +35,28 @@ def cattaneo_case():
                      tang=1, cksi=.00304, eps=1e-8, maxgs=100, maxin=100)
 
 
+def cattaneo_force_case():
+    """examples/cattaneo.inp case 2: T=1 shift with Fx = -7/8 mu Fn and Fy = 0 prescribed (F=2),
+    Fn = 9.1954 (N=1), 19x19 grid, polyethylene (nu 0.42, G 200), mu 0.4, eps 1e-4."""
+    c = grid_case(19, 19, -1.26667, -1.26667, .13333, .13333,
+                  prmudf=[0.0100, 0.0, 0.0100, 0.0, 0.0, 0.0], norm=1, fn=9.1954, tang=1,
+                  fstat=0.4, nu=0.42, g=200.0, eps=1e-4, maxgs=100, maxin=100)
+    c.update(force=2, fxrel=-0.875, fyrel=0.0, gausei=0)
+    return c
+
+
+# examples/cattaneo.ref_out:88-102, the Newton iterations of case 2 as printed (4 digits):
+# (iteration, cksi, Fx/(mu Fn), ceta, Fy/(mu Fn), cumulative TangCG iterations)
+CATTANEO_FORCE_TRACE = [
+    (1, 4.000e-06, -5.424e-04, 3.000e-06, -4.068e-04, 20),
+    (2, 6.453e-03, -0.7410, 0.0, 5.200e-06, 33),
+    (3, 7.620e-03, -0.8367, 4.460e-08, -4.188e-06, 42),
+    (4, 8.087e-03, -0.8704, 8.681e-09, -2.461e-06, 50),
+    (5, 8.150e-03, -0.8747, -1.243e-08, 6.659e-07, 55),
+    (6, 8.155e-03, -0.8750, -6.716e-09, 9.615e-07, 59),
+]
+
+
 def table_cases(mx=11, ncase=3220):
     """The 3220 steady-rolling Hertz cases of src/test_table.f90:91-305 (B=0, T=3)."""
     ellip = [0.2, 0.5, 1.0, 2.0, 5.0]

@@ -161,3 +161,47 @@ def test_steadygs_wide_rows(cntc, oracle):
     print("wide rows: gpu iterations", its, "oracle", o[4].iters())
     assert its[4] & 2
     compare(g, o, 1e-6, 1e-5, "wide rows", max_el_diff=4)
+
+
+def test_cattaneo_force_golden(cntc, oracle):
+    """examples/cattaneo.inp case 2 through the C ABI (T=1, F=2: Fx, Fy prescribed, Newton on
+    the shift): examples/cattaneo.ref_out:88-139 and the oracle."""
+    c = C.cattaneo_force_case()
+    g = run_gpu(cntc, c, ire=51)
+    o = run_oracle(oracle, c)
+    compare(g, o, 1e-8, 1e-7, "cattaneo F=2")
+    cksi, ceta, _ = cntc.getcreepages(51, 1)
+    ocksi, oceta, _ = o[4].creepages()
+    print("creepages gpu", cksi, ceta, "oracle", ocksi, oceta, flush=True)
+    assert abs(cksi - 8.155e-3) < 5e-7 and abs(ceta - (-6.716e-9)) < 5e-12
+    assert abs(cksi - ocksi) <= 1e-10 * abs(ocksi) and abs(ceta - oceta) <= 1e-6 * abs(oceta)
+    el = g[3]
+    assert (el > 0).sum() == 177 and (el == 1).sum() == 45 and (el == 2).sum() == 132
+    assert cntc.getiterations(51, 1)[3] == 59
+    S = cntc.getsensitivities(51, 1)
+    assert float("%.3g" % S[1, 1]) == -67.6 and float("%.3g" % S[2, 2]) == -117.0
+    So = o[4].sens()
+    assert np.allclose(S, So, rtol=1e-8, atol=0)
+
+
+@pytest.mark.parametrize("force,gausei", [(1, 5), (2, 0), (2, 5)])
+def test_force_prescribed_rolling(cntc, oracle, force, gausei):
+    """T=3 with Fx (F=1) or Fx and Fy (F=2) prescribed: Newton on the creepages with GDsteady
+    or SteadyGS inside, against the oracle; the creepages found and the sensitivities (S=3,
+    whose right-hand side adds the solved creepages) agree too."""
+    c = C.hertz_case(6.0, 4.0, 24, 24, 8200.0, cphi=0.0005, eps=1e-9, maxgs=2000, maxin=40,
+                     gausei=gausei)
+    c["ceta"] = 0.0002
+    c.update(force=force, fxrel=-0.6, fyrel=0.3, sens=3)
+    ire = 52 + force * 2 + (gausei == 0)
+    g = run_gpu(cntc, c, ire=ire)
+    o = run_oracle(oracle, c)
+    compare(g, o, 1e-8, 1e-7, "rolling F=%d G=%d" % (force, gausei))
+    cr = cntc.getcreepages(ire, 1)
+    ocr = o[4].creepages()
+    print("creepages gpu", cr, "oracle", ocr, flush=True)
+    assert abs(cr[0] - ocr[0]) <= 1e-8 * abs(ocr[0]) and abs(cr[1] - ocr[1]) <= 1e-8 * max(abs(ocr[1]), 1e-6)
+    S, So = cntc.getsensitivities(ire, 1), o[4].sens()
+    for k in range(4):
+        scale = np.abs(So[:, k]).max()
+        assert np.abs(S[:, k] - So[:, k]).max() <= 1e-5 * scale, (k, S, So)

@@ -3,7 +3,8 @@ outputs, closed forms and brute force.  No GPU is used.
 
 Golden data: testbank/test_table_mx11.ref_fx (tests/golden/test_table_mx11_fx.csv),
 testbank/test_table_mx11.ref_out:1-5 (Hertz rho, cp), testbank/testbank.ref_out:44-47
-(Cattaneo case 1).
+(Cattaneo case 1), examples/cattaneo.ref_out:88-139 (Cattaneo with prescribed forces: the
+Newton iteration on the creepages, F=2).
 """
 import math
 
@@ -214,6 +215,50 @@ def test_cattaneo_golden(oracle):
     assert (el > 0).sum() == 177 and (el == 1).sum() == 45 and (el == 2).sum() == 132
 
 
+def _sig(x, ref, digits=4):
+    """x printed with `digits` significant digits equals ref."""
+    if ref == 0.0:
+        return abs(x) < 5e-7
+    return float("%.*e" % (digits - 1, x)) == pytest.approx(ref, rel=1e-12)
+
+
+def test_cattaneo_force_golden(oracle):
+    """examples/cattaneo.inp case 2 (T=1, F=2: Fx, Fy prescribed, Newton on the shift):
+    examples/cattaneo.ref_out:88-139 -- shift 8.155e-3, Fx -3.218, C/A/S 177/45/132,
+    ItCG 59, sensitivities dFx/dcksi -67.6 and dFy/dceta -117 as printed."""
+    c = C.cattaneo_force_case()
+    P = oracle.Problem()
+    inp, h = C.to_oracle(oracle, c)
+    assert P.calculate(inp, h) == 0
+    fn, fx, fy, mz = P.forces()
+    cksi, ceta, _ = P.creepages()
+    el = P.eldiv()
+    assert abs(fn - 9.195) < 5e-4 and abs(fx - (-3.218)) < 5e-4 and abs(fy) < 5e-4
+    assert _sig(cksi, 8.155e-3) and _sig(ceta, -6.716e-9)
+    assert (el > 0).sum() == 177 and (el == 1).sum() == 45 and (el == 2).sum() == 132
+    assert P.iters()[3] == 59
+    S = P.sens()
+    assert float("%.3g" % S[1, 1]) == -67.6 and float("%.3g" % S[2, 2]) == -117.0
+
+
+@pytest.mark.parametrize("row", C.CATTANEO_FORCE_TRACE, ids=lambda r: "nr%d" % r[0])
+def test_cattaneo_force_newton_trace(oracle, row):
+    """Each Newton iteration of the golden case (examples/cattaneo.ref_out:88-101), by
+    stopping after `nr` iterations: creepages and forces to the printed 4 digits and the
+    cumulative TangCG iterations exactly."""
+    nr, cksi_r, fx_r, ceta_r, fy_r, itcg_r = row
+    c = dict(C.cattaneo_force_case(), maxin=1)
+    inp, h = C.to_oracle(oracle, c)
+    inp.maxnr = nr
+    P = oracle.Problem()
+    assert P.calculate(inp, h) == 0
+    fn, fx, fy, _ = P.forces()
+    cksi, ceta, _ = P.creepages()
+    assert _sig(cksi, cksi_r) and _sig(fx / (0.4 * fn), fx_r)
+    assert _sig(ceta, ceta_r) and _sig(fy / (0.4 * fn), fy_r)
+    assert P.iters()[3] == itcg_r
+
+
 def test_gdsteady_stagnation_falls_back(oracle):
     """A batch patch on which GDsteady stagnates: SteadyGS takes over (src/m_solvpt.f90:466-485)
     and the result no longer depends on rounding-level changes of the input."""
@@ -229,3 +274,29 @@ def test_gdsteady_stagnation_falls_back(oracle):
         assert P.stagnated() == 3
         res.append(np.array(P.forces()[:3]))
     assert np.max(np.abs(res[0] - res[1])) <= 1e-7 * np.abs(res[0][0])
+
+
+def test_force_prescribed_rolling_reaches_target(oracle):
+    """T=3 with Fx prescribed (F=1, GDsteady) and with Fx, Fy prescribed (F=2, SteadyGS): the
+    Newton iteration on the creepages reaches the prescribed forces within eps, and a
+    solve with the creepages it found, prescribed (F=0), gives the same forces."""
+    for force, gausei in ((1, 5), (2, 0)):
+        c = C.hertz_case(6.0, 4.0, 24, 24, 8200.0, cphi=0.0005, eps=1e-9, maxgs=2000, maxin=40,
+                         gausei=gausei)
+        c["ceta"] = 0.0002
+        c.update(force=force, fxrel=-0.6, fyrel=0.3)
+        P = oracle.Problem()
+        inp, h = C.to_oracle(oracle, c)
+        assert P.calculate(inp, h) == 0
+        fn, fx, fy, _ = P.forces()
+        cksi, ceta, _ = P.creepages()
+        assert abs(fx / (0.3 * fn) - (-0.6)) <= 2e-8
+        if force == 2:
+            assert abs(fy / (0.3 * fn) - 0.3) <= 2e-8
+        else:
+            assert ceta == 0.0002
+        g = dict(c, force=0, cksi=cksi, ceta=ceta)
+        Q = oracle.Problem()
+        inp, h = C.to_oracle(oracle, g)
+        assert Q.calculate(inp, h) == 0
+        assert np.allclose(Q.forces()[1:3], (fx, fy), rtol=0, atol=1e-6 * fn)
