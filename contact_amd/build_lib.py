@@ -12,7 +12,7 @@ SOURCES = ["kernels.hip", "engine.cpp", "cntc_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++20", "-fPIC", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-Wall", "-Wno-unused-result"]
+         "-Wall", "-Wno-unused-result"] + os.environ.get("CNTC_EXTRA_FLAGS", "").split()
 
 
 def _compile(src):

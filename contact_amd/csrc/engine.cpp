@@ -371,10 +371,13 @@ private:
                 for (int i = 0; i < op.nitems; i++) pe.bytes += op.K >= 3 ? r[(size_t)i * op.K + 2] : 0.0;
                 pe.launches++;
             }
-            if (const char* tv = std::getenv("CNTC_GS_TIMING"); tv && tv[0] == '1' && op.K >= 5)
-                for (int i = 0; i < op.nitems; i++)
-                    std::fprintf(stderr, "steadygs item %d: sweeps %.0f, row sums %.3f s, sequential %.3f s\n", i,
-                                 r[(size_t)i * op.K], r[(size_t)i * op.K + 3] * 1e-8, r[(size_t)i * op.K + 4] * 1e-8);
+            if (const char* tv = std::getenv("CNTC_GS_TIMING"); tv && tv[0] == '1' && op.K >= 10)
+                for (int i = 0; i < op.nitems; i++) {
+                    const double* q = r + (size_t)i * op.K;
+                    std::fprintf(stderr, "steadygs item %d: sweeps %.0f, row sums %.3f s, sequential %.3f s "
+                                 "(own row %.3f, search %.3f, element %.3f, chain %.3f)\n", i, q[0],
+                                 q[3] * 1e-8, q[4] * 1e-8, q[6] * 1e-8, q[7] * 1e-8, q[8] * 1e-8, q[9] * 1e-8);
+                }
             std::vector<int> sl = op.slots;
             for (int i = 0; i < op.nitems; i++) {
                 if (op.dest[i])

@@ -1653,7 +1653,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_guard(const GsItem* items, con
 // in adhesion: the elements beyond it depend on its unchanged traction only.  The current
 // row lives in LDS; dp is written through to global memory for the summation.
 // Returns (itgs, dif, algorithmic flops: 8 per coefficient-difference product of the sums,
-// 0, 0, dif of the first sweep).
+// 0, 0, dif of the first sweep, 0, 0, 0, 0).
 __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, const int* gstart,
                                                        int nitems, double* partials)
 {
@@ -1838,12 +1838,11 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, cons
         __syncthreads();
     }
     if (tid == 0) {
-        partials[(size_t)blockIdx.x * 6] = (double)itgs;
-        partials[(size_t)blockIdx.x * 6 + 1] = dif;
-        partials[(size_t)blockIdx.x * 6 + 2] = 8.0 * itgs * (double)I.k * (double)len;
-        partials[(size_t)blockIdx.x * 6 + 3] = 0.0;
-        partials[(size_t)blockIdx.x * 6 + 4] = 0.0;
-        partials[(size_t)blockIdx.x * 6 + 5] = dif1;
+        partials[(size_t)blockIdx.x * 10] = (double)itgs;
+        partials[(size_t)blockIdx.x * 10 + 1] = dif;
+        partials[(size_t)blockIdx.x * 10 + 2] = 8.0 * itgs * (double)I.k * (double)len;
+        for (int q = 3; q < 10; q++) partials[(size_t)blockIdx.x * 10 + q] = 0.0;
+        partials[(size_t)blockIdx.x * 10 + 5] = dif1;
     }
 }
 
@@ -1855,7 +1854,8 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, cons
 // current row's own influence and runs on one wavefront without work-group barriers.
 // Summation order differs from gf3_AijPj; the sums are the same.  Returns (itgs, dif,
 // algorithmic flops, wall-clock ticks (100 MHz) spent in the row sums and in the sequential
-// parts, dif of the first sweep).
+// parts, dif of the first sweep, and -- built with GS_PHASE_TIMING -- the ticks of the four
+// steps of the sequential part).
 
 // LDS of k_stdygs2 without the staging area; the staging area holds `sr` source rows
 __host__ __device__ inline size_t gs2_base_lds(int mx, int my)
@@ -1934,6 +1934,9 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
     int itgs = 0;
     double dif = 2.0, difid = 1.0, dif1 = 0.0;
     unsigned long long t_rows = 0, t_inrow = 0;   // thread 0: wall-clock ticks of the two phases
+    // with GS_PHASE_TIMING: ticks of the in-row steps (own-row sum, upstream search, element
+    // problem, chain rebuild), thread 0
+    unsigned long long ph[4] = {0, 0, 0, 0};
     while (dif >= difid && itgs < I.maxgs) {
         itgs++;
         double difacc = 0.0;   // thread 0
@@ -2009,6 +2012,9 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                 const int lane = tid, j0 = rj0[iy], j1 = rj1[iy];
                 for (int ix = t0; ix < t0 + nt; ix++) {
                     if (rel[ix] < ADHES) continue;
+#ifdef GS_PHASE_TIMING
+                    const unsigned long long c0 = wall_clock64();
+#endif
                     double qsx = 0.0, qsy = 0.0;
                     for (int jx = j0 + lane; jx <= j1; jx += 64) {
                         int o = ix - jx + mx;
@@ -2020,6 +2026,9 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                         qsx += __shfl_xor(qsx, off);
                         qsy += __shfl_xor(qsy, off);
                     }
+#ifdef GS_PHASE_TIMING
+                    const unsigned long long c1 = wall_clock64();
+#endif
                     // first upstream element not in adhesion (:2972-2977): the largest jx in
                     // [1, ix-1] with another state, else 0 (-1 at the trailing edge)
                     int jx = ix >= 1 ? 0 : -1;
@@ -2031,6 +2040,9 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                             break;
                         }
                     }
+#ifdef GS_PHASE_TIMING
+                    const unsigned long long c2 = wall_clock64();
+#endif
                     double Dx = 0.0, Dy = 0.0;
                     if (lane == 0) {
                         const size_t ii = r0 + ix;
@@ -2064,6 +2076,9 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                     // handles 64 upstream elements per step.
                     Dx = __shfl(Dx, 0);
                     Dy = __shfl(Dy, 0);
+#ifdef GS_PHASE_TIMING
+                    const unsigned long long c3 = wall_clock64();
+#endif
                     int hi = ix - 1;
                     while (hi >= 0) {
                         const int kx = hi - lane;
@@ -2126,6 +2141,15 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                         __builtin_amdgcn_wave_barrier();
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     }
+#ifdef GS_PHASE_TIMING
+                    if (lane == 0) {
+                        const unsigned long long c4 = wall_clock64();
+                        ph[0] += c1 - c0;
+                        ph[1] += c2 - c1;
+                        ph[2] += c3 - c2;
+                        ph[3] += c4 - c3;
+                    }
+#endif
                 }
             }
             __syncthreads();
@@ -2168,12 +2192,13 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
     if (tid == 0) {
         double len = 0.0;
         for (int jy = 0; jy < my; jy++) len += rj1[jy] >= rj0[jy] ? rj1[jy] - rj0[jy] + 1 : 0;
-        partials[(size_t)blockIdx.x * 6] = (double)itgs;
-        partials[(size_t)blockIdx.x * 6 + 1] = dif;
-        partials[(size_t)blockIdx.x * 6 + 2] = 8.0 * itgs * (double)I.k * len;
-        partials[(size_t)blockIdx.x * 6 + 3] = (double)t_rows;
-        partials[(size_t)blockIdx.x * 6 + 4] = (double)t_inrow;
-        partials[(size_t)blockIdx.x * 6 + 5] = dif1;
+        partials[(size_t)blockIdx.x * 10] = (double)itgs;
+        partials[(size_t)blockIdx.x * 10 + 1] = dif;
+        partials[(size_t)blockIdx.x * 10 + 2] = 8.0 * itgs * (double)I.k * len;
+        partials[(size_t)blockIdx.x * 10 + 3] = (double)t_rows;
+        partials[(size_t)blockIdx.x * 10 + 4] = (double)t_inrow;
+        partials[(size_t)blockIdx.x * 10 + 5] = dif1;
+        for (int q = 0; q < 4; q++) partials[(size_t)blockIdx.x * 10 + 6 + q] = (double)ph[q];
     }
 }
 
@@ -2208,7 +2233,7 @@ int op_nred(int kind, const void* item)
     case OP_GD_DIAGRES: return 3;
     case OP_SETUP: return 2;
     case OP_FORCES: return 5;
-    case OP_STDYGS: return 6;
+    case OP_STDYGS: return 10;
     case OP_GUARD: return 5;
     case OP_SENS_EXT: return 4;
     default: return 0;
