@@ -1007,13 +1007,26 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
                 fk0 = pr[1] * six - pr[0] * siy;
                 fk1 = pr[0] * pr[0] + pr[1] * pr[1] - trc_bound * trc_bound;
             }
-            double alph0 = atan2(pold2, pold1), alph1 = atan2(pr[1], pr[0]);
-            double dalph = alph1 - alph0;
-            if (dalph < -PI) dalph += 2.0 * PI;
-            if (dalph > PI) dalph -= 2.0 * PI;
-            alph1 = alph0 + omegas * dalph;
-            pr[0] = trc_bound * cos(alph1);
-            pr[1] = trc_bound * sin(alph1);
+            if (omegas == 1.0) {
+                // alph0 + (alph1 - alph0) is alph1 up to rounding (the 2 pi wrap keeps the
+                // direction): the traction at the bound in the Newton solution's direction
+                double pabs = sqrt(pr[0] * pr[0] + pr[1] * pr[1]);
+                if (pabs > 0.0) {
+                    pr[0] = trc_bound * (pr[0] / pabs);
+                    pr[1] = trc_bound * (pr[1] / pabs);
+                } else {
+                    pr[0] = trc_bound;
+                    pr[1] = 0.0;
+                }
+            } else {
+                double alph0 = atan2(pold2, pold1), alph1 = atan2(pr[1], pr[0]);
+                double dalph = alph1 - alph0;
+                if (dalph < -PI) dalph += 2.0 * PI;
+                if (dalph > PI) dalph -= 2.0 * PI;
+                alph1 = alph0 + omegas * dalph;
+                pr[0] = trc_bound * cos(alph1);
+                pr[1] = trc_bound * sin(alph1);
+            }
             if (fabs(pr[0]) > fabs(pr[1])) {
                 if (pr[0] * six > 0.0) violat = 1;
             } else {
@@ -1621,7 +1634,10 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_sens(const SensItem* items, co
 // SteadyGS, src/m_solvpt.f90:2825-3254 (chi = 0, no plasticity, no third body)
 // ------------------------------------------------------------------------------------------
 
-constexpr int GS_THREADS = 1024;
+#ifndef GS_THREADS_N
+#define GS_THREADS_N 512
+#endif
+constexpr int GS_THREADS = GS_THREADS_N;
 
 // Rows with an interior element in the trailing-edge column ix = 1: steady rolling then
 // needs ConvexGS (src/m_stang.f90:131-143, 185-190).  One thread per row.  Also returns the
@@ -1852,6 +1868,9 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, cons
 // (each source row's coefficient row and differences staged through LDS, alternating between
 // two buffers, one barrier per source row), so that the sequential part along the row only adds the
 // current row's own influence and runs on one wavefront without work-group barriers.
+// The row sums are Toeplitz products along x: a thread owns GS_TB consecutive targets and
+// takes the sources in chunks of GS_QB, so that one window of GS_TB + GS_QB - 1 coefficients
+// per block serves GS_TB x GS_QB products from registers.
 // Summation order differs from gf3_AijPj; the sums are the same.  Returns (itgs, dif,
 // algorithmic flops, wall-clock ticks (100 MHz) spent in the row sums and in the sequential
 // parts, dif of the first sweep, and -- built with GS_PHASE_TIMING -- the ticks of the four
@@ -1863,7 +1882,19 @@ __host__ __device__ inline size_t gs2_base_lds(int mx, int my)
     return (size_t)(10 * mx) * sizeof(double) + 2 * GS_THREADS * sizeof(double) +
            (size_t)4 * my * sizeof(int) + mx + 64;
 }
-__host__ __device__ inline size_t gs2_row_lds(int mx) { return (size_t)(10 * mx) * sizeof(double); }
+#ifndef GS_TB_N
+#define GS_TB_N 8
+#endif
+constexpr int GS_TB = GS_TB_N, GS_QB = 4;
+// doubles of one staging buffer: 4 coefficient windows of up to 2 mx + GS_TB + GS_QB values
+// (each padded to a multiple of 4) and 2 x (mx + GS_QB) differences; this also holds the
+// 4 x 2 mx values of the own-row coefficient row
+__host__ __device__ inline int gs2_window(int mx) { return (2 * mx + GS_TB + GS_QB + 3) / 4 * 4; }
+__host__ __device__ inline int gs2_stage_stride(int mx)
+{
+    return (4 * gs2_window(mx) + 2 * (mx + GS_QB) + 7) / 8 * 8;
+}
+__host__ __device__ inline size_t gs2_row_lds(int mx) { return (size_t)gs2_stage_stride(mx) * sizeof(double); }
 // two staging buffers: the next source row is written while others still read the current
 __host__ __device__ inline int gs2_stage_rows(int mx, int my) { (void)mx; (void)my; return 2; }
 __host__ __device__ inline size_t gs2_lds(int mx, int my)
@@ -1944,50 +1975,98 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
             if (ri1[iy] < ri0[iy]) continue;
             const unsigned long long tr0 = wall_clock64();
             const int t0 = ri0[iy], nt = ri1[iy] - t0 + 1;
-            const int P = T / nt > 0 ? T / nt : 1;
-            const int ti = tid % nt, part = tid / nt;
-            const bool active = part < P && ti < nt;
+            const int ntg = (nt + GS_TB - 1) / GS_TB, ntp = ntg * GS_TB;
+            const int P = T / ntg > 0 ? T / ntg : 1;
+            const int tg = tid % ntg, part = tid / ntg;
+            const bool active = part < P;
             const size_t r0 = (size_t)iy * mx;
-            double ax = 0.0, ay = 0.0;
+            const int SW = gs2_stage_stride(mx);
+            double ax[GS_TB], ay[GS_TB];
+#pragma unroll
+            for (int i = 0; i < GS_TB; i++) ax[i] = ay[i] = 0.0;
             int sbuf = 0;
             for (int jy = 0; jy < my; jy++) {
                 const int j0 = rj0[jy], j1 = rj1[jy];
                 if (jy == iy || j1 < j0) continue;
-                const int d = iy - jy, xlo = t0 - j1, w = t0 + nt - 1 - j0 - xlo + 1;
+                // sources padded with zero differences to nch chunks; window S[k], k = t - q + Q - 1,
+                // holds the coefficient of target t0 + t and source j0 + q (zero outside the block)
+                const int d = iy - jy, nq = j1 - j0 + 1, nch = (nq + GS_QB - 1) / GS_QB, Q = nch * GS_QB;
+                const int Wn = ntp + Q - 1, Wp = (Wn + 1 + 3) / 4 * 4;
+                const int col0 = t0 - j0 - (Q - 1) + mx;
                 // two buffers used in turn: one barrier per source row suffices
-                double* sc = stg + (size_t)sbuf * 10 * mx;
+                double* sc = stg + (size_t)sbuf * SW;
                 sbuf ^= 1;
-                for (int e = tid; e < 4 * w; e += T) {
-                    int b = e / w, q = e - b * w;
-                    sc[b * M2 + q] = cfb[b][(size_t)(d + my) * M2 + (xlo + q + mx)];
+                for (int e = tid; e < 4 * Wp; e += T) {
+                    int b = e / Wp, k = e - b * Wp, col = col0 + k;
+                    sc[e] = (k < Wn && col >= 0 && col < M2) ? cfb[b][(size_t)(d + my) * M2 + col] : 0.0;
                 }
-                for (int e = tid; e <= j1 - j0; e += T) {
-                    sc[8 * mx + e] = dpx[(size_t)jy * mx + j0 + e];
-                    sc[9 * mx + e] = dpy[(size_t)jy * mx + j0 + e];
+                double* sdx = sc + 4 * Wp;
+                double* sdy = sdx + Q;
+                for (int e = tid; e < Q; e += T) {
+                    const bool in = e < nq;
+                    sdx[e] = in ? dpx[(size_t)jy * mx + j0 + e] : 0.0;
+                    sdy[e] = in ? dpy[(size_t)jy * mx + j0 + e] : 0.0;
                 }
                 __syncthreads();
                 if (active) {
-                    const int ix = t0 + ti;
-                    for (int q = part; q <= j1 - j0; q += P) {
-                        int o = ix - (j0 + q) - xlo;
-                        double qx = sc[8 * mx + q], qy = sc[9 * mx + q];
-                        ax += sc[o] * qx + sc[M2 + o] * qy;
-                        ay += sc[2 * M2 + o] * qx + sc[3 * M2 + o] * qy;
+                    for (int c = part; c < nch; c += P) {
+                        const int kmin = tg * GS_TB - c * GS_QB + (Q - GS_QB);
+                        double qx[GS_QB], qy[GS_QB];
+#pragma unroll
+                        for (int j = 0; j < GS_QB; j++) {
+                            qx[j] = sdx[c * GS_QB + j];
+                            qy[j] = sdy[c * GS_QB + j];
+                        }
+#pragma unroll
+                        for (int h = 0; h < 2; h++) {
+                            // blocks (0, 1) give the x sums, blocks (2, 3) the y sums
+                            const double2* wa = (const double2*)(sc + (2 * h) * Wp + kmin);
+                            const double2* wb = (const double2*)(sc + (2 * h + 1) * Wp + kmin);
+                            double va[GS_TB + GS_QB], vb[GS_TB + GS_QB];
+#pragma unroll
+                            for (int m = 0; m < (GS_TB + GS_QB) / 2; m++) {
+                                const double2 u = wa[m], v = wb[m];
+                                va[2 * m] = u.x;
+                                va[2 * m + 1] = u.y;
+                                vb[2 * m] = v.x;
+                                vb[2 * m + 1] = v.y;
+                            }
+#pragma unroll
+                            for (int i = 0; i < GS_TB; i++) {
+                                double acc = h == 0 ? ax[i] : ay[i];
+#pragma unroll
+                                for (int j = 0; j < GS_QB; j++) {
+                                    const int m = i + GS_QB - 1 - j;
+                                    acc += va[m] * qx[j] + vb[m] * qy[j];
+                                }
+                                if (h == 0) ax[i] = acc;
+                                else ay[i] = acc;
+                            }
+                        }
                     }
                 }
             }
             __syncthreads();
-            red[2 * tid] = ax;
-            red[2 * tid + 1] = ay;
-            __syncthreads();
-            for (int c = tid; c < nt; c += T) {
-                double sx = 0.0, sy = 0.0;
-                for (int p = 0; p < P; p++) {
-                    sx += red[2 * (p * nt + c)];
-                    sy += red[2 * (p * nt + c) + 1];
+            // sums over the parts, one target of each group per pass
+            for (int i = 0; i < GS_TB; i++) {
+                if (active) {
+                    red[2 * (part * ntg + tg)] = ax[i];
+                    red[2 * (part * ntg + tg) + 1] = ay[i];
                 }
-                ucx[t0 + c] = sx;
-                ucy[t0 + c] = sy;
+                __syncthreads();
+                for (int g = tid; g < ntg; g += T) {
+                    const int c = g * GS_TB + i;
+                    if (c < nt) {
+                        double sx = 0.0, sy = 0.0;
+                        for (int p = 0; p < P; p++) {
+                            sx += red[2 * (p * ntg + g)];
+                            sy += red[2 * (p * ntg + g) + 1];
+                        }
+                        ucx[t0 + c] = sx;
+                        ucy[t0 + c] = sy;
+                    }
+                }
+                __syncthreads();
             }
             for (int ix = tid; ix < mx; ix += T) {
                 rpx[ix] = I.px[r0 + ix];
@@ -2010,8 +2089,15 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
             const unsigned long long tr1 = wall_clock64();
             if (tid < 64) {
                 const int lane = tid, j0 = rj0[iy], j1 = rj1[iy];
+                // first upstream element not in adhesion (:2972-2977): the largest jx in [1, ix-1]
+                // with another state, else 0 (-1 at the trailing edge), carried along the row;
+                // the elements before the first interior one are exterior
+                int lastna = t0 - 1 >= 1 ? t0 - 1 : 0;
                 for (int ix = t0; ix < t0 + nt; ix++) {
-                    if (rel[ix] < ADHES) continue;
+                    if (rel[ix] < ADHES) {
+                        if (ix >= 1) lastna = ix;
+                        continue;
+                    }
 #ifdef GS_PHASE_TIMING
                     const unsigned long long c0 = wall_clock64();
 #endif
@@ -2029,17 +2115,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
 #ifdef GS_PHASE_TIMING
                     const unsigned long long c1 = wall_clock64();
 #endif
-                    // first upstream element not in adhesion (:2972-2977): the largest jx in
-                    // [1, ix-1] with another state, else 0 (-1 at the trailing edge)
-                    int jx = ix >= 1 ? 0 : -1;
-                    for (int hi = ix - 1; hi > 0; hi -= 64) {
-                        const int kx = hi - lane;
-                        const unsigned long long m = __ballot(kx >= 1 && rel[kx] != ADHES);
-                        if (m) {
-                            jx = hi - (__ffsll((long long)m) - 1);
-                            break;
-                        }
-                    }
+                    const int jx = ix >= 1 ? lastna : -1;
 #ifdef GS_PHASE_TIMING
                     const unsigned long long c2 = wall_clock64();
 #endif
@@ -2150,6 +2226,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                         ph[3] += c4 - c3;
                     }
 #endif
+                    if (ix >= 1 && rel[ix] != ADHES) lastna = ix;
                 }
             }
             __syncthreads();
