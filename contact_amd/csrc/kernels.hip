@@ -940,9 +940,12 @@ __global__ void __launch_bounds__(GD_THREADS) k_gd_update(const GdItem* items, c
 
 // plstrc without plasticity (elmtrc), src/m_solvpt.f90:3258-3807; the plastic branch only
 // reports a violated constraint when tau_c0 = 1e20 and k_tau = 0
+// scale_dir: with omegas = 1 put the slip traction on the bound along the Newton solution
+// directly instead of through atan2, cos and sin (the same direction up to rounding); used on
+// the sequential path of SteadyGS, where the transcendental chain is latency
 __device__ void elmtrc(int8_t& el, const double c11, const double c12, const double c21,
                        const double c22, double eps, double pr[3], double mus, double s[2],
-                       double omegah = 1.0, double omegas = 1.0)
+                       double omegah = 1.0, double omegas = 1.0, bool scale_dir = false)
 {
     const int maxnr = 10;
     const double prc = 0.0001, taucs = 1e20, PI = 3.14159265358979323846;
@@ -1007,7 +1010,7 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
                 fk0 = pr[1] * six - pr[0] * siy;
                 fk1 = pr[0] * pr[0] + pr[1] * pr[1] - trc_bound * trc_bound;
             }
-            if (omegas == 1.0) {
+            if (scale_dir && omegas == 1.0) {
                 // alph0 + (alph1 - alph0) is alph1 up to rounding (the 2 pi wrap keeps the
                 // direction): the traction at the bound in the Newton solution's direction
                 double pabs = sqrt(pr[0] * pr[0] + pr[1] * pr[1]);
@@ -2130,7 +2133,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                         double sv[2] = {rwx[ix] + I.ga_inv * (ucx[ix] + qsx), rwy[ix] + I.ga_inv * (ucy[ix] + qsy)};
                         double pr[3] = {rpx[ix], rpy[ix], rpn[ix]};
                         int8_t e8 = rel[ix];
-                        elmtrc(e8, a11, a12, a21, a22, I.eps, pr, I.mu, sv, I.omegah, I.omegas);
+                        elmtrc(e8, a11, a12, a21, a22, I.eps, pr, I.mu, sv, I.omegah, I.omegas, true);
                         Dx = pr[0] - rpx[ix];
                         Dy = pr[1] - rpy[ix];
                         difacc += Dx * Dx + Dy * Dy;
