@@ -1888,7 +1888,10 @@ __host__ __device__ inline size_t gs2_base_lds(int mx, int my)
 #ifndef GS_TB_N
 #define GS_TB_N 8
 #endif
-constexpr int GS_TB = GS_TB_N, GS_QB = 4;
+#ifndef GS_QB_N
+#define GS_QB_N 4
+#endif
+constexpr int GS_TB = GS_TB_N, GS_QB = GS_QB_N;
 // doubles of one staging buffer: 4 coefficient windows of up to 2 mx + GS_TB + GS_QB values
 // (each padded to a multiple of 4) and 2 x (mx + GS_QB) differences; this also holds the
 // 4 x 2 mx values of the own-row coefficient row
@@ -1899,7 +1902,13 @@ __host__ __device__ inline int gs2_stage_stride(int mx)
 }
 __host__ __device__ inline size_t gs2_row_lds(int mx) { return (size_t)gs2_stage_stride(mx) * sizeof(double); }
 // two staging buffers: the next source row is written while others still read the current
-__host__ __device__ inline int gs2_stage_rows(int mx, int my) { (void)mx; (void)my; return 2; }
+// source rows staged per barrier (GS_R, when two groups of them fit next to the fixed part)
+constexpr int GS_R = 2;
+__host__ __device__ inline int gs2_rows_per_stage(int mx, int my)
+{
+    return gs2_base_lds(mx, my) + (size_t)2 * GS_R * gs2_row_lds(mx) <= MAX_LDS ? GS_R : 1;
+}
+__host__ __device__ inline int gs2_stage_rows(int mx, int my) { return 2 * gs2_rows_per_stage(mx, my); }
 __host__ __device__ inline size_t gs2_lds(int mx, int my)
 {
     return gs2_base_lds(mx, my) + (size_t)gs2_stage_rows(mx, my) * gs2_row_lds(mx);
@@ -1987,63 +1996,84 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
             double ax[GS_TB], ay[GS_TB];
 #pragma unroll
             for (int i = 0; i < GS_TB; i++) ax[i] = ay[i] = 0.0;
-            int sbuf = 0;
-            for (int jy = 0; jy < my; jy++) {
-                const int j0 = rj0[jy], j1 = rj1[jy];
-                if (jy == iy || j1 < j0) continue;
-                // sources padded with zero differences to nch chunks; window S[k], k = t - q + Q - 1,
-                // holds the coefficient of target t0 + t and source j0 + q (zero outside the block)
-                const int d = iy - jy, nq = j1 - j0 + 1, nch = (nq + GS_QB - 1) / GS_QB, Q = nch * GS_QB;
-                const int Wn = ntp + Q - 1, Wp = (Wn + 1 + 3) / 4 * 4;
-                const int col0 = t0 - j0 - (Q - 1) + mx;
-                // two buffers used in turn: one barrier per source row suffices
-                double* sc = stg + (size_t)sbuf * SW;
-                sbuf ^= 1;
-                for (int e = tid; e < 4 * Wp; e += T) {
-                    int b = e / Wp, k = e - b * Wp, col = col0 + k;
-                    sc[e] = (k < Wn && col >= 0 && col < M2) ? cfb[b][(size_t)(d + my) * M2 + col] : 0.0;
-                }
-                double* sdx = sc + 4 * Wp;
-                double* sdy = sdx + Q;
-                for (int e = tid; e < Q; e += T) {
-                    const bool in = e < nq;
-                    sdx[e] = in ? dpx[(size_t)jy * mx + j0 + e] : 0.0;
-                    sdy[e] = in ? dpy[(size_t)jy * mx + j0 + e] : 0.0;
+            // source rows staged R at a time into one of two groups of buffers used in turn: one
+            // barrier per group suffices
+            const int R = gs2_rows_per_stage(mx, my);
+            int sgrp = 0, jy = 0;
+            while (true) {
+                int rows[GS_R], nr = 0;
+                for (; jy < my && nr < R; jy++)
+                    if (jy != iy && rj1[jy] >= rj0[jy]) rows[nr++] = jy;
+                if (nr == 0) break;
+                double* grp = stg + (size_t)sgrp * R * SW;
+                sgrp ^= 1;
+                for (int r = 0; r < nr; r++) {
+                    // sources padded with zero differences to nch chunks; window S[k],
+                    // k = t - q + Q - 1, holds the coefficient of target t0 + t and source j0 + q
+                    // (zero outside the block)
+                    const int sy = rows[r], j0 = rj0[sy], nq = rj1[sy] - j0 + 1;
+                    const int Q = (nq + GS_QB - 1) / GS_QB * GS_QB;
+                    const int Wn = ntp + Q - 1, Wp = (Wn + 1 + 3) / 4 * 4;
+                    const int col0 = t0 - j0 - (Q - 1) + mx;
+                    double* sc = grp + (size_t)r * SW;
+#pragma unroll
+                    for (int bb = 0; bb < 4; bb++) {
+                        const double* crow = cfb[bb] + (size_t)(iy - sy + my) * M2;
+                        for (int k = tid; k < Wp; k += T) {
+                            const int col = col0 + k;
+                            sc[bb * Wp + k] = (k < Wn && col >= 0 && col < M2) ? crow[col] : 0.0;
+                        }
+                    }
+                    double* sdx = sc + 4 * Wp;
+                    double* sdy = sdx + Q;
+                    for (int e = tid; e < Q; e += T) {
+                        const bool in = e < nq;
+                        sdx[e] = in ? dpx[(size_t)sy * mx + j0 + e] : 0.0;
+                        sdy[e] = in ? dpy[(size_t)sy * mx + j0 + e] : 0.0;
+                    }
                 }
                 __syncthreads();
                 if (active) {
-                    for (int c = part; c < nch; c += P) {
-                        const int kmin = tg * GS_TB - c * GS_QB + (Q - GS_QB);
-                        double qx[GS_QB], qy[GS_QB];
+                    for (int r = 0; r < nr; r++) {
+                        const int sy = rows[r], nq = rj1[sy] - rj0[sy] + 1;
+                        const int nch = (nq + GS_QB - 1) / GS_QB, Q = nch * GS_QB;
+                        const int Wp = (ntp + Q + 3) / 4 * 4;
+                        const double* sc = grp + (size_t)r * SW;
+                        const double* sdx = sc + 4 * Wp;
+                        const double* sdy = sdx + Q;
+                        for (int c = part; c < nch; c += P) {
+                            const int kmin = tg * GS_TB - c * GS_QB + (Q - GS_QB);
+                            double qx[GS_QB], qy[GS_QB];
 #pragma unroll
-                        for (int j = 0; j < GS_QB; j++) {
-                            qx[j] = sdx[c * GS_QB + j];
-                            qy[j] = sdy[c * GS_QB + j];
-                        }
-#pragma unroll
-                        for (int h = 0; h < 2; h++) {
-                            // blocks (0, 1) give the x sums, blocks (2, 3) the y sums
-                            const double2* wa = (const double2*)(sc + (2 * h) * Wp + kmin);
-                            const double2* wb = (const double2*)(sc + (2 * h + 1) * Wp + kmin);
-                            double va[GS_TB + GS_QB], vb[GS_TB + GS_QB];
-#pragma unroll
-                            for (int m = 0; m < (GS_TB + GS_QB) / 2; m++) {
-                                const double2 u = wa[m], v = wb[m];
-                                va[2 * m] = u.x;
-                                va[2 * m + 1] = u.y;
-                                vb[2 * m] = v.x;
-                                vb[2 * m + 1] = v.y;
+                            for (int j = 0; j < GS_QB; j++) {
+                                qx[j] = sdx[c * GS_QB + j];
+                                qy[j] = sdy[c * GS_QB + j];
                             }
 #pragma unroll
-                            for (int i = 0; i < GS_TB; i++) {
-                                double acc = h == 0 ? ax[i] : ay[i];
+                            for (int h = 0; h < 2; h++) {
+                                // blocks (0, 1) give the x sums, blocks (2, 3) the y sums
+                                const double2* wa = (const double2*)(sc + (2 * h) * Wp + kmin);
+                                const double2* wb = (const double2*)(sc + (2 * h + 1) * Wp + kmin);
+                                double va[GS_TB + GS_QB], vb[GS_TB + GS_QB];
 #pragma unroll
-                                for (int j = 0; j < GS_QB; j++) {
-                                    const int m = i + GS_QB - 1 - j;
-                                    acc += va[m] * qx[j] + vb[m] * qy[j];
+                                for (int m = 0; m < (GS_TB + GS_QB) / 2; m++) {
+                                    const double2 u = wa[m], v = wb[m];
+                                    va[2 * m] = u.x;
+                                    va[2 * m + 1] = u.y;
+                                    vb[2 * m] = v.x;
+                                    vb[2 * m + 1] = v.y;
                                 }
-                                if (h == 0) ax[i] = acc;
-                                else ay[i] = acc;
+#pragma unroll
+                                for (int i = 0; i < GS_TB; i++) {
+                                    double acc = h == 0 ? ax[i] : ay[i];
+#pragma unroll
+                                    for (int j = 0; j < GS_QB; j++) {
+                                        const int m = i + GS_QB - 1 - j;
+                                        acc += va[m] * qx[j] + vb[m] * qy[j];
+                                    }
+                                    if (h == 0) ax[i] = acc;
+                                    else ay[i] = acc;
+                                }
                             }
                         }
                     }
