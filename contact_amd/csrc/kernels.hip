@@ -2152,28 +2152,34 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
 #ifdef GS_PHASE_TIMING
                     const unsigned long long c2 = wall_clock64();
 #endif
-                    double Dx = 0.0, Dy = 0.0;
-                    if (lane == 0) {
-                        const size_t ii = r0 + ix;
+                    // the element problem, solved by every lane on the same (broadcast) data so that
+                    // its change D is known wavefront-wide; lane 0 stores
+                    const int8_t eup = ix >= 1 ? rel[ix - 1] : EXTER;
+                    double Dx, Dy;
+                    {
                         const int b0 = mx, bj = mx + (jx - ix);
                         double a11 = I.ga_inv * (stc[b0] - stc[bj]);
                         double a12 = I.ga_inv * (stc[M2 + b0] - stc[M2 + bj]);
                         double a21 = I.ga_inv * (stc[2 * M2 + b0] - stc[2 * M2 + bj]);
                         double a22 = I.ga_inv * (stc[3 * M2 + b0] - stc[3 * M2 + bj]);
                         double sv[2] = {rwx[ix] + I.ga_inv * (ucx[ix] + qsx), rwy[ix] + I.ga_inv * (ucy[ix] + qsy)};
-                        double pr[3] = {rpx[ix], rpy[ix], rpn[ix]};
+                        const double opx = rpx[ix], opy = rpy[ix];
+                        double pr[3] = {opx, opy, rpn[ix]};
                         int8_t e8 = rel[ix];
                         elmtrc(e8, a11, a12, a21, a22, I.eps, pr, I.mu, sv, I.omegah, I.omegas, true);
-                        Dx = pr[0] - rpx[ix];
-                        Dy = pr[1] - rpy[ix];
-                        difacc += Dx * Dx + Dy * Dy;
-                        rdpx[ix] += Dx;
-                        rdpy[ix] += Dy;
-                        rpx[ix] = pr[0];
-                        rpy[ix] = pr[1];
-                        rel[ix] = e8;
-                        I.sx[ii] = sv[0];
-                        I.sy[ii] = sv[1];
+                        Dx = pr[0] - opx;
+                        Dy = pr[1] - opy;
+                        if (lane == 0) {
+                            const size_t ii = r0 + ix;
+                            difacc += Dx * Dx + Dy * Dy;
+                            rdpx[ix] += Dx;
+                            rdpy[ix] += Dy;
+                            rpx[ix] = pr[0];
+                            rpy[ix] = pr[1];
+                            rel[ix] = e8;
+                            I.sx[ii] = sv[0];
+                            I.sy[ii] = sv[1];
+                        }
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
@@ -2183,12 +2189,27 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
                     // p(ix) until one exceeds its traction bound (clipped there, D becomes that
                     // element's change) or a non-adhesion element ends the chain.  The wavefront
                     // handles 64 upstream elements per step.
-                    Dx = __shfl(Dx, 0);
-                    Dy = __shfl(Dy, 0);
 #ifdef GS_PHASE_TIMING
                     const unsigned long long c3 = wall_clock64();
 #endif
                     int hi = ix - 1;
+                    if (ix >= 1 && eup != ADHES) {
+                        // the upstream neighbour ends the chain at once
+                        if (lane == 0) {
+                            const int kx = ix - 1;
+                            if (eup == SLIP || eup == PLAST) {
+                                rdpx[kx] = rpx[kx] - rpx[ix];
+                                rdpy[kx] = rpy[kx] - rpy[ix];
+                            } else {
+                                rdpx[kx] = -rpx[ix];
+                                rdpy[kx] = -rpy[ix];
+                            }
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        hi = -1;
+                    }
                     while (hi >= 0) {
                         const int kx = hi - lane;
                         const bool valid = kx >= 0;
