@@ -477,6 +477,7 @@ private:
     };
     void flush()
     {
+        const auto tf0 = std::chrono::steady_clock::now();
         // collect requests level by level; side-stream operations are set aside (a task
         // waiting on one has nothing queued behind it)
         std::vector<Req*> flat, async_reqs;
@@ -612,7 +613,12 @@ private:
             async_wait_[r->slot] = 1;
         }
         if (res > 0) HIPCHK(hipMemcpyAsync(hres_, dres_, res * sizeof(double), hipMemcpyDeviceToHost, s_));
+        const auto tf1 = std::chrono::steady_clock::now();
         HIPCHK(hipStreamSynchronize(s_));
+        const auto tf2 = std::chrono::steady_clock::now();
+        flush_issue_s += std::chrono::duration<double>(tf1 - tf0).count();
+        flush_wait_s += std::chrono::duration<double>(tf2 - tf1).count();
+        flush_launches += (long)launches.size();
         for (size_t k = 0; k < pkeys.size(); k++) {
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, ev_pool[2 * k], ev_pool[2 * k + 1]));
@@ -650,6 +656,9 @@ private:
 
 public:
     long nflush = 0;
+    // host seconds spent issuing flushes and waiting for them, kernel launches issued
+    double flush_issue_s = 0.0, flush_wait_s = 0.0;
+    long flush_launches = 0;
 };
 
 // awaiter of a request with results
@@ -792,6 +801,9 @@ Engine::Engine()
 Engine::~Engine()
 {
     serve_stop();
+    if (const char* v = std::getenv("CNTC_FLUSH_STATS"); v && v[0] == '1' && sched_)
+        std::fprintf(stderr, "flushes %ld, launches %ld, host issue %.3f s, wait %.3f s\n", sched_->nflush,
+                     sched_->flush_launches, sched_->flush_issue_s, sched_->flush_wait_s);
     sched_.reset();
     for (auto& t : tw_) (void)hipFree(t.second);
     if (stream_) (void)hipStreamDestroy(stream_);
