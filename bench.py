@@ -31,6 +31,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# The engine runs SteadyGS launches (one work-group per patch, seconds long on the largest
+# patches) on side streams beside the main stream; with HIP's default of 4 hardware queues per
+# process only 3 of them run at once.  Eight queues let 7 side streams overlap (set before the
+# HIP runtime starts; an explicit larger setting is kept).
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 IRE_MIXED, IRE_FP64 = 1, 301
 
 

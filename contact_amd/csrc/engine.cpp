@@ -192,6 +192,14 @@ class Scheduler {
 public:
     explicit Scheduler(hipStream_t s) : s_(s)
     {
+        // one hardware queue stays with the main stream: side streams that share it would hold
+        // the main stream's work behind their long launches
+        int hwq = 4;
+        if (const char* v = std::getenv("GPU_MAX_HW_QUEUES")) hwq = std::max(2, std::atoi(v));
+        int nside = std::min(7, hwq - 1);
+        if (const char* v = std::getenv("CNTC_SIDE_STREAMS")) nside = std::max(1, std::min(hwq - 1, std::atoi(v)));
+        side_.assign(nside, nullptr);
+        if (const char* v = std::getenv("CNTC_PRIO_EXTRA")) PRIO_EXTRA = std::max(0, std::min(64, std::atoi(v)));
         for (auto& st : side_) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         cap_ = 64ull << 20;
         HIPCHK(hipHostMalloc((void**)&hstage_, cap_, hipHostMallocDefault));
@@ -218,7 +226,12 @@ public:
         q_.assign(nslots, {});
         waiter_.assign(nslots, nullptr);
         async_wait_.assign(nslots, 0);
+        prio_.assign(nslots, 0);
     }
+    // patches marked here get flushes of their own between the full ones (PRIO_EXTRA per full
+    // flush), so that the largest patches of a batch, which set its length, are not held to
+    // the pace of every other patch's work; each patch's own sequence of operations is unchanged
+    void set_priority(int slot, bool p) { prio_[slot] = p ? 1 : 0; }
     void enqueue(int slot, int kind, const AnyItem& it)
     {
         Req r;
@@ -250,12 +263,16 @@ public:
     // false when nothing was pending
     bool step(const std::vector<int>& slots)
     {
-        bool any = false;
+        bool any = false, anyp = false;
         for (int sl : slots)
-            if (!q_[sl].empty()) any = true;
+            if (!q_[sl].empty()) {
+                any = true;
+                if (prio_[sl]) anyp = true;
+            }
         if (any) {
-            flush();
-            resume_ready(slots);
+            const bool only = anyp && (prio_turn_++ % (PRIO_EXTRA + 1)) != 0;
+            flush(only);
+            resume_ready(slots, only);
         }
         launch_pending(!any_queued());
         if (!async_.empty()) any = complete_async(false) > 0 || any;
@@ -269,13 +286,19 @@ public:
         for (size_t i = 0; i < tasks.size(); i++) tasks[i].h.resume();
         for (;;) {
             if (any_queued()) {
-                flush();
-                resume_ready(slots);
+                bool anyp = false;
+                for (size_t sl = 0; sl < q_.size(); sl++)
+                    if (prio_[sl] && !q_[sl].empty()) anyp = true;
+                const bool only = anyp && (prio_turn_++ % (PRIO_EXTRA + 1)) != 0;
+                flush(only);
+                resume_ready(slots, only);
             }
             launch_pending(!any_queued());
             // side-stream operations: wait for one only when nothing else can proceed
             if (!async_.empty()) complete_async(!any_queued() && pending_.empty());
             if (!any_queued() && async_.empty() && pending_.empty()) break;
+            // side-stream work waits for a stream: poll gently
+            if (!any_queued() && !pending_.empty()) std::this_thread::sleep_for(std::chrono::microseconds(50));
         }
         for (auto& t : tasks)
             if (!t.h.done()) throw std::runtime_error("scheduler: a patch solve did not finish");
@@ -283,10 +306,10 @@ public:
 
 private:
     // resume the tasks of `slots` whose results arrived (not those waiting on a side stream)
-    void resume_ready(const std::vector<int>& slots)
+    void resume_ready(const std::vector<int>& slots, bool prio_only = false)
     {
         for (int sl : slots)
-            if (waiter_[sl] && !async_wait_[sl]) {
+            if (waiter_[sl] && !async_wait_[sl] && (!prio_only || prio_[sl])) {
                 auto h = waiter_[sl];
                 waiter_[sl] = nullptr;
                 h.resume();
@@ -307,6 +330,8 @@ private:
         size_t res_off = 0;
         std::vector<int> slots;
         std::vector<double*> dest;
+        double t_launch = 0.0;       // host clock at launch (CNTC_GS_TIMING)
+        long max_n = 0;
     };
     void free_async(AsyncOp& op)
     {
@@ -371,6 +396,10 @@ private:
                 for (int i = 0; i < op.nitems; i++) pe.bytes += op.K >= 3 ? r[(size_t)i * op.K + 2] : 0.0;
                 pe.launches++;
             }
+            if (const char* tv = std::getenv("CNTC_GS_TIMING"); tv && tv[0] == '2')
+                std::fprintf(stderr, "steadygs launch of %d items (largest %ld elements): launched at %.2f s, "
+                             "done by %.2f s\n", op.nitems, op.max_n, op.t_launch,
+                             std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count() - t_origin);
             if (const char* tv = std::getenv("CNTC_GS_TIMING"); tv && tv[0] == '1' && op.K >= 10)
                 for (int i = 0; i < op.nitems; i++) {
                     const double* q = r + (size_t)i * op.K;
@@ -408,23 +437,38 @@ private:
     }
     void launch_pending(bool main_idle)
     {
+        (void)main_idle;
         if (pending_.empty()) return;
-        bool idle_side = false;
-        for (auto st : side_) idle_side = idle_side || side_idle(st);
-        if (!main_idle && pending_.size() < 256 && !(idle_side && pending_.size() >= 16)) return;
-        std::vector<Req*> v1, v2;
         // CNTC_GS_KERNEL=1 selects the variant without row staging for every size (testing)
         const char* forced = std::getenv("CNTC_GS_KERNEL");
         bool force_v1 = forced && forced[0] == '1';
-        for (auto& r : pending_) {
-            bool small = !force_v1 && op_lds(r.kind, &r.item, 1) <= MAX_LDS;
-            (small ? v2 : v1).push_back(&r);
+        // largest patches first; one launch per idle side stream, each of patches of similar
+        // size (down to a quarter of the launch's largest, at most 64), because a launch
+        // returns when its longest item ends and a launch queued behind another on the same
+        // stream waits for all of it
+        std::vector<Req*> order;
+        for (auto& r : pending_) order.push_back(&r);
+        auto bigger = [](const Req* a, const Req* b) { return a->item.gs.n > b->item.gs.n; };
+        std::stable_sort(order.begin(), order.end(), bigger);
+        size_t k = 0;
+        std::vector<Req> rest;
+        for (auto st : side_) {
+            if (k >= order.size()) break;
+            if (!side_idle(st)) continue;
+            const long top = order[k]->item.gs.n;
+            std::vector<Req*> v1, v2;
+            for (size_t c = 0; k < order.size() && c < 64 && 4L * order[k]->item.gs.n >= top; k++, c++) {
+                Req* r = order[k];
+                bool small = !force_v1 && op_lds(r->kind, &r->item, 1) <= MAX_LDS;
+                (small ? v2 : v1).push_back(r);
+            }
+            if (!v2.empty()) launch_async(v2, 1, st);
+            if (!v1.empty()) launch_async(v1, 0, st);
         }
-        if (!v2.empty()) launch_async(v2, 1);
-        if (!v1.empty()) launch_async(v1, 0);
-        pending_.clear();
+        for (; k < order.size(); k++) rest.push_back(*order[k]);
+        pending_.swap(rest);
     }
-    void launch_async(const std::vector<Req*>& reqs, int variant)
+    void launch_async(const std::vector<Req*>& reqs, int variant, hipStream_t stream)
     {
         int kind = reqs[0]->kind, nitems = (int)reqs.size();
         size_t isz = item_size(kind);
@@ -433,13 +477,13 @@ private:
         size_t items_b = align(isz * nitems), gs_b = align(sizeof(int) * (nitems + 1));
         size_t res_b = align(sizeof(double) * K * nitems);
         AsyncOp op = take_async(items_b + gs_b + res_b);
-        // an idle side stream if there is one (operations on one stream run one after another)
-        op.stream = side_[next_side_++ % NSIDE];
-        for (auto st : side_)
-            if (side_idle(st)) { op.stream = st; break; }
+        op.stream = stream;
         op.nitems = nitems;
         op.K = K;
         op.res_off = items_b + gs_b;
+        op.t_launch = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count() - t_origin;
+        op.max_n = 0;
+        for (auto* r : reqs) op.max_n = std::max(op.max_n, (long)r->item.gs.n);
         size_t lds = 0;
         int* gs = (int*)(op.host + items_b);
         int tot = 0;
@@ -475,7 +519,8 @@ private:
         int kind;
         std::vector<int> reqs;   // indices into flat list
     };
-    void flush()
+    // prio_only: the queues of the priority patches only; the others stay queued
+    void flush(bool prio_only = false)
     {
         const auto tf0 = std::chrono::steady_clock::now();
         // collect requests level by level; side-stream operations are set aside (a task
@@ -483,11 +528,14 @@ private:
         std::vector<Req*> flat, async_reqs;
         std::vector<std::vector<Group>> levels;
         size_t maxlen = 0;
-        for (auto& q : q_) maxlen = std::max(maxlen, q.size());
+        auto taken = [&](size_t sl) { return !prio_only || prio_[sl]; };
+        for (size_t sl = 0; sl < q_.size(); sl++)
+            if (taken(sl)) maxlen = std::max(maxlen, q_[sl].size());
         for (size_t lev = 0; lev < maxlen; lev++) {
             std::map<int, Group> byk;
-            for (auto& q : q_)
-                if (lev < q.size()) {
+            for (size_t sl = 0; sl < q_.size(); sl++) {
+                auto& q = q_[sl];
+                if (taken(sl) && lev < q.size()) {
                     if (is_async_kind(q[lev].kind)) {
                         async_reqs.push_back(&q[lev]);
                         continue;
@@ -498,6 +546,7 @@ private:
                     g.kind = q[lev].kind;
                     g.reqs.push_back(idx);
                 }
+            }
             std::vector<Group> gs;
             for (auto& kv : byk) gs.push_back(kv.second);
             levels.push_back(gs);
@@ -632,17 +681,22 @@ private:
                 int K = op_nred(rm.first->kind, &rm.first->item);
                 for (int k = 0; k < K; k++) rm.first->res[k] = hres_[rm.second + k];
             }
-        for (auto& q : q_) q.clear();
+        for (size_t sl = 0; sl < q_.size(); sl++)
+            if (taken(sl)) q_[sl].clear();
         nflush++;
     }
 
     hipStream_t s_;
-    static constexpr int NSIDE = 3;
-    hipStream_t side_[NSIDE] = {};
-    unsigned next_side_ = 0;
+    // side streams (CNTC_SIDE_STREAMS, default and at most GPU_MAX_HW_QUEUES - 1, at most 7 by
+    // default): operations on one stream run one after another, so more streams let more
+    // SteadyGS launches run at once
+    std::vector<hipStream_t> side_;
     std::vector<AsyncOp> async_, async_free_;
     std::vector<Req> pending_;
     std::vector<char> async_wait_;
+    std::vector<char> prio_;
+    int PRIO_EXTRA = 2;        // CNTC_PRIO_EXTRA
+    long prio_turn_ = 0;
     std::vector<std::vector<Req>> q_;
     std::vector<std::coroutine_handle<>> waiter_;
     char* hstage_ = nullptr;
@@ -656,6 +710,7 @@ private:
 
 public:
     long nflush = 0;
+    double t_origin = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
     // host seconds spent issuing flushes and waiting for them, kernel launches issued
     double flush_issue_s = 0.0, flush_wait_s = 0.0;
     long flush_launches = 0;
@@ -2494,6 +2549,10 @@ int Engine::solve(const std::vector<Patch*>& patches)
     sched_->resize((int)ok.size());
     std::vector<Task> tasks;
     std::vector<int> slots;
+    // the largest patches of a big batch set its length: give them flushes of their own
+    long maxn = 0;
+    for (Patch* p : ok) maxn = std::max(maxn, (long)p->n);
+    for (size_t i = 0; i < ok.size(); i++) sched_->set_priority((int)i, ok.size() >= 16 && 2L * ok[i]->n >= maxn);
     for (size_t i = 0; i < ok.size(); i++) {
         ok[i]->slot = (int)i;
         tasks.push_back(calculate(Ctx{sched_.get(), ok[i], (int)i, this}));
@@ -2609,6 +2668,13 @@ void Engine::serve_loop()
                 queue_.pop_front();
             }
         }
+        // the largest patches of a big admission set its length: give them flushes of their own
+        long maxn = 0;
+        for (auto& ab : admit) {
+            Patch* p = tickets_at(ab.first);
+            maxn = std::max(maxn, (long)std::max(1, p->in.mx) * std::max(1, p->in.my));
+        }
+        const bool many = admit.size() >= 16;
         for (auto& ab : admit) {
             Patch* p = tickets_at(ab.first);
             bool ok = false;
@@ -2626,6 +2692,7 @@ void Engine::serve_loop()
             int slot = free_slots.back();
             free_slots.pop_back();
             p->slot = slot;
+            sched_->set_priority(slot, many && 2L * p->n >= maxn);
             active.push_back(Act{ab.first, slot, ab.second, calculate(Ctx{sched_.get(), p, slot, this})});
             active.back().task.h.resume();
         }

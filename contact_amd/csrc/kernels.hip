@@ -1903,7 +1903,7 @@ __host__ __device__ inline int gs2_stage_stride(int mx)
 __host__ __device__ inline size_t gs2_row_lds(int mx) { return (size_t)gs2_stage_stride(mx) * sizeof(double); }
 // two staging buffers: the next source row is written while others still read the current
 // source rows staged per barrier (GS_R, when two groups of them fit next to the fixed part)
-constexpr int GS_R = 2;
+constexpr int GS_R = 1;
 __host__ __device__ inline int gs2_rows_per_stage(int mx, int my)
 {
     return gs2_base_lds(mx, my) + (size_t)2 * GS_R * gs2_row_lds(mx) <= MAX_LDS ? GS_R : 1;
