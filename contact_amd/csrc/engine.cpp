@@ -439,9 +439,10 @@ private:
     {
         (void)main_idle;
         if (pending_.empty()) return;
-        // CNTC_GS_KERNEL=1 selects the variant without row staging for every size (testing)
+        // CNTC_GS_KERNEL=1 selects the variant without row staging for every size, 2 the staged
+        // variant without the overlap of row sums and walk (testing)
         const char* forced = std::getenv("CNTC_GS_KERNEL");
-        bool force_v1 = forced && forced[0] == '1';
+        const int maxvar = forced && forced[0] == '1' ? 0 : forced && forced[0] == '2' ? 1 : 2;
         // largest patches first; one launch per idle side stream, each of patches of similar
         // size (down to a quarter of the launch's largest, at most 64), because a launch
         // returns when its longest item ends and a launch queued behind another on the same
@@ -456,14 +457,16 @@ private:
             if (k >= order.size()) break;
             if (!side_idle(st)) continue;
             const long top = order[k]->item.gs.n;
-            std::vector<Req*> v1, v2;
+            std::vector<Req*> byvar[3];
             for (size_t c = 0; k < order.size() && c < 64 && 4L * order[k]->item.gs.n >= top; k++, c++) {
                 Req* r = order[k];
-                bool small = !force_v1 && op_lds(r->kind, &r->item, 1) <= MAX_LDS;
-                (small ? v2 : v1).push_back(r);
+                int v = maxvar;
+                // (2 KB left for the kernels' static LDS)
+                while (v > 0 && op_lds(r->kind, &r->item, v) + 2048 > MAX_LDS) v--;
+                byvar[v].push_back(r);
             }
-            if (!v2.empty()) launch_async(v2, 1, st);
-            if (!v1.empty()) launch_async(v1, 0, st);
+            for (int v = 2; v >= 0; v--)
+                if (!byvar[v].empty()) launch_async(byvar[v], v, st);
         }
         for (; k < order.size(); k++) rest.push_back(*order[k]);
         pending_.swap(rest);
@@ -2071,7 +2074,8 @@ static Task stdygs(Ctx C, int k, double omegah, double omegas, int maxgs, double
     it.gs = gs_item(P, k, omegah, omegas, maxgs, eps);
     it.gs.p0x = p0x;
     it.gs.p0y = p0y;
-    if (op_lds(OP_STDYGS, &it.gs, 0) > MAX_LDS && op_lds(OP_STDYGS, &it.gs, 1) > MAX_LDS) co_return ERR_OTHER;
+    if (op_lds(OP_STDYGS, &it.gs, 0) + 2048 > MAX_LDS && op_lds(OP_STDYGS, &it.gs, 1) + 2048 > MAX_LDS)
+        co_return ERR_OTHER;
     Res r = co_await C.opr(OP_STDYGS, it);
     int itgs = (int)r.v[0];
     double dif = r.v[1], dif1 = r.v[5];

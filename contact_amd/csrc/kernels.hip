@@ -2333,6 +2333,498 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
     }
 }
 
+// SteadyGS with the row sums of the next row overlapped with the walk along the current row:
+// while wavefront 0 walks row iy element by element, the other wavefronts sum the influence
+// of every source row but iy and iy' on the next target row iy' (those rows do not change
+// while row iy is walked); row iy's own contribution to row iy' is added once its walk has
+// ended.  Same sweeps and element updates as k_stdygs2; only the summation order of the
+// row sums differs.  Returns the same values as k_stdygs2, with the ticks of the row-sum
+// phases and of wavefront 0's walks in place of the two phase times.
+__host__ __device__ inline size_t gs3_base_lds(int mx, int my)
+{
+    return (size_t)(18 * mx) * sizeof(double) + 2 * GS_THREADS * sizeof(double) +
+           (size_t)4 * my * sizeof(int) + mx + 64;
+}
+__host__ __device__ inline size_t gs3_stage_off(int mx, int my)
+{
+    return (gs3_base_lds(mx, my) + 63) / 64 * 64;
+}
+// a helper thread owns GS3_TB consecutive targets and takes the sources in chunks of GS3_QB
+// (4 x 8: as many products per coefficient load as 8 x 4, with half the accumulators, which
+// stay live while wavefront 0 walks)
+#ifndef GS3_TB_N
+#define GS3_TB_N 4
+#endif
+#ifndef GS3_QB_N
+#define GS3_QB_N 8
+#endif
+constexpr int GS3_TB = GS3_TB_N, GS3_QB = GS3_QB_N;
+__host__ __device__ inline int gs3_window(int mx) { return (2 * mx + GS3_TB + GS3_QB + 3) / 4 * 4; }
+__host__ __device__ inline int gs3_stage_stride(int mx)
+{
+    return (4 * gs3_window(mx) + 2 * (mx + GS3_QB) + 7) / 8 * 8;
+}
+__host__ __device__ inline size_t gs3_lds(int mx, int my)
+{
+    return gs3_stage_off(mx, my) + 2 * (size_t)gs3_stage_stride(mx) * sizeof(double);
+}
+
+__global__ void __launch_bounds__(GS_THREADS) k_stdygs3(const GsItem* items, const int* gstart,
+                                                        int nitems, double* partials)
+{
+    extern __shared__ double gs_lds[];
+    __shared__ double red_s[16 * MAXK];
+    __shared__ double s_ctl[4];
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GsItem& I = items[it];
+    const int mx = I.mx, my = I.my, n = I.n, tid = threadIdx.x, T = blockDim.x;
+    const int M2 = 2 * mx;
+    // helpers: the wavefronts after the first
+    const int TH = T - 64, hid = tid - 64;
+    const bool helper = tid >= 64;
+    double* rpx = gs_lds;
+    double* rpy = rpx + mx;
+    double* rdpx = rpy + mx;
+    double* rdpy = rdpx + mx;
+    double* rg = rdpy + mx;
+    double* rpn = rg + mx;
+    double* rwx = rpn + mx;
+    double* rwy = rwx + mx;
+    double* ucx = rwy + mx;                // sums over the other rows, per column
+    double* ucy = ucx + mx;
+    double* stc = ucy + mx;                // own-row coefficients, offsets (q - mx, 0)
+    double* red = stc + 4 * M2;            // 2 partial sums per thread
+    int* rj0 = (int*)(red + 2 * GS_THREADS);   // summation range per row
+    int* rj1 = rj0 + my;
+    int* ri0 = rj1 + my;                   // interior range per row
+    int* ri1 = ri0 + my;
+    int8_t* rel = (int8_t*)(ri1 + my);
+    double* stg = gs_lds + gs3_stage_off(mx, my) / 8;
+    const int SW = gs3_stage_stride(mx);
+    double* dpx = I.dpx;
+    double* dpy = I.dpy;
+    const double* cfb[4] = {I.cf[0][0], I.cf[0][1], I.cf[1][0], I.cf[1][1]};
+
+    for (int i = tid; i < n; i += T) {
+        int ix = i % mx;
+        if (ix != mx - 1) {
+            dpx[i] = I.px[i] - I.px[i + 1];
+            dpy[i] = I.py[i] - I.py[i + 1];
+        } else {
+            dpx[i] = I.px[i];
+            dpy[i] = I.py[i];
+        }
+    }
+    for (int jy = tid; jy < my; jy += T) {
+        const int8_t* row = I.el + (size_t)jy * mx;
+        int first = 0;
+        while (first < mx - 1 && row[first] <= EXTER) first++;
+        int last = -1, lo = mx;
+        for (int ix = 0; ix < mx; ix++)
+            if (row[ix] >= ADHES) {
+                last = ix;
+                if (ix < lo) lo = ix;
+            }
+        rj0[jy] = first - 1 < 0 ? 0 : first - 1;
+        rj1[jy] = last + 1 > mx - 1 ? mx - 1 : last + 1;
+        ri0[jy] = lo;
+        ri1[jy] = last;
+    }
+    for (int e = tid; e < 4 * M2; e += T) {
+        int b = e / M2, q = e - b * M2;
+        stc[e] = cfb[b][(size_t)my * M2 + q];
+    }
+    __syncthreads();
+    int nsrc = 0;   // source rows
+    for (int jy = 0; jy < my; jy++) nsrc += rj1[jy] >= rj0[jy] ? 1 : 0;
+
+    // ---- the row sums of target row tr over the source rows other than tr and skip, into the
+    // helpers' accumulators; wavefront 0 meanwhile walks its row (walk(b, nb): the share of
+    // the row due after b of nb source rows)
+    double ax[GS3_TB], ay[GS3_TB];
+    int sgrp = 0;
+    auto row_sums = [&](int tr, int skip, int only, auto&& walk) {
+        const int t0 = ri0[tr], nt = ri1[tr] - t0 + 1;
+        const int ntg = (nt + GS3_TB - 1) / GS3_TB, ntp = ntg * GS3_TB;
+        const int P = TH / ntg > 0 ? TH / ntg : 1;
+        const int tg = hid % ntg, part = hid / ntg;
+        const bool active = helper && part < P;
+        const int nb = only >= 0 ? 1 : nsrc - (rj1[tr] >= rj0[tr]) - (skip >= 0 && rj1[skip] >= rj0[skip]);
+        int jy = 0, b = 0;
+        while (true) {
+            int sy = -1;
+            if (only >= 0) {
+                if (b == 0) sy = only;
+            } else {
+                for (; jy < my; jy++)
+                    if (jy != tr && jy != skip && rj1[jy] >= rj0[jy]) {
+                        sy = jy++;
+                        break;
+                    }
+            }
+            if (sy < 0) break;
+            double* sc = stg + (size_t)sgrp * SW;
+            sgrp ^= 1;
+            const int j0 = rj0[sy], nq = rj1[sy] - j0 + 1;
+            const int Q = (nq + GS3_QB - 1) / GS3_QB * GS3_QB;
+            const int Wn = ntp + Q - 1, Wp = (Wn + 1 + 3) / 4 * 4;
+            if (helper) {
+                // window S[k], k = t - q + Q - 1: coefficient of target t0 + t and source j0 + q
+                const int col0 = t0 - j0 - (Q - 1) + mx;
+#pragma unroll
+                for (int bb = 0; bb < 4; bb++) {
+                    const double* crow = cfb[bb] + (size_t)(tr - sy + my) * M2;
+                    for (int k = hid; k < Wp; k += TH) {
+                        const int col = col0 + k;
+                        sc[bb * Wp + k] = (k < Wn && col >= 0 && col < M2) ? crow[col] : 0.0;
+                    }
+                }
+                double* sdx = sc + 4 * Wp;
+                double* sdy = sdx + Q;
+                for (int e = hid; e < Q; e += TH) {
+                    const bool in = e < nq;
+                    sdx[e] = in ? dpx[(size_t)sy * mx + j0 + e] : 0.0;
+                    sdy[e] = in ? dpy[(size_t)sy * mx + j0 + e] : 0.0;
+                }
+            } else {
+                walk(b, nb);
+            }
+            __syncthreads();
+            b++;
+            if (active) {
+                const int nch = Q / GS3_QB;
+                const double* sdx = sc + 4 * Wp;
+                const double* sdy = sdx + Q;
+                for (int c = part; c < nch; c += P) {
+                    const int kmin = tg * GS3_TB - c * GS3_QB + (Q - GS3_QB);
+                    double qx[GS3_QB], qy[GS3_QB];
+#pragma unroll
+                    for (int j = 0; j < GS3_QB; j++) {
+                        qx[j] = sdx[c * GS3_QB + j];
+                        qy[j] = sdy[c * GS3_QB + j];
+                    }
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        // blocks (0, 1) give the x sums, blocks (2, 3) the y sums
+                        const double2* wa = (const double2*)(sc + (2 * h) * Wp + kmin);
+                        const double2* wb = (const double2*)(sc + (2 * h + 1) * Wp + kmin);
+                        double va[GS3_TB + GS3_QB], vb[GS3_TB + GS3_QB];
+#pragma unroll
+                        for (int m = 0; m < (GS3_TB + GS3_QB) / 2; m++) {
+                            const double2 u = wa[m], v = wb[m];
+                            va[2 * m] = u.x;
+                            va[2 * m + 1] = u.y;
+                            vb[2 * m] = v.x;
+                            vb[2 * m + 1] = v.y;
+                        }
+#pragma unroll
+                        for (int i = 0; i < GS3_TB; i++) {
+                            double acc = h == 0 ? ax[i] : ay[i];
+#pragma unroll
+                            for (int j = 0; j < GS3_QB; j++) {
+                                const int m = i + GS3_QB - 1 - j;
+                                acc += va[m] * qx[j] + vb[m] * qy[j];
+                            }
+                            if (h == 0) ax[i] = acc;
+                            else ay[i] = acc;
+                        }
+                    }
+                }
+            }
+        }
+        if (!helper) walk(nb, nb);
+    };
+    // sums over the helpers' parts (target row tr)
+    auto reduce_sums = [&](int tr) {
+        const int t0 = ri0[tr], nt = ri1[tr] - t0 + 1;
+        const int ntg = (nt + GS3_TB - 1) / GS3_TB;
+        const int P = TH / ntg > 0 ? TH / ntg : 1;
+        const int tg = hid % ntg, part = hid / ntg;
+        const bool active = helper && part < P;
+        __syncthreads();
+        for (int i = 0; i < GS3_TB; i++) {
+            if (active) {
+                red[2 * (part * ntg + tg)] = ax[i];
+                red[2 * (part * ntg + tg) + 1] = ay[i];
+            }
+            __syncthreads();
+            for (int g = tid; g < ntg; g += T) {
+                const int c = g * GS3_TB + i;
+                if (c < nt) {
+                    double sx = 0.0, sy = 0.0;
+                    for (int p = 0; p < P; p++) {
+                        sx += red[2 * (p * ntg + g)];
+                        sy += red[2 * (p * ntg + g) + 1];
+                    }
+                    ucx[t0 + c] = sx;
+                    ucy[t0 + c] = sy;
+                }
+            }
+            __syncthreads();
+        }
+    };
+    auto clear_acc = [&]() {
+#pragma unroll
+        for (int i = 0; i < GS3_TB; i++) ax[i] = ay[i] = 0.0;
+    };
+    auto load_row = [&](int iy) {
+        const size_t r0 = (size_t)iy * mx;
+        for (int ix = tid; ix < mx; ix += T) {
+            rpx[ix] = I.px[r0 + ix];
+            rpy[ix] = I.py[r0 + ix];
+            rdpx[ix] = dpx[r0 + ix];
+            rdpy[ix] = dpy[r0 + ix];
+            double pn = I.pn[r0 + ix];
+            rpn[ix] = pn;
+            rg[ix] = I.mu * pn;
+            rwx[ix] = I.wsx[r0 + ix];
+            rwy[ix] = I.wsy[r0 + ix];
+            rel[ix] = I.el[r0 + ix];
+        }
+    };
+    auto store_row = [&](int iy) {
+        const size_t r0 = (size_t)iy * mx;
+        for (int ix = tid; ix < mx; ix += T) {
+            I.px[r0 + ix] = rpx[ix];
+            I.py[r0 + ix] = rpy[ix];
+            dpx[r0 + ix] = rdpx[ix];
+            dpy[r0 + ix] = rdpy[ix];
+            I.el[r0 + ix] = rel[ix];
+        }
+    };
+    auto next_row = [&](int iy) {
+        int r = iy + 1;
+        while (r < my && ri1[r] < ri0[r]) r++;
+        return r;
+    };
+
+    int itgs = 0;
+    double dif = 2.0, difid = 1.0, dif1 = 0.0;
+    unsigned long long t_rows = 0, t_walk = 0;   // thread 0: wall-clock ticks
+    auto nowalk = [&](int, int) {};
+    while (dif >= difid && itgs < I.maxgs) {
+        itgs++;
+        double difacc = 0.0;   // thread 0
+        int iy = next_row(-1);
+        if (iy < my) {
+            const unsigned long long tr0 = wall_clock64();
+            clear_acc();
+            row_sums(iy, -1, -1, nowalk);
+            reduce_sums(iy);
+            load_row(iy);
+            __syncthreads();
+            if (tid == 0) t_rows += wall_clock64() - tr0;
+        }
+        while (iy < my) {
+            const int nx = next_row(iy);
+            // the walk along row iy by wavefront 0 (one element at a time)
+            const int t0 = ri0[iy], nt = ri1[iy] - t0 + 1;
+            const int j0 = rj0[iy], j1 = rj1[iy];
+            const size_t r0 = (size_t)iy * mx;
+            const int lane = tid;
+            // first upstream element not in adhesion (:2972-2977): the largest jx in [1, ix-1]
+            // with another state, else 0 (-1 at the trailing edge), carried along the row;
+            // the elements before the first interior one are exterior
+            int lastna = t0 - 1 >= 1 ? t0 - 1 : 0;
+            int ixw = t0;
+            auto walk = [&](int b, int nb) {
+                const int iend = b >= nb ? t0 + nt : t0 + (int)(((long)nt * (b + 1) + nb - 1) / nb);
+                if (ixw >= iend) return;
+                const unsigned long long w0 = wall_clock64();
+                for (; ixw < iend; ixw++) {
+                    const int ix = ixw;
+                    if (rel[ix] < ADHES) {
+                        if (ix >= 1) lastna = ix;
+                        continue;
+                    }
+                    double qsx = 0.0, qsy = 0.0;
+                    for (int jx = j0 + lane; jx <= j1; jx += 64) {
+                        int o = ix - jx + mx;
+                        double qx = rdpx[jx], qy = rdpy[jx];
+                        qsx += stc[o] * qx + stc[M2 + o] * qy;
+                        qsy += stc[2 * M2 + o] * qx + stc[3 * M2 + o] * qy;
+                    }
+                    for (int off = 32; off > 0; off >>= 1) {
+                        qsx += __shfl_xor(qsx, off);
+                        qsy += __shfl_xor(qsy, off);
+                    }
+                    const int jx = ix >= 1 ? lastna : -1;
+                    // the element problem, solved by every lane on the same (broadcast) data so
+                    // that its change D is known wavefront-wide; lane 0 stores
+                    const int8_t eup = ix >= 1 ? rel[ix - 1] : EXTER;
+                    double Dx, Dy;
+                    {
+                        const int b0 = mx, bj = mx + (jx - ix);
+                        double a11 = I.ga_inv * (stc[b0] - stc[bj]);
+                        double a12 = I.ga_inv * (stc[M2 + b0] - stc[M2 + bj]);
+                        double a21 = I.ga_inv * (stc[2 * M2 + b0] - stc[2 * M2 + bj]);
+                        double a22 = I.ga_inv * (stc[3 * M2 + b0] - stc[3 * M2 + bj]);
+                        double sv[2] = {rwx[ix] + I.ga_inv * (ucx[ix] + qsx), rwy[ix] + I.ga_inv * (ucy[ix] + qsy)};
+                        const double opx = rpx[ix], opy = rpy[ix];
+                        double pr[3] = {opx, opy, rpn[ix]};
+                        int8_t e8 = rel[ix];
+                        elmtrc(e8, a11, a12, a21, a22, I.eps, pr, I.mu, sv, I.omegah, I.omegas, true);
+                        Dx = pr[0] - opx;
+                        Dy = pr[1] - opy;
+                        if (lane == 0) {
+                            const size_t ii = r0 + ix;
+                            difacc += Dx * Dx + Dy * Dy;
+                            rdpx[ix] += Dx;
+                            rdpy[ix] += Dy;
+                            rpx[ix] = pr[0];
+                            rpy[ix] = pr[1];
+                            rel[ix] = e8;
+                            I.sx[ii] = sv[0];
+                            I.sy[ii] = sv[1];
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    // Rebuild the upstream part of the row (:3086-3127).  The adhesion elements
+                    // upstream satisfy p(k) = p(k+1) + dp(k), so they all move by the change D
+                    // of p(ix) until one exceeds its traction bound (clipped there, D becomes
+                    // that element's change) or a non-adhesion element ends the chain.  The
+                    // wavefront handles 64 upstream elements per step.
+                    int hi = ix - 1;
+                    if (ix >= 1 && eup != ADHES) {
+                        // the upstream neighbour ends the chain at once
+                        if (lane == 0) {
+                            const int kx = ix - 1;
+                            if (eup == SLIP || eup == PLAST) {
+                                rdpx[kx] = rpx[kx] - rpx[ix];
+                                rdpy[kx] = rpy[kx] - rpy[ix];
+                            } else {
+                                rdpx[kx] = -rpx[ix];
+                                rdpy[kx] = -rpy[ix];
+                            }
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        hi = -1;
+                    }
+                    while (hi >= 0) {
+                        const int kx = hi - lane;
+                        const bool valid = kx >= 0;
+                        const int8_t ek = valid ? rel[kx] : EXTER;
+                        const unsigned long long smask = __ballot(!valid || ek != ADHES);
+                        const int sl = smask ? __ffsll((long long)smask) - 1 : 64;
+                        double ox = 0.0, oy = 0.0, qx = 0.0, qy = 0.0;
+                        bool clip = false;
+                        if (lane < sl) {
+                            ox = rpx[kx];
+                            oy = rpy[kx];
+                            qx = ox + Dx;
+                            qy = oy + Dy;
+                            clip = sqrt(qx * qx + qy * qy) > rg[kx];
+                        }
+                        const unsigned long long cmask = __ballot(clip);
+                        const int cl = cmask ? __ffsll((long long)cmask) - 1 : 64;
+                        const int nmove = sl < cl ? sl : cl;
+                        if (lane < nmove) {
+                            rpx[kx] = qx;
+                            rpy[kx] = qy;
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        if (cl < sl) {
+                            // clip at lane cl and continue below it with that element's change
+                            double nDx = 0.0, nDy = 0.0;
+                            if (lane == cl) {
+                                double ptabs = sqrt(qx * qx + qy * qy), ptbnd = rg[kx];
+                                qx = qx * ptbnd / ptabs;
+                                qy = qy * ptbnd / ptabs;
+                                rdpx[kx] = qx - rpx[kx + 1];
+                                rdpy[kx] = qy - rpy[kx + 1];
+                                rpx[kx] = qx;
+                                rpy[kx] = qy;
+                                nDx = qx - ox;
+                                nDy = qy - oy;
+                            }
+                            Dx = __shfl(nDx, cl);
+                            Dy = __shfl(nDy, cl);
+                            hi -= cl + 1;
+                        } else if (sl < 64) {
+                            // the first non-adhesion element ends the chain
+                            if (lane == sl && valid) {
+                                if (ek == SLIP || ek == PLAST) {
+                                    rdpx[kx] = rpx[kx] - rpx[kx + 1];
+                                    rdpy[kx] = rpy[kx] - rpy[kx + 1];
+                                } else if (rel[kx + 1] >= ADHES) {
+                                    rdpx[kx] = -rpx[kx + 1];
+                                    rdpy[kx] = -rpy[kx + 1];
+                                }
+                            }
+                            hi = -1;
+                        } else {
+                            hi -= 64;
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    }
+                    if (ix >= 1 && rel[ix] != ADHES) lastna = ix;
+                }
+                if (lane == 0) t_walk += wall_clock64() - w0;
+            };
+            const unsigned long long tr0 = wall_clock64();
+            if (nx < my) {
+                // the next row's sums over the rows that do not change meanwhile
+                clear_acc();
+                row_sums(nx, iy, -1, walk);
+            } else if (!helper) {
+                walk(1, 1);
+            }
+            __syncthreads();
+            store_row(iy);
+            __syncthreads();
+            if (nx < my) {
+                // ... and row iy's contribution, now final
+                if (rj1[iy] >= rj0[iy]) row_sums(nx, -1, iy, nowalk);
+                reduce_sums(nx);
+                load_row(nx);
+                __syncthreads();
+            }
+            if (tid == 0) t_rows += wall_clock64() - tr0;
+            iy = nx;
+        }
+        double sq[1] = {0.0};
+        if (I.p0x) {
+            // sensitivity solve: rms of the change from the initial estimate (:3138-3144)
+            for (int i = tid; i < n; i += T) {
+                double bx = I.px[i] + -1.0 * I.p0x[i], by = I.py[i] + -1.0 * I.p0y[i];
+                sq[0] += bx * bx + by * by;
+            }
+        } else {
+            for (int i = tid; i < n; i += T) sq[0] += I.px[i] * I.px[i] + I.py[i] * I.py[i];
+        }
+        group_reduce<1>(sq, 0, red_s);
+        if (tid == 0) {
+            double rms = sqrt(sq[0] / (double)(2 * n > 1 ? 2 * n : 1));
+            s_ctl[0] = sqrt(difacc / (2.0 * I.k));
+            s_ctl[1] = I.eps * fmax(1e-6, I.facnel * rms);
+        }
+        __syncthreads();
+        dif = s_ctl[0];
+        difid = s_ctl[1];
+        if (itgs == 1) dif1 = dif;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double len = 0.0;
+        for (int jy = 0; jy < my; jy++) len += rj1[jy] >= rj0[jy] ? rj1[jy] - rj0[jy] + 1 : 0;
+        partials[(size_t)blockIdx.x * 10] = (double)itgs;
+        partials[(size_t)blockIdx.x * 10 + 1] = dif;
+        partials[(size_t)blockIdx.x * 10 + 2] = 8.0 * itgs * (double)I.k * len;
+        partials[(size_t)blockIdx.x * 10 + 3] = (double)t_rows;
+        partials[(size_t)blockIdx.x * 10 + 4] = (double)t_walk;
+        partials[(size_t)blockIdx.x * 10 + 5] = dif1;
+        for (int q = 6; q < 10; q++) partials[(size_t)blockIdx.x * 10 + q] = 0.0;
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // host-side launchers
 // ------------------------------------------------------------------------------------------
@@ -2470,7 +2962,8 @@ void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, i
     case OP_SETUP: hipLaunchKernelGGL(k_setup, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_FORCES: hipLaunchKernelGGL(k_forces, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, d_partials); break;
     case OP_STDYGS:
-        if (conv_stage == 1) hipLaunchKernelGGL(k_stdygs2, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, d_partials);
+        if (conv_stage == 2) hipLaunchKernelGGL(k_stdygs3, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, d_partials);
+        else if (conv_stage == 1) hipLaunchKernelGGL(k_stdygs2, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, d_partials);
         else hipLaunchKernelGGL(k_stdygs, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, d_partials);
         break;
     case OP_GUARD: hipLaunchKernelGGL(k_guard, ge, te, 0, s, (const GsItem*)d_items, d_gstart, nitems, d_partials); break;
@@ -2578,6 +3071,7 @@ size_t op_lds(int kind, const void* item, int stage)
     }
     if (kind == OP_STDYGS) {
         const GsItem* g = (const GsItem*)item;
+        if (stage == 2) return gs3_lds(g->mx, g->my);
         if (stage == 1) return gs2_lds(g->mx, g->my);
         return (size_t)g->mx * (5 * sizeof(double) + 1) + (size_t)(2 * g->my + 1) * sizeof(int) + 16;
     }

@@ -130,14 +130,18 @@ def test_steadygs_table_golden(cntc, oracle):
     print("table G=0: worst golden difference %.2e" % worst)
 
 
-@pytest.mark.parametrize("kernel", ["staged", "direct"])
+@pytest.mark.parametrize("kernel", ["overlapped", "staged", "direct"])
 def test_gdsteady_stagnation_fallback(cntc, oracle, kernel, monkeypatch):
     """Hertzian patches on which GDsteady stagnates: both solvers fall back to SteadyGS
-    (src/m_solvpt.f90:466-485) and must agree to the solver tolerance.  Both SteadyGS kernels
-    (row-staged for rows up to ~480 elements, direct summation beyond) are checked."""
+    (src/m_solvpt.f90:466-485) and must agree to the solver tolerance.  The three SteadyGS
+    kernels are checked: row sums overlapped with the walk along the previous row (the
+    default where its LDS fits, rows up to ~480 elements), row sums before each row's walk,
+    and direct summation (rows too wide for staging)."""
     from contact_amd import workloads as W
     if kernel == "direct":
         monkeypatch.setenv("CNTC_GS_KERNEL", "1")
+    elif kernel == "staged":
+        monkeypatch.setenv("CNTC_GS_KERNEL", "2")
     cases = W.hertz_batch(n=2048)
     for k in (381, 1505):
         c = cases[k]
@@ -152,7 +156,7 @@ def test_gdsteady_stagnation_fallback(cntc, oracle, kernel, monkeypatch):
 
 def test_steadygs_wide_rows(cntc, oracle):
     """SteadyGS (G=0) on rows of 520 elements: the row-staged kernel then needs more than 64 KB
-    of LDS per work-group (gfx950 allows 160 KB)."""
+    of LDS per work-group (gfx950 allows 160 KB), and the overlapped one does not fit."""
     c = C.hertz_case(12.0, 3.0, 520, 16, 82000.0, cksi=0.0008, ceta=-0.0004, cphi=0.0002,
                      eps=1e-6, maxgs=299, maxin=5, gausei=0)
     g = run_gpu(cntc, c)
