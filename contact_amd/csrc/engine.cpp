@@ -200,6 +200,7 @@ public:
         if (const char* v = std::getenv("CNTC_SIDE_STREAMS")) nside = std::max(1, std::min(hwq - 1, std::atoi(v)));
         side_.assign(nside, nullptr);
         if (const char* v = std::getenv("CNTC_PRIO_EXTRA")) PRIO_EXTRA = std::max(0, std::min(64, std::atoi(v)));
+        if (const char* v = std::getenv("CNTC_PRIO_DIV")) PRIO_DIV = std::max(1, std::min(64, std::atoi(v)));
         for (auto& st : side_) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         cap_ = 64ull << 20;
         HIPCHK(hipHostMalloc((void**)&hstage_, cap_, hipHostMallocDefault));
@@ -232,6 +233,7 @@ public:
     // flush), so that the largest patches of a batch, which set its length, are not held to
     // the pace of every other patch's work; each patch's own sequence of operations is unchanged
     void set_priority(int slot, bool p) { prio_[slot] = p ? 1 : 0; }
+    int prio_div() const { return PRIO_DIV; }
     void enqueue(int slot, int kind, const AnyItem& it)
     {
         Req r;
@@ -699,6 +701,8 @@ private:
     std::vector<char> async_wait_;
     std::vector<char> prio_;
     int PRIO_EXTRA = 2;        // CNTC_PRIO_EXTRA
+    // patches of at least 1/PRIO_DIV of the batch's largest element count get priority
+    int PRIO_DIV = 4;          // CNTC_PRIO_DIV
     long prio_turn_ = 0;
     std::vector<std::vector<Req>> q_;
     std::vector<std::coroutine_handle<>> waiter_;
@@ -2556,7 +2560,7 @@ int Engine::solve(const std::vector<Patch*>& patches)
     // the largest patches of a big batch set its length: give them flushes of their own
     long maxn = 0;
     for (Patch* p : ok) maxn = std::max(maxn, (long)p->n);
-    for (size_t i = 0; i < ok.size(); i++) sched_->set_priority((int)i, ok.size() >= 16 && 2L * ok[i]->n >= maxn);
+    for (size_t i = 0; i < ok.size(); i++) sched_->set_priority((int)i, ok.size() >= 16 && (long)sched_->prio_div() * ok[i]->n >= maxn);
     for (size_t i = 0; i < ok.size(); i++) {
         ok[i]->slot = (int)i;
         tasks.push_back(calculate(Ctx{sched_.get(), ok[i], (int)i, this}));
@@ -2696,7 +2700,7 @@ void Engine::serve_loop()
             int slot = free_slots.back();
             free_slots.pop_back();
             p->slot = slot;
-            sched_->set_priority(slot, many && 2L * p->n >= maxn);
+            sched_->set_priority(slot, many && (long)sched_->prio_div() * p->n >= maxn);
             active.push_back(Act{ab.first, slot, ab.second, calculate(Ctx{sched_.get(), p, slot, this})});
             active.back().task.h.resume();
         }
