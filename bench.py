@@ -53,7 +53,7 @@ def kernel_name(op):
     conv = {"conv32_fwd": "k_rowfwd<float>", "conv32_mul": "k_colmul<float>",
             "conv32_inv": "k_rowinv<float>", "conv_fwd": "k_rowfwd<double>",
             "conv_mul": "k_colmul<double>", "conv_inv": "k_rowinv<double>",
-            "stdygs_staged": "k_stdygs2"}
+            "stdygs_staged": "k_stdygs2", "stdygs_overlap": "k_stdygs3"}
     return conv.get(op, "k_" + op)
 
 

@@ -371,6 +371,8 @@ def amdprofileget(lenarr=256):
             name += ("_fwd", "_mul", "_inv")[stage]
         elif name == "stdygs" and stage == 1:
             name = "stdygs_staged"
+        elif name == "stdygs" and stage == 2:
+            name = "stdygs_overlap"
         out[name] = (float(sec[k]), float(byt[k]), int(nl[k]))
     return out
 
