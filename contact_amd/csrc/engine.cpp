@@ -754,6 +754,13 @@ struct Ctx {
         S->enqueue(slot, kind, it);
         return ResAwait{S, slot, {}};
     }
+    // an operation whose results land in dst when the flush that runs it ends; the task goes
+    // on issuing (a later co_await of the same flush makes them readable)
+    void opr_into(int kind, const AnyItem& it, double* dst)
+    {
+        S->enqueue(slot, kind, it);
+        S->set_result_dest(slot, dst);
+    }
     // masked BLAS-1 helpers (src/m_gridfunc.f90)
     AnyItem vitem(int sub, int mask, double* x0, double* x1, double* y0, double* y1, double a, double b)
     {
@@ -1748,11 +1755,32 @@ static Task gdsteady(Ctx C, const double cf00[4], int maxgd, double eps, int* it
         bool stop_j = false, has_bracket = false, prev_bracket;
         double alpha_j = 0.0, alpha_prv = -1.0, dst_cur = 0.0, dst_prv1 = 0.0, dst_prv2 = 0.0, ddrho;
         double sum_dq2 = 0.0, sum_r2 = 0.0;
+        // After the first iteration the first two trial steps are known before any probe
+        // result: 0, then 0.6 alpha0 with alpha0 = (alpha0 + alpha) / 2 (set at j = 0 below,
+        // taken at j = 1).  Both probes go into one flush; probes only reduce, so the kernels
+        // and their order are those of one probe per flush.
+        double spec[2][sizeof(Res::v) / sizeof(double)];
+        double spec_alpha = 0.0;
+        int nspec = 0;
+        if (itgd > 1) {
+            spec_alpha = 0.0 + 0.6 * (0.5 * (alpha0 + alpha));
+            AnyItem p0 = it;
+            p0.gd.alpha = 0.0;
+            p0.gd.first = 1;
+            C.opr_into(OP_GD_PROBE, p0, spec[0]);
+            AnyItem p1 = it;
+            p1.gd.alpha = spec_alpha;
+            p1.gd.first = 0;
+            Res rs = co_await C.opr(OP_GD_PROBE, p1);
+            std::memcpy(spec[1], rs.v, sizeof(rs.v));
+            nspec = 2;
+        }
         while (!stop_j) {
             AnyItem pr = it;
             pr.gd.alpha = alpha_j;
             pr.gd.first = j == 0 ? 1 : 0;
-            r = co_await C.opr(OP_GD_PROBE, pr);
+            if (j < nspec && alpha_j == (j == 0 ? 0.0 : spec_alpha)) std::memcpy(r.v, spec[j], sizeof(r.v));
+            else r = co_await C.opr(OP_GD_PROBE, pr);
             double rhonew = r.v[0];
             double drho_da = r.v[1] + r.v[3] + alpha_j * (r.v[2] + r.v[4]) + alpha_j * alpha_j * r.v[5];
             if (j == 0) {
@@ -1892,11 +1920,14 @@ static Task gdsteady(Ctx C, const double cf00[4], int maxgd, double eps, int* it
         C.op(OP_GD_UPDATE, iu);
         C.set(M_ALL, P.sx, 0.0, P.sy);
         C.conv_t(0, M_INT, P.sx, P.sy, G.dpx, G.dpy, G.wsx, G.wsy);
-        r = co_await C.opr(OP_GD_MOVES, it);
-        lchanged = r.v[0] > 0;
+        // the moves count and the residual terms in one flush (nothing between them reads the
+        // count)
+        double moves[sizeof(Res::v) / sizeof(double)];
+        C.opr_into(OP_GD_MOVES, it, moves);
         AnyItem idr = it;
         idr.gd.first = 0;
         r = co_await C.opr(OP_GD_DIAGRES, idr);
+        lchanged = moves[0] > 0;
         resrms2 = r.v[2];
         double facdif;
         if (beta >= 0.1) facdif = 1.0;
