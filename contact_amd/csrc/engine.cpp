@@ -332,6 +332,7 @@ private:
         size_t res_off = 0;
         std::vector<int> slots;
         std::vector<double*> dest;
+        std::vector<long> ns;        // element counts of the items (CNTC_GS_TIMING)
         double t_launch = 0.0;       // host clock at launch (CNTC_GS_TIMING)
         long max_n = 0;
     };
@@ -353,6 +354,7 @@ private:
                 async_free_.erase(async_free_.begin() + k);
                 op.slots.clear();
                 op.dest.clear();
+                op.ns.clear();
                 return op;
             }
         AsyncOp op;
@@ -405,8 +407,9 @@ private:
             if (const char* tv = std::getenv("CNTC_GS_TIMING"); tv && tv[0] == '1' && op.K >= 10)
                 for (int i = 0; i < op.nitems; i++) {
                     const double* q = r + (size_t)i * op.K;
-                    std::fprintf(stderr, "steadygs item %d: sweeps %.0f, row sums %.3f s, sequential %.3f s "
-                                 "(own row %.3f, search %.3f, element %.3f, chain %.3f)\n", i, q[0],
+                    std::fprintf(stderr, "steadygs item %d (%ld elements): sweeps %.0f, row sums %.3f s, sequential %.3f s "
+                                 "(own row %.3f, search %.3f, element %.3f, chain %.3f)\n", i,
+                                 i < (int)op.ns.size() ? op.ns[i] : 0L, q[0],
                                  q[3] * 1e-8, q[4] * 1e-8, q[6] * 1e-8, q[7] * 1e-8, q[8] * 1e-8, q[9] * 1e-8);
                 }
             std::vector<int> sl = op.slots;
@@ -499,6 +502,7 @@ private:
             lds = std::max(lds, op_lds(kind, &reqs[i]->item, variant));
             op.slots.push_back(reqs[i]->slot);
             op.dest.push_back(reqs[i]->res);
+            op.ns.push_back((long)reqs[i]->item.gs.n);
             async_wait_[reqs[i]->slot] = 1;
         }
         gs[nitems] = tot;
