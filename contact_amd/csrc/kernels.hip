@@ -3010,8 +3010,8 @@ double op_bytes(int kind, const void* item, int stage)
     case OP_ST_AS: case OP_ST_SA: return ((const StItem*)item)->n * 42.0;
     case OP_GD_DP: { const GdItem* g = (const GdItem*)item; return (double)g->n * 32.0; }
     case OP_GD_PROJECT: { const GdItem* g = (const GdItem*)item; return (double)g->n * 73.0; }
-    // six 2N FP64 vectors per probe (SURVEY-style count: dp, dv, s, q, n, v), two more (r) on
-    // the first probe of a line search
+    // six 2N FP64 vectors per probe (the algorithmic count: dp, dv, s, q, n, v), two more (r)
+    // on the first probe of a line search
     case OP_GD_PROBE: { const GdItem* g = (const GdItem*)item; return (double)g->n * (96.0 + (g->first ? 32.0 : 0.0)); }
     case OP_GD_UPDATE: { const GdItem* g = (const GdItem*)item; return (double)g->n * 121.0; }
     case OP_GD_MOVES: { const GdItem* g = (const GdItem*)item; return (double)g->n * 66.0; }
