@@ -2403,6 +2403,9 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs3(const GsItem* items, con
     const int SW = gs3_stage_stride(mx);
     double* dpx = I.dpx;
     double* dpy = I.dpy;
+    // the xx, xy and yy coefficient blocks: the coefficient kernel writes the xy and yx blocks
+    // from one value, so the yx block is the xy block
+    const double* cfs[3] = {I.cf[0][0], I.cf[0][1], I.cf[1][1]};
     const double* cfb[4] = {I.cf[0][0], I.cf[0][1], I.cf[1][0], I.cf[1][1]};
 
     for (int i = tid; i < n; i += T) {
@@ -2472,14 +2475,14 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs3(const GsItem* items, con
                 // window S[k], k = t - q + Q - 1: coefficient of target t0 + t and source j0 + q
                 const int col0 = t0 - j0 - (Q - 1) + mx;
 #pragma unroll
-                for (int bb = 0; bb < 4; bb++) {
-                    const double* crow = cfb[bb] + (size_t)(tr - sy + my) * M2;
+                for (int bb = 0; bb < 3; bb++) {
+                    const double* crow = cfs[bb] + (size_t)(tr - sy + my) * M2;
                     for (int k = hid; k < Wp; k += TH) {
                         const int col = col0 + k;
                         sc[bb * Wp + k] = (k < Wn && col >= 0 && col < M2) ? crow[col] : 0.0;
                     }
                 }
-                double* sdx = sc + 4 * Wp;
+                double* sdx = sc + 3 * Wp;
                 double* sdy = sdx + Q;
                 for (int e = hid; e < Q; e += TH) {
                     const bool in = e < nq;
@@ -2493,7 +2496,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs3(const GsItem* items, con
             b++;
             if (active) {
                 const int nch = Q / GS3_QB;
-                const double* sdx = sc + 4 * Wp;
+                const double* sdx = sc + 3 * Wp;
                 const double* sdy = sdx + Q;
                 for (int c = part; c < nch; c += P) {
                     const int kmin = tg * GS3_TB - c * GS3_QB + (Q - GS3_QB);
@@ -2503,31 +2506,32 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs3(const GsItem* items, con
                         qx[j] = sdx[c * GS3_QB + j];
                         qy[j] = sdy[c * GS3_QB + j];
                     }
+                    // windows of the xx, xy (= yx) and yy blocks
+                    const double2* w0 = (const double2*)(sc + kmin);
+                    const double2* w1 = (const double2*)(sc + Wp + kmin);
+                    const double2* w3 = (const double2*)(sc + 2 * Wp + kmin);
+                    double v0[GS3_TB + GS3_QB], v1[GS3_TB + GS3_QB], v3[GS3_TB + GS3_QB];
 #pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        // blocks (0, 1) give the x sums, blocks (2, 3) the y sums
-                        const double2* wa = (const double2*)(sc + (2 * h) * Wp + kmin);
-                        const double2* wb = (const double2*)(sc + (2 * h + 1) * Wp + kmin);
-                        double va[GS3_TB + GS3_QB], vb[GS3_TB + GS3_QB];
+                    for (int m = 0; m < (GS3_TB + GS3_QB) / 2; m++) {
+                        const double2 a = w0[m], b2 = w1[m], d = w3[m];
+                        v0[2 * m] = a.x;
+                        v0[2 * m + 1] = a.y;
+                        v1[2 * m] = b2.x;
+                        v1[2 * m + 1] = b2.y;
+                        v3[2 * m] = d.x;
+                        v3[2 * m + 1] = d.y;
+                    }
 #pragma unroll
-                        for (int m = 0; m < (GS3_TB + GS3_QB) / 2; m++) {
-                            const double2 u = wa[m], v = wb[m];
-                            va[2 * m] = u.x;
-                            va[2 * m + 1] = u.y;
-                            vb[2 * m] = v.x;
-                            vb[2 * m + 1] = v.y;
+                    for (int i = 0; i < GS3_TB; i++) {
+                        double accx = ax[i], accy = ay[i];
+#pragma unroll
+                        for (int j = 0; j < GS3_QB; j++) {
+                            const int m = i + GS3_QB - 1 - j;
+                            accx += v0[m] * qx[j] + v1[m] * qy[j];
+                            accy += v1[m] * qx[j] + v3[m] * qy[j];
                         }
-#pragma unroll
-                        for (int i = 0; i < GS3_TB; i++) {
-                            double acc = h == 0 ? ax[i] : ay[i];
-#pragma unroll
-                            for (int j = 0; j < GS3_QB; j++) {
-                                const int m = i + GS3_QB - 1 - j;
-                                acc += va[m] * qx[j] + vb[m] * qy[j];
-                            }
-                            if (h == 0) ax[i] = acc;
-                            else ay[i] = acc;
-                        }
+                        ax[i] = accx;
+                        ay[i] = accy;
                     }
                 }
             }
