@@ -1446,6 +1446,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_elascf(const CoefItem* items, 
                 c23 = c23 + ek * t23(ix + 1, iy);
                 c23 = c23 - ek * t23(ix + 1, iy + 1);
             }
+            // the xy and yx blocks hold one value (k_stdygs3 stages only the xy block)
             c[0] = c11; c[1] = c21; c[2] = c13;
             c[3] = c21; c[4] = c22; c[5] = c23;
             c[6] = -c13; c[7] = -c23; c[8] = c33;
