@@ -2361,6 +2361,8 @@ __host__ __device__ inline size_t gs3_stage_off(int mx, int my)
 #endif
 constexpr int GS3_TB = GS3_TB_N, GS3_QB = GS3_QB_N;
 __host__ __device__ inline int gs3_window(int mx) { return (2 * mx + GS3_TB + GS3_QB + 3) / 4 * 4; }
+// one staging buffer: room for four coefficient windows (the row sums use three: xx, xy, yy)
+// and the two difference rows
 __host__ __device__ inline int gs3_stage_stride(int mx)
 {
     return (4 * gs3_window(mx) + 2 * (mx + GS3_QB) + 7) / 8 * 8;
