@@ -13,10 +13,16 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(_HERE, "libcntc_amd.so")
 
-CNTC = dict(if_units=1933, un_cntc=1934, un_spck=1935, un_si=1936, ic_pvtime=1970,
-            ic_bound=1971, ic_tang=1972, ic_norm=1973, ic_force=1974, ic_iestim=1983,
-            ic_output=1984, ic_flow=1985, if_idebug=2000, if_precision=2009,
-            fld_h=1, fld_mu=2, fld_px=3, fld_py=4, fld_pn=5, fld_sx=15, fld_sy=16,
+CNTC = dict(if_units=1933, un_cntc=1934, un_spck=1935, un_si=1936, ic_config=1967,
+            ic_pvtime=1970, ic_bound=1971, ic_tang=1972, ic_norm=1973, ic_force=1974,
+            ic_frclaw=1976, ic_discns=1977, ic_inflcf=1978, ic_mater=1979, ic_exrhs=1980,
+            ic_xflow=1981, ic_heat=1982, ic_iestim=1983, ic_output=1984, ic_flow=1985,
+            ic_return=1986, ic_matfil=1987, ic_sens=1988, ic_sbsout=1991, ic_sbsfil=1992,
+            if_idebug=2000, if_wrtinp=2002, if_ncase=2005, if_precision=2009,
+            fld_h=1, fld_mu=2, fld_px=3, fld_py=4, fld_pn=5, fld_ux=7, fld_uy=8, fld_un=9,
+            fld_taucrt=11, fld_uplsx=12, fld_uplsy=13, fld_sx=15, fld_sy=16, fld_wx=22,
+            fld_wy=23, mt_tim=2011, mt_xr=2012, mt_yr=2013, mt_xw=2014, mt_yw=2015, mt_sw=2016,
+            mt_run=2021, mt_axle=2022, mt_side=2023,
             err_allow=-12, err_norm=-27, err_tang=-28, err_icp=-31, err_other=-99)
 
 _lib = None
@@ -41,6 +47,12 @@ _PROTOS = {
     "cntc_setundeformeddistc": [_pi, _pi, _pi, _pi, _pd],
     "cntc_settangentialforces": [_pi, _pi, _pd, _pd],
     "cntc_setcreepages": [_pi, _pi, _pd, _pd, _pd],
+    "cntc_setextrarigidslip": [_pi, _pi, _pi, _pd, _pd],
+    "cntc_setmetadata": [_pi, _pi, _pi, _pi, _pd],
+    "cntc_getflags": [_pi, _pi, _pi, _pi, _pi],
+    "cntc_getparameters": [_pi, _pi, _pi, _pi, _pd],
+    "cntc_getreferencevelocity": [_pi, _pi, _pd],
+    "cntc_getdisplacements": [_pi, _pi, _pi, _pd, _pd, _pd],
     "cntc_calculate": [_pi, _pi, _pi],
     "cntc_calculatebatch": [_pi, _pi, _pi, _pi],
     "cntc_servestart": [_pd, _pi],
@@ -191,6 +203,47 @@ def setcreepages(ire, icp, vx, vy, phi):
     lib().cntc_setcreepages(_i(ire), _i(icp), _d(vx), _d(vy), _d(phi))
 
 
+def setextrarigidslip(ire, icp, wx, wy):
+    """Extra term (wx, wy) of the tangential right-hand side per element (E digit 1)."""
+    a, ap = _darr(np.ravel(wx))
+    b, bp = _darr(np.ravel(wy))
+    lib().cntc_setextrarigidslip(_i(ire), _i(icp), _i(min(len(a), len(b))), ap, bp)
+
+
+def setmetadata(ire, icp, params, values):
+    p, pp = _iarr(params)
+    v, vp = _darr(values)
+    lib().cntc_setmetadata(_i(ire), _i(icp), _i(len(p)), pp, vp)
+
+
+def getflags(ire, icp, params):
+    p, pp = _iarr(params)
+    v = np.zeros(len(p), dtype=np.int32)
+    lib().cntc_getflags(_i(ire), _i(icp), _i(len(p)), pp, v.ctypes.data_as(_pi))
+    return v
+
+
+def getparameters(ire, icp, itask, lenarr):
+    v = np.zeros(lenarr)
+    lib().cntc_getparameters(_i(ire), _i(icp), _i(itask), _i(lenarr), v.ctypes.data_as(_pd))
+    return v
+
+
+def getreferencevelocity(ire=1, icp=1):
+    v = c_double(0)
+    lib().cntc_getreferencevelocity(_i(ire), _i(icp), ctypes.byref(v))
+    return v.value
+
+
+def getdisplacements(ire=1, icp=1):
+    """(un, ux, uy) of the solution, each of shape (my, mx)."""
+    mx, my = getnumelements(ire, icp)
+    un, ux, uy = (np.zeros(mx * my) for _ in range(3))
+    lib().cntc_getdisplacements(_i(ire), _i(icp), _i(mx * my), un.ctypes.data_as(_pd),
+                                ux.ctypes.data_as(_pd), uy.ctypes.data_as(_pd))
+    return un.reshape(my, mx), ux.reshape(my, mx), uy.reshape(my, mx)
+
+
 def calculate(ire=1, icp=1, idebug=1):
     ierr = c_int(-999)
     lib().cntc_calculate(_i(ire), _i(icp), ctypes.byref(ierr))
@@ -238,7 +291,7 @@ def servestop():
     lib().cntc_servestop()
 
 
-def gethertzcontact(ire=1, icp=1, lenarr=7):
+def gethertzcontact(ire=1, icp=1, lenarr=10):
     v = np.zeros(lenarr)
     lib().cntc_gethertzcontact(_i(ire), _i(icp), _i(lenarr), v.ctypes.data_as(_pd))
     return v
@@ -351,8 +404,9 @@ OP_NAMES = ["vec", "ncg_negpn", "ncg_dd", "ncg_expand", "ncg_hmin", "sn_contract
 
 
 def amdprofile(on):
-    """Start (True; clears earlier records) or stop per-kernel HIP-event timing."""
-    lib().cntc_amdprofile(_i(1 if on else 0))
+    """Start (True: every kernel, or 2: only the SteadyGS, GD probe and FFT product kernels;
+    clears earlier records) or stop (False) per-kernel HIP-event timing."""
+    lib().cntc_amdprofile(_i(2 if on == 2 else (1 if on else 0)))
 
 
 def amdprofileget(lenarr=256):

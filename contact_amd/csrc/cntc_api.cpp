@@ -40,6 +40,10 @@ Scaling select_units(int iunits)
 struct Problem {
     Patch patch;
     Scaling scl;
+    std::map<int, int> flags;        // control digits without effect on this path (getflags)
+    std::map<int, double> meta;      // cntc_setmetadata values, internal units
+    int ncase = 0;                   // cases calculated (CNTC_if_ncase)
+    double fkin = 0.30;              // kinetic friction of L=0 (getparameters)
     double tcpu = 0, twall = 0;
     bool calculated = false;
     int ticket = 0;          // request of the serving loop in flight for this problem
@@ -75,6 +79,7 @@ void settle(Problem* P)
         P->last_rc = rc;
         P->twall += lat;
         P->calculated = true;
+        P->ncase++;
         S.engine->forget(P->ticket);
     }
     S.tickets.erase(P->ticket);
@@ -171,6 +176,20 @@ void cntc_setflags(int* ire, int* icp, int* lenflg, int* params, int* values)
         case CNTC_if_precision: in.precision = v ? 1 : 0; break;
         case CNTC_ic_sens: in.sens = std::max(0, std::min(3, v)); break;   // :844-846
         case CNTC_ic_force: in.force3 = std::max(0, std::min(2, v)); break;  // :834-842, module 3
+        case CNTC_ic_exrhs: in.rztang = std::max(0, std::min(3, v)); break;   // :883-885
+        case CNTC_ic_config: P->flags[params[i]] = std::max(0, std::min(5, v)); break;
+        case CNTC_ic_bound: P->flags[params[i]] = std::max(0, std::min(6, v)); break;
+        case CNTC_ic_discns: if (v >= 0 && v <= 9) P->flags[params[i]] = v; break;
+        case CNTC_ic_inflcf: if (v >= 0 && v <= 4) P->flags[params[i]] = v; break;
+        case CNTC_ic_matfil: case CNTC_ic_sbsfil: P->flags[params[i]] = (v < 0 || v > 2) ? 0 : v; break;
+        case CNTC_ic_output: P->flags[params[i]] = std::max(0, std::min(5, v)); break;
+        case CNTC_ic_sbsout: P->flags[params[i]] = std::max(0, std::min(4, v)) == 3 ? 2 : std::max(0, std::min(4, v)); break;
+        case CNTC_ic_flow: P->flags[params[i]] = std::max(0, std::min(9, v)); break;
+        case CNTC_ic_return: P->flags[params[i]] = std::max(0, std::min(3, v)); break;
+        case CNTC_ic_xflow: P->flags[params[i]] = 1; break;
+        case CNTC_if_wrtinp: case CNTC_ic_ifmeth: case CNTC_ic_ifvari: case CNTC_if_ncase:
+            P->flags[params[i]] = v;
+            break;
         default: break;   // output, flow, file flags: no effect on the solution
         }
     }
@@ -264,30 +283,62 @@ void cntc_setfrictionmethod(int* ire, int* icp, int* imeth, int* nparam, double*
     Problem* P = activate(*ire, *icp);
     if (!P || *imeth != 0 || *nparam != 2) return;
     P->patch.in.fstat = params[0];
+    P->fkin = params[1];
 }
 
-// src/contact_addon.f90:1972-2093
+// src/contact_addon.f90:1972-2093.  The method is checked before the parameter count, as the
+// reference does (-6 takes 6 parameters, the others 5).  The rectangular (-4, -5) and SDEC (-6)
+// options prescribe the normal pressure instead of solving for it; this library does not
+// implement them, so the problem is marked and cntc_calculate returns CNTC_err_other for it
+// instead of solving the previous geometry.
 void cntc_sethertzcontact(int* ire, int* icp, int* ipotcn, int* nparam, double* params)
 {
     std::lock_guard<std::mutex> g(store().mtx);
     Problem* P = activate(*ire, *icp);
-    if (!P || *nparam != 5) return;
+    if (!P) return;
+    static const int nparam_loc[6] = {6, 5, 5, 5, 5, 5};   // ipotcn -6 .. -1
+    if (*ipotcn < -6 || *ipotcn > -1) {
+        if (store().idebug > 0) std::fprintf(stderr, "cntc_sethertzcontact: method IPOTCN=%d does not exist\n", *ipotcn);
+        return;
+    }
+    if (*nparam != nparam_loc[*ipotcn + 6]) {
+        if (store().idebug > 0)
+            std::fprintf(stderr, "cntc_sethertzcontact: method IPOTCN=%d needs %d parameters\n", *ipotcn,
+                         nparam_loc[*ipotcn + 6]);
+        return;
+    }
     CaseInput& in = P->patch.in;
     double L = P->scl.len;
+    in.unsupported = nullptr;
+    in.mx = std::max(1, (int)std::lround(params[0]));
+    in.my = std::max(1, (int)std::lround(params[1]));
     if (*ipotcn == -3) {
-        in.mx = std::max(1, (int)std::lround(params[0]));
-        in.my = std::max(1, (int)std::lround(params[1]));
         in.hz_aa = std::max(1e-6, params[2]) * L;
         in.hz_bb = std::max(1e-6, params[3]) * L;
         in.hz_scale = std::max(1e-6, params[4]);
+    } else if (*ipotcn == -2) {
+        in.hz_a1 = std::max(1e-12, params[2]) / L;
+        in.hz_aob = std::max(1e-6, params[3]);
+        in.hz_scale = std::max(1e-6, params[4]);
     } else if (*ipotcn == -1) {
-        in.mx = std::max(1, (int)std::lround(params[0]));
-        in.my = std::max(1, (int)std::lround(params[1]));
         in.hz_a1 = std::max(1e-12, params[2]) / L;
         in.hz_b1 = std::max(1e-12, params[3]) / L;
         in.hz_scale = std::max(1e-6, params[4]);
     } else {
-        return;
+        // stored for the getters as the reference stores them (:2024-2049)
+        if (*ipotcn == -6) {
+            in.hz_aa = std::max(1e-6, params[2]) * L;
+            in.hz_bneg = std::max(1e-6, params[3]) * L;
+            in.hz_bpos = std::max(1e-6, params[4]) * L;
+            in.hz_scale = std::max(1e-6, params[5]);
+        } else {
+            if (*ipotcn == -5) in.hz_aa = std::max(1e-6, params[2]) * L;
+            else in.hz_a1 = std::max(1e-6, params[2]) / L;
+            in.hz_bb = std::max(1e-6, params[3]) * L;
+            in.hz_scale = std::max(1e-6, params[4]);
+        }
+        in.unsupported = *ipotcn == -6 ? "Hertzian option IPOTCN=-6 (SDEC) is not supported"
+                                       : "Hertzian rectangular contact (IPOTCN=-4, -5) is not supported";
     }
     in.ipotcn = *ipotcn;
 }
@@ -300,6 +351,7 @@ void cntc_setpotcontact(int* ire, int* icp, int* ipotcn, int* nparam, double* pa
     if (!P || *ipotcn < 1 || *ipotcn > 4 || *nparam != 6) return;
     CaseInput& in = P->patch.in;
     double L = P->scl.len;
+    in.unsupported = nullptr;
     in.mx = std::max(1, (int)std::lround(params[0]));
     in.my = std::max(1, (int)std::lround(params[1]));
     if (*ipotcn == 1) {
@@ -393,6 +445,45 @@ void cntc_setcreepages(int* ire, int* icp, double* vx, double* vy, double* phi)
     }
 }
 
+// src/contact_addon.f90:2534-2593: E digit 1 and the extra term of the tangential right-hand
+// side, sized for the current potential contact (elements beyond lenarr get 0)
+void cntc_setextrarigidslip(int* ire, int* icp, int* lenarr, double* wx, double* wy)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    CaseInput& in = P->patch.in;
+    const int npot = std::max(1, in.mx) * std::max(1, in.my);
+    in.rztang = 1;
+    in.exrhs.assign(2 * (size_t)npot, 0.0);
+    for (int i = 0; i < std::min(*lenarr, npot); i++) {
+        in.exrhs[i] = wx[i];
+        in.exrhs[npot + i] = wy[i];
+    }
+}
+
+// src/contact_addon.f90:1055-1124: metadata are kept with the problem (lengths in mm); they do
+// not enter the solution.  Code 0 marks an unused position; unknown codes are ignored.
+void cntc_setmetadata(int* ire, int* icp, int* lenmta, int* params, double* values)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    for (int i = 0; i < *lenmta; i++) {
+        int c = params[i];
+        double v = values[i];
+        if (c == 0) continue;
+        if (c == CNTC_mt_sw || c == CNTC_mt_xr || c == CNTC_mt_yr || c == CNTC_mt_xw || c == CNTC_mt_yw)
+            P->meta[c] = P->scl.len * v;
+        else if (c == CNTC_mt_tim)
+            P->meta[c] = v;
+        else if (c == CNTC_mt_run || c == CNTC_mt_axle || c == CNTC_mt_side)
+            P->meta[c] = (double)std::lround(v);
+        else if (store().idebug > 0)
+            std::fprintf(stderr, "cntc_setmetadata: unknown metadata code=%d is ignored\n", c);
+    }
+}
+
 static void calculate_many(int n, const int* ire, const int* icp, int* ierror)
 {
     Store& S = store();
@@ -438,6 +529,7 @@ static void calculate_many(int n, const int* ire, const int* icp, int* ierror)
         if (!P || j >= pr.size() || P != pr[j]) continue;
         ierror[k] = P->patch.out.ierror;
         P->calculated = true;
+        P->ncase++;
         P->twall += P->patch.out.t_done > 0 ? P->patch.out.t_done : tw;
         P->tcpu += tc / pr.size();
         j++;
@@ -477,8 +569,13 @@ void cntc_gethertzcontact(int* ire, int* icp, int* lenarr, double* v)
     if (!P) return;
     const auto& o = P->patch.out;
     double L = P->scl.len;
-    double vals[7] = {o.hz_a1 * L, o.hz_b1 * L, o.hz_aa / L, o.hz_bb / L, o.hz_rho / L, o.hz_cp / L, P->patch.in.hz_scale};
-    for (int k = 0; k < std::min(*lenarr, 7); k++) v[k] = vals[k];
+    // aob: prescribed for ipotcn = -2, else aa/bb (1 for an empty ellipse), src/m_hertz.f90:75-82;
+    // bneg, bpos: the SDEC inputs (ipotcn = -6)
+    double aob = P->patch.in.ipotcn == -2 ? P->patch.in.hz_aob
+                 : (o.hz_aa <= 1e-20 && o.hz_bb <= 1e-20) ? 1.0 : o.hz_aa / o.hz_bb;
+    double vals[10] = {o.hz_a1 * L, o.hz_b1 * L, o.hz_aa / L, o.hz_bb / L, o.hz_rho / L, o.hz_cp / L,
+                       P->patch.in.hz_scale, P->patch.in.hz_bneg / L, P->patch.in.hz_bpos / L, aob};
+    for (int k = 0; k < std::min(*lenarr, 10); k++) v[k] = vals[k];
 }
 
 void cntc_getnumelements(int* ire, int* icp, int* mx, int* my)
@@ -633,11 +730,130 @@ void cntc_getfielddata(int* ire, int* icp, int* ifld, int* lenarr, double* fld)
     case CNTC_fld_px: v = field_of(P, 1); fac = s.area * s.body; break;
     case CNTC_fld_py: v = field_of(P, 2); fac = s.area * s.body; break;
     case CNTC_fld_pn: v = field_of(P, 0); fac = s.area; break;
+    case CNTC_fld_ux: v = field_of(P, 7); fac = 1.0 / s.len; break;
+    case CNTC_fld_uy: v = field_of(P, 8); fac = 1.0 / s.len; break;
+    case CNTC_fld_un: v = field_of(P, 6); fac = 1.0 / s.len; break;
+    // M=0: no plasticity, so taucrt stays at tau_c0 = 0 (src/m_hierarch_data.f90:1462) and the
+    // plastic deformation upls at 0 (init_curr_data, src/m_sdis.f90:625)
+    case CNTC_fld_taucrt: case CNTC_fld_uplsx: case CNTC_fld_uplsy: v.assign(P->patch.n, 0.0); break;
     case CNTC_fld_sx: v = field_of(P, 3); fac = s.body / dq; break;
     case CNTC_fld_sy: v = field_of(P, 4); fac = s.body / dq; break;
-    default: return;
+    case CNTC_fld_wx: v = field_of(P, 9); fac = s.body / dq; break;
+    case CNTC_fld_wy: v = field_of(P, 10); fac = s.body / dq; break;
+    default:
+        // src/contact_addon.f90:5787-5790 (unknown code) and :5799-5801 (temperature fields,
+        // which need the H digit): fld is left unchanged
+        if (store().idebug > 0) std::fprintf(stderr, "cntc_getfielddata: field %d is not available\n", *ifld);
+        return;
     }
     for (int k = 0; k < std::min(*lenarr, (int)v.size()); k++) fld[k] = fac * v[k];
+}
+
+// src/contact_addon.f90:5889-5915
+void cntc_getdisplacements(int* ire, int* icp, int* lenarr, double* un, double* ux, double* uy)
+{
+    int fx = CNTC_fld_ux, fy = CNTC_fld_uy, fn = CNTC_fld_un;
+    cntc_getfielddata(ire, icp, &fx, lenarr, ux);
+    cntc_getfielddata(ire, icp, &fy, lenarr, uy);
+    cntc_getfielddata(ire, icp, &fn, lenarr, un);
+}
+
+// src/contact_addon.f90:4001-4160; digits this library keeps without using them come back as
+// set, or at the module-3 defaults of ic_init (src/m_hierarch_data.f90:966-1014)
+void cntc_getflags(int* ire, int* icp, int* lenflg, int* params, int* values)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    const CaseInput& in = P->patch.in;
+    auto stored = [&](int code, int dflt) {
+        auto it = P->flags.find(code);
+        return it == P->flags.end() ? dflt : it->second;
+    };
+    for (int i = 0; i < *lenflg; i++) {
+        switch (params[i]) {
+        case 0: break;
+        case CNTC_ic_config: values[i] = stored(CNTC_ic_config, 1); break;
+        case CNTC_ic_pvtime: values[i] = in.pvtime; break;
+        case CNTC_ic_bound: values[i] = stored(CNTC_ic_bound, 0); break;
+        case CNTC_ic_tang: values[i] = in.tang; break;
+        case CNTC_ic_norm: values[i] = in.norm; break;
+        case CNTC_ic_force: values[i] = in.force3; break;
+        case CNTC_ic_sens: values[i] = in.sens; break;
+        case CNTC_ic_frclaw: values[i] = 0; break;
+        case CNTC_ic_discns: values[i] = stored(CNTC_ic_discns, 2); break;
+        case CNTC_ic_npomax: values[i] = 0; break;   // module 1 only
+        case CNTC_ic_inflcf: values[i] = stored(CNTC_ic_inflcf, 2); break;
+        case CNTC_ic_ifmeth: values[i] = stored(CNTC_ic_ifmeth, 0); break;
+        case CNTC_ic_ifvari: values[i] = stored(CNTC_ic_ifvari, 0); break;
+        case CNTC_ic_mater: values[i] = 0; break;
+        case CNTC_ic_xflow: values[i] = stored(CNTC_ic_xflow, 0); break;
+        case CNTC_ic_heat: values[i] = 0; break;
+        case CNTC_ic_iestim: values[i] = in.iestim; break;
+        case CNTC_ic_matfil: values[i] = stored(CNTC_ic_matfil, 1); break;
+        case CNTC_ic_output: values[i] = stored(CNTC_ic_output, 2); break;
+        case CNTC_ic_sbsfil: values[i] = stored(CNTC_ic_sbsfil, 1); break;
+        case CNTC_ic_sbsout: values[i] = stored(CNTC_ic_sbsout, 1); break;
+        case CNTC_ic_flow: values[i] = stored(CNTC_ic_flow, 4); break;
+        case CNTC_ic_return: values[i] = stored(CNTC_ic_return, 1); break;
+        case CNTC_if_wrtinp: values[i] = stored(CNTC_if_wrtinp, 0); break;
+        case CNTC_if_ncase: values[i] = stored(CNTC_if_ncase, 0) + P->ncase; break;
+        default:
+            if (store().idebug > 0)
+                std::fprintf(stderr, "cntc_getflags: unknown parameter code=%d is ignored\n", params[i]);
+        }
+    }
+}
+
+// src/contact_addon.f90:4164-4258.  Parameters of options this library does not implement
+// (visco-elastic, plastic, FASTSIM flexibilities, friction memory) are 0, their defaults.
+void cntc_getparameters(int* ire, int* icp, int* itask, int* lenarr, double* values)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    const CaseInput& in = P->patch.in;
+    const Scaling& s = P->scl;
+    const int n = *lenarr;
+    auto put = [&](int k, double v) { if (n >= k) values[k - 1] = v; };
+    if (*itask == 1) {
+        double dq = P->patch.dq > 0 ? P->patch.dq : (in.dq > 0 ? in.dq : in.dx);
+        put(1, in.veloc / s.veloc);
+        put(2, in.chi / s.angle);
+        put(3, dq / s.len);
+        put(4, 0.0);
+        put(5, 0.0);
+        put(6, 0.0);
+    } else if (*itask == 2) {
+        double ga = 2.0 / (1.0 / in.gg1 + 1.0 / in.gg2);
+        double nu = ga * (in.poiss1 / in.gg1 + in.poiss2 / in.gg2) / 2.0;
+        double ak = (ga / 4.0) * ((1.0 - 2.0 * in.poiss1) / in.gg1 - (1.0 - 2.0 * in.poiss2) / in.gg2);
+        put(1, in.gg1 * s.area);
+        put(2, in.gg2 * s.area);
+        put(3, ga * s.area);
+        put(4, in.poiss1);
+        put(5, in.poiss2);
+        put(6, nu);
+        put(7, ak);
+        for (int k = 8; k <= std::min(n, 22); k++) values[k - 1] = 0.0;
+    } else if (*itask == 3) {
+        put(1, 0.0);       // L = 0
+        put(2, 1.0);       // one friction value
+        put(3, 0.0);
+        put(4, 0.0);
+        put(5, 0.0);
+        put(6, in.fstat);
+        put(7, P->fkin);
+    }
+}
+
+// src/contact_addon.f90:4987-5017
+void cntc_getreferencevelocity(int* ire, int* icp, double* veloc)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Problem* P = activate(*ire, *icp);
+    if (!P) return;
+    *veloc = P->patch.in.veloc / P->scl.veloc;
 }
 
 void cntc_gettractions(int* ire, int* icp, int* lenarr, double* pn, double* px, double* py)
@@ -768,7 +984,7 @@ void cntc_amdprofile(int* mode)
 {
     std::lock_guard<std::mutex> g(store().mtx);
     Engine* E = engine();
-    if (E) E->set_profile(*mode != 0);
+    if (E) E->set_profile(*mode);
 }
 
 void cntc_amdprofileget(int* lenarr, int* nkinds, int* kinds, double* seconds, double* bytes, double* launches)

@@ -256,7 +256,15 @@ public:
         double seconds = 0, bytes = 0;
         long launches = 0;
     };
-    bool profile = false;
+    // 0: off; 1: every kernel; 2: only the kernels whose rooflines are reported (SteadyGS, the
+    // GD line-search probe and the FFT product stages), so that timing the rest costs nothing
+    int profile = 0;
+    bool timed_kind(int kind) const
+    {
+        return profile == 1 || (profile == 2 && (kind == OP_STDYGS || kind == OP_GD_PROBE ||
+                                                 kind == OP_CONV || kind == OP_CONV32));
+    }
+    std::mutex prof_mtx;             // prof and nflush are read by the caller's thread
     std::map<int, ProfEntry> prof;   // key kind*4 + stage
     std::vector<hipEvent_t> ev_pool;
     void set_waiter(int slot, std::coroutine_handle<> h) { waiter_[slot] = h; }
@@ -395,6 +403,7 @@ private:
             if (op.timed) {
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, op.t0, op.t1));
+                std::lock_guard<std::mutex> lk(prof_mtx);
                 auto& pe = prof[op.kind * 4 + op.variant];
                 pe.seconds += ms * 1e-3;
                 for (int i = 0; i < op.nitems; i++) pe.bytes += op.K >= 3 ? r[(size_t)i * op.K + 2] : 0.0;
@@ -512,7 +521,7 @@ private:
         HIPCHK(hipMemcpyAsync(op.dev, op.host, items_b + gs_b, hipMemcpyHostToDevice, op.stream));
         op.kind = kind;
         op.variant = variant;
-        op.timed = profile;
+        op.timed = timed_kind(kind);
         if (op.timed) HIPCHK(hipEventRecord(op.t0, op.stream));
         launch_op(kind, op.dev, (const int*)(op.dev + items_b), nitems, tot, (double*)(op.dev + op.res_off),
                   variant, lds, op.stream);
@@ -638,14 +647,17 @@ private:
         std::vector<std::pair<int, double>> pkeys;
         size_t nev = 0;
         if (profile) {
-            while (ev_pool.size() < 2 * launches.size()) {
+            size_t need = 0;
+            for (auto& L : launches) need += timed_kind(L.kind) && L.ngroups > 0 ? 2 : 0;
+            while (ev_pool.size() < need) {
                 hipEvent_t ev;
                 HIPCHK(hipEventCreate(&ev));
                 ev_pool.push_back(ev);
             }
         }
         for (auto& L : launches) {
-            if (profile && L.ngroups > 0) {
+            const bool timed = timed_kind(L.kind) && L.ngroups > 0;
+            if (timed) {
                 double bytes = 0;
                 for (int i = 0; i < L.nitems; i++)
                     bytes += op_bytes(L.kind, hstage_ + L.items_off + item_size(L.kind) * i, L.stage);
@@ -660,7 +672,7 @@ private:
                               L.kind, L.stage, hipGetErrorString(le), L.ngroups, L.lds);
                 throw std::runtime_error(msg);
             }
-            if (profile && L.ngroups > 0) HIPCHK(hipEventRecord(ev_pool[nev++], s_));
+            if (timed) HIPCHK(hipEventRecord(ev_pool[nev++], s_));
             if (L.K > 0)
                 launch_finalize((const int*)(dstage_ + L.gstart_off), (const int*)(dstage_ + L.modes_off),
                                 L.nitems, L.K, dpart_ + L.part_off, dres_ + L.res_off, s_);
@@ -677,6 +689,7 @@ private:
         flush_issue_s += std::chrono::duration<double>(tf1 - tf0).count();
         flush_wait_s += std::chrono::duration<double>(tf2 - tf1).count();
         flush_launches += (long)launches.size();
+        std::lock_guard<std::mutex> lk(prof_mtx);
         for (size_t k = 0; k < pkeys.size(); k++) {
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, ev_pool[2 * k], ev_pool[2 * k + 1]));
@@ -1011,6 +1024,61 @@ static bool hertz3d(double e_star, int ipotcn, int norm, double& a1, double& b1,
     return true;
 }
 
+// elliptic_to_curv, src/m_hertz.f90:95-213: the curvature b1 for which the Hertz ellipse of
+// curvature a1 has semi-axis ratio aob -- b1 halved or doubled from b1 = a1 until aob is
+// bracketed, then regula falsi to a relative tolerance of 10 epshz
+static bool elliptic_to_curv(double e_star, int norm, double a1, double aob, double& b1, double& aa,
+                             double& bb, double& pen, double& fn, double& cp, double& rho)
+{
+    const double epshz = sp(1e-9f), epsaob = 10.0 * epshz;
+    int iter = 0;
+    double a1w = a1;
+    b1 = a1;
+    auto solve = [&]() { return hertz3d(e_star, -1, norm, a1w, b1, aa, bb, pen, fn, cp, rho); };
+    if (!solve()) return false;
+    if (aa < TINY && bb < TINY) return true;
+    double b1low, b1high, aoblow, aobhigh;
+    if (aa / bb > aob) {
+        b1high = b1;
+        aobhigh = aa / bb;
+        while (iter <= 100 && aa / bb > aob) {
+            iter++;
+            b1high = b1;
+            aobhigh = aa / bb;
+            b1 *= 0.5;
+            if (!solve()) return false;
+        }
+        b1low = b1;
+        aoblow = aa / bb;
+    } else {
+        b1low = b1;
+        aoblow = aa / bb;
+        while (iter <= 100 && aa / bb < aob) {
+            iter++;
+            b1low = b1;
+            aoblow = aa / bb;
+            b1 *= 2.0;
+            if (!solve()) return false;
+        }
+        b1high = b1;
+        aobhigh = aa / bb;
+    }
+    while (iter <= 100 && std::fabs(aa / bb - aob) > epsaob * aob) {
+        iter++;
+        double rico = (aobhigh - aoblow) / (b1high - b1low);
+        b1 = b1low + (aob - aoblow) / rico;
+        if (!solve()) return false;
+        if (aa / bb > aob) {
+            b1high = b1;
+            aobhigh = aa / bb;
+        } else {
+            b1low = b1;
+            aoblow = aa / bb;
+        }
+    }
+    return true;
+}
+
 // Host part of contac's set-up (src/m_scontc.f90:37-176): checks, Hertz, grid, h and w.
 // device bytes of one patch's arena (grid functions, coefficients, spectra, workspaces)
 size_t patch_bytes(int mx, int my)
@@ -1024,7 +1092,7 @@ size_t patch_bytes(int mx, int my)
     size_t pH = (size_t)mx + 1, pL = 2 * (size_t)my;
     size_t ws = std::max((size_t)3 * Ly * H, pH * pL) * 16;
     size_t emb = std::max((size_t)Lx * Ly, nb) * 8;
-    return al(n) + al(37 * n * 8) + al(9 * nb * 8) * 2 + al(3 * nb * 8) + al(9 * bsz * 16) * 2 +
+    return al(n) + al(40 * n * 8) + al(9 * nb * 8) * 2 + al(3 * nb * 8) + al(9 * bsz * 16) * 2 +
            al(3 * bsz * 16) + al(9 * bsz * 8) + al(3 * bsz * 8) + al(3 * emb) + 2 * al(ws) + 4096;
 }
 
@@ -1033,7 +1101,12 @@ bool Engine::prepare(Patch& P)
     CaseInput& in = P.in;
     CaseResult& R = P.out;
     R = CaseResult();
-    bool hertzian = in.ipotcn == -1 || in.ipotcn == -3;
+    if (in.unsupported) {
+        R.ierror = ERR_OTHER;
+        last_error = in.unsupported;
+        return false;
+    }
+    bool hertzian = in.ipotcn == -1 || in.ipotcn == -2 || in.ipotcn == -3;
     if (in.tang < 0 || in.tang > 3 || in.ipotcn == 0 || (!hertzian && in.ibase == 0)) {
         R.ierror = ERR_OTHER;
         last_error = "potential contact or undeformed distance not specified";
@@ -1055,11 +1128,16 @@ bool Engine::prepare(Patch& P)
     int ibase = in.ibase;
     double prm[6];
     std::memcpy(prm, in.prmudf, sizeof prm);
-    if (in.ipotcn == -1 || in.ipotcn == -3) {
+    if (hertzian) {
+        // hzsol, src/m_hertz.f90:29-90
         double e_star = P.ga / (1.0 - P.nu);
         R.hz_a1 = in.hz_a1; R.hz_b1 = in.hz_b1; R.hz_aa = in.hz_aa; R.hz_bb = in.hz_bb;
-        if (!hertz3d(e_star, in.ipotcn, in.norm, R.hz_a1, R.hz_b1, R.hz_aa, R.hz_bb, P.pen, P.fntrue,
-                     R.hz_cp, R.hz_rho)) { R.ierror = ERR_OTHER; return false; }
+        bool ok = in.ipotcn == -2
+                      ? elliptic_to_curv(e_star, in.norm, in.hz_a1, in.hz_aob, R.hz_b1, R.hz_aa, R.hz_bb,
+                                         P.pen, P.fntrue, R.hz_cp, R.hz_rho)
+                      : hertz3d(e_star, in.ipotcn, in.norm, R.hz_a1, R.hz_b1, R.hz_aa, R.hz_bb, P.pen,
+                                P.fntrue, R.hz_cp, R.hz_rho);
+        if (!ok) { R.ierror = ERR_OTHER; last_error = "Hertz solution failed"; return false; }
         ibase = 1;
         std::memset(prm, 0, sizeof prm);
         prm[0] = R.hz_a1;
@@ -1150,11 +1228,12 @@ bool Engine::prepare(Patch& P)
     char* b = P.mem.base;
     auto take = [&](size_t bytes) { char* r = b; b += al(bytes); return r; };
     P.el = (int8_t*)take(n);
-    double* d = (double*)take(37 * n * 8);
+    double* d = (double*)take(40 * n * 8);
     P.pn = d; P.px = d + n; P.py = d + 2 * n; P.sx = d + 3 * n; P.sy = d + 4 * n;
     P.pvx = d + 5 * n; P.pvy = d + 6 * n; P.pvn = d + 7 * n;
     P.h = d + 8 * n; P.hx = d + 9 * n; P.hy = d + 10 * n; P.wsx = d + 11 * n; P.wsy = d + 12 * n;
     for (int k = 0; k < 24; k++) P.w[k] = d + (13 + k) * n;
+    P.un = d + 37 * n; P.ux = d + 38 * n; P.uy = d + 39 * n;
     P.cf = (double*)take(9 * nb * 8);
     P.cfv = (double*)take(9 * nb * 8);
     P.cfm = (double*)take(3 * nb * 8);
@@ -1176,6 +1255,16 @@ bool Engine::prepare(Patch& P)
     }
     if (ibase == 9)
         HIPCHK(hipMemcpyAsync(P.h, P.h_host.data(), n * 8, hipMemcpyHostToDevice, stream_));
+    if (in.rztang == 1) {
+        // extra rigid slip of cntc_setextrarigidslip, sized for the grid it was given for
+        if (in.exrhs.size() != 2 * n) {
+            R.ierror = ERR_OTHER;
+            last_error = "extra rigid slip was set for another grid size";
+            return false;
+        }
+        HIPCHK(hipMemcpyAsync(P.hx, in.exrhs.data(), n * 8, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(P.hy, in.exrhs.data() + n, n * 8, hipMemcpyHostToDevice, stream_));
+    }
     // init_curr_data, src/m_sdis.f90:587-768: eldiv0 (src/m_sdis.f90:818-1012) on the host for
     // a non-Hertzian zero initial estimate; the Hertzian estimate is formed on the GPU
     if (in.iestim == 0 && ihertz == 0) {
@@ -2180,6 +2269,7 @@ static SetupItem setup_item(Patch& P)
     S.pn = P.pn; S.px = P.px; S.py = P.py; S.pvn = P.pvn; S.pvx = P.pvx; S.pvy = P.pvy;
     S.xc1 = P.xc1; S.yc1 = P.yc1; S.dx = P.dx; S.dy = P.dy;
     for (int k = 0; k < 6; k++) S.prm[k] = P.prm[k];
+    S.facx = S.facy = 1.0;
     return S;
 }
 
@@ -2530,6 +2620,11 @@ static Task calculate(Ctx C)
         S.cphi = P.in.cphi; S.dq = P.dq;
         S.xofs = P.xofs; S.yofs = P.yofs;
         S.hz_aa = R.hz_aa; S.hz_bb = R.hz_bb; S.pnmax = P.hz_pnmax;
+        // E digit (src/m_sdis.f90:517-553): 1 extra rigid slip (copied into hx, hy by prepare),
+        // 2 no spin in wy, 3 no spin in wx
+        S.exrhs = P.in.rztang == 1 ? 1 : 0;
+        S.facy = P.in.rztang == 2 ? 0.0 : 1.0;
+        S.facx = P.in.rztang == 3 ? 0.0 : 1.0;
         Res r = co_await C.opr(OP_SETUP, it);
         P.hmax = r.v[0];
         P.hmin = -r.v[1];
@@ -2560,6 +2655,13 @@ static Task calculate(Ctx C)
             R.ittang = ittang;
         }
     }
+    // displacement differences of the solution (soutpt, src/m_soutpt.f90:371-397): us = A ps on
+    // the interior, 0 elsewhere (init_curr_data clears us for every case, src/m_sdis.f90:624);
+    // with K = 0 the normal and tangential parts decouple.  Read by cntc_getdisplacements.
+    C.set(M_ALL, P.un, 0.0);
+    C.set(M_ALL, P.ux, 0.0, P.uy);
+    C.conv_n(0, M_INT, P.un, P.pn);
+    if (P.in.tang != 0) C.conv_t(0, M_INT, P.ux, P.uy, P.px, P.py);
     // forces (soutpt, src/m_soutpt.f90:424-450)
     AnyItem fi;
     fi.setup = setup_item(P);
@@ -2773,15 +2875,17 @@ Patch* Engine::tickets_at(int id)
     return tickets_[id].p;
 }
 
-void Engine::set_profile(bool on)
+void Engine::set_profile(int mode)
 {
-    sched_->profile = on;
+    std::lock_guard<std::mutex> lk(sched_->prof_mtx);
+    sched_->profile = mode == 1 || mode == 2 ? mode : 0;
     sched_->prof.clear();
     sched_->nflush = 0;
 }
 
 int Engine::get_profile(int len, int* kinds, double* seconds, double* bytes, long* launches) const
 {
+    std::lock_guard<std::mutex> lk(sched_->prof_mtx);
     int k = 0;
     for (auto& kv : sched_->prof) {
         if (k < len) {
@@ -2795,7 +2899,11 @@ int Engine::get_profile(int len, int* kinds, double* seconds, double* bytes, lon
     return k;
 }
 
-long Engine::rounds() const { return sched_->nflush; }
+long Engine::rounds() const
+{
+    std::lock_guard<std::mutex> lk(sched_->prof_mtx);
+    return sched_->nflush;
+}
 
 void Engine::get_field(const Patch& p, int what, double* out) const
 {
@@ -2807,6 +2915,11 @@ void Engine::get_field(const Patch& p, int what, double* out) const
     case 3: src = p.sx; break;
     case 4: src = p.sy; break;
     case 5: src = p.h; break;
+    case 6: src = p.un; break;
+    case 7: src = p.ux; break;
+    case 8: src = p.uy; break;
+    case 9: src = p.hx; break;
+    case 10: src = p.hy; break;
     }
     if (!src || p.n <= 0) return;
     HIPCHK(hipMemcpyAsync(out, src, (size_t)p.n * 8, hipMemcpyDeviceToHost, stream_));

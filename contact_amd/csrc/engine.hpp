@@ -24,10 +24,14 @@ struct CaseInput {
     int tang = 3, norm = 0, pvtime = 2, iestim = 0, gausei = 0;
     double poiss1 = 0.28, poiss2 = 0.28, gg1 = 1.0, gg2 = 1.0;
     double fstat = 0.30;
-    int ipotcn = 0;                 // 1..4 grid forms, -1/-3 Hertzian; 0 = not yet set
+    int ipotcn = 0;                 // 1..4 grid forms, -1/-2/-3 Hertzian; 0 = not yet set
+    // set when the last potential-contact call asked for an option this library does not solve
+    // (Hertzian rectangles and SDEC, ipotcn -4..-6): calculate reports it instead of solving
+    // the previous geometry
+    const char* unsupported = nullptr;
     int mx = 1, my = 1;
     double xl = 0, yl = 0, xh = 0, yh = 0, xc1 = 0, yc1 = 0, xcm = 0, ycm = 0, dx = 1, dy = 1;
-    double hz_a1 = 0, hz_b1 = 0, hz_aa = 0, hz_bb = 0, hz_scale = 1.0;
+    double hz_a1 = 0, hz_b1 = 0, hz_aa = 0, hz_bb = 0, hz_aob = 1.0, hz_scale = 1.0, hz_bneg = 0, hz_bpos = 0;
     int ibase = 0;                  // 1 quadratic, 9 element-wise; 0 = not yet set
     double prmudf[6] = {0, 0, 0, 0, 0, 0};
     std::vector<double> h_elem;
@@ -47,6 +51,10 @@ struct CaseInput {
     // creepages cksi (and ceta) then being solved for and kept in cksi, ceta
     int force3 = 0;
     double fxrel = 0, fyrel = 0;
+    // E digit (src/m_sdis.f90:517-573): 1 adds the extra rigid slip exrhs (x then y, mx*my each,
+    // cntc_setextrarigidslip); 2 drops the spin term from wy, 3 from wx
+    int rztang = 0;
+    std::vector<double> exrhs;
 };
 
 // Results of one case.
@@ -99,6 +107,7 @@ struct Patch {
     double *pvx = nullptr, *pvy = nullptr, *pvn = nullptr;
     double *h = nullptr, *hx = nullptr, *hy = nullptr, *wsx = nullptr, *wsy = nullptr;
     double* w[24] = {};              // work vectors
+    double *un = nullptr, *ux = nullptr, *uy = nullptr;   // displacement differences A ps
     double *cf = nullptr, *cfv = nullptr, *cfm = nullptr;  // coefficient blocks cs, cv, ms
     void *spec = nullptr, *specv = nullptr, *specm = nullptr, *spec32 = nullptr, *specm32 = nullptr;
     double* embed = nullptr;         // padded real scratch for spectra (Lx*Ly, or 4*cf_mx*cf_my)
@@ -138,7 +147,7 @@ public:
     std::string last_error;
     std::chrono::steady_clock::time_point t_solve0;   // start of the current solve call
     // per-kernel timing (HIP events around every launch); key = kind*4 + stage
-    void set_profile(bool on);
+    void set_profile(int mode);
     int get_profile(int len, int* kinds, double* seconds, double* bytes, long* launches) const;
     long rounds() const;
 

@@ -1541,9 +1541,11 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_setup(const SetupItem* items, 
         } else {
             h = I.h[i];
         }
-        // rigid slip w = (xi - phi (y + yofs), eta + phi (x + xofs)), times -dq
-        double vx = 1.0 * -(y + I.yofs - 0.0) * I.cphi + 0.0;
-        double vy = 1.0 * (x + I.xofs - 0.0) * I.cphi + 0.0;
+        // rigid slip w = (xi - phi (y + yofs), eta + phi (x + xofs)) + exrhs, times -dq
+        // (set_tang_rhs, src/m_sdis.f90:556-579)
+        const double ex = I.exrhs ? I.hx[i] : 0.0, ey = I.exrhs ? I.hy[i] : 0.0;
+        double vx = I.facx * -(y + I.yofs - 0.0) * I.cphi + ex;
+        double vy = I.facy * (x + I.xofs - 0.0) * I.cphi + ey;
         vx += I.cksi;
         vy += I.ceta;
         I.hx[i] = vx * -I.dq;

@@ -209,6 +209,8 @@ struct ConvItem {
 struct SetupItem {
     int n, mx, my, ibase;          // ibase 1: quadratic gap from prm; otherwise h is given
     int prev, zero_pn, zero_pt, hertz;
+    int exrhs, pad0;               // exrhs: hx, hy hold the extra rigid slip on entry
+    double facx, facy;             // spin factors of the rigid slip (E digit 2, 3 zero one)
     int8_t* el;
     double* h; double* hx; double* hy;
     double* pn; double* px; double* py;

@@ -139,6 +139,10 @@ def apply_case(cntc, ire, icp, c):
         cntc.setsolverflags(ire, icp, 0, [c["maxgs"], c["maxin"], 30, 1], [c["eps"]])
     if c["ipotcn"] == -3:
         cntc.sethertzcontact(ire, icp, -3, [c["mx"], c["my"], c["aa"], c["bb"], c["scale"]])
+    elif c["ipotcn"] == -2:
+        cntc.sethertzcontact(ire, icp, -2, [c["mx"], c["my"], c["a1"], c["aob"], c["scale"]])
+    elif c["ipotcn"] == -1:
+        cntc.sethertzcontact(ire, icp, -1, [c["mx"], c["my"], c["a1"], c["b1"], c["scale"]])
     else:
         cntc.setpotcontact(ire, icp, 1, [c["mx"], c["my"], c["xl"], c["yl"], c["dx"], c["dy"]])
         if c.get("h") is not None:
@@ -155,6 +159,10 @@ def apply_case(cntc, ire, icp, c):
     if c.get("force") is not None:
         cntc.setflags(ire, icp, [1974], [c["force"]])
         cntc.settangentialforces(ire, icp, c.get("fxrel", 0.0), c.get("fyrel", 0.0))
+    if c.get("rztang") is not None:
+        cntc.setflags(ire, icp, [1980], [c["rztang"]])
+    if c.get("exrhs") is not None:
+        cntc.setextrarigidslip(ire, icp, c["exrhs"][0], c["exrhs"][1])
     if c.get("sens") is not None:
         cntc.setflags(ire, icp, [1988], [c["sens"]])
     if c.get("mxsens") is not None or c.get("epsens") is not None:

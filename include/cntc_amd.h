@@ -55,7 +55,9 @@ void cntc_setreferencevelocity(int *ire, int *icp, double *veloc);
 void cntc_setrollingstepsize(int *ire, int *icp, double *chi, double *dq);
 /* src/contact_addon.f90:1671 cntc_setFrictionMethod (L=0) */
 void cntc_setfrictionmethod(int *ire, int *icp, int *imeth, int *nparam, double *params);
-/* src/contact_addon.f90:1972-2093 cntc_setHertzContact (ipotcn -1, -3) */
+/* src/contact_addon.f90:1972-2093 cntc_setHertzContact (ipotcn -1, -2, -3; -4, -5 and -6 are
+   accepted with the reference's parameter counts but make cntc_calculate return
+   CNTC_err_other: this library does not implement the prescribed-pressure options) */
 void cntc_sethertzcontact(int *ire, int *icp, int *ipotcn, int *nparam, double *rparam);
 /* src/contact_addon.f90:2097-2257 cntc_setPotContact (ipotcn 1..4) */
 void cntc_setpotcontact(int *ire, int *icp, int *ipotcn, int *nparam, double *rparam);
@@ -70,6 +72,19 @@ void cntc_setundeformeddistc(int *ire, int *icp, int *ibase, int *nparam, double
 void cntc_settangentialforces(int *ire, int *icp, double *fx, double *fy);
 /* src/contact_addon.f90:2491-2530 cntc_setCreepages */
 void cntc_setcreepages(int *ire, int *icp, double *vx, double *vy, double *phi);
+/* src/contact_addon.f90:2534-2593 cntc_setExtraRigidSlip: extra term (wx, wy) of the tangential
+   right-hand side per element of the potential contact (sets the E digit to 1) */
+void cntc_setextrarigidslip(int *ire, int *icp, int *lenarr, double *wx, double *wy);
+/* src/contact_addon.f90:1055-1124 cntc_setMetadata: metadata codes CNTC_mt_*, kept with the
+   problem; they do not change the solution */
+void cntc_setmetadata(int *ire, int *icp, int *lenmta, int *params, double *values);
+/* src/contact_addon.f90:4001-4160 cntc_getFlags: control digits of the problem */
+void cntc_getflags(int *ire, int *icp, int *lenflg, int *params, int *values);
+/* src/contact_addon.f90:4164-4258 cntc_getParameters: itask 1 (veloc, chi, dq, spinxo, spinyo,
+   tau_c0), 2 (material), 3 (friction) */
+void cntc_getparameters(int *ire, int *icp, int *itask, int *lenarr, double *values);
+/* src/contact_addon.f90:4987-5017 cntc_getReferenceVelocity */
+void cntc_getreferencevelocity(int *ire, int *icp, double *veloc);
 
 /* ---- calculation ----------------------------------------------------------------------- */
 
@@ -130,8 +145,12 @@ void cntc_getelementdivision(int *ire, int *icp, int *lenarr, int *eldiv);
 void cntc_getmaximumpressure(int *ire, int *icp, double *pnmax);
 /* src/contact_addon.f90:5591 cntc_getMaximumTraction */
 void cntc_getmaximumtraction(int *ire, int *icp, double *ptmax);
-/* src/contact_addon.f90:5672+ cntc_getFieldData (h, mu, px, py, pn, sx, sy, wx, wy) */
+/* src/contact_addon.f90:5672+ cntc_getFieldData (h, mu, px, py, pn, ux, uy, un, taucrt,
+   uplsx, uplsy, sx, sy, wx, wy; an unknown code leaves fld unchanged, as in the reference) */
 void cntc_getfielddata(int *ire, int *icp, int *ifld, int *lenarr, double *fld);
+/* src/contact_addon.f90:5889-5915 cntc_getDisplacements: displacement differences un, ux, uy of
+   the solution (A ps on the interior, 0 elsewhere) */
+void cntc_getdisplacements(int *ire, int *icp, int *lenarr, double *un, double *ux, double *uy);
 /* src/contact_addon.f90:5827-5855 cntc_getTractions */
 void cntc_gettractions(int *ire, int *icp, int *lenarr, double *pn, double *px, double *py);
 /* src/contact_addon.f90 cntc_getMicroslip */
@@ -150,7 +169,9 @@ void cntc_resetcalculationtime(int *ire, int *icp);
 /* ---- kernel timing (no counterpart in the reference) ------------------------------------ */
 
 /* For benchmark and profiling callers: mode 1 records HIP events around every kernel launch
-   of subsequent calculations (resetting earlier records), mode 0 stops recording. */
+   of subsequent calculations (resetting earlier records), mode 2 only around the SteadyGS,
+   GD line-search probe and FFT product kernels, mode 0 stops recording.  Safe to call while
+   the serving loop runs. */
 void cntc_amdprofile(int *mode);
 /* Per kernel (kinds[k] = operation*4 + stage): seconds, algorithmic bytes and launches
    accumulated since recording started; *nkinds receives the number of kernels recorded. */
@@ -169,21 +190,34 @@ void cntc_finalizelast(void);
 #define CNTC_un_cntc    1934
 #define CNTC_un_spck    1935
 #define CNTC_un_si      1936
+#define CNTC_ic_config  1967
 #define CNTC_ic_pvtime  1970
 #define CNTC_ic_bound   1971
 #define CNTC_ic_tang    1972
 #define CNTC_ic_norm    1973
 #define CNTC_ic_force   1974
+#define CNTC_ic_frclaw  1976
 #define CNTC_ic_discns  1977
 #define CNTC_ic_inflcf  1978
+#define CNTC_ic_mater   1979
+#define CNTC_ic_exrhs   1980
+#define CNTC_ic_xflow   1981
+#define CNTC_ic_heat    1982
 #define CNTC_ic_iestim  1983
 #define CNTC_ic_output  1984
 #define CNTC_ic_flow    1985
+#define CNTC_ic_return  1986
 #define CNTC_ic_matfil  1987
 #define CNTC_ic_sens    1988
+#define CNTC_ic_ifmeth  1989
+#define CNTC_ic_ifvari  1990
+#define CNTC_ic_sbsout  1991
+#define CNTC_ic_sbsfil  1992
+#define CNTC_ic_npomax  1993
 #define CNTC_if_idebug  2000
 #define CNTC_if_wrtinp  2002
 #define CNTC_if_openmp  2003
+#define CNTC_if_ncase   2005
 /* No counterpart in the reference: 0 = FP64 products (default), 1 = FP32 FFT products inside
    the FP64 refinement (residual, projections and forces stay FP64). */
 #define CNTC_if_precision 2009
@@ -193,10 +227,27 @@ void cntc_finalizelast(void);
 #define CNTC_fld_px   3
 #define CNTC_fld_py   4
 #define CNTC_fld_pn   5
+#define CNTC_fld_ux   7
+#define CNTC_fld_uy   8
+#define CNTC_fld_un   9
+#define CNTC_fld_taucrt 11
+#define CNTC_fld_uplsx 12
+#define CNTC_fld_uplsy 13
 #define CNTC_fld_sx   15
 #define CNTC_fld_sy   16
 #define CNTC_fld_wx   22
 #define CNTC_fld_wy   23
+
+/* metadata codes of cntc_setmetadata (src/caddon_flags.inc:155-163) */
+#define CNTC_mt_tim   2011
+#define CNTC_mt_xr    2012
+#define CNTC_mt_yr    2013
+#define CNTC_mt_xw    2014
+#define CNTC_mt_yw    2015
+#define CNTC_mt_sw    2016
+#define CNTC_mt_run   2021
+#define CNTC_mt_axle  2022
+#define CNTC_mt_side  2023
 
 #define CNTC_err_allow  (-12)
 #define CNTC_err_norm   (-27)

@@ -812,6 +812,59 @@ static int hzcalc3d(double e_star, double epshz, int ipotcn, double *a1, double 
     return 0;
 }
 
+/* elliptic_to_curv, src/m_hertz.f90:95-213: the curvature b1 at which the Hertz ellipse of
+   curvature a1 has the semi-axis ratio aob (halving/doubling from b1 = a1, then regula falsi) */
+static int elliptic_to_curv(double e_star, double epshz, int ic_norm, double *a1, double aob,
+                            double *b1, double *aa, double *bb, double *pen, double *fntrue,
+                            double *cp, double *rho)
+{
+    int iter = 0;
+    double epsaob = 10.0 * epshz, b1low, b1high, aoblow, aobhigh;
+    *b1 = *a1;
+    if (hzcalc3d(e_star, epshz, -1, a1, b1, aa, bb, ic_norm, pen, fntrue, cp, rho) != 0) return -1;
+    if (*aa < TINY && *bb < TINY) return 0;
+    if (*aa / *bb > aob) {
+        b1high = *b1;
+        aobhigh = *aa / *bb;
+        while (iter <= 100 && *aa / *bb > aob) {
+            iter++;
+            b1high = *b1;
+            aobhigh = *aa / *bb;
+            *b1 = *b1 * 0.5;
+            if (hzcalc3d(e_star, epshz, -1, a1, b1, aa, bb, ic_norm, pen, fntrue, cp, rho) != 0) return -1;
+        }
+        b1low = *b1;
+        aoblow = *aa / *bb;
+    } else {
+        b1low = *b1;
+        aoblow = *aa / *bb;
+        while (iter <= 100 && *aa / *bb < aob) {
+            iter++;
+            b1low = *b1;
+            aoblow = *aa / *bb;
+            *b1 = *b1 * 2.0;
+            if (hzcalc3d(e_star, epshz, -1, a1, b1, aa, bb, ic_norm, pen, fntrue, cp, rho) != 0) return -1;
+        }
+        b1high = *b1;
+        aobhigh = *aa / *bb;
+    }
+    while (iter <= 100 && fabs(*aa / *bb - aob) > epsaob * aob) {
+        iter++;
+        double daob = aob - aoblow;
+        double rico = (aobhigh - aoblow) / (b1high - b1low);
+        *b1 = b1low + daob / rico;
+        if (hzcalc3d(e_star, epshz, -1, a1, b1, aa, bb, ic_norm, pen, fntrue, cp, rho) != 0) return -1;
+        if (*aa / *bb > aob) {
+            b1high = *b1;
+            aobhigh = *aa / *bb;
+        } else {
+            b1low = *b1;
+            aoblow = *aa / *bb;
+        }
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* defaults                                                                                     */
 /* ------------------------------------------------------------------------------------------ */
@@ -2627,12 +2680,16 @@ int orc_calculate(orc_problem *P, const orc_input *inp)
     double prmudf[6];
     memcpy(prmudf, inp->prmudf, sizeof prmudf);
     int ibase = inp->ibase;
-    if (inp->ipotcn == -1 || inp->ipotcn == -3) {
+    if (inp->ipotcn == -1 || inp->ipotcn == -2 || inp->ipotcn == -3) {
+        /* hzsol, src/m_hertz.f90:29-90 */
         double e_star = P->ga / (1.0 - P->nu);
         P->hz_a1 = inp->hz_a1; P->hz_b1 = inp->hz_b1; P->hz_aa = inp->hz_aa; P->hz_bb = inp->hz_bb;
-        if (hzcalc3d(e_star, sp(1e-9f), inp->ipotcn, &P->hz_a1, &P->hz_b1, &P->hz_aa, &P->hz_bb,
-                     inp->norm, &P->pen, &P->fntrue, &P->hz_cp, &P->hz_rho) != 0)
-            return P->ierror = CNTC_ERR_OTHER;
+        int herr = inp->ipotcn == -2
+                       ? elliptic_to_curv(e_star, sp(1e-9f), inp->norm, &P->hz_a1, inp->hz_aob, &P->hz_b1,
+                                          &P->hz_aa, &P->hz_bb, &P->pen, &P->fntrue, &P->hz_cp, &P->hz_rho)
+                       : hzcalc3d(e_star, sp(1e-9f), inp->ipotcn, &P->hz_a1, &P->hz_b1, &P->hz_aa, &P->hz_bb,
+                                  inp->norm, &P->pen, &P->fntrue, &P->hz_cp, &P->hz_rho);
+        if (herr != 0) return P->ierror = CNTC_ERR_OTHER;
         ibase = 1;
         memset(prmudf, 0, sizeof prmudf);
         prmudf[0] = P->hz_a1;
@@ -2700,13 +2757,16 @@ int orc_calculate(orc_problem *P, const orc_input *inp)
         else
             return P->ierror = CNTC_ERR_OTHER;
     }
-    /* set_tang_rhs, src/m_sdis.f90:498-583 (E=0, no spin offset) */
+    /* set_tang_rhs, src/m_sdis.f90:498-583 (no spin offset; E digit 0-3) */
     {
         double xofs = is_roll ? cos(P->chi) * P->dq * (1.0 / 6.0) : 0.0;
         double yofs = is_roll ? sin(P->chi) * P->dq * (1.0 / 6.0) : 0.0;
+        double facx = inp->rztang == 3 ? 0.0 : 1.0, facy = inp->rztang == 2 ? 0.0 : 1.0;
+        int use_ex = inp->rztang == 1 && inp->exrhs;
         for (int i = 0; i < npot; i++) {
-            P->hs[0][i] = 1.0 * -(P->y[i] + yofs - 0.0) * P->cphi + 0.0;
-            P->hs[1][i] = 1.0 * (P->x[i] + xofs - 0.0) * P->cphi + 0.0;
+            double ex = use_ex ? inp->exrhs[i] : 0.0, ey = use_ex ? inp->exrhs[npot + i] : 0.0;
+            P->hs[0][i] = facx * -(P->y[i] + yofs - 0.0) * P->cphi + ex;
+            P->hs[1][i] = facy * (P->x[i] + xofs - 0.0) * P->cphi + ey;
             if (inp->force3 == 0) P->hs[0][i] += P->cksi;   /* F=0: cksi prescribed */
             if (inp->force3 <= 1) P->hs[1][i] += P->ceta;   /* F<=1: ceta prescribed */
             P->hs[0][i] *= -P->dq;

@@ -38,11 +38,12 @@ typedef struct orc_input {
     /* Coulomb friction, L=0 */
     double fstat;
     /* potential contact: ipotcn=3 form (centre of first element and step sizes), or
-       Hertzian ipotcn=-1 (curvatures a1, b1) / -3 (semi-axes aa, bb) with scale */
+       Hertzian ipotcn=-1 (curvatures a1, b1) / -2 (curvature a1, ellipticity aob) / -3
+       (semi-axes aa, bb) with scale */
     int ipotcn;
     int mx, my;
     double xc1, yc1, dx, dy;
-    double hz_a1, hz_b1, hz_aa, hz_bb, hz_scale;
+    double hz_a1, hz_b1, hz_aa, hz_bb, hz_scale, hz_aob;
     /* undeformed distance: ibase=1 quadratic (prmudf[0..5]) or ibase=9 element-wise */
     int ibase;
     double prmudf[6];
@@ -64,6 +65,11 @@ typedef struct orc_input {
        2 Fx and Fy prescribed as fractions of mu*Fn (cntc_setTangentialForces, :2597-2650) */
     int force3;
     double fxrel, fyrel;
+    /* E digit (src/m_sdis.f90:517-553): 1 adds exrhs (2*mx*my values, x then y, from
+       cntc_setExtraRigidSlip, src/contact_addon.f90:2534-2593); 2 drops the spin term from wy,
+       3 from wx */
+    int rztang;
+    const double *exrhs;
 } orc_input;
 
 /* fills an input record with the reference's module-3 defaults */

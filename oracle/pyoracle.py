@@ -27,6 +27,7 @@ class OrcInput(ctypes.Structure):
         ("dx", ctypes.c_double), ("dy", ctypes.c_double),
         ("hz_a1", ctypes.c_double), ("hz_b1", ctypes.c_double),
         ("hz_aa", ctypes.c_double), ("hz_bb", ctypes.c_double), ("hz_scale", ctypes.c_double),
+        ("hz_aob", ctypes.c_double),
         ("ibase", ctypes.c_int), ("prmudf", ctypes.c_double * 6),
         ("h_elem", ctypes.POINTER(ctypes.c_double)),
         ("pen", ctypes.c_double), ("fn", ctypes.c_double), ("cksi", ctypes.c_double),
@@ -40,6 +41,7 @@ class OrcInput(ctypes.Structure):
         ("omegah", ctypes.c_double), ("omegas", ctypes.c_double),
         ("sens", ctypes.c_int), ("mxsens", ctypes.c_int), ("epsens", ctypes.c_double),
         ("force3", ctypes.c_int), ("fxrel", ctypes.c_double), ("fyrel", ctypes.c_double),
+        ("rztang", ctypes.c_int), ("exrhs", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -270,6 +272,14 @@ def case_input(c):
         inp.ipotcn = -3
         inp.mx, inp.my = c["mx"], c["my"]
         inp.hz_aa, inp.hz_bb, inp.hz_scale = c["aa"], c["bb"], c["scale"]
+    elif c["ipotcn"] == -2:
+        inp.ipotcn = -2
+        inp.mx, inp.my = c["mx"], c["my"]
+        inp.hz_a1, inp.hz_aob, inp.hz_scale = c["a1"], c["aob"], c["scale"]
+    elif c["ipotcn"] == -1:
+        inp.ipotcn = -1
+        inp.mx, inp.my = c["mx"], c["my"]
+        inp.hz_a1, inp.hz_b1, inp.hz_scale = c["a1"], c["b1"], c["scale"]
     else:
         inp.ipotcn = 3
         inp.mx, inp.my = c["mx"], c["my"]
@@ -292,6 +302,15 @@ def case_input(c):
     if fabs > 1.0:      # cntc_setTangentialForces normalises |(fx, fy)| to 1
         fx, fy = fx / fabs, fy / fabs
     inp.fxrel, inp.fyrel = fx, fy
+    if c.get("rztang") is not None:
+        inp.rztang = c["rztang"]
+    if c.get("exrhs") is not None:
+        # extra rigid slip, x then y (cntc_setExtraRigidSlip); kept alive with the record
+        inp.rztang = 1
+        ex = np.ascontiguousarray(np.concatenate([np.ravel(c["exrhs"][0]), np.ravel(c["exrhs"][1])]),
+                                  dtype=np.float64)
+        inp._exrhs_keep = ex
+        inp.exrhs = _dptr(ex)
     inp.sens = c.get("sens", 1)
     inp.mxsens = c.get("mxsens", 30)
     inp.epsens = c.get("epsens", 1e-4)

@@ -300,3 +300,29 @@ def test_force_prescribed_rolling_reaches_target(oracle):
         inp, h = C.to_oracle(oracle, g)
         assert Q.calculate(inp, h) == 0
         assert np.allclose(Q.forces()[1:3], (fx, fy), rtol=0, atol=1e-6 * fn)
+
+
+# ---- Hertz option -2: curvature a1 and ellipticity aob (src/m_hertz.f90:95-213) -------------
+
+@pytest.mark.parametrize("aob", [0.3, 1.0, 2.7])
+def test_hertz_ellipticity_option(oracle, aob):
+    """elliptic_to_curv finds the curvature b1 whose Hertz ellipse has aa/bb = aob.  Pinned by
+    the other two Hertz options: ipotcn -1 with the b1 found gives the same ellipse, and
+    ipotcn -3 with that ellipse's semi-axes gives back the curvatures a1 and b1."""
+    from contact_amd import workloads as W
+    a1 = 1.0 / 920.0
+    base = dict(mx=10, my=10, scale=1.3, fn=1e5, tang=0)
+    rc, _, P2 = oracle.solve_case(W.base_case(ipotcn=-2, a1=a1, aob=aob, **base))
+    assert rc == 0
+    h2 = P2.hertz()
+    b1, aa, bb = h2[1], h2[2], h2[3]
+    assert h2[0] == pytest.approx(a1, rel=1e-15)
+    assert aa / bb == pytest.approx(aob, rel=1e-7)
+    rc, _, P1 = oracle.solve_case(W.base_case(ipotcn=-1, a1=a1, b1=b1, **base))
+    assert rc == 0
+    h1 = P1.hertz()
+    assert h1[2] == pytest.approx(aa, rel=1e-14) and h1[3] == pytest.approx(bb, rel=1e-14)
+    rc, _, P3 = oracle.solve_case(W.base_case(ipotcn=-3, aa=aa, bb=bb, **base))
+    assert rc == 0
+    h3 = P3.hertz()
+    assert h3[0] == pytest.approx(a1, rel=1e-7) and h3[1] == pytest.approx(b1, rel=1e-7)
