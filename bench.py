@@ -45,7 +45,7 @@ sys.path.insert(0, ROOT)
 if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
     os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
-IRE_MIXED, IRE_FP64 = 1, 301
+IRE_MIXED, IRE_FP64 = 1, 460       # two mixed-precision sets of 228 result elements, then FP64
 
 
 def ids(k, base):
@@ -209,8 +209,10 @@ def serve_leg(cntc, W, seconds, rate, nchan=64, seed=77, ire0=801, sens_frac=0.1
     err = cntc.calculatebatch([i for i, _ in ids], [j for _, j in ids])
     if np.any(err != 0):
         raise SystemExit("serving warm-up failed: %s" % sorted(set(err)))
-    for c in state:
+    # later steps continue from the previous one (P=0) and start from its tractions (I=3)
+    for (ire, icp), c in zip(ids, state):
         c.update(pvtime=0, iestim=3)
+        cntc.setflags(ire, icp, [1970, 1983], [0, 3])
     cntc.servestart(200e9)
     n_req = max(1, int(seconds * rate))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
@@ -278,9 +280,8 @@ def forces_of(cntc, idl):
     return np.array([cntc.getcontactforces(i, j) for i, j in idl])
 
 
-def solve_batch(cntc, idl, label):
-    """Submit every problem of `idl` to the serving loop at once and wait for all of them,
-    reporting progress as patches complete.  Returns (error codes, latencies in s)."""
+def submit_batch(cntc, idl, label):
+    """Submit every problem of `idl` to the serving loop; returns the open batch."""
     t0 = time.perf_counter()
     tick = {}
     for k, (i, j) in enumerate(idl):
@@ -288,9 +289,16 @@ def solve_batch(cntc, idl, label):
         if t <= 0:
             raise SystemExit("%s: submit failed (%d)" % (label, t))
         tick[t] = k
-    err = np.zeros(len(idl), dtype=np.int64)
-    lat = np.zeros(len(idl))
-    last = t0
+    return dict(label=label, t0=t0, tick=tick, n=len(idl))
+
+
+def finish_batch(cntc, b):
+    """Wait for every patch of an open batch, reporting progress as patches complete.
+    Returns (error codes, latencies in s, seconds from submit to the last result)."""
+    tick, n = b["tick"], b["n"]
+    err = np.zeros(n, dtype=np.int64)
+    lat = np.zeros(n)
+    last = time.perf_counter()
     while tick:
         for t in list(tick):
             d, rc, l = cntc.poll(t)
@@ -300,10 +308,12 @@ def solve_batch(cntc, idl, label):
         now = time.perf_counter()
         if now - last >= 15.0 and tick:
             last = now
-            print("# %s: %d/%d patches done at %.0f s" % (label, len(idl) - len(tick), len(idl), now - t0),
+            print("# %s: %d/%d patches done at %.0f s" % (b["label"], n - len(tick), n, now - b["t0"]),
                   file=sys.stderr, flush=True)
         time.sleep(0.002)
-    return err, lat
+    if np.any(err != 0):
+        raise SystemExit("%s: %d patches failed (codes %s)" % (b["label"], np.sum(err != 0), sorted(set(err))))
+    return err, lat, time.perf_counter() - b["t0"]
 
 
 def rel_err(a, b):
@@ -377,59 +387,78 @@ def main():
 
     from contact_amd import cntc
     cntc.lib()
-    mixed = [ids(k, IRE_MIXED) for k in range(npatch)]
+    # two sets of problems, so that a batch can be submitted while the previous one finishes
+    sets = [[ids(k, IRE_MIXED + 229 * h) for k in range(npatch)] for h in range(2)]
+    mixed = sets[0]
     fp64 = [ids(k, IRE_FP64) for k in range(npatch)]
-    for k in range(npatch):
-        W.apply_case(cntc, *mixed[k], cases[k])
+    for h in range(2):
+        for k in range(npatch):
+            W.apply_case(cntc, *sets[h][k], cases[k])
 
-    # all solves go through the continuous-batching loop: a batch is every patch submitted at once
+    # all solves go through the continuous-batching loop: a step is one batch of every patch,
+    # submitted at once; the next batch is submitted as soon as this one is (at most two in
+    # flight), so the loop stays full while a batch's slowest patches finish
     cntc.servestart(200e9)
 
-    def step(label):
-        for k in range(npatch):
-            new_case(cntc, *mixed[k], cases[k])
-        e, l = solve_batch(cntc, mixed, label)
-        if np.any(e != 0):
-            raise SystemExit("mixed batch: %d patches failed (codes %s)" % (np.sum(e != 0), sorted(set(e))))
-        return l
+    def submit_step(s, label):
+        idl = sets[s % 2]
+        for k in range(npatch):       # waits for this set's previous batch, two steps back
+            new_case(cntc, *idl[k], cases[k])
+        return submit_batch(cntc, idl, label)
 
     # time kept back for the legs after the timed steps (plain FP64 subset, serving, report)
     reserve = 45.0 if args.plain > 0 or args.serve_seconds > 0 else 5.0
+
+    def run_steps(nsteps, label, can_start):
+        """Up to `nsteps` batches, two in flight; `can_start(done_latencies)` decides whether
+        another batch may be submitted.  Returns (steps run, per-batch latencies, last
+        batch's per-patch latencies)."""
+        open_, done, last_lat = [], [], np.array([np.nan])
+        for s in range(nsteps):
+            if not can_start(done, s):
+                break
+            if len(open_) == 2:
+                _, last_lat, tb = finish_batch(cntc, open_.pop(0))
+                done.append(tb)
+            open_.append(submit_step(s, "%s %d" % (label, s)))
+        while open_:
+            _, last_lat, tb = finish_batch(cntc, open_.pop(0))
+            done.append(tb)
+            print("# %s: batch done (%.1f s from submit)" % (label, tb), file=sys.stderr, flush=True)
+        return len(done), done, last_lat
+
     t0 = time.perf_counter()
-    warm_done, t_warm = 0, []
-    for w in range(args.warmup):
-        # at least one warm-up step (it builds the coefficient spectra); the rest only while
-        # they leave room for at least three timed steps of the same length
-        if w > 0 and left() - reserve < 4.0 * max(t_warm) * 1.1:
-            break
-        t1 = time.perf_counter()
-        step("warmup %d" % w)
-        t_warm.append(time.perf_counter() - t1)
-        warm_done += 1
+    # warm-up: both sets once (their coefficient spectra are built on first use); more only
+    # while everything asked for still fits the budget
+    def warm_ok(done, s):
+        if s < 2:
+            return True
+        per = (time.perf_counter() - t0) / max(1, s)
+        return left() - reserve >= (args.steps + args.warmup - s) * per * 1.1
+    warm_done, t_warm, _ = run_steps(max(2, args.warmup), "warmup", warm_ok)
     t_first = time.perf_counter() - t0
 
     cntc.amdprofile(2)
     device_sync()
     t0 = time.perf_counter()
-    steps_done, t_step = 0, []
-    lat = np.array([np.nan])
-    for s in range(args.steps):
-        # whole steps only: stop when the next one (at the longest step time so far) would not
-        # finish inside the budget
-        est = max(t_step) if t_step else (max(t_warm) if t_warm else 0.0)
-        if s > 0 and left() - reserve < est * 1.1:
+
+    def timed_ok(done, s):
+        # whole batches only: submit another while it can finish inside the budget (a batch
+        # takes at most the longest batch latency seen, counted from now)
+        if s == 0:
+            return True
+        est = max(done + t_warm) if (done or t_warm) else 0.0
+        if left() - reserve < est * 1.1:
             print("# stopping after %d of %d timed steps: the wall-clock budget (%.0f s) is spent"
                   % (s, args.steps, args.budget), file=sys.stderr, flush=True)
-            break
-        t1 = time.perf_counter()
-        lat = step("step %d" % s)
-        t_step.append(time.perf_counter() - t1)
-        steps_done += 1
-        print("# step %d done at %.2f s" % (s, time.perf_counter() - t0), file=sys.stderr, flush=True)
+            return False
+        return True
+    steps_done, t_step, lat = run_steps(args.steps, "step", timed_ok)
     device_sync()
     elapsed = time.perf_counter() - t0
     prof = cntc.amdprofileget()
     cntc.amdprofile(False)
+    mixed = sets[(steps_done - 1) % 2]
 
     FM = forces_of(cntc, mixed)
     its = np.array([cntc.getiterations(i, j) for i, j in mixed])
@@ -516,7 +545,10 @@ def main():
         "warmup_requested": args.warmup,
         "budget_s": args.budget,
         "ms_per_step": round(elapsed / max(1, steps_done) * 1e3, 2),
-        "ms_per_step_min_max": [round(float(np.min(t_step_a)) * 1e3, 1), round(float(np.max(t_step_a)) * 1e3, 1)],
+        "batch_latency_ms": {"mean": round(float(np.mean(t_step_a)) * 1e3, 1),
+                             "min": round(float(np.min(t_step_a)) * 1e3, 1),
+                             "max": round(float(np.max(t_step_a)) * 1e3, 1),
+                             "what": "submit of a batch to its last result; batches overlap by up to one"},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -535,13 +567,13 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
                        "what": "batch submit to patch result, last timed step"},
-        "steadygs_fallbacks": int(np.sum(flags >= 2)),
+        "steadygs_fallbacks": int(np.sum((flags & 2) != 0)),
         "element_iterations_per_s": round(elem_iters * steps_done / elapsed, 0),
         "plain_fp64": {"patches_per_s": round(plain_rate, 3), "patches": len(sub),
                        "latency_ms_p50": round(float(np.percentile(t_plain, 50)) * 1e3, 2),
                        "latency_ms_p99": round(float(np.percentile(t_plain, 99)) * 1e3, 2)},
         "speedup_vs_plain_fp64": round(value / plain_rate, 2),
-        "first_call_patches_per_s": round(npatch * max(1, warm_done) / t_first, 2) if warm_done else None,
+        "warmup_patches_per_s": round(npatch * max(1, warm_done) / t_first, 2) if warm_done else None,
         "device_time_per_wall": round(kernel_sec / elapsed, 3),
         "device_time_what": "HIP-event time of the recorded kernels (SteadyGS, GD probe, FFT products) / wall",
         "kernels": kernels,

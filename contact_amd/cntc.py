@@ -400,7 +400,7 @@ OP_NAMES = ["vec", "ncg_negpn", "ncg_dd", "ncg_expand", "ncg_hmin", "sn_contract
             "sn_final", "aij_ones", "tcg_nt", "tcg_rset", "tcg_zprep", "tcg_qpost", "tcg_pupd",
             "tcg_as", "tcg_sa", "st_as", "st_sa", "gd_dp", "gd_project", "gd_probe", "gd_update",
             "gd_moves", "gd_diagres", "conv", "conv32", "elascf", "embed", "extract", "setup",
-            "forces", "stdygs", "guard"]
+            "forces", "stdygs", "guard", "sens", "sens_ext", "ledge"]
 
 
 def amdprofile(on):

@@ -139,6 +139,9 @@ void cntc_initialize(int* ire, int* imodul, int* ifcver, int* ierror, const char
         return;
     }
     store().ire_module[*ire] = 3;
+    // set-up: the engine (stream, staging buffers) is made here when a GPU is present, so
+    // that later calls need not allocate it
+    (void)engine();
     if (ierror) *ierror = 0;
 }
 
@@ -178,7 +181,7 @@ void cntc_setflags(int* ire, int* icp, int* lenflg, int* params, int* values)
         case CNTC_ic_force: in.force3 = std::max(0, std::min(2, v)); break;  // :834-842, module 3
         case CNTC_ic_exrhs: in.rztang = std::max(0, std::min(3, v)); break;   // :883-885
         case CNTC_ic_config: P->flags[params[i]] = std::max(0, std::min(5, v)); break;
-        case CNTC_ic_bound: P->flags[params[i]] = std::max(0, std::min(6, v)); break;
+        case CNTC_ic_bound: in.bound = std::max(0, std::min(6, v)); break;
         case CNTC_ic_discns: if (v >= 0 && v <= 9) P->flags[params[i]] = v; break;
         case CNTC_ic_inflcf: if (v >= 0 && v <= 4) P->flags[params[i]] = v; break;
         case CNTC_ic_matfil: case CNTC_ic_sbsfil: P->flags[params[i]] = (v < 0 || v > 2) ? 0 : v; break;
@@ -287,10 +290,10 @@ void cntc_setfrictionmethod(int* ire, int* icp, int* imeth, int* nparam, double*
 }
 
 // src/contact_addon.f90:1972-2093.  The method is checked before the parameter count, as the
-// reference does (-6 takes 6 parameters, the others 5).  The rectangular (-4, -5) and SDEC (-6)
-// options prescribe the normal pressure instead of solving for it; this library does not
-// implement them, so the problem is marked and cntc_calculate returns CNTC_err_other for it
-// instead of solving the previous geometry.
+// reference does (-6 takes 6 parameters, the others 5).  The rectangular options (-4, -5) set
+// the B digit to 2 (prescribed pressure).  The SDEC option (-6) is not implemented: the problem
+// is marked and cntc_calculate returns CNTC_err_other for it instead of solving the previous
+// geometry.
 void cntc_sethertzcontact(int* ire, int* icp, int* ipotcn, int* nparam, double* params)
 {
     std::lock_guard<std::mutex> g(store().mtx);
@@ -331,14 +334,15 @@ void cntc_sethertzcontact(int* ire, int* icp, int* ipotcn, int* nparam, double* 
             in.hz_bneg = std::max(1e-6, params[3]) * L;
             in.hz_bpos = std::max(1e-6, params[4]) * L;
             in.hz_scale = std::max(1e-6, params[5]);
+            in.unsupported = "Hertzian option IPOTCN=-6 (SDEC) is not supported";
         } else {
+            // rectangular (line) contact: semi-elliptical pressure in x, prescribed (B=2)
             if (*ipotcn == -5) in.hz_aa = std::max(1e-6, params[2]) * L;
             else in.hz_a1 = std::max(1e-6, params[2]) / L;
             in.hz_bb = std::max(1e-6, params[3]) * L;
             in.hz_scale = std::max(1e-6, params[4]);
+            in.bound = 2;
         }
-        in.unsupported = *ipotcn == -6 ? "Hertzian option IPOTCN=-6 (SDEC) is not supported"
-                                       : "Hertzian rectangular contact (IPOTCN=-4, -5) is not supported";
     }
     in.ipotcn = *ipotcn;
 }
@@ -775,7 +779,7 @@ void cntc_getflags(int* ire, int* icp, int* lenflg, int* params, int* values)
         case 0: break;
         case CNTC_ic_config: values[i] = stored(CNTC_ic_config, 1); break;
         case CNTC_ic_pvtime: values[i] = in.pvtime; break;
-        case CNTC_ic_bound: values[i] = stored(CNTC_ic_bound, 0); break;
+        case CNTC_ic_bound: values[i] = in.bound; break;
         case CNTC_ic_tang: values[i] = in.tang; break;
         case CNTC_ic_norm: values[i] = in.norm; break;
         case CNTC_ic_force: values[i] = in.force3; break;
@@ -879,7 +883,7 @@ void cntc_getiterations(int* ire, int* icp, int* lenarr, int* values)
     Problem* P = solved(*ire, *icp);
     if (!P) return;
     const auto& o = P->patch.out;
-    int v[5] = {o.itnorm, o.itcg, o.ittang, o.itgs, o.gd_stagnated + 2 * o.used_stdygs};
+    int v[5] = {o.itnorm, o.itcg, o.ittang, o.itgs, o.gd_stagnated + 2 * o.used_stdygs + 4 * o.prio};
     for (int k = 0; k < std::min(*lenarr, 5); k++) values[k] = v[k];
 }
 

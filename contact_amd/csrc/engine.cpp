@@ -151,6 +151,7 @@ union AnyItem {
     SetupItem setup;
     GsItem gs;
     SensItem sens;
+    LedgeItem ledge;
     AnyItem() { std::memset(this, 0, sizeof(*this)); }
 };
 
@@ -173,6 +174,7 @@ static size_t item_size(int kind)
     case OP_SETUP: case OP_FORCES: return sizeof(SetupItem);
     case OP_STDYGS: case OP_GUARD: return sizeof(GsItem);
     case OP_SENS: case OP_SENS_EXT: return sizeof(SensItem);
+    case OP_LEDGE: return sizeof(LedgeItem);
     }
     return 0;
 }
@@ -208,6 +210,8 @@ public:
         HIPCHK(hipHostMalloc((void**)&hres_, rescap_ * sizeof(double), hipHostMallocDefault));
         HIPCHK(hipMalloc((void**)&dres_, rescap_ * sizeof(double)));
         HIPCHK(hipMalloc((void**)&dpart_, partcap_ * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&dcnt_, cntcap_ * sizeof(unsigned)));
+        HIPCHK(hipMemset(dcnt_, 0, cntcap_ * sizeof(unsigned)));
     }
     ~Scheduler()
     {
@@ -220,6 +224,7 @@ public:
         (void)hipHostFree(hres_);
         (void)hipFree(dres_);
         (void)hipFree(dpart_);
+        (void)hipFree(dcnt_);
     }
     void resize(int nslots)
     {
@@ -523,8 +528,9 @@ private:
         op.variant = variant;
         op.timed = timed_kind(kind);
         if (op.timed) HIPCHK(hipEventRecord(op.t0, op.stream));
-        launch_op(kind, op.dev, (const int*)(op.dev + items_b), nitems, tot, (double*)(op.dev + op.res_off),
-                  variant, lds, op.stream);
+        // one work-group per item: its partials are the item's results
+        launch_op(kind, op.dev, (const int*)(op.dev + items_b), nitems, tot,
+                  RedOut{(double*)(op.dev + op.res_off), nullptr, nullptr, nullptr}, variant, lds, op.stream);
         HIPCHK(hipGetLastError());
         if (op.timed) HIPCHK(hipEventRecord(op.t1, op.stream));
         HIPCHK(hipMemcpyAsync(op.host + op.res_off, op.dev + op.res_off, res_b, hipMemcpyDeviceToHost,
@@ -572,13 +578,13 @@ private:
         // stage items, group tables and result slots
         struct Launch {
             int kind, nitems, K, stage;
-            size_t items_off, gstart_off, modes_off, part_off, res_off;
+            size_t items_off, gstart_off, modes_off, part_off, res_off, cnt_off;
             int ngroups;
             size_t lds;
         };
         std::vector<Launch> launches;
         size_t off = 0;
-        size_t part = 0, res = 0;
+        size_t part = 0, res = 0, cnt = 0;
         auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
         std::vector<std::pair<Req*, size_t>> resmap;
         for (auto& lev : levels)
@@ -606,7 +612,10 @@ private:
                     size_t lds = 0;
                     for (int i = 0; i < nitems; i++) {
                         gs[i] = tot;
-                        tot += op_groups(kind, &flat[g.reqs[i]]->item, st);
+                        const int ng = op_groups(kind, &flat[g.reqs[i]]->item, st);
+                        // every item of a reduction must have a group to form its result
+                        if (K > 0 && ng <= 0) throw std::runtime_error("scheduler: reduction item without work-groups");
+                        tot += ng;
                         lds = std::max(lds, op_lds(kind, &flat[g.reqs[i]]->item, st));
                     }
                     gs[nitems] = tot;
@@ -619,7 +628,9 @@ private:
                     off = align(off + sizeof(int) * nitems);
                     L.part_off = part;
                     L.res_off = res;
+                    L.cnt_off = cnt;
                     if (K > 0) {
+                        cnt += nitems;
                         part += (size_t)tot * K;
                         for (int i = 0; i < nitems; i++)
                             resmap.push_back({flat[g.reqs[i]], res + (size_t)i * K});
@@ -634,6 +645,13 @@ private:
             HIPCHK(hipFree(dpart_));
             partcap_ = part * 2;
             HIPCHK(hipMalloc((void**)&dpart_, partcap_ * sizeof(double)));
+        }
+        if (cnt > cntcap_) {
+            HIPCHK(hipStreamSynchronize(s_));
+            HIPCHK(hipFree(dcnt_));
+            cntcap_ = cnt * 2;
+            HIPCHK(hipMalloc((void**)&dcnt_, cntcap_ * sizeof(unsigned)));
+            HIPCHK(hipMemset(dcnt_, 0, cntcap_ * sizeof(unsigned)));
         }
         if (res > rescap_) {
             HIPCHK(hipStreamSynchronize(s_));
@@ -664,8 +682,10 @@ private:
                 pkeys.push_back({L.kind * 4 + L.stage, bytes});
                 HIPCHK(hipEventRecord(ev_pool[nev++], s_));
             }
+            RedOut ro{dpart_ + L.part_off, L.K > 0 ? dcnt_ + L.cnt_off : nullptr, dres_ + L.res_off,
+                      (const int*)(dstage_ + L.modes_off)};
             launch_op(L.kind, dstage_ + L.items_off, (const int*)(dstage_ + L.gstart_off), L.nitems,
-                      L.ngroups, dpart_ + L.part_off, L.stage, L.lds, s_);
+                      L.ngroups, ro, L.stage, L.lds, s_);
             if (hipError_t le = hipGetLastError(); le != hipSuccess) {
                 char msg[256];
                 std::snprintf(msg, sizeof msg, "launch of operation %d stage %d failed (%s): %d groups, %zu B LDS",
@@ -673,9 +693,6 @@ private:
                 throw std::runtime_error(msg);
             }
             if (timed) HIPCHK(hipEventRecord(ev_pool[nev++], s_));
-            if (L.K > 0)
-                launch_finalize((const int*)(dstage_ + L.gstart_off), (const int*)(dstage_ + L.modes_off),
-                                L.nitems, L.K, dpart_ + L.part_off, dres_ + L.res_off, s_);
         }
         HIPCHK(hipGetLastError());
         for (Req* r : async_reqs) {
@@ -731,6 +748,9 @@ private:
     size_t rescap_ = 1 << 20;
     double* dpart_ = nullptr;
     size_t partcap_ = 4 << 20;
+    // one arrival counter per reduction item of a flush; each item's last group resets its own
+    unsigned* dcnt_ = nullptr;
+    size_t cntcap_ = 1 << 16;
 
 public:
     long nflush = 0;
@@ -1092,7 +1112,7 @@ size_t patch_bytes(int mx, int my)
     size_t pH = (size_t)mx + 1, pL = 2 * (size_t)my;
     size_t ws = std::max((size_t)3 * Ly * H, pH * pL) * 16;
     size_t emb = std::max((size_t)Lx * Ly, nb) * 8;
-    return al(n) + al(40 * n * 8) + al(9 * nb * 8) * 2 + al(3 * nb * 8) + al(9 * bsz * 16) * 2 +
+    return al(n) + al(42 * n * 8) + al(9 * nb * 8) * 2 + al(3 * nb * 8) + al(9 * bsz * 16) * 2 +
            al(3 * bsz * 16) + al(9 * bsz * 8) + al(3 * bsz * 8) + al(3 * emb) + 2 * al(ws) + 4096;
 }
 
@@ -1106,7 +1126,24 @@ bool Engine::prepare(Patch& P)
         last_error = in.unsupported;
         return false;
     }
-    bool hertzian = in.ipotcn == -1 || in.ipotcn == -2 || in.ipotcn == -3;
+    bool hertzian = in.ipotcn <= -1 && in.ipotcn >= -5;
+    const bool hertz2d = in.ipotcn == -4 || in.ipotcn == -5;
+    if (in.bound != 0 && in.bound != 2) {
+        R.ierror = ERR_OTHER;
+        last_error = "the B digit supports 0 and 2";
+        return false;
+    }
+    if (in.bound == 2 && !hertzian) {
+        // src/m_sinput.f90:176-181: B=2 needs a Hertzian potential contact
+        R.ierror = ERR_OTHER;
+        last_error = "B=2 needs a Hertzian potential contact";
+        return false;
+    }
+    if (hertz2d && in.norm != 1) {
+        R.ierror = ERR_OTHER;
+        last_error = "2D Hertz needs N=1";
+        return false;
+    }
     if (in.tang < 0 || in.tang > 3 || in.ipotcn == 0 || (!hertzian && in.ibase == 0)) {
         R.ierror = ERR_OTHER;
         last_error = "potential contact or undeformed distance not specified";
@@ -1132,19 +1169,46 @@ bool Engine::prepare(Patch& P)
         // hzsol, src/m_hertz.f90:29-90
         double e_star = P.ga / (1.0 - P.nu);
         R.hz_a1 = in.hz_a1; R.hz_b1 = in.hz_b1; R.hz_aa = in.hz_aa; R.hz_bb = in.hz_bb;
-        bool ok = in.ipotcn == -2
-                      ? elliptic_to_curv(e_star, in.norm, in.hz_a1, in.hz_aob, R.hz_b1, R.hz_aa, R.hz_bb,
-                                         P.pen, P.fntrue, R.hz_cp, R.hz_rho)
-                      : hertz3d(e_star, in.ipotcn, in.norm, R.hz_a1, R.hz_b1, R.hz_aa, R.hz_bb, P.pen,
-                                P.fntrue, R.hz_cp, R.hz_rho);
+        bool ok = true;
+        if (hertz2d) {
+            // hzcalc2d, src/m_hertz.f90:217-263: line contact of half-width bb
+            const double fn_2d = P.fntrue / (2.0 * R.hz_bb);
+            R.hz_b1 = 0.0;
+            if (in.ipotcn == -4) {
+                const double rx = 1.0 / (2.0 * R.hz_a1);
+                R.hz_aa = std::sqrt(4.0 * rx * fn_2d / (PI * e_star));
+            } else {
+                const double rx = PI * R.hz_aa * R.hz_aa * e_star / (4.0 * fn_2d);
+                R.hz_a1 = 1.0 / (2.0 * rx);
+            }
+            R.hz_rho = 2.0 / (R.hz_a1 + R.hz_b1);
+            R.hz_cp = std::sqrt(4.0 * R.hz_aa * R.hz_bb / PI);
+        } else if (in.ipotcn == -2) {
+            ok = elliptic_to_curv(e_star, in.norm, in.hz_a1, in.hz_aob, R.hz_b1, R.hz_aa, R.hz_bb, P.pen,
+                                  P.fntrue, R.hz_cp, R.hz_rho);
+        } else {
+            ok = hertz3d(e_star, in.ipotcn, in.norm, R.hz_a1, R.hz_b1, R.hz_aa, R.hz_bb, P.pen, P.fntrue,
+                         R.hz_cp, R.hz_rho);
+        }
         if (!ok) { R.ierror = ERR_OTHER; last_error = "Hertz solution failed"; return false; }
         ibase = 1;
         std::memset(prm, 0, sizeof prm);
         prm[0] = R.hz_a1;
         prm[2] = R.hz_b1;
+        // potcon_hertz, src/m_hierarch_data.f90:1889-1959
         double xl = -in.hz_scale * std::max(1e-9, R.hz_aa), yl = -in.hz_scale * std::max(1e-9, R.hz_bb);
         dx = (-xl - xl) / mx;
         dy = (-yl - yl) / my;
+        if (hertz2d) {
+            // n_int interior rows of the strip: dy such that n_int dy = 2 bb
+            int n_int = my;
+            if (in.hz_scale > 1.0) {
+                n_int = std::max(1, (int)std::lround(my / in.hz_scale));
+                if ((my - n_int) % 2 == 1) n_int = n_int + 1;
+            }
+            dy = 2.0 * std::max(1e-9, R.hz_bb) / n_int;
+            yl = -my * dy / 2.0;
+        }
         xc1 = xl + 0.5 * dx;
         yc1 = yl + 0.5 * dy;
     } else if (in.ipotcn == 1) {
@@ -1181,7 +1245,10 @@ bool Engine::prepare(Patch& P)
     }
     if (!is_roll) { P.chi = 0.0; P.dq = 1.0; }
     if (std::fabs(P.chi) > 0.01) { R.ierror = ERR_OTHER; last_error = "rolling direction chi=0 only"; return false; }
-    if (in.tang == 2 && std::fabs(P.dq - dx) > 1e-9 * dx) { R.ierror = ERR_OTHER; last_error = "transient rolling needs dq = dx"; return false; }
+    // transient rolling with dq > dx: elements entering the contact area near the leading edge
+    // spend part of the step in it (sxbnd/subnd, src/m_leadedge.f90); with dq <= dx every
+    // interior element has facdt = 1 and no leading-edge position is used
+    P.ledge = in.tang == 2 && P.dq > dx * (1.0 + 1e-12);
     // set_norm_rhs, src/m_sdis.f90:321-494: the quadratic gap is formed on the GPU; the host
     // keeps a copy only where it is needed (element-wise input, or eldiv0's initial estimate)
     int ihertz = in.ipotcn < 0 ? -in.ipotcn : 0;
@@ -1228,12 +1295,13 @@ bool Engine::prepare(Patch& P)
     char* b = P.mem.base;
     auto take = [&](size_t bytes) { char* r = b; b += al(bytes); return r; };
     P.el = (int8_t*)take(n);
-    double* d = (double*)take(40 * n * 8);
+    double* d = (double*)take(42 * n * 8);
     P.pn = d; P.px = d + n; P.py = d + 2 * n; P.sx = d + 3 * n; P.sy = d + 4 * n;
     P.pvx = d + 5 * n; P.pvy = d + 6 * n; P.pvn = d + 7 * n;
     P.h = d + 8 * n; P.hx = d + 9 * n; P.hy = d + 10 * n; P.wsx = d + 11 * n; P.wsy = d + 12 * n;
     for (int k = 0; k < 24; k++) P.w[k] = d + (13 + k) * n;
     P.un = d + 37 * n; P.ux = d + 38 * n; P.uy = d + 39 * n;
+    P.facdt = d + 40 * n; P.lix = d + 41 * n;
     P.cf = (double*)take(9 * nb * 8);
     P.cfv = (double*)take(9 * nb * 8);
     P.cfm = (double*)take(3 * nb * 8);
@@ -1300,8 +1368,11 @@ bool Engine::prepare(Patch& P)
         for (size_t i = 0; i < n; i++) P.el_host[i] = (P.h_host[i] - hsmin < facpen * pentru) ? ADHES : EXTER;
         HIPCHK(hipMemcpyAsync(P.el, P.el_host.data(), n, hipMemcpyHostToDevice, stream_));
     }
-    P.hertz_init = in.iestim == 0 && ihertz >= 1;
-    P.hz_pnmax = P.hertz_init ? 3.0 * P.fntrue / (2.0 * PI * R.hz_aa * R.hz_bb) : 0.0;
+    // Hertzian pressure: prescribed (B=2) or the initial estimate of I=0 (src/m_sdis.f90:641-670)
+    P.hertz_init = ihertz >= 1 && (in.bound == 2 || (in.iestim == 0 && in.bound == 0));
+    P.hz_pnmax = !P.hertz_init ? 0.0
+                 : ihertz <= 3 ? 3.0 * P.fntrue / (2.0 * PI * R.hz_aa * R.hz_bb)
+                               : P.fntrue / (PI * R.hz_aa * R.hz_bb);
     P.xofs = is_roll ? std::cos(P.chi) * P.dq * (1.0 / 6.0) : 0.0;
     P.yofs = is_roll ? std::sin(P.chi) * P.dq * (1.0 / 6.0) : 0.0;
     return true;
@@ -2258,6 +2329,7 @@ static void add_rigid_slip(Ctx& C, double* wbx, double* wby, double* wsx, double
     q.n = P.n; q.mx = P.mx; q.mode = 1; q.which = which;
     q.el = P.el; q.wbx = wbx; q.wby = wby; q.wsx = wsx; q.wsy = wsy;
     q.xc1 = P.xc1; q.yc1 = P.yc1; q.dx = P.dx; q.dy = P.dy; q.dq = P.dq; q.d = d;
+    q.facdt = P.ledge ? P.facdt : nullptr;
     C.op(OP_SENS, wi);
 }
 
@@ -2543,12 +2615,31 @@ static Task stang(Ctx C, int* ittang_out)
     }
     // stang_rhs: ws = -facdt h_t (- A'_tt(cv) pv_t for T=2) on the interior; facdt = 1 for
     // interior elements when dq = dx (src/m_leadedge.f90:290-318)
-    C.set(M_ALL, P.wsx, 0.0, P.wsy);
-    C.axpy(M_INT, -1.0, P.hx, P.wsx, P.hy, P.wsy);
-    if (P.in.tang == 2) {
-        double *ux = P.w[20], *uy = P.w[21];
+    if (P.ledge) {
+        // transient rolling with dq > dx: facdt and the leading-edge positions from the element
+        // division (sxbnd), then (1a) in the interior and (1b) at the leading edge, where the
+        // previous-time displacement is replaced by ubnd, interpolated from A pv (cs)
+        AnyItem li;
+        LedgeItem& L = li.ledge;
+        L.n = P.n; L.mx = P.mx; L.my = P.my; L.mode = 0;
+        L.el = P.el; L.facdt = P.facdt; L.lix = P.lix;
+        L.xc1 = P.xc1; L.dx = P.dx; L.dq = P.dq; L.fxdfac = 1.0;   // TangCG: edge 1 dx beyond ix0
+        C.op(OP_LEDGE, li);
+        double *ux = P.w[20], *uy = P.w[21], *bx = P.w[18], *by = P.w[19];
         C.conv_t(1, M_INT, ux, uy, P.pvx, P.pvy);
-        C.axpy(M_INT, -1.0, ux, P.wsx, uy, P.wsy);
+        C.conv_t(0, M_ALL, bx, by, P.pvx, P.pvy);
+        L.mode = 1;
+        L.hx = P.hx; L.hy = P.hy; L.ux = ux; L.uy = uy; L.bx = bx; L.by = by;
+        L.wsx = P.wsx; L.wsy = P.wsy;
+        C.op(OP_LEDGE, li);
+    } else {
+        C.set(M_ALL, P.wsx, 0.0, P.wsy);
+        C.axpy(M_INT, -1.0, P.hx, P.wsx, P.hy, P.wsy);
+        if (P.in.tang == 2) {
+            double *ux = P.w[20], *uy = P.w[21];
+            C.conv_t(1, M_INT, ux, uy, P.pvx, P.pvy);
+            C.axpy(M_INT, -1.0, ux, P.wsx, uy, P.wsy);
+        }
     }
     // with forces prescribed, solvpt rebuilds its right-hand side from wsfix1 (kept in w[22],
     // w[23]) and the current creepages at every call
@@ -2607,7 +2698,9 @@ static Task calculate(Ctx C)
         S.prev = P.in.pvtime == 0;
         S.zero_pn = P.in.iestim == 0;
         S.zero_pt = P.in.iestim == 0 || P.in.tang == 0;
-        S.hertz = P.hertz_init;
+        S.hertz = !P.hertz_init ? 0 : ((P.in.ipotcn == -4 || P.in.ipotcn == -5) ? 2 : 1);
+        S.regularize = P.in.iestim == 1;
+        S.mu = P.in.fstat;
         // set_tang_rhs (src/m_sdis.f90:566-575): a creepage enters the rigid slip only when it
         // is prescribed (F=0: both; F=1: ceta); init_curr_data starts the force-controlled ones
         // from cksi = 1e-6, ceta = 0 (:759-763)
@@ -2642,7 +2735,8 @@ static Task calculate(Ctx C)
     }
     co_await setup_coefficients(C, P.in.tang == 2);
     int itnorm = 0, ittang = 0;
-    int rc = co_await snorm(C, &itnorm);
+    // B=2: the Hertzian pressure is the normal solution (panprc skips NORM, src/m_scontc.f90:409-411)
+    int rc = P.in.bound == 2 ? 0 : co_await snorm(C, &itnorm);
     if (rc == 0) {
         R.itnorm = itnorm;
         C.set(M_EXT, P.px, 0.0, P.py);
@@ -2660,7 +2754,7 @@ static Task calculate(Ctx C)
     // with K = 0 the normal and tangential parts decouple.  Read by cntc_getdisplacements.
     C.set(M_ALL, P.un, 0.0);
     C.set(M_ALL, P.ux, 0.0, P.uy);
-    C.conv_n(0, M_INT, P.un, P.pn);
+    if (P.in.bound != 2) C.conv_n(0, M_INT, P.un, P.pn);   // B=2: normal part 0 (:379-380)
     if (P.in.tang != 0) C.conv_t(0, M_INT, P.ux, P.uy, P.px, P.py);
     // forces (soutpt, src/m_soutpt.f90:424-450)
     AnyItem fi;
@@ -2697,7 +2791,11 @@ int Engine::solve(const std::vector<Patch*>& patches)
     // the largest patches of a big batch set its length: give them flushes of their own
     long maxn = 0;
     for (Patch* p : ok) maxn = std::max(maxn, (long)p->n);
-    for (size_t i = 0; i < ok.size(); i++) sched_->set_priority((int)i, ok.size() >= 16 && (long)sched_->prio_div() * ok[i]->n >= maxn);
+    for (size_t i = 0; i < ok.size(); i++) {
+        const bool pr = ok.size() >= 16 && (long)sched_->prio_div() * ok[i]->n >= maxn;
+        sched_->set_priority((int)i, pr);
+        if (pr) ok[i]->out.prio = 1;
+    }
     for (size_t i = 0; i < ok.size(); i++) {
         ok[i]->slot = (int)i;
         tasks.push_back(calculate(Ctx{sched_.get(), ok[i], (int)i, this}));
@@ -2813,13 +2911,6 @@ void Engine::serve_loop()
                 queue_.pop_front();
             }
         }
-        // the largest patches of a big admission set its length: give them flushes of their own
-        long maxn = 0;
-        for (auto& ab : admit) {
-            Patch* p = tickets_at(ab.first);
-            maxn = std::max(maxn, (long)std::max(1, p->in.mx) * std::max(1, p->in.my));
-        }
-        const bool many = admit.size() >= 16;
         for (auto& ab : admit) {
             Patch* p = tickets_at(ab.first);
             bool ok = false;
@@ -2837,11 +2928,24 @@ void Engine::serve_loop()
             int slot = free_slots.back();
             free_slots.pop_back();
             p->slot = slot;
-            sched_->set_priority(slot, many && (long)sched_->prio_div() * p->n >= maxn);
             active.push_back(Act{ab.first, slot, ab.second, calculate(Ctx{sched_.get(), p, slot, this})});
             active.back().task.h.resume();
         }
         if (active.empty()) continue;
+        if (!admit.empty()) {
+            // the largest patches of a big active set set its length: give them flushes of their
+            // own.  Decided over everything active, so that patches admitted over several
+            // rounds (a large batch arriving while the first requests start) are ranked together.
+            long maxn = 0;
+            for (auto& a : active) maxn = std::max(maxn, (long)tickets_at(a.ticket)->n);
+            const bool many = active.size() >= 16;
+            for (auto& a : active) {
+                Patch* p = tickets_at(a.ticket);
+                const bool pr = many && (long)sched_->prio_div() * p->n >= maxn;
+                sched_->set_priority(a.slot, pr);
+                if (pr) p->out.prio = 1;
+            }
+        }
         std::vector<int> slots;
         for (auto& a : active) slots.push_back(a.slot);
         try {

@@ -22,6 +22,9 @@ namespace cntc {
 // src/m_global_data.f90:336-350).
 struct CaseInput {
     int tang = 3, norm = 0, pvtime = 2, iestim = 0, gausei = 0;
+    // B digit: 0 normal problem solved; 2 Hertzian elliptical pressure prescribed, normal
+    // problem skipped (src/m_sdis.f90:641-709, src/m_scontc.f90:409-411)
+    int bound = 0;
     double poiss1 = 0.28, poiss2 = 0.28, gg1 = 1.0, gg2 = 1.0;
     double fstat = 0.30;
     int ipotcn = 0;                 // 1..4 grid forms, -1/-2/-3 Hertzian; 0 = not yet set
@@ -65,6 +68,7 @@ struct CaseResult {
     int itnorm = 0, ittang = 0, itcg = 0, itgs = 0;
     int gd_stagnated = 0;
     int used_stdygs = 0;
+    int prio = 0;                    // solved with flushes of its own (largest patches of a batch)
     double t_done = 0;               // seconds from the start of the solve call to this result
     double hz_a1 = 0, hz_b1 = 0, hz_aa = 0, hz_bb = 0, hz_rho = 0, hz_cp = 0;
 };
@@ -108,6 +112,8 @@ struct Patch {
     double *h = nullptr, *hx = nullptr, *hy = nullptr, *wsx = nullptr, *wsy = nullptr;
     double* w[24] = {};              // work vectors
     double *un = nullptr, *ux = nullptr, *uy = nullptr;   // displacement differences A ps
+    double *facdt = nullptr, *lix = nullptr;  // leading edge of transient rolling (P.ledge)
+    bool ledge = false;              // T=2 with dq > dx
     double *cf = nullptr, *cfv = nullptr, *cfm = nullptr;  // coefficient blocks cs, cv, ms
     void *spec = nullptr, *specv = nullptr, *specm = nullptr, *spec32 = nullptr, *specm32 = nullptr;
     double* embed = nullptr;         // padded real scratch for spectra (Lx*Ly, or 4*cf_mx*cf_my)

@@ -80,26 +80,52 @@ __device__ inline void group_reduce(double (&v)[K], int mode, double* lds)
     }
 }
 
-template <int K>
-__device__ inline void write_partials(double (&v)[K], int mode, double* partials, int kstride)
+__device__ inline double combine_identity(int mode)
 {
-    __shared__ double red_lds[16 * MAXK];
-    group_reduce<K>(v, mode, red_lds);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < K; k++) partials[(size_t)blockIdx.x * kstride + k] = v[k];
+    return mode == 0 ? 0.0 : (mode == 1 ? -INFINITY : INFINITY);
 }
 
-// finalize: one thread per (item, k) adds the partials of the item's groups in order
-__global__ void k_finalize(const int* gstart, const int* modes, int nitems, int K,
-                           const double* partials, double* results)
+// The last work-group of an item to finish combines the item's group partials: one wave, lane
+// l taking groups l, l + 64, ... in order, then a butterfly over the lanes -- a fixed order
+// that depends only on the item's group count, so an item's result is the same whichever
+// other items share the launch.
+template <int K>
+__device__ inline void final_reduce(const RedOut& ro, const int* gstart, int it)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nitems * K) return;
-    int i = t / K, k = t - i * K;
-    int mode = modes[i];
-    double x = mode == 0 ? 0.0 : (mode == 1 ? -INFINITY : INFINITY);
-    for (int g = gstart[i]; g < gstart[i + 1]; g++) x = combine(x, partials[(size_t)g * K + k], mode);
-    results[(size_t)i * K + k] = x;
+    __threadfence();   // acquire: the other groups' partials
+    if (threadIdx.x >= 64) return;
+    const int lane = threadIdx.x, g0 = gstart[it], g1 = gstart[it + 1];
+    const int mode = ro.modes[it];
+    for (int k = 0; k < K; k++) {
+        double x = combine_identity(mode);
+        for (int g = g0 + lane; g < g1; g += 64) x = combine(x, ro.partials[(size_t)g * K + k], mode);
+        for (int off = 32; off > 0; off >>= 1) x = combine(x, __shfl_xor(x, off), mode);
+        if (lane == 0) ro.results[(size_t)it * K + k] = x;
+    }
+    if (lane == 0) ro.count[it] = 0u;   // ready for the next launch that uses this counter
+}
+
+// Reduce K values over the work-group and store the group's partials; with ro.count set, the
+// item's last group also forms the item's results (no separate finalize launch).  Must be
+// reached by every thread of the group.
+template <int K>
+__device__ inline void write_partials(double (&v)[K], int mode, const RedOut& ro, const int* gstart, int it)
+{
+    __shared__ double red_lds[16 * MAXK];
+    __shared__ int s_last;
+    group_reduce<K>(v, mode, red_lds);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < K; k++) ro.partials[(size_t)blockIdx.x * K + k] = v[k];
+        int last = 0;
+        if (ro.count) {
+            __threadfence();   // release: this group's partials before its arrival
+            const unsigned ng = (unsigned)(gstart[it + 1] - gstart[it]);
+            last = atomicAdd(&ro.count[it], 1u) == ng - 1u;
+        }
+        s_last = last;
+    }
+    __syncthreads();
+    if (s_last) final_reduce<K>(ro, gstart, it);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -107,7 +133,7 @@ __global__ void k_finalize(const int* gstart, const int* modes, int nitems, int 
 // ------------------------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(ELEM_THREADS) k_vec(const VecItem* items, const int* gstart,
-                                                      int nitems, double* partials)
+                                                      int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const VecItem& I = items[it];
@@ -137,13 +163,10 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_vec(const VecItem* items, cons
             }
         }
     }
-    if (mode != 0) {
-        double v1[1] = {acc[0]};
-        write_partials<1>(v1, mode, partials, 2);
-        if (threadIdx.x == 0) partials[(size_t)blockIdx.x * 2 + 1] = 0.0;
-    } else {
-        write_partials<2>(acc, 0, partials, 2);
-    }
+    // max/min: the second value is 0 (the count of the sums); the group reduction of
+    // zeros under max or min stays 0
+    if (mode != 0) acc[1] = 0.0;
+    write_partials<2>(acc, mode, ro, gstart, it);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -153,7 +176,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_vec(const VecItem* items, cons
 // interior elements with negative pressure move to the exterior (src/m_solvpn.f90:302-311);
 // returns (#moved, #interior after, sum pn)
 __global__ void __launch_bounds__(ELEM_THREADS) k_ncg_negpn(const NcgItem* items, const int* gstart,
-                                                            int nitems, double* partials)
+                                                            int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const NcgItem& I = items[it];
@@ -172,13 +195,13 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_ncg_negpn(const NcgItem* items
         if (s >= ADHES) acc[1] += 1.0;
         acc[2] += I.p[i];
     }
-    write_partials<3>(acc, 0, partials, 3);
+    write_partials<3>(acc, 0, ro, gstart, it);
 }
 
 // elements at the deepest interpenetration enter: mode 1 at the start of NormCG with N=1
 // (src/m_solvpn.f90:140-152), mode 2 when no contact is left (src/m_solvpn.f90:316-326)
 __global__ void __launch_bounds__(ELEM_THREADS) k_ncg_hmin(const NcgItem* items, const int* gstart,
-                                                           int nitems, double* partials)
+                                                           int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const NcgItem& I = items[it];
@@ -194,12 +217,12 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_ncg_hmin(const NcgItem* items,
             if (I.mode == 2) I.p[i] = 0.0;
         }
     }
-    write_partials<1>(acc, 0, partials, 1);
+    write_partials<1>(acc, 0, ro, gstart, it);
 }
 
 // dd = A ps - rhs on all elements; returns (sum dd, count) over the interior
 __global__ void __launch_bounds__(ELEM_THREADS) k_ncg_dd(const NcgItem* items, const int* gstart,
-                                                         int nitems, double* partials)
+                                                         int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const NcgItem& I = items[it];
@@ -212,13 +235,13 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_ncg_dd(const NcgItem* items, c
         I.dd[i] = d;
         if (I.el[i] >= ADHES) { acc[0] += d; acc[1] += 1.0; }
     }
-    write_partials<2>(acc, 0, partials, 2);
+    write_partials<2>(acc, 0, ro, gstart, it);
 }
 
 // proj_avg of dd, residual, and interpenetration check of the exterior
 // (src/m_solvpn.f90:351-390); returns #elements that entered
 __global__ void __launch_bounds__(ELEM_THREADS) k_ncg_expand(const NcgItem* items, const int* gstart,
-                                                             int nitems, double* partials)
+                                                             int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const NcgItem& I = items[it];
@@ -244,12 +267,12 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_ncg_expand(const NcgItem* item
         }
         if (s == EXTER) I.v[i] = 0.0;
     }
-    write_partials<1>(acc, 0, partials, 1);
+    write_partials<1>(acc, 0, ro, gstart, it);
 }
 
 // snorm contract step (src/m_snorm.f90:196-219): returns (#moved out, #small pn)
 __global__ void __launch_bounds__(ELEM_THREADS) k_sn_contract(const NcgItem* items, const int* gstart,
-                                                              int nitems, double* partials)
+                                                              int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const NcgItem& I = items[it];
@@ -267,12 +290,12 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_sn_contract(const NcgItem* ite
             if (fabs(I.p[i]) < I.errpn) acc[1] += 1.0;
         }
     }
-    write_partials<2>(acc, 0, partials, 2);
+    write_partials<2>(acc, 0, ro, gstart, it);
 }
 
 // snorm expand step (src/m_snorm.f90:253-286): returns (#moved in, #small dd)
 __global__ void __launch_bounds__(ELEM_THREADS) k_sn_expand(const NcgItem* items, const int* gstart,
-                                                            int nitems, double* partials)
+                                                            int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const NcgItem& I = items[it];
@@ -290,12 +313,12 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_sn_expand(const NcgItem* items
             if (fabs(d) < I.tol) acc[1] += 1.0;
         }
     }
-    write_partials<2>(acc, 0, partials, 2);
+    write_partials<2>(acc, 0, ro, gstart, it);
 }
 
 // final regularisation (src/m_snorm.f90:315-325)
 __global__ void __launch_bounds__(ELEM_THREADS) k_sn_final(const NcgItem* items, const int* gstart,
-                                                           int nitems, double* partials)
+                                                           int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const NcgItem& I = items[it];
@@ -310,14 +333,14 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_sn_final(const NcgItem* items,
             acc[0] += 1.0;
         }
     }
-    write_partials<1>(acc, 0, partials, 1);
+    write_partials<1>(acc, 0, ro, gstart, it);
 }
 
 // gf3_AijPj with unit tractions at one element: sum over rows jy of the coefficients for
 // jx in [row1st-1, rowlst+1] of the current contact area (src/m_aijpj.f90:158-185,
 // src/m_gridfunc.f90:582-637).  One group per item, one thread per row.
 __global__ void __launch_bounds__(ROW_THREADS) k_aij_ones(const AijItem* items, const int* gstart,
-                                                          int nitems, double* partials)
+                                                          int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const AijItem& I = items[it];
@@ -342,7 +365,7 @@ __global__ void __launch_bounds__(ROW_THREADS) k_aij_ones(const AijItem* items, 
         }
         acc[0] = rowsum;
     }
-    write_partials<1>(acc, 0, partials, 1);
+    write_partials<1>(acc, 0, ro, gstart, it);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -357,7 +380,7 @@ __device__ inline void nt_from_p(double px, double py, double& nx, double& ny)
 }
 
 __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_nt(const TcgItem* items, const int* gstart,
-                                                         int nitems, double* partials)
+                                                         int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const TcgItem& I = items[it];
@@ -374,7 +397,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_nt(const TcgItem* items, c
 
 // r = -s, projected on t in the slip area
 __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_rset(const TcgItem* items, const int* gstart,
-                                                           int nitems, double* partials)
+                                                           int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const TcgItem& I = items[it];
@@ -396,7 +419,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_rset(const TcgItem* items,
 
 // Jacobian diagonal scaling and slip projection of z; returns (z.q, v.q) over all elements
 __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_zprep(const TcgItem* items, const int* gstart,
-                                                            int nitems, double* partials)
+                                                            int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const TcgItem& I = items[it];
@@ -424,12 +447,12 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_zprep(const TcgItem* items
         acc[0] += zx * I.qx[i] + zy * I.qy[i];
         acc[1] += I.vx[i] * I.qx[i] + I.vy[i] * I.qy[i];
     }
-    write_partials<2>(acc, 0, partials, 2);
+    write_partials<2>(acc, 0, ro, gstart, it);
 }
 
 // q: slip projection and curvature term; returns (r.v, v.q)
 __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_qpost(const TcgItem* items, const int* gstart,
-                                                            int nitems, double* partials)
+                                                            int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const TcgItem& I = items[it];
@@ -454,13 +477,13 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_qpost(const TcgItem* items
         acc[0] += I.rx[i] * I.vx[i] + I.ry[i] * I.vy[i];
         acc[1] += I.vx[i] * qx + I.vy[i] * qy;
     }
-    write_partials<2>(acc, 0, partials, 2);
+    write_partials<2>(acc, 0, ro, gstart, it);
 }
 
 // pold = p; p += alpha v (interior); slip clipped to the bound; returns (sum v^2, sum p^2), or
 // sum (p - p0)^2 in a sensitivity solve (src/m_solvpt.f90:2209-2216)
 __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_pupd(const TcgItem* items, const int* gstart,
-                                                           int nitems, double* partials)
+                                                           int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const TcgItem& I = items[it];
@@ -492,12 +515,12 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_pupd(const TcgItem* items,
         }
         acc[1] += px * px + py * py;
     }
-    write_partials<2>(acc, 0, partials, 2);
+    write_partials<2>(acc, 0, ro, gstart, it);
 }
 
 // adhesion elements above the traction bound move to slip; returns #moved
 __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_as(const TcgItem* items, const int* gstart,
-                                                         int nitems, double* partials)
+                                                         int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const TcgItem& I = items[it];
@@ -518,12 +541,12 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_as(const TcgItem* items, c
             }
         }
     }
-    write_partials<1>(acc, 0, partials, 1);
+    write_partials<1>(acc, 0, ro, gstart, it);
 }
 
 // slip elements with s.p > 0 move to adhesion; pold -= p; returns (#moved, sum (pold-p)^2)
 __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_sa(const TcgItem* items, const int* gstart,
-                                                         int nitems, double* partials)
+                                                         int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const TcgItem& I = items[it];
@@ -544,7 +567,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_sa(const TcgItem* items, c
         I.poldy[i] = dy;
         acc[1] += dx * dx + dy * dy;
     }
-    write_partials<2>(acc, 0, partials, 2);
+    write_partials<2>(acc, 0, ro, gstart, it);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -552,7 +575,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_sa(const TcgItem* items, c
 // ------------------------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(ELEM_THREADS) k_st_as(const StItem* items, const int* gstart,
-                                                        int nitems, double* partials)
+                                                        int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const StItem& I = items[it];
@@ -573,11 +596,11 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_st_as(const StItem* items, con
             }
         }
     }
-    write_partials<1>(acc, 0, partials, 1);
+    write_partials<1>(acc, 0, ro, gstart, it);
 }
 
 __global__ void __launch_bounds__(ELEM_THREADS) k_st_sa(const StItem* items, const int* gstart,
-                                                        int nitems, double* partials)
+                                                        int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const StItem& I = items[it];
@@ -596,7 +619,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_st_sa(const StItem* items, con
             }
         }
     }
-    write_partials<1>(acc, 0, partials, 1);
+    write_partials<1>(acc, 0, ro, gstart, it);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -622,7 +645,7 @@ __host__ __device__ inline int gd_row_stride(int mx) { return mx | 1; }
 
 // compute_dp, src/gdsteady.f90:610-635: dp = backward differences along x (element-wise)
 __global__ void __launch_bounds__(ELEM_THREADS) k_gd_dp(const GdItem* items, const int* gstart,
-                                                        int nitems, double* partials)
+                                                        int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const GdItem& I = items[it];
@@ -643,7 +666,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_gd_dp(const GdItem* items, con
 
 // project_searchdir, src/gdsteady.f90:639-803: dv from the residual r, with v // t in slip
 __global__ void __launch_bounds__(GD_THREADS) k_gd_project(const GdItem* items, const int* gstart,
-                                                           int nitems, double* partials)
+                                                           int nitems, RedOut ro)
 {
     extern __shared__ double gd_lds[];
     int it = find_item(gstart, nitems, blockIdx.x);
@@ -753,7 +776,7 @@ __global__ void __launch_bounds__(GD_THREADS) k_gd_project(const GdItem* items, 
 // (rho, c0adh, c1adh, c0slp, c1slp, c2slp) and, when I.first, also
 // (sum r^2, sum r.q dscl, sum_int (dscl q)^2, sum_int r^2).
 __global__ void __launch_bounds__(GD_THREADS) k_gd_probe(const GdItem* items, const int* gstart,
-                                                         int nitems, double* partials)
+                                                         int nitems, RedOut ro)
 {
     extern __shared__ double gd_lds[];
     int it = find_item(gstart, nitems, blockIdx.x);
@@ -852,12 +875,12 @@ __global__ void __launch_bounds__(GD_THREADS) k_gd_probe(const GdItem* items, co
             acc[0] += d2 * ex * ex;
         }
     }
-    write_partials<10>(acc, 0, partials, 10);
+    write_partials<10>(acc, 0, ro, gstart, it);
 }
 
 // pold = p; dp += alpha dv on the interior; apply_trcbnd (src/gdsteady.f90:807-847); n from p
 __global__ void __launch_bounds__(GD_THREADS) k_gd_update(const GdItem* items, const int* gstart,
-                                                          int nitems, double* partials)
+                                                          int nitems, RedOut ro)
 {
     extern __shared__ double gd_lds[];
     int it = find_item(gstart, nitems, blockIdx.x);
@@ -1050,7 +1073,7 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
 
 // element-state moves after a GD step (src/gdsteady.f90:375-418); returns #changes
 __global__ void __launch_bounds__(ELEM_THREADS) k_gd_moves(const GdItem* items, const int* gstart,
-                                                           int nitems, double* partials)
+                                                           int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const GdItem& I = items[it];
@@ -1081,7 +1104,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_gd_moves(const GdItem* items, 
             }
         }
     }
-    write_partials<1>(acc, 0, partials, 1);
+    write_partials<1>(acc, 0, ro, gstart, it);
 }
 
 // angle-split of a slip element (compute_diagscaling, src/gdsteady.f90:927-980); returns
@@ -1121,7 +1144,7 @@ __device__ inline bool gd_slip_split(const GdItem& I, int ii, int8_t& elnew, dou
 // compute_diagscaling and the scaled residual (src/gdsteady.f90:901-1042, 429-439); when
 // not I.first also pold -= p on the interior.  Returns (sum (pold)^2, sum p^2, sum_int r^2).
 __global__ void __launch_bounds__(ELEM_THREADS) k_gd_diagres(const GdItem* items, const int* gstart,
-                                                             int nitems, double* partials)
+                                                             int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const GdItem& I = items[it];
@@ -1185,7 +1208,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_gd_diagres(const GdItem* items
         acc[1] += px * px + py * py;
         if (s >= ADHES) acc[2] += rx * rx + ry * ry;
     }
-    write_partials<3>(acc, 0, partials, 3);
+    write_partials<3>(acc, 0, ro, gstart, it);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1513,7 +1536,7 @@ __global__ void k_to_float(size_t n, const double* in, float* out)
 // ------------------------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(ELEM_THREADS) k_setup(const SetupItem* items, const int* gstart,
-                                                        int nitems, double* partials)
+                                                        int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const SetupItem& I = items[it];
@@ -1558,20 +1581,34 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_setup(const SetupItem* items, 
         }
         if (I.hertz) {
             double ax = x / I.hz_aa, by = y / I.hz_bb;
-            double f = 1.0 - ax * ax - by * by;
+            double f = I.hertz == 1 ? 1.0 - ax * ax - by * by : 1.0 - ax * ax;
+            if (I.hertz == 2 && fabs(y) > I.hz_bb) f = 0.0;
             double pn = I.pnmax * sqrt(fmax(0.0, f));
             I.pn[i] = pn;
             I.el[i] = pn > (double)1e-20f ? ADHES : EXTER;
+        }
+        if (I.regularize) {
+            // I=1 (src/m_sdis.f90:722-735): exterior tractions 0, slip tractions on the bound
+            const int8_t e = I.el[i];
+            if (e <= EXTER) {
+                I.pn[i] = 0.0;
+                I.px[i] = 0.0;
+                I.py[i] = 0.0;
+            } else if (e == SLIP) {
+                const double pabs = fmax(1e-10, sqrt(I.px[i] * I.px[i] + I.py[i] * I.py[i]));
+                I.px[i] = I.mu * I.pn[i] * I.px[i] / pabs;
+                I.py[i] = I.mu * I.pn[i] * I.py[i] / pabs;
+            }
         }
         if (h > (double)1e29f) I.el[i] = EXTER;
         acc[0] = fmax(acc[0], h);
         acc[1] = fmax(acc[1], -h);
     }
-    write_partials<2>(acc, 1, partials, 2);
+    write_partials<2>(acc, 1, ro, gstart, it);
 }
 
 __global__ void __launch_bounds__(ELEM_THREADS) k_forces(const SetupItem* items, const int* gstart,
-                                                         int nitems, double* partials)
+                                                         int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const SetupItem& I = items[it];
@@ -1588,7 +1625,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_forces(const SetupItem* items,
         acc[3] += I.px[i] * y;
         acc[4] += I.py[i] * x;
     }
-    write_partials<5>(acc, 0, partials, 5);
+    write_partials<5>(acc, 0, ro, gstart, it);
 }
 
 
@@ -1598,7 +1635,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_forces(const SetupItem* items,
 // ------------------------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(ELEM_THREADS) k_sens(const SensItem* items, const int* gstart,
-                                                       int nitems, double* partials)
+                                                       int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const SensItem& I = items[it];
@@ -1616,11 +1653,13 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_sens(const SensItem* items, co
             if (I.el[i] >= ADHES) {
                 int ix = i % I.mx, iy = i / I.mx;
                 double x = I.xc1 + ix * I.dx, y = I.yc1 + iy * I.dy;
-                if (I.which == 0) wx = wx + I.d * I.dq;
-                else if (I.which == 1) wy = wy + I.d * I.dq;
+                // facdt < 1 only near the leading edge of transient rolling with dq > dx
+                const double d = I.facdt ? I.facdt[i] * I.d : I.d;
+                if (I.which == 0) wx = wx + d * I.dq;
+                else if (I.which == 1) wy = wy + d * I.dq;
                 else {
-                    wx = wx - I.d * y * I.dq;
-                    wy = wy + I.d * x * I.dq;
+                    wx = wx - d * y * I.dq;
+                    wy = wy + d * x * I.dq;
                 }
             }
             I.wsx[i] = wx;
@@ -1633,7 +1672,62 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_sens(const SensItem* items, co
             acc[3] = fmax(acc[3], iy);
         }
     }
-    if (I.mode == 2) write_partials<4>(acc, 1, partials, 4);
+    if (I.mode == 2) write_partials<4>(acc, 1, ro, gstart, it);
+}
+
+// leading edge of transient rolling, sxbnd and stang_rhs (1b) (src/m_leadedge.f90:92-394,
+// src/m_stang.f90:838-935); see LedgeItem
+__global__ void __launch_bounds__(ELEM_THREADS) k_ledge(const LedgeItem* items, const int* gstart,
+                                                        int nitems, RedOut ro)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const LedgeItem& I = items[it];
+    (void)ro;
+    if (I.mode == 0) {
+        const int iy = (blockIdx.x - gstart[it]) * ELEM_THREADS + threadIdx.x;
+        if (iy >= I.my) return;
+        const int mx = I.mx;
+        const size_t r0 = (size_t)iy * mx;
+        int ix0 = -1;   // nearest leading-edge position at or after ix (interior, exterior next)
+        for (int ix = mx - 1; ix >= 0; ix--) {
+            const size_t ii = r0 + ix;
+            const int8_t e = I.el[ii];
+            if (e >= ADHES && (ix == mx - 1 || I.el[ii + 1] <= EXTER)) ix0 = ix;
+            double f = 1.0, l = -1.0;
+            if (e <= EXTER) {
+                f = 0.0;
+            } else if (ix + 3 <= mx) {   // 1-based ix + 2 <= mx: room to estimate the edge
+                const double xbnd = (I.xc1 + ix0 * I.dx) + I.fxdfac * I.dx;
+                f = fmin(1.0, (xbnd - (I.xc1 + ix * I.dx)) / I.dq);
+                if (f < 0.9999) l = (double)ix0;
+            }
+            I.facdt[ii] = f;
+            I.lix[ii] = l;
+        }
+        return;
+    }
+    const int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        const int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        double wx = 0.0, wy = 0.0;
+        if (I.el[i] >= ADHES) {
+            const double f = I.facdt[i];
+            wx = -f * I.hx[i];
+            wy = -f * I.hy[i];
+            const int j0 = (int)I.lix[i];
+            if (j0 < 0) {
+                wx = wx - I.ux[i];
+                wy = wy - I.uy[i];
+            } else if (j0 + 3 <= I.mx) {   // subnd: 1-based ix0 + 2 <= mx, else ubnd = 0
+                const size_t r = (size_t)(i / I.mx) * I.mx + j0;
+                wx = wx - ((1.0 - I.fxdfac) * I.bx[r] + I.fxdfac * I.bx[r + 1]);
+                wy = wy - ((1.0 - I.fxdfac) * I.by[r] + I.fxdfac * I.by[r + 1]);
+            }
+        }
+        I.wsx[i] = wx;
+        I.wsy[i] = wy;
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1649,7 +1743,7 @@ constexpr int GS_THREADS = GS_THREADS_N;
 // needs ConvexGS (src/m_stang.f90:131-143, 185-190).  One thread per row.  Also returns the
 // tangential coefficients of element (0,0) (blocks xx, xy, yx, yy, unscaled).
 __global__ void __launch_bounds__(ELEM_THREADS) k_guard(const GsItem* items, const int* gstart,
-                                                        int nitems, double* partials)
+                                                        int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const GsItem& I = items[it];
@@ -1663,7 +1757,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_guard(const GsItem* items, con
         acc[3] = I.cf[1][0][k0];
         acc[4] = I.cf[1][1][k0];
     }
-    write_partials<5>(acc, 0, partials, 5);
+    write_partials<5>(acc, 0, ro, gstart, it);
 }
 
 // One work-group per patch runs all sweeps.  Per interior element (rows in order, each row
@@ -1677,7 +1771,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_guard(const GsItem* items, con
 // Returns (itgs, dif, algorithmic flops: 8 per coefficient-difference product of the sums,
 // 0, 0, dif of the first sweep, 0, 0, 0, 0).
 __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, const int* gstart,
-                                                       int nitems, double* partials)
+                                                       int nitems, RedOut ro)
 {
     extern __shared__ double gs_lds[];
     __shared__ double red[16 * MAXK];
@@ -1860,11 +1954,11 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs(const GsItem* items, cons
         __syncthreads();
     }
     if (tid == 0) {
-        partials[(size_t)blockIdx.x * 10] = (double)itgs;
-        partials[(size_t)blockIdx.x * 10 + 1] = dif;
-        partials[(size_t)blockIdx.x * 10 + 2] = 8.0 * itgs * (double)I.k * (double)len;
-        for (int q = 3; q < 10; q++) partials[(size_t)blockIdx.x * 10 + q] = 0.0;
-        partials[(size_t)blockIdx.x * 10 + 5] = dif1;
+        ro.partials[(size_t)blockIdx.x * 10] = (double)itgs;
+        ro.partials[(size_t)blockIdx.x * 10 + 1] = dif;
+        ro.partials[(size_t)blockIdx.x * 10 + 2] = 8.0 * itgs * (double)I.k * (double)len;
+        for (int q = 3; q < 10; q++) ro.partials[(size_t)blockIdx.x * 10 + q] = 0.0;
+        ro.partials[(size_t)blockIdx.x * 10 + 5] = dif1;
     }
 }
 
@@ -1918,7 +2012,7 @@ __host__ __device__ inline size_t gs2_lds(int mx, int my)
 }
 
 __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, const int* gstart,
-                                                        int nitems, double* partials)
+                                                        int nitems, RedOut ro)
 {
     extern __shared__ double gs_lds[];
     __shared__ double red_s[16 * MAXK];
@@ -2326,13 +2420,13 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
     if (tid == 0) {
         double len = 0.0;
         for (int jy = 0; jy < my; jy++) len += rj1[jy] >= rj0[jy] ? rj1[jy] - rj0[jy] + 1 : 0;
-        partials[(size_t)blockIdx.x * 10] = (double)itgs;
-        partials[(size_t)blockIdx.x * 10 + 1] = dif;
-        partials[(size_t)blockIdx.x * 10 + 2] = 8.0 * itgs * (double)I.k * len;
-        partials[(size_t)blockIdx.x * 10 + 3] = (double)t_rows;
-        partials[(size_t)blockIdx.x * 10 + 4] = (double)t_inrow;
-        partials[(size_t)blockIdx.x * 10 + 5] = dif1;
-        for (int q = 0; q < 4; q++) partials[(size_t)blockIdx.x * 10 + 6 + q] = (double)ph[q];
+        ro.partials[(size_t)blockIdx.x * 10] = (double)itgs;
+        ro.partials[(size_t)blockIdx.x * 10 + 1] = dif;
+        ro.partials[(size_t)blockIdx.x * 10 + 2] = 8.0 * itgs * (double)I.k * len;
+        ro.partials[(size_t)blockIdx.x * 10 + 3] = (double)t_rows;
+        ro.partials[(size_t)blockIdx.x * 10 + 4] = (double)t_inrow;
+        ro.partials[(size_t)blockIdx.x * 10 + 5] = dif1;
+        for (int q = 0; q < 4; q++) ro.partials[(size_t)blockIdx.x * 10 + 6 + q] = (double)ph[q];
     }
 }
 
@@ -2375,7 +2469,7 @@ __host__ __device__ inline size_t gs3_lds(int mx, int my)
 }
 
 __global__ void __launch_bounds__(GS_THREADS) k_stdygs3(const GsItem* items, const int* gstart,
-                                                        int nitems, double* partials)
+                                                        int nitems, RedOut ro)
 {
     extern __shared__ double gs_lds[];
     __shared__ double red_s[16 * MAXK];
@@ -2824,13 +2918,13 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs3(const GsItem* items, con
     if (tid == 0) {
         double len = 0.0;
         for (int jy = 0; jy < my; jy++) len += rj1[jy] >= rj0[jy] ? rj1[jy] - rj0[jy] + 1 : 0;
-        partials[(size_t)blockIdx.x * 10] = (double)itgs;
-        partials[(size_t)blockIdx.x * 10 + 1] = dif;
-        partials[(size_t)blockIdx.x * 10 + 2] = 8.0 * itgs * (double)I.k * len;
-        partials[(size_t)blockIdx.x * 10 + 3] = (double)t_rows;
-        partials[(size_t)blockIdx.x * 10 + 4] = (double)t_walk;
-        partials[(size_t)blockIdx.x * 10 + 5] = dif1;
-        for (int q = 6; q < 10; q++) partials[(size_t)blockIdx.x * 10 + q] = 0.0;
+        ro.partials[(size_t)blockIdx.x * 10] = (double)itgs;
+        ro.partials[(size_t)blockIdx.x * 10 + 1] = dif;
+        ro.partials[(size_t)blockIdx.x * 10 + 2] = 8.0 * itgs * (double)I.k * len;
+        ro.partials[(size_t)blockIdx.x * 10 + 3] = (double)t_rows;
+        ro.partials[(size_t)blockIdx.x * 10 + 4] = (double)t_walk;
+        ro.partials[(size_t)blockIdx.x * 10 + 5] = dif1;
+        for (int q = 6; q < 10; q++) ro.partials[(size_t)blockIdx.x * 10 + q] = 0.0;
     }
 }
 
@@ -2907,6 +3001,10 @@ int op_groups(int kind, const void* item, int conv_stage)
     case OP_STDYGS: return 1;
     case OP_SENS: case OP_SENS_EXT: return ngroups_elem(((const SensItem*)item)->n);
     case OP_GUARD: return (((const GsItem*)item)->my + ELEM_THREADS - 1) / ELEM_THREADS;
+    case OP_LEDGE: {
+        const LedgeItem* l = (const LedgeItem*)item;
+        return l->mode == 0 ? (l->my + ELEM_THREADS - 1) / ELEM_THREADS : ngroups_elem(l->n);
+    }
     case OP_ELASCF: { const CoefItem* c = (const CoefItem*)item; return (int)(((size_t)(2 * c->mx) * (2 * c->my) + ELEM_THREADS - 1) / ELEM_THREADS); }
     case OP_EMBED: { const EmbedItem* e = (const EmbedItem*)item; return (int)(((size_t)e->Lx * e->Ly + ELEM_THREADS - 1) / ELEM_THREADS); }
     case OP_EXTRACT: { const ExtractItem* e = (const ExtractItem*)item; return (int)(((size_t)(2 * e->cf_mx) * (2 * e->cf_my) + ELEM_THREADS - 1) / ELEM_THREADS); }
@@ -2939,44 +3037,45 @@ static size_t conv_lds(int kind, const ConvItem* c, int stage)
 }
 
 void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, int ngroups,
-               double* d_partials, int conv_stage, size_t lds_bytes, hipStream_t s)
+               RedOut ro, int conv_stage, size_t lds_bytes, hipStream_t s)
 {
     if (ngroups <= 0) return;
     dim3 ge(ngroups), te(ELEM_THREADS), tr(ROW_THREADS), tf(FFT_THREADS);
     switch (kind) {
-    case OP_VEC: hipLaunchKernelGGL(k_vec, ge, te, 0, s, (const VecItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_NCG_NEGPN: hipLaunchKernelGGL(k_ncg_negpn, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_NCG_DD: hipLaunchKernelGGL(k_ncg_dd, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_NCG_EXPAND: hipLaunchKernelGGL(k_ncg_expand, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_NCG_HMIN: hipLaunchKernelGGL(k_ncg_hmin, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_SN_CONTRACT: hipLaunchKernelGGL(k_sn_contract, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_SN_EXPAND: hipLaunchKernelGGL(k_sn_expand, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_SN_FINAL: hipLaunchKernelGGL(k_sn_final, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_AIJ_ONES: hipLaunchKernelGGL(k_aij_ones, ge, tr, 0, s, (const AijItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_TCG_NT: hipLaunchKernelGGL(k_tcg_nt, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_TCG_RSET: hipLaunchKernelGGL(k_tcg_rset, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_TCG_ZPREP: hipLaunchKernelGGL(k_tcg_zprep, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_TCG_QPOST: hipLaunchKernelGGL(k_tcg_qpost, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_TCG_PUPD: hipLaunchKernelGGL(k_tcg_pupd, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_TCG_AS: hipLaunchKernelGGL(k_tcg_as, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_TCG_SA: hipLaunchKernelGGL(k_tcg_sa, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_ST_AS: hipLaunchKernelGGL(k_st_as, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_ST_SA: hipLaunchKernelGGL(k_st_sa, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_GD_DP: hipLaunchKernelGGL(k_gd_dp, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_GD_PROJECT: hipLaunchKernelGGL(k_gd_project, ge, dim3(GD_THREADS), lds_bytes, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_GD_PROBE: hipLaunchKernelGGL(k_gd_probe, ge, dim3(GD_THREADS), lds_bytes, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_GD_UPDATE: hipLaunchKernelGGL(k_gd_update, ge, dim3(GD_THREADS), lds_bytes, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_GD_MOVES: hipLaunchKernelGGL(k_gd_moves, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_GD_DIAGRES: hipLaunchKernelGGL(k_gd_diagres, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_SETUP: hipLaunchKernelGGL(k_setup, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_FORCES: hipLaunchKernelGGL(k_forces, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_VEC: hipLaunchKernelGGL(k_vec, ge, te, 0, s, (const VecItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_NCG_NEGPN: hipLaunchKernelGGL(k_ncg_negpn, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_NCG_DD: hipLaunchKernelGGL(k_ncg_dd, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_NCG_EXPAND: hipLaunchKernelGGL(k_ncg_expand, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_NCG_HMIN: hipLaunchKernelGGL(k_ncg_hmin, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_SN_CONTRACT: hipLaunchKernelGGL(k_sn_contract, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_SN_EXPAND: hipLaunchKernelGGL(k_sn_expand, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_SN_FINAL: hipLaunchKernelGGL(k_sn_final, ge, te, 0, s, (const NcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_AIJ_ONES: hipLaunchKernelGGL(k_aij_ones, ge, tr, 0, s, (const AijItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_TCG_NT: hipLaunchKernelGGL(k_tcg_nt, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_TCG_RSET: hipLaunchKernelGGL(k_tcg_rset, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_TCG_ZPREP: hipLaunchKernelGGL(k_tcg_zprep, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_TCG_QPOST: hipLaunchKernelGGL(k_tcg_qpost, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_TCG_PUPD: hipLaunchKernelGGL(k_tcg_pupd, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_TCG_AS: hipLaunchKernelGGL(k_tcg_as, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_TCG_SA: hipLaunchKernelGGL(k_tcg_sa, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_ST_AS: hipLaunchKernelGGL(k_st_as, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_ST_SA: hipLaunchKernelGGL(k_st_sa, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_GD_DP: hipLaunchKernelGGL(k_gd_dp, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_GD_PROJECT: hipLaunchKernelGGL(k_gd_project, ge, dim3(GD_THREADS), lds_bytes, s, (const GdItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_GD_PROBE: hipLaunchKernelGGL(k_gd_probe, ge, dim3(GD_THREADS), lds_bytes, s, (const GdItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_GD_UPDATE: hipLaunchKernelGGL(k_gd_update, ge, dim3(GD_THREADS), lds_bytes, s, (const GdItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_GD_MOVES: hipLaunchKernelGGL(k_gd_moves, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_GD_DIAGRES: hipLaunchKernelGGL(k_gd_diagres, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_SETUP: hipLaunchKernelGGL(k_setup, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_FORCES: hipLaunchKernelGGL(k_forces, ge, te, 0, s, (const SetupItem*)d_items, d_gstart, nitems, ro); break;
     case OP_STDYGS:
-        if (conv_stage == 2) hipLaunchKernelGGL(k_stdygs3, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, d_partials);
-        else if (conv_stage == 1) hipLaunchKernelGGL(k_stdygs2, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, d_partials);
-        else hipLaunchKernelGGL(k_stdygs, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, d_partials);
+        if (conv_stage == 2) hipLaunchKernelGGL(k_stdygs3, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, ro);
+        else if (conv_stage == 1) hipLaunchKernelGGL(k_stdygs2, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, ro);
+        else hipLaunchKernelGGL(k_stdygs, ge, dim3(GS_THREADS), lds_bytes, s, (const GsItem*)d_items, d_gstart, nitems, ro);
         break;
-    case OP_GUARD: hipLaunchKernelGGL(k_guard, ge, te, 0, s, (const GsItem*)d_items, d_gstart, nitems, d_partials); break;
-    case OP_SENS: case OP_SENS_EXT: hipLaunchKernelGGL(k_sens, ge, te, 0, s, (const SensItem*)d_items, d_gstart, nitems, d_partials); break;
+    case OP_GUARD: hipLaunchKernelGGL(k_guard, ge, te, 0, s, (const GsItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_SENS: case OP_SENS_EXT: hipLaunchKernelGGL(k_sens, ge, te, 0, s, (const SensItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_LEDGE: hipLaunchKernelGGL(k_ledge, ge, te, 0, s, (const LedgeItem*)d_items, d_gstart, nitems, ro); break;
     case OP_ELASCF: hipLaunchKernelGGL(k_elascf, ge, te, 0, s, (const CoefItem*)d_items, d_gstart, nitems); break;
     case OP_EMBED: hipLaunchKernelGGL(k_embed, ge, te, 0, s, (const EmbedItem*)d_items, d_gstart, nitems); break;
     case OP_EXTRACT: hipLaunchKernelGGL(k_extract, ge, te, 0, s, (const ExtractItem*)d_items, d_gstart, nitems); break;
@@ -3086,16 +3185,6 @@ size_t op_lds(int kind, const void* item, int stage)
     }
     return 0;
 }
-
-void launch_finalize(const int* d_gstart, const int* d_modes, int nitems, int K,
-                     const double* d_partials, double* d_results, hipStream_t s)
-{
-    int n = nitems * K;
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_finalize, dim3((n + 255) / 256), dim3(256), 0, s, d_gstart, d_modes, nitems, K,
-                       d_partials, d_results);
-}
-
 
 void launch_rollcoef(int n, const double* cs, double* cv, hipStream_t s)
 {

@@ -22,12 +22,11 @@ int op_groups(int kind, const void* item, int conv_stage);
 double op_bytes(int kind, const void* item, int stage);
 // dynamic LDS bytes of one work-group for this item and stage
 size_t op_lds(int kind, const void* item, int stage);
-// launch one batched kernel over `nitems` items whose work-groups are given by d_gstart
+// launch one batched kernel over `nitems` items whose work-groups are given by d_gstart; a
+// kernel with reductions writes its group partials to ro.partials and, when ro.count is set,
+// each item's results to ro.results (ro.count: one zeroed counter per item)
 void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, int ngroups,
-               double* d_partials, int conv_stage, size_t lds_bytes, hipStream_t s);
-// per-item reduction of the group partials
-void launch_finalize(const int* d_gstart, const int* d_modes, int nitems, int K,
-                     const double* d_partials, double* d_results, hipStream_t s);
+               RedOut ro, int conv_stage, size_t lds_bytes, hipStream_t s);
 
 void launch_rollcoef(int n, const double* cs, double* cv, hipStream_t s);
 void launch_to_float(size_t n, const double* in, float* out, hipStream_t s);

@@ -3,13 +3,23 @@
 // Every device operation of the engine is launched once for all patches that asked for it in
 // the same scheduler round.  Each participating patch contributes one item; the work-groups of
 // a launch are split between items by a prefix table (item i owns groups [gstart[i],
-// gstart[i+1])).  Reductions are deterministic: every group writes its partial sums, and a
-// finalize pass adds the partials of each item in group order.
+// gstart[i+1])).  Reductions are deterministic: every group writes its partial sums, and the
+// item's last group to finish combines them in an order fixed by the item's group count.
 #pragma once
 
 #include <cstdint>
 
 namespace cntc {
+
+// Where a launch's reductions go: group partials [group * K + k]; with `count` set (one
+// counter per item, zero between launches) the last group of each item also combines them
+// into results [item * K + k] with the item's combine mode (modes[item]: 0 sum, 1 max, 2 min).
+struct RedOut {
+    double* partials;
+    unsigned* count;
+    double* results;
+    const int* modes;
+};
 
 // element-state codes, src/m_gridfunc.f90:30-37
 enum : int8_t { EXTER = 0, ADHES = 1, SLIP = 2, PLAST = 3 };
@@ -53,6 +63,7 @@ enum OpKind : int {
     OP_GUARD,
     OP_SENS,
     OP_SENS_EXT,
+    OP_LEDGE,
     OP_COUNT_KINDS
 };
 
@@ -184,6 +195,23 @@ struct SensItem {
     const double* wbx; const double* wby;
     double* wsx; double* wsy;
     double xc1, yc1, dx, dy, dq, d;
+    const double* facdt;       // mode 1: fraction of the step in contact (null: 1)
+};
+
+// leading edge of transient rolling with dq > dx (sxbnd, stang_rhs (1b), src/m_leadedge.f90,
+// src/m_stang.f90:838-935).  mode 0, one thread per row: facdt and the leading-edge position
+// lix (0-based ix0, or -1 where facdt >= 0.9999).  mode 1, per element: the fixed right-hand
+// side ws = -facdt h - (A' pv)(cv) on the interior, with the displacement at the leading edge
+// interpolated from b = A pv (cs) between ix0 and ix0 + 1 where lix >= 0.
+struct LedgeItem {
+    int n, mx, my, mode;
+    const int8_t* el;
+    double* facdt; double* lix;
+    const double* hx; const double* hy;
+    const double* ux; const double* uy;     // A' pv_t (cv), interior
+    const double* bx; const double* by;     // A pv_t (cs), all elements
+    double* wsx; double* wsy;
+    double xc1, dx, dq, fxdfac;
 };
 
 // one FFT product (gf3_VecAijPj via fft_VecAijPj, src/m_aijpj.f90:258-451, 712-1015)
@@ -208,8 +236,10 @@ struct ConvItem {
 // (sum pn, sum px, sum py, sum px*y, sum py*x).
 struct SetupItem {
     int n, mx, my, ibase;          // ibase 1: quadratic gap from prm; otherwise h is given
-    int prev, zero_pn, zero_pt, hertz;
-    int exrhs, pad0;               // exrhs: hx, hy hold the extra rigid slip on entry
+    int prev, zero_pn, zero_pt, hertz;   // hertz: 1 elliptical (3D), 2 strip |y| <= bb (2D)
+    int exrhs, regularize;         // exrhs: hx, hy hold the extra rigid slip on entry;
+                                   // regularize: I=1 (exterior 0, slip on the bound)
+    double mu;
     double facx, facy;             // spin factors of the rigid slip (E digit 2, 3 zero one)
     int8_t* el;
     double* h; double* hx; double* hy;

@@ -143,6 +143,9 @@ def apply_case(cntc, ire, icp, c):
         cntc.sethertzcontact(ire, icp, -2, [c["mx"], c["my"], c["a1"], c["aob"], c["scale"]])
     elif c["ipotcn"] == -1:
         cntc.sethertzcontact(ire, icp, -1, [c["mx"], c["my"], c["a1"], c["b1"], c["scale"]])
+    elif c["ipotcn"] in (-4, -5):
+        first = c["a1"] if c["ipotcn"] == -4 else c["aa"]
+        cntc.sethertzcontact(ire, icp, c["ipotcn"], [c["mx"], c["my"], first, c["bb"], c["scale"]])
     else:
         cntc.setpotcontact(ire, icp, 1, [c["mx"], c["my"], c["xl"], c["yl"], c["dx"], c["dy"]])
         if c.get("h") is not None:
@@ -161,6 +164,8 @@ def apply_case(cntc, ire, icp, c):
         cntc.settangentialforces(ire, icp, c.get("fxrel", 0.0), c.get("fyrel", 0.0))
     if c.get("rztang") is not None:
         cntc.setflags(ire, icp, [1980], [c["rztang"]])
+    if c.get("bound") is not None:
+        cntc.setflags(ire, icp, [1971], [c["bound"]])
     if c.get("exrhs") is not None:
         cntc.setextrarigidslip(ire, icp, c["exrhs"][0], c["exrhs"][1])
     if c.get("sens") is not None:

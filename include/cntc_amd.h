@@ -158,7 +158,8 @@ void cntc_getmicroslip(int *ire, int *icp, int *lenarr, double *sx, double *sy);
 /* No counterpart in the reference, which prints these counts in its flow output
    (src/m_snorm.f90:329-331, src/m_stang.f90:57): values = {NORM outer iterations, NormCG
    iterations, TANG outer iterations, tangential solver iterations, solver flags} of the last
-   case; flags bit 0: GDsteady stagnated, bit 1: SteadyGS ran. */
+   case; flags bit 0: GDsteady stagnated, bit 1: SteadyGS ran, bit 2: solved among the largest
+   patches of a batch, with flushes of their own. */
 void cntc_getiterations(int *ire, int *icp, int *lenarr, int *values);
 /* src/contact_addon.f90:6011-6040 cntc_getCalculationTime (for cntc_calculatebatch, twall is
    the time from the start of the batch to this problem's result) */

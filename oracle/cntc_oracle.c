@@ -336,6 +336,7 @@ struct orc_problem {
     /* grid functions: [0]=x, [1]=y, [2]=n */
     double *hs[3];
     double *ps[3], *pv[3], *ss[2], *facdt;
+    int *lix;          /* per element: 0-based ix of its leading-edge position when facdt < 0.9999, else -1 */
     eldiv igs, igv;
     int have_solution;
     /* kinematics and results */
@@ -2114,9 +2115,11 @@ static int tangcg(orc_problem *P, int maxcg, double eps, double **ws, const doub
 /* stang (Coulomb, creepages prescribed), src/m_stang.f90:28-951, src/m_solvpt.f90:51-493       */
 /* ------------------------------------------------------------------------------------------ */
 
-/* sxbnd's facdt, src/m_leadedge.f90:92-332 (ubnd is needed only when facdt<1, i.e. dq>dx;
-   that case is reported as not covered) */
-static int set_facdt(orc_problem *P, int tang)
+/* sxbnd, src/m_leadedge.f90:92-332: per element the fraction facdt of the time step spent in
+   the contact area and, for elements entering it (facdt < 0.9999), the leading-edge position
+   (the interior element ix0 before the exterior) whose displacement ubnd replaces the previous
+   time's; the edge lies fxdfac = 1 element beyond ix0 (2 with SteadyGS, :215-224) */
+static int set_facdt(orc_problem *P, int tang, double fxdfac)
 {
     int mx = P->mx, my = P->my, *el = P->igs.el;
     int is_roll = tang == 2 || tang == 3;
@@ -2134,17 +2137,31 @@ static int set_facdt(orc_problem *P, int tang)
         for (int ix = 1; ix <= mx; ix++) {
             while (j < nb && ix > xb[j]) j++;
             int ii = ix - 1 + iy * mx;
+            P->lix[ii] = -1;
             if (!is_roll) P->facdt[ii] = 1.0;
             else if (el[ii] <= ORC_EXTER) P->facdt[ii] = 0.0;
             else if (ix + 2 > mx) P->facdt[ii] = 1.0;
             else {
-                double xbnd = P->x[xb[j] - 1 + iy * mx] + ixinc * 1.0 * P->dx;
+                double xbnd = P->x[xb[j] - 1 + iy * mx] + ixinc * fxdfac * P->dx;
                 P->facdt[ii] = dmin(1.0, ixinc * (xbnd - P->x[ii]) / P->dq);
-                if (P->facdt[ii] < 0.9999 && tang == 2) return CNTC_ERR_OTHER;
+                if (P->facdt[ii] < 0.9999) P->lix[ii] = xb[j] - 1;
             }
         }
     }
     return 0;
+}
+
+/* subnd, src/m_leadedge.f90:336-394: the tangential displacement difference at the leading edge
+   of element ii's row, interpolated between ix0 and ix0 + 1 from the previous tractions pv (all
+   components, direct sums with the coefficients cs and pv's element division igv) */
+static double subnd(orc_problem *P, int ii, int ik, double fxdfac)
+{
+    int mx = P->mx, iy = ii / mx, ix0 = P->lix[ii] + 1;   /* 1-based */
+    if (ix0 + 2 > mx) return 0.0;
+    int i0 = ix0 + iy * mx;                                /* 1-based index of ix0 */
+    double u0 = aijpj(P, i0, ik, P->pv, -3, &P->cs, &P->igv);
+    double u1 = aijpj(P, i0 + 1, ik, P->pv, -3, &P->cs, &P->igv);
+    return (1.0 - fxdfac) * u0 + fxdfac * u1;
 }
 
 /* tang_solver, src/m_solvpt.f90:382-493: GDsteady for T=3 with G=5, restarted by SteadyGS when
@@ -2450,7 +2467,7 @@ static int stang(orc_problem *P, const orc_input *inp, int *ittang_out)
     for (int i = 0; i < npot; i++)
         if (el[i] >= ORC_ADHES) k++;
     if (k <= 0) { free(mus); return CNTC_ERR_OTHER; }
-    int rc = set_facdt(P, inp->tang);
+    int rc = set_facdt(P, inp->tang, 1.0);
     if (rc) { free(mus); return rc; }
     /* steady rolling needs exterior elements at the trailing edge; otherwise the reference
        switches to ConvexGS (src/m_stang.f90:131-143, 185-190), which is not restated here */
@@ -2479,12 +2496,14 @@ static int stang(orc_problem *P, const orc_input *inp, int *ittang_out)
     if (inp->tang != 3) vecaijpj(P, el, ALLINT, uvn, -2, P->pv, 3, &P->cv);
     else vecaijpj(P, el, ALLINT, uvn, -2, P->ps, 3, &P->cv);
     if (inp->tang != 3) vecaijpj(P, el, ALLINT, uvt, -2, P->pv, -2, &P->cv);
+    areas(&P->igv, mx, my);
     for (int ik = 0; ik < 2; ik++)
         for (int i = 0; i < npot; i++)
             if (el[i] >= ORC_ADHES) {
                 double wsrig = -P->facdt[i] * P->hs[ik][i];
-                if (inp->tang != 3) ws[ik][i] = wsrig + usn[ik][i] - uvn[ik][i] - uvt[ik][i];
-                else ws[ik][i] = wsrig + usn[ik][i] - uvn[ik][i];
+                if (inp->tang == 3) ws[ik][i] = wsrig + usn[ik][i] - uvn[ik][i];
+                else if (P->lix[i] < 0) ws[ik][i] = wsrig + usn[ik][i] - uvn[ik][i] - uvt[ik][i];
+                else ws[ik][i] = wsrig + usn[ik][i] - subnd(P, i, ik + 1, 1.0);   /* (1b) */
             }
 
     gdpar G;
@@ -2559,6 +2578,7 @@ static void free_grid_arrays(orc_problem *p)
     }
     for (int k = 0; k < 2; k++) { free(p->ss[k]); p->ss[k] = NULL; }
     free(p->facdt); p->facdt = NULL;
+    free(p->lix); p->lix = NULL;
     free(p->igs.el); free(p->igs.row1st); free(p->igs.rowlst);
     free(p->igv.el); free(p->igv.row1st); free(p->igv.rowlst);
     memset(&p->igs, 0, sizeof p->igs);
@@ -2680,23 +2700,53 @@ int orc_calculate(orc_problem *P, const orc_input *inp)
     double prmudf[6];
     memcpy(prmudf, inp->prmudf, sizeof prmudf);
     int ibase = inp->ibase;
-    if (inp->ipotcn == -1 || inp->ipotcn == -2 || inp->ipotcn == -3) {
+    int hertz2d = inp->ipotcn == -4 || inp->ipotcn == -5;
+    if (inp->bound != 0 && inp->bound != 2) return P->ierror = CNTC_ERR_OTHER;
+    if (inp->bound == 2 && !(inp->ipotcn <= -1 && inp->ipotcn >= -5)) return P->ierror = CNTC_ERR_OTHER;
+    if (hertz2d && inp->norm != 1) return P->ierror = CNTC_ERR_OTHER;
+    if (inp->ipotcn <= -1 && inp->ipotcn >= -5) {
         /* hzsol, src/m_hertz.f90:29-90 */
         double e_star = P->ga / (1.0 - P->nu);
         P->hz_a1 = inp->hz_a1; P->hz_b1 = inp->hz_b1; P->hz_aa = inp->hz_aa; P->hz_bb = inp->hz_bb;
-        int herr = inp->ipotcn == -2
-                       ? elliptic_to_curv(e_star, sp(1e-9f), inp->norm, &P->hz_a1, inp->hz_aob, &P->hz_b1,
-                                          &P->hz_aa, &P->hz_bb, &P->pen, &P->fntrue, &P->hz_cp, &P->hz_rho)
-                       : hzcalc3d(e_star, sp(1e-9f), inp->ipotcn, &P->hz_a1, &P->hz_b1, &P->hz_aa, &P->hz_bb,
-                                  inp->norm, &P->pen, &P->fntrue, &P->hz_cp, &P->hz_rho);
+        int herr = 0;
+        if (hertz2d) {
+            /* hzcalc2d, src/m_hertz.f90:217-263 */
+            double fn_2d = P->fntrue / (2.0 * P->hz_bb), rx;
+            P->hz_b1 = 0.0;
+            if (inp->ipotcn == -4) {
+                rx = 1.0 / (2.0 * P->hz_a1);
+                P->hz_aa = sqrt(4.0 * rx * fn_2d / (PI * e_star));
+            } else {
+                rx = PI * P->hz_aa * P->hz_aa * e_star / (4.0 * fn_2d);
+                P->hz_a1 = 1.0 / (2.0 * rx);
+            }
+            P->hz_rho = 2.0 / (P->hz_a1 + P->hz_b1);
+            P->hz_cp = sqrt(4.0 * P->hz_aa * P->hz_bb / PI);
+        } else if (inp->ipotcn == -2) {
+            herr = elliptic_to_curv(e_star, sp(1e-9f), inp->norm, &P->hz_a1, inp->hz_aob, &P->hz_b1,
+                                    &P->hz_aa, &P->hz_bb, &P->pen, &P->fntrue, &P->hz_cp, &P->hz_rho);
+        } else {
+            herr = hzcalc3d(e_star, sp(1e-9f), inp->ipotcn, &P->hz_a1, &P->hz_b1, &P->hz_aa, &P->hz_bb,
+                            inp->norm, &P->pen, &P->fntrue, &P->hz_cp, &P->hz_rho);
+        }
         if (herr != 0) return P->ierror = CNTC_ERR_OTHER;
         ibase = 1;
         memset(prmudf, 0, sizeof prmudf);
         prmudf[0] = P->hz_a1;
         prmudf[2] = P->hz_b1;
+        /* potcon_hertz, src/m_hierarch_data.f90:1889-1959 */
         double xl = -inp->hz_scale * dmax(1e-9, P->hz_aa), yl = -inp->hz_scale * dmax(1e-9, P->hz_bb);
         dx = (-xl - xl) / mx;
         dy = (-yl - yl) / my;
+        if (hertz2d) {
+            int n_int = my;
+            if (inp->hz_scale > 1.0) {
+                n_int = imax(1, (int)lround(my / inp->hz_scale));
+                if ((my - n_int) % 2 == 1) n_int = n_int + 1;
+            }
+            dy = 2.0 * dmax(1e-9, P->hz_bb) / n_int;
+            yl = -my * dy / 2.0;
+        }
         xc1 = xl + 0.5 * dx;
         yc1 = yl + 0.5 * dy;
     } else if (inp->ipotcn != 3) {
@@ -2712,6 +2762,7 @@ int orc_calculate(orc_problem *P, const orc_input *inp)
         for (int k = 0; k < 3; k++) { P->hs[k] = dalloc(n); P->ps[k] = dalloc(n); P->pv[k] = dalloc(n); }
         for (int k = 0; k < 2; k++) P->ss[k] = dalloc(n);
         P->facdt = dalloc(n);
+        P->lix = (int *)calloc((size_t)n, sizeof(int));
         eldiv_alloc(&P->igs, n, my);
         eldiv_alloc(&P->igv, n, my);
     }
@@ -2782,16 +2833,38 @@ int orc_calculate(orc_problem *P, const orc_input *inp)
     if (inp->iestim == 0 || inp->tang == 0) {
         for (int i = 0; i < npot; i++) { P->ps[0][i] = 0.0; P->ps[1][i] = 0.0; }
     }
-    int use_iestim_hertz = inp->iestim == 0 && ihertz >= 1;
-    if (use_iestim_hertz) {
-        double pnmax = 3.0 * P->fntrue / (2.0 * PI * P->hz_aa * P->hz_bb);
-        for (int i = 0; i < npot; i++) {
-            double pnfac = 1.0 - sqr(P->x[i] / P->hz_aa) - sqr(P->y[i] / P->hz_bb);
-            P->ps[2][i] = pnmax * sqrt(dmax(0.0, pnfac));
+    int use_iestim_hertz = inp->iestim == 0 && inp->bound == 0 && ihertz >= 1;
+    if (inp->bound == 2 || use_iestim_hertz) {
+        if (ihertz <= 3) {
+            double pnmax = 3.0 * P->fntrue / (2.0 * PI * P->hz_aa * P->hz_bb);
+            for (int i = 0; i < npot; i++) {
+                double pnfac = 1.0 - sqr(P->x[i] / P->hz_aa) - sqr(P->y[i] / P->hz_bb);
+                P->ps[2][i] = pnmax * sqrt(dmax(0.0, pnfac));
+            }
+        } else {
+            /* 2D elliptical traction bound, uniform in y */
+            double pnmax = P->fntrue / (PI * P->hz_aa * P->hz_bb);
+            for (int i = 0; i < npot; i++) {
+                double pnfac = 1.0 - sqr(P->x[i] / P->hz_aa);
+                if (fabs(P->y[i]) > P->hz_bb) pnfac = 0.0;
+                P->ps[2][i] = pnmax * sqrt(dmax(0.0, pnfac));
+            }
         }
         for (int i = 0; i < npot; i++) P->igs.el[i] = P->ps[2][i] > sp(1e-20f) ? ORC_ADHES : ORC_EXTER;
     } else if (inp->iestim == 0) {
         eldiv0(P, inp);
+    }
+    if (inp->iestim == 1) {
+        /* regularise the tractions (:722-735): exterior 0, slip on the traction bound */
+        for (int i = 0; i < npot; i++) {
+            if (P->igs.el[i] <= ORC_EXTER) {
+                P->ps[0][i] = P->ps[1][i] = P->ps[2][i] = 0.0;
+            } else if (P->igs.el[i] == ORC_SLIP) {
+                double pabs = dmax(1e-10, sqrt(sqr(P->ps[0][i]) + sqr(P->ps[1][i])));
+                P->ps[0][i] = P->muscal * P->ps[2][i] * P->ps[0][i] / pabs;
+                P->ps[1][i] = P->muscal * P->ps[2][i] * P->ps[1][i] / pabs;
+            }
+        }
     }
     for (int i = 0; i < npot; i++)
         if (P->hs[2][i] > sp(1e29f)) P->igs.el[i] = ORC_EXTER;
@@ -2819,7 +2892,8 @@ int orc_calculate(orc_problem *P, const orc_input *inp)
     while (dif > difid && itout < maxout && itnorm_tot >= 0 && ittang_tot >= 0) {
         itout++;
         int it = 0;
-        rc = snorm(P, inp, &it);
+        /* B=2: the Hertzian pressure is the normal solution (src/m_scontc.f90:409-411) */
+        rc = inp->bound == 2 ? 0 : snorm(P, inp, &it);
         if (rc) break;
         itnorm_tot = it >= 0 ? itnorm_tot + it : -1;
         if (inp->tang == 3) memcpy(P->igv.el, P->igs.el, sizeof(int) * npot);
@@ -2872,6 +2946,19 @@ int orc_calculate(orc_problem *P, const orc_input *inp)
 /* ------------------------------------------------------------------------------------------ */
 /* getters                                                                                      */
 /* ------------------------------------------------------------------------------------------ */
+
+/* displacement differences of the last solution, soutpt (src/m_soutpt.f90:371-397): us = A ps
+   on the interior (AllInt, matfil_surf < 2), 0 elsewhere (init_curr_data clears us,
+   src/m_sdis.f90:624) */
+void orc_get_displacements(orc_problem *P, double *un, double *ux, double *uy)
+{
+    int n = P->npot;
+    double *U[3] = {ux, uy, un};
+    for (int k = 0; k < 3; k++)
+        for (int i = 0; i < n; i++) U[k][i] = 0.0;
+    vecaijpj(P, P->igs.el, ALLINT, U, 3, P->ps, -3, &P->cs);
+    vecaijpj(P, P->igs.el, ALLINT, U, -2, P->ps, -3, &P->cs);
+}
 
 int orc_get_mx(const orc_problem *p) { return p->mx; }
 int orc_get_my(const orc_problem *p) { return p->my; }

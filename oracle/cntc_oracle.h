@@ -39,7 +39,7 @@ typedef struct orc_input {
     double fstat;
     /* potential contact: ipotcn=3 form (centre of first element and step sizes), or
        Hertzian ipotcn=-1 (curvatures a1, b1) / -2 (curvature a1, ellipticity aob) / -3
-       (semi-axes aa, bb) with scale */
+       (semi-axes aa, bb) / -4 (curvature a1, half-width bb) / -5 (half-sizes aa, bb), with scale */
     int ipotcn;
     int mx, my;
     double xc1, yc1, dx, dy;
@@ -70,6 +70,9 @@ typedef struct orc_input {
        3 from wx */
     int rztang;
     const double *exrhs;
+    /* B digit (src/m_sdis.f90:641-709, src/m_scontc.f90:409-411): 0 normal problem solved, 2
+       Hertzian elliptical pressure prescribed (set by the Hertz options -4 and -5 as well) */
+    int bound;
 } orc_input;
 
 /* fills an input record with the reference's module-3 defaults */
@@ -95,6 +98,8 @@ void orc_get_slip(const orc_problem *p, double *sx, double *sy);
 void orc_get_eldiv(const orc_problem *p, int *el);
 void orc_get_h(const orc_problem *p, double *h);
 double orc_get_pen(const orc_problem *p);
+/* displacement differences un, ux, uy of the last solution (A ps on the interior) */
+void orc_get_displacements(orc_problem *p, double *un, double *ux, double *uy);
 void orc_get_hertz(const orc_problem *p, double *a1, double *b1, double *aa, double *bb,
                    double *rho, double *cp);
 void orc_get_iters(const orc_problem *p, int *itnorm, int *ittang, int *itcg, int *itgs);

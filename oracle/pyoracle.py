@@ -42,6 +42,7 @@ class OrcInput(ctypes.Structure):
         ("sens", ctypes.c_int), ("mxsens", ctypes.c_int), ("epsens", ctypes.c_double),
         ("force3", ctypes.c_int), ("fxrel", ctypes.c_double), ("fyrel", ctypes.c_double),
         ("rztang", ctypes.c_int), ("exrhs", ctypes.POINTER(ctypes.c_double)),
+        ("bound", ctypes.c_int),
     ]
 
 
@@ -75,6 +76,7 @@ def lib():
         L.orc_get_pen.argtypes = [vp]
         L.orc_get_pen.restype = ctypes.c_double
         L.orc_get_hertz.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.orc_get_displacements.argtypes = [vp, dp, dp, dp]
         L.orc_get_iters.argtypes = [vp, ip, ip, ip, ip]
         L.orc_get_stagnated.argtypes = [vp]
         L.orc_get_stagnated.restype = ctypes.c_int
@@ -162,6 +164,13 @@ class Problem:
         v = [ctypes.c_double() for _ in range(4)]
         lib().orc_get_grid(self._p, *[ctypes.byref(x) for x in v])
         return tuple(x.value for x in v)
+
+    def displacements(self):
+        """(un, ux, uy) = A ps on the interior, 0 elsewhere (flat, x fastest)."""
+        mx, my = self.shape
+        un, ux, uy = (np.zeros(mx * my) for _ in range(3))
+        lib().orc_get_displacements(self._p, _dptr(un), _dptr(ux), _dptr(uy))
+        return un, ux, uy
 
     def hertz(self):
         v = [ctypes.c_double() for _ in range(6)]
@@ -280,6 +289,15 @@ def case_input(c):
         inp.ipotcn = -1
         inp.mx, inp.my = c["mx"], c["my"]
         inp.hz_a1, inp.hz_b1, inp.hz_scale = c["a1"], c["b1"], c["scale"]
+    elif c["ipotcn"] in (-4, -5):
+        inp.ipotcn = c["ipotcn"]
+        inp.mx, inp.my = c["mx"], c["my"]
+        if c["ipotcn"] == -4:
+            inp.hz_a1 = c["a1"]
+        else:
+            inp.hz_aa = c["aa"]
+        inp.hz_bb, inp.hz_scale = c["bb"], c["scale"]
+        inp.bound = 2
     else:
         inp.ipotcn = 3
         inp.mx, inp.my = c["mx"], c["my"]
@@ -304,6 +322,8 @@ def case_input(c):
     inp.fxrel, inp.fyrel = fx, fy
     if c.get("rztang") is not None:
         inp.rztang = c["rztang"]
+    if c.get("bound") is not None:
+        inp.bound = c["bound"]
     if c.get("exrhs") is not None:
         # extra rigid slip, x then y (cntc_setExtraRigidSlip); kept alive with the record
         inp.rztang = 1

@@ -118,3 +118,33 @@ def apply_cntc(cntc, ire, icp, c):
 def to_oracle(O, c):
     """The oracle input record of case `c` (same settings as apply_cntc)."""
     return O.case_input(c)
+
+
+def testbank_chain():
+    """testbank/testbank.inp:245-301, cases 13-16, as case dicts (mm, N): a Cattaneo shift with
+    the Hertzian pressure prescribed (B=2) and Fx prescribed (F=1), then transient rolling
+    (T=2, continuation P=0, regularised warm start I=1) at dq = 0.2 < dx = 0.2343 with Fx
+    prescribed (two steps), then at dq = 0.8 > dx with the creepage prescribed and the Hertz
+    geometry given by curvature and ellipticity (ipotcn -2): the leading-edge correction of
+    src/m_leadedge.f90 is active only in the last case.  Between cases the creepage carries
+    over (the .inp cases 14 and 15 give Fx, not cksi)."""
+    from contact_amd import workloads as W
+    common = dict(norm=1, fn=0.7563, fstat=0.4013, g=1.0, nu=0.28, eps=1e-6, maxgs=80, maxin=100,
+                  gausei=0, mx=41, my=41, scale=1.17143, ceta=0.0, cphi=0.0)
+    c13 = W.base_case(**common, tang=1, pvtime=2, iestim=0, bound=2, force=1, fxrel=0.657,
+                      ipotcn=-3, aa=4.1, bb=4.1, cksi=0.0, veloc=1.0)
+    c14 = W.base_case(**common, tang=2, pvtime=0, iestim=1, bound=0, force=1, fxrel=0.657,
+                      ipotcn=-3, aa=4.1, bb=4.1, cksi=None, dq=0.2, veloc=1000.0)
+    c16 = W.base_case(**common, tang=2, pvtime=0, iestim=1, bound=0, force=0, ipotcn=-2,
+                      a1=0.2963e-2, aob=1.0, cksi=-0.002124, dq=0.8, veloc=1000.0)
+    return [c13, c14, dict(c14), c16]
+
+
+# testbank/testbank.ref_out: per case (creepage cksi (shift for case 13), Fx/(mu Fn), #adhesion,
+# #slip, cumulative TangCG iterations, dFx/dcksi or None) as printed
+TESTBANK_CHAIN_GOLDEN = {
+    13: (-2.432e-02, 0.6570, 485, 488, 74, -22.4),      # :1296, :1330-1339
+    14: (-2.175e-03, 0.6570, 668, 305, 77, -5.27),      # :1407, :1443-1450
+    15: (-2.185e-03, 0.6570, 652, 321, 22, -5.21),      # :1514, :1536-1543
+    16: (-2.124e-03, 0.6563, 610, 363, 15, None),       # :1605, :1642
+}

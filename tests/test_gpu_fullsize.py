@@ -1,7 +1,8 @@
 """Full-size rough non-Hertzian patches (gap = Hertz quadratic + seeded Gaussian roughness,
 RMS 1 um, correlation length 4 elements, Fn = 120 kN, xi = 0.002, eta = 0.001, phi = 0.1 rad/m):
 
-* 256^2 and 512^2 in FP64 against the oracle's results stored in tests/golden (made by
+* 256^2 in FP64 at eps 1e-12 against the oracle's full solution at eps 1e-12, and 512^2 at
+  eps 1e-12 against the oracle's forces and a traction sample (tests/golden, made by
   tests/golden/make_rough_fixture.py);
 * 1024^2 with FP32 products in the FP64 refinement against the FP64 solve of the same patch.
 """
@@ -25,21 +26,48 @@ def solve(cntc, ire, c):
     return rc, f, pn, px, py, el
 
 
-@pytest.mark.parametrize("n", [256, 512])
-def test_rough_patch_fp64_vs_oracle(cntc, n):
-    g = np.load(os.path.join(HERE, "golden", "rough_patch_%d.npz" % n))
-    rc, f, pn, px, py, el = solve(cntc, 40 + n // 256, W.rough_patch(n=n))
-    print("n", n, "gpu", f, cntc.getiterations(40 + n // 256, 1), "oracle", g["forces"], g["iters"])
-    assert rc == 0
-    fo = g["forces"]
-    scale = np.linalg.norm(fo[:3])
-    assert np.max(np.abs(f[:3] - fo[:3])) <= 1e-8 * scale
-    s = g["stride"]
-    pmax = g["sums"][3]
-    for a, b in ((pn, g["pn"]), (px, g["px"]), (py, g["py"])):
-        assert np.max(np.abs(a[::s] - b)) <= 1e-6 * pmax
+def test_rough_patch_256_tight_vs_oracle(cntc):
+    """256^2 at eps 1e-12 against the oracle at eps 1e-12, every element: forces to 1e-9 of each
+    component, tractions to 1e-8 of each component's largest value, element states identical
+    except where an element lies on a state boundary within 1e-8 of the largest pressure."""
+    from tests.test_gpu_configs import force_err, traction_err, states_agree
+    g = np.load(os.path.join(HERE, "golden", "rough_patch_256_eps12.npz"))
+    c = W.rough_patch(n=256, eps=1e-12)
+    rc, f, pn, px, py, el = solve(cntc, 42, c)
+    assert rc == 0 and int(g["rc"]) == 0
+    ref = [g["pn"], g["px"], g["py"]]
+    ef = force_err(f, g["forces"], c["fstat"], c["fn"])
+    ep = traction_err([pn, px, py], ref)
+    ndiff, bad = states_agree(el, g["el"], [pn, px, py], ref, c["fstat"], 1e-8)
+    print("256^2 eps 1e-12: gpu", f, cntc.getiterations(42, 1), "oracle", g["forces"], g["iters"],
+          "force err", ef, "traction err %.2e" % ep, "states differing", ndiff)
+    assert ef.max() <= 1e-9
+    assert ep <= 1e-8
+    assert not bad, bad[:10]
+
+
+def test_rough_patch_512_tight_vs_oracle(cntc):
+    """512^2 at eps 1e-12 against the oracle at eps 1e-12: forces to 1e-9 of each component,
+    tractions to 1e-8 on a sample of every 97th element, element counts and the sampled
+    states identical except on a state boundary within 1e-8 of the largest pressure."""
+    from tests.test_gpu_configs import force_err, traction_err, states_agree
+    g = np.load(os.path.join(HERE, "golden", "rough_patch_512_eps12.npz"))
+    c = W.rough_patch(n=512, eps=1e-12)
+    rc, f, pn, px, py, el = solve(cntc, 43, c)
+    assert rc == 0 and int(g["rc"]) == 0
+    s = int(g["stride"])
+    got = [pn[::s], px[::s], py[::s]]
+    ref = [g["pn"], g["px"], g["py"]]
+    ef = force_err(f, g["forces"], c["fstat"], c["fn"])
+    ep = traction_err(got, ref)
+    ndiff, bad = states_agree(el[::s], g["el"], got, ref, c["fstat"], 1e-8)
     counts = np.bincount(el, minlength=4)
-    assert np.abs(counts - g["counts"]).sum() <= 8, (counts, g["counts"])
+    print("512^2 eps 1e-12: gpu", f, cntc.getiterations(43, 1), "oracle", g["forces"], g["iters"],
+          "force err", ef, "traction err %.2e" % ep, "counts", counts, g["counts"])
+    assert ef.max() <= 1e-9
+    assert ep <= 1e-8
+    assert not bad, bad[:10]
+    assert np.abs(counts - g["counts"]).sum() <= 4, (counts, g["counts"])
 
 
 def test_rough_patch_1024_mixed_vs_fp64(cntc):

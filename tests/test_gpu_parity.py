@@ -77,12 +77,22 @@ def test_table_first_cases(cntc, oracle):
 
 
 def test_steady_rolling_40(cntc, oracle):
-    """40x40 steady rolling Hertz patch at tight tolerance (GDsteady)."""
-    c = C.hertz_case(7.193, 5.411, 40, 40, 1e5, cksi=-0.001, eps=1e-10, maxgs=2000, maxin=20,
+    """The 40x40 steady-rolling Hertz patch (Fn 100 kN, xi = -0.001, GDsteady) at eps 1e-12:
+    Fn, Fx each to 1e-9 of itself (Fy, zero here, to 1e-9 of mu Fn), tractions to 1e-8 of
+    each component's largest value, element states identical."""
+    from tests.test_gpu_configs import traction_err
+    c = C.hertz_case(7.193, 5.411, 40, 40, 1e5, cksi=-0.001, eps=1e-12, maxgs=2000, maxin=20,
                      scale=1.3)
     g = run_gpu(cntc, c)
     o = run_oracle(oracle, c)
-    compare(g, o, 1e-8, 1e-6, "hertz40", max_el_diff=0)
+    assert g[0] == o[0] == 0
+    # Fn, Fx relative to themselves; Fy vanishes by symmetry (eta = phi = 0): relative to mu Fn
+    den = np.array([abs(o[1][0]), abs(o[1][1]), 0.3 * o[1][0]])
+    ef = np.abs(np.array(g[1][:3]) - np.array(o[1][:3])) / den
+    ep = traction_err(g[2], o[2])
+    print("hertz40 eps 1e-12: gpu", g[1], "oracle", o[1], "force err", ef, "traction err %.2e" % ep)
+    assert ef.max() <= 1e-9 and ep <= 1e-8
+    assert np.array_equal(g[3], o[3])
 
 
 def test_batch_equals_single(cntc, oracle):

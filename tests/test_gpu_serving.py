@@ -124,3 +124,28 @@ def test_serving_channels_warm_start(cntc):
     for ch in range(len(base)):
         for s in range(steps):
             assert got[ch][s] == ref[ch][s][1], (ch, s)
+
+
+def test_transient_rolling_reaches_the_steady_state(cntc, oracle):
+    """The GPU's T=2 path (continuation P=0, warm start I=3, rolled coefficients) stepped at fixed
+    creepages reaches the steady-rolling (T=3, SteadyGS) solution of the oracle: the fixed point
+    of transient rolling satisfies the steady-rolling equations."""
+    base = C.hertz_case(6.0, 4.0, 30, 24, 80000.0, cksi=-0.0008, ceta=0.0003, eps=1e-12,
+                        maxgs=2000, maxin=50, gausei=0)
+    P3 = oracle.Problem()
+    assert P3.calculate(*C.to_oracle(oracle, base)) == 0
+    f3, p3, el3 = np.array(P3.forces()), P3.tractions(), P3.eldiv()
+    dist = []
+    for step in range(450):
+        c = dict(base, tang=2, pvtime=2 if step == 0 else 0, iestim=0 if step == 0 else 3)
+        C.apply_cntc(cntc, 31, 1, c)
+        assert cntc.calculate(31, 1) == 0
+        f = np.array(cntc.getcontactforces(31, 1))
+        dist.append(np.abs(f[:3] - f3[:3]).max() / abs(f3[1]))
+    print("GPU T=2 distance to the oracle's T=3 solution after 1, 100, 200, 450 steps:",
+          dist[0], dist[99], dist[199], dist[-1])
+    assert dist[0] > 1e-2 and dist[99] > 1e-5
+    assert dist[-1] <= 1e-10
+    p = [a.ravel() for a in cntc.gettractions(31, 1)]
+    assert max(np.abs(a - b).max() for a, b in zip(p, p3)) <= 1e-9 * np.abs(p3[0]).max()
+    assert np.array_equal(cntc.getelementdivision(31, 1).ravel(), el3)

@@ -326,3 +326,100 @@ def test_hertz_ellipticity_option(oracle, aob):
     assert rc == 0
     h3 = P3.hertz()
     assert h3[0] == pytest.approx(a1, rel=1e-7) and h3[1] == pytest.approx(b1, rel=1e-7)
+
+
+def test_displacements_close_the_gap_and_the_shift(oracle):
+    """The displacement differences us = A ps of a solution (soutpt, src/m_soutpt.f90:371-397)
+    satisfy the contact conditions they were solved for: in contact the deformed gap
+    h + un - pen is closed, and in adhesion under the Cattaneo shift (T=1, dq=1) the
+    tangential displacement cancels the rigid shift: ux = -cksi, uy = 0."""
+    c = C.cattaneo_case()
+    c["eps"] = 1e-12
+    P = oracle.Problem()
+    inp, h = C.to_oracle(oracle, c)
+    assert P.calculate(inp, h) == 0
+    un, ux, uy = P.displacements()
+    el = P.eldiv()
+    hs, pen = P.h(), P.pen()
+    con, adh = el > 0, el == 1
+    assert adh.sum() > 20
+    assert np.abs(un[con] + hs[con] - pen).max() <= 1e-12 * pen
+    assert np.abs(ux[adh] + c["cksi"]).max() <= 1e-12 * c["cksi"]
+    assert np.abs(uy[adh]).max() <= 1e-12 * c["cksi"]
+    assert np.all(un[~con] == 0.0) and np.all(ux[~con] == 0.0)
+
+
+# ---- transient rolling (T=2) pinned by its steady state --------------------------------------
+
+def test_transient_rolling_reaches_the_steady_state(oracle):
+    """Transient rolling (T=2, dq = dx) stepped with continuation (P=0: pv = ps, cv rolled by dq)
+    and warm starts (I=3) at fixed creepages reaches a fixed point; its equations are then those
+    of steady rolling (T=3), whose SteadyGS solution the table goldens pin.  So the fixed point
+    must equal the T=3 solution: this pins set_prev_data, the rolled coefficients cv and csv,
+    the pv terms of stang_rhs and TangCG's warm start (src/m_sdis.f90:208-317,
+    src/m_visc.f90:319-358, src/m_stang.f90:749-951)."""
+    base = C.hertz_case(6.0, 4.0, 30, 24, 80000.0, cksi=-0.0008, ceta=0.0003, eps=1e-12,
+                        maxgs=2000, maxin=50, gausei=0)
+    P3 = oracle.Problem()
+    assert P3.calculate(*C.to_oracle(oracle, base)) == 0
+    f3, p3, el3 = np.array(P3.forces()), P3.tractions(), P3.eldiv()
+    P = oracle.Problem()
+    dist = []
+    for step in range(450):     # the distance to the fixed point falls by ~0.93 per step
+        c = dict(base, tang=2, pvtime=2 if step == 0 else 0, iestim=0 if step == 0 else 3)
+        assert P.calculate(*C.to_oracle(oracle, c)) == 0
+        f = np.array(P.forces())
+        dist.append(np.abs(f[:3] - f3[:3]).max() / abs(f3[1]))
+    print("T=2 distance to the T=3 solution after 1, 100, 200, 450 steps:", dist[0], dist[99],
+          dist[199], dist[-1])
+    assert dist[0] > 1e-2 and dist[99] > 1e-5      # genuinely transient at first
+    assert dist[-1] <= 1e-12
+    p = P.tractions()
+    assert max(np.abs(a - b).max() for a, b in zip(p, p3)) <= 1e-10 * np.abs(p3[0]).max()
+    assert np.array_equal(P.eldiv(), el3)
+
+
+def test_testbank_transient_chain_goldens(oracle):
+    """testbank cases 13-16 (testbank/testbank.inp:245-301) against the printed results
+    of testbank/testbank.ref_out: B=2 (prescribed Hertz pressure) shift with Fx prescribed, then
+    transient rolling with continuation and I=1 at dq < dx (two steps) and at dq > dx, where the
+    leading-edge correction (sxbnd/subnd) enters the right-hand side.  Pins the oracle's T=2 path,
+    its force-prescribed Newton iteration and sensitivity, the Hertz option -2 and B=2."""
+    P = oracle.Problem()
+    for k, c in zip((13, 14, 15, 16), C.testbank_chain()):
+        c = dict(c)
+        if c["cksi"] is None:
+            c["cksi"] = P.creepages()[0]
+        assert P.calculate(*C.to_oracle(oracle, c)) == 0, k
+        fn, fx, fy, mz = P.forces()
+        el = P.eldiv()
+        cksi = P.creepages()[0]
+        g = C.TESTBANK_CHAIN_GOLDEN[k]
+        got = (cksi, fx / (c["fstat"] * fn), int((el == 1).sum()), int((el == 2).sum()), P.iters()[3],
+               P.sens()[1][1])
+        print(k, got)
+        assert abs(got[0] - g[0]) <= 0.6e-3 * abs(g[0]) + 1e-15, (k, got, g)   # 4 printed digits
+        assert abs(got[1] - g[1]) <= 0.6e-4, (k, got, g)
+        assert got[2:5] == g[2:5], (k, got, g)
+        if g[5] is not None:
+            assert abs(got[5] - g[5]) <= 0.06 if abs(g[5]) > 10 else abs(got[5] - g[5]) <= 0.006
+
+
+def test_line_contact_hertz_options(oracle):
+    """The 2D Hertz options are inverse to each other (hzcalc2d, src/m_hertz.f90:217-263): -5
+    with half-length aa gives the curvature a1 for which -4 gives back aa; the prescribed
+    semi-elliptical pressure (B=2) carries the normal force (its discrete integral to the
+    grid's resolution) and is uniform in y over |y| <= bb."""
+    from contact_amd import workloads as W
+    base = dict(mx=40, my=20, bb=3.0, scale=1.2, fn=5e4, eps=1e-10, maxgs=500, maxin=30, tang=0)
+    rc, _, P5 = oracle.solve_case(W.base_case(ipotcn=-5, aa=5.0, **base))
+    assert rc == 0
+    a1 = P5.hertz()[0]
+    rc, _, P4 = oracle.solve_case(W.base_case(ipotcn=-4, a1=a1, **base))
+    assert rc == 0
+    assert P4.hertz()[2] == pytest.approx(5.0, rel=1e-14)
+    pn = P5.tractions()[0].reshape(20, 40)
+    xc1, yc1, dx, dy = P5.grid()
+    assert pn.sum() * dx * dy == pytest.approx(5e4, rel=2e-2)
+    inside = np.abs(yc1 + np.arange(20) * dy) <= 3.0
+    assert np.allclose(pn[inside], pn[inside][0], rtol=0, atol=0) and np.all(pn[~inside] == 0)
