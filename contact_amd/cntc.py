@@ -55,6 +55,7 @@ _PROTOS = {
     "cntc_getdisplacements": [_pi, _pi, _pi, _pd, _pd, _pd],
     "cntc_calculate": [_pi, _pi, _pi],
     "cntc_calculatebatch": [_pi, _pi, _pi, _pi],
+    "cntc_calculatenoalloc": [_pi, _pi, _pi],
     "cntc_servestart": [_pd, _pi],
     "cntc_submit": [_pi, _pi, _pi],
     "cntc_poll": [_pi, _pi, _pi, _pd],
@@ -247,6 +248,13 @@ def getdisplacements(ire=1, icp=1):
 def calculate(ire=1, icp=1, idebug=1):
     ierr = c_int(-999)
     lib().cntc_calculate(_i(ire), _i(icp), ctypes.byref(ierr))
+    return ierr.value
+
+
+def calculatenoalloc(ire=1, icp=1):
+    """cntc_calculate without device allocation (buffers from the library's fixed array)."""
+    ierr = c_int(-999)
+    lib().cntc_calculatenoalloc(_i(ire), _i(icp), ctypes.byref(ierr))
     return ierr.value
 
 

@@ -139,8 +139,8 @@ void cntc_initialize(int* ire, int* imodul, int* ifcver, int* ierror, const char
         return;
     }
     store().ire_module[*ire] = 3;
-    // set-up: the engine (stream, staging buffers) is made here when a GPU is present, so
-    // that later calls need not allocate it
+    // set-up: the engine (stream, staging buffers, the no-allocation array's address) is made
+    // here when a GPU is present, so that later calls need not allocate it
     (void)engine();
     if (ierror) *ierror = 0;
 }
@@ -547,6 +547,46 @@ void cntc_calculate(int* ire, int* icp, int* ierror)
     int e = 0;
     calculate_many(1, ire, icp, &e);
     *ierror = e;
+}
+
+// No counterpart in the reference: cntc_calculate without device allocation, for callers that
+// cannot allocate (code under graph capture, tight serving loops).  Every device buffer of the
+// solve comes from the engine's fixed array (64 MiB per GPU); the solution is then kept in
+// host memory with the problem.  Needs the engine made by an earlier call (cntc_initialize);
+// rejects patches whose buffers do not fit, and calls while the serving loop runs.
+void cntc_calculatenoalloc(int* ire, int* icp, int* ierror)
+{
+    std::lock_guard<std::mutex> g(store().mtx);
+    Store& S = store();
+    *ierror = 0;
+    Problem* P = activate(*ire, *icp);
+    if (!P) { *ierror = CNTC_err_icp; return; }
+    Engine* E = S.engine.get();
+    if (!E) {
+        if (S.idebug > 0) std::fprintf(stderr, "cntc_calculatenoalloc: call cntc_initialize first\n");
+        *ierror = CNTC_err_other;
+        return;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    std::clock_t c0 = std::clock();
+    int rc = 0;
+    try {
+        rc = E->solve_noalloc(&P->patch);
+    } catch (const std::exception& ex) {
+        if (S.idebug > 0) std::fprintf(stderr, "cntc_calculatenoalloc: %s\n", ex.what());
+        P->patch.out.ierror = CNTC_err_other;
+    }
+    if (rc < 0) {
+        if (S.idebug > 0) std::fprintf(stderr, "cntc_calculatenoalloc: %s\n", E->last_error.c_str());
+        *ierror = CNTC_err_other;
+        return;
+    }
+    *ierror = P->patch.out.ierror;
+    P->calculated = true;
+    P->ncase++;
+    P->twall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    P->tcpu += double(std::clock() - c0) / CLOCKS_PER_SEC;
+    if (S.idebug > 0 && !E->last_error.empty()) std::fprintf(stderr, "cntc_calculatenoalloc: %s\n", E->last_error.c_str());
 }
 
 void cntc_calculatebatch(int* nprob, const int* ire, const int* icp, int* ierror)

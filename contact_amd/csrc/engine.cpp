@@ -190,6 +190,19 @@ struct Req {
 // side stream, so that the batched launches of the other patches are not held behind them
 static bool is_async_kind(int kind) { return kind == OP_STDYGS; }
 
+// Device buffers of the scheduler's flushes: item staging, group partials, results and the
+// per-item arrival counters of the fused reductions
+struct FlushBufs {
+    char* stage = nullptr;
+    size_t stage_cap = 0;
+    double* part = nullptr;
+    size_t part_cap = 0;      // doubles
+    double* res = nullptr;
+    size_t res_cap = 0;       // doubles
+    unsigned* cnt = nullptr;
+    size_t cnt_cap = 0;       // counters
+};
+
 class Scheduler {
 public:
     explicit Scheduler(hipStream_t s) : s_(s)
@@ -238,6 +251,35 @@ public:
     // flush), so that the largest patches of a batch, which set its length, are not held to
     // the pace of every other patch's work; each patch's own sequence of operations is unchanged
     void set_priority(int slot, bool p) { prio_[slot] = p ? 1 : 0; }
+    // Fixed buffers: flushes use `b` (no growth: a flush that needs more throws) and every
+    // operation, SteadyGS included, runs on the main stream (no side-stream buffers).  Used by
+    // the entry point that allocates nothing; restore() returns to the scheduler's own buffers.
+    void use_fixed(const FlushBufs& b)
+    {
+        saved_ = FlushBufs{dstage_, cap_, dpart_, partcap_, dres_, rescap_, dcnt_, cntcap_};
+        dstage_ = b.stage;
+        dpart_ = b.part;
+        dres_ = b.res;
+        dcnt_ = b.cnt;
+        fixed_stage_cap_ = std::min(b.stage_cap, cap_);
+        fixed_part_cap_ = b.part_cap;
+        fixed_res_cap_ = std::min(b.res_cap, rescap_);
+        fixed_cnt_cap_ = b.cnt_cap;
+        fixed_ = true;
+    }
+    void restore()
+    {
+        if (!fixed_) return;
+        dstage_ = saved_.stage;
+        dpart_ = saved_.part;
+        dres_ = saved_.res;
+        dcnt_ = saved_.cnt;
+        fixed_ = false;
+    }
+    bool fixed() const { return fixed_; }
+    // the end of the pinned staging buffer, which fixed-mode flushes never reach (they use
+    // at most the fixed staging capacity from the start)
+    char* host_tail(size_t bytes) { return hstage_ + ((cap_ - bytes) & ~(size_t)255); }
     int prio_div() const { return PRIO_DIV; }
     void enqueue(int slot, int kind, const AnyItem& it)
     {
@@ -560,7 +602,7 @@ private:
             for (size_t sl = 0; sl < q_.size(); sl++) {
                 auto& q = q_[sl];
                 if (taken(sl) && lev < q.size()) {
-                    if (is_async_kind(q[lev].kind)) {
+                    if (is_async_kind(q[lev].kind) && !fixed_) {
                         async_reqs.push_back(&q[lev]);
                         continue;
                     }
@@ -593,13 +635,21 @@ private:
                 size_t isz = item_size(kind);
                 int nitems = (int)g.reqs.size();
                 size_t items_off = off;
-                if (off + isz * nitems > cap_) throw std::runtime_error("scheduler: staging overflow");
+                if (off + isz * nitems > stage_cap()) throw std::runtime_error("scheduler: staging overflow");
                 for (int i = 0; i < nitems; i++)
                     std::memcpy(hstage_ + off + isz * i, &flat[g.reqs[i]]->item, isz);
                 off = align(off + isz * nitems);
                 int K = op_nred(kind, &flat[g.reqs[0]]->item);
                 int nst = (kind == OP_CONV || kind == OP_CONV32) ? 3 : 1;
-                for (int st = 0; st < nst; st++) {
+                int st0 = 0;
+                if (kind == OP_STDYGS) {
+                    // SteadyGS on the main stream (fixed-buffer mode): the overlapped variant
+                    // where every item's LDS fits, else the staged one, else the direct one
+                    st0 = 2;
+                    for (int i = 0; i < nitems; i++)
+                        while (st0 > 0 && op_lds(kind, &flat[g.reqs[i]]->item, st0) + 2048 > MAX_LDS) st0--;
+                }
+                for (int st = st0; st < st0 + nst; st++) {
                     Launch L;
                     L.kind = kind;
                     L.nitems = nitems;
@@ -636,24 +686,26 @@ private:
                             resmap.push_back({flat[g.reqs[i]], res + (size_t)i * K});
                         res += (size_t)nitems * K;
                     }
-                    if (off > cap_) throw std::runtime_error("scheduler: staging overflow");
+                    if (off > stage_cap()) throw std::runtime_error("scheduler: staging overflow");
                     launches.push_back(L);
                 }
             }
-        if (part > partcap_) {
+        if (fixed_ && (part > fixed_part_cap_ || cnt > fixed_cnt_cap_ || res > fixed_res_cap_))
+            throw std::runtime_error("scheduler: a flush needs more than the fixed buffers hold");
+        if (!fixed_ && part > partcap_) {
             HIPCHK(hipStreamSynchronize(s_));
             HIPCHK(hipFree(dpart_));
             partcap_ = part * 2;
             HIPCHK(hipMalloc((void**)&dpart_, partcap_ * sizeof(double)));
         }
-        if (cnt > cntcap_) {
+        if (!fixed_ && cnt > cntcap_) {
             HIPCHK(hipStreamSynchronize(s_));
             HIPCHK(hipFree(dcnt_));
             cntcap_ = cnt * 2;
             HIPCHK(hipMalloc((void**)&dcnt_, cntcap_ * sizeof(unsigned)));
             HIPCHK(hipMemset(dcnt_, 0, cntcap_ * sizeof(unsigned)));
         }
-        if (res > rescap_) {
+        if (!fixed_ && res > rescap_) {
             HIPCHK(hipStreamSynchronize(s_));
             HIPCHK(hipHostFree(hres_));
             HIPCHK(hipFree(dres_));
@@ -684,6 +736,8 @@ private:
             }
             RedOut ro{dpart_ + L.part_off, L.K > 0 ? dcnt_ + L.cnt_off : nullptr, dres_ + L.res_off,
                       (const int*)(dstage_ + L.modes_off)};
+            // SteadyGS writes its one group's values as the item's results
+            if (L.kind == OP_STDYGS) ro = RedOut{dres_ + L.res_off, nullptr, nullptr, nullptr};
             launch_op(L.kind, dstage_ + L.items_off, (const int*)(dstage_ + L.gstart_off), L.nitems,
                       L.ngroups, ro, L.stage, L.lds, s_);
             if (hipError_t le = hipGetLastError(); le != hipSuccess) {
@@ -751,6 +805,11 @@ private:
     // one arrival counter per reduction item of a flush; each item's last group resets its own
     unsigned* dcnt_ = nullptr;
     size_t cntcap_ = 1 << 16;
+    // fixed-buffer mode (use_fixed): caps of the caller's buffers, the scheduler's own saved
+    bool fixed_ = false;
+    size_t fixed_stage_cap_ = 0, fixed_part_cap_ = 0, fixed_res_cap_ = 0, fixed_cnt_cap_ = 0;
+    FlushBufs saved_;
+    size_t stage_cap() const { return fixed_ ? fixed_stage_cap_ : cap_; }
 
 public:
     long nflush = 0;
@@ -898,10 +957,31 @@ struct Ctx {
 // engine basics
 // ------------------------------------------------------------------------------------------
 
+// The fixed device array of the solve that allocates nothing (cntc_calculatenoalloc): 64 MiB
+// per GPU, held until the process ends.  Host code never uses the name for its memory: the
+// engine looks up the array's device address on its GPU at set-up and uses that address.
+// Layout per call: the patch arena and its twiddle tables, then the scheduler's staging,
+// partials, results and arrival counters.
+constexpr size_t NOALLOC_BYTES = 64ull << 20;
+constexpr size_t NOALLOC_STAGE = 4ull << 20, NOALLOC_PART = 2ull << 20, NOALLOC_RES = 1ull << 20,
+                 NOALLOC_CNT = 256ull << 10;
+constexpr size_t NOALLOC_ARENA = NOALLOC_BYTES - NOALLOC_STAGE - NOALLOC_PART - NOALLOC_RES - NOALLOC_CNT;
+static __device__ __align__(256) char scratch[NOALLOC_BYTES];
+
+size_t Engine::noalloc_bytes() { return NOALLOC_BYTES; }
+size_t Engine::noalloc_patch_limit() { return NOALLOC_ARENA; }
+
 Engine::Engine()
 {
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     sched_.reset(new Scheduler(stream_));
+    // set-up of the no-allocation path on this GPU (the current device): the fixed array's
+    // device address
+    hipError_t e = hipGetSymbolAddress((void**)&noalloc_base_, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        noalloc_base_ = nullptr;
+        (void)hipGetLastError();
+    }
 }
 
 Engine::~Engine()
@@ -913,6 +993,17 @@ Engine::~Engine()
     sched_.reset();
     for (auto& t : tw_) (void)hipFree(t.second);
     if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// twiddle table exp(-2 pi i e / L), e < L, into host memory h (2L values)
+template <typename T>
+static void twiddle_table(int L, T* h)
+{
+    for (int e = 0; e < L; e++) {
+        long double a = -2.0L * 3.141592653589793238462643383279502884L * e / L;
+        h[2 * e] = (T)cosl(a);
+        h[2 * e + 1] = (T)sinl(a);
+    }
 }
 
 const void* Engine::twiddles(int L, bool fp32)
@@ -1291,8 +1382,28 @@ bool Engine::prepare(Patch& P)
     size_t ws = std::max((size_t)3 * P.Ly * H, pH * pL) * 16;
     size_t emb = std::max((size_t)P.Lx * P.Ly, (size_t)(2 * mx) * (2 * my)) * 8;
     size_t need = patch_bytes(mx, my);
-    P.mem.ensure(need);
-    char* b = P.mem.base;
+    // twiddle tables of the no-allocation path, after the arena: FP64 and FP32 at M, Lx, Ly,
+    // FP64 at mx, 2mx, 2my (the preconditioner's transform)
+    const int twl[9] = {M, P.Lx, P.Ly, M, P.Lx, P.Ly, mx, 2 * mx, 2 * my};
+    const int twsz[9] = {16, 16, 16, 8, 8, 8, 16, 16, 16};
+    size_t twbytes = 0;
+    for (int k = 0; k < 9; k++) twbytes += al((size_t)twl[k] * twsz[k]);
+    char* b;
+    if (noalloc_) {
+        if (need + twbytes > NOALLOC_ARENA) {
+            R.ierror = ERR_OTHER;
+            char msg[160];
+            std::snprintf(msg, sizeof msg, "a %dx%d patch needs %zu bytes, more than the %zu of the "
+                          "no-allocation array", mx, my, need + twbytes, NOALLOC_ARENA);
+            last_error = msg;
+            return false;
+        }
+        b = noalloc_base_;
+    } else {
+        P.mem.ensure(need);
+        b = P.mem.base;
+    }
+    char* const arena = b;
     auto take = [&](size_t bytes) { char* r = b; b += al(bytes); return r; };
     P.el = (int8_t*)take(n);
     double* d = (double*)take(42 * n * 8);
@@ -1313,14 +1424,44 @@ bool Engine::prepare(Patch& P)
     P.embed = (double*)take(3 * emb);
     P.wsS = take(ws);
     P.wsT = take(ws);
-    P.twM = twiddles(M, false); P.twLx = twiddles(P.Lx, false); P.twLy = twiddles(P.Ly, false);
-    P.twM32 = twiddles(M, true); P.twLx32 = twiddles(P.Lx, true); P.twLy32 = twiddles(P.Ly, true);
-    P.ptwM = twiddles(mx, false); P.ptwLx = twiddles(2 * mx, false); P.ptwLy = twiddles(2 * my, false);
+    if (noalloc_) {
+        // tables computed into the pinned staging tail and copied into the array
+        char* host = sched_->host_tail(twbytes);
+        char* dev = b;
+        const void** dst[9] = {&P.twM, &P.twLx, &P.twLy, &P.twM32, &P.twLx32, &P.twLy32,
+                               &P.ptwM, &P.ptwLx, &P.ptwLy};
+        size_t o = 0;
+        for (int k = 0; k < 9; k++) {
+            if (twsz[k] == 16) twiddle_table(twl[k], (double*)(host + o));
+            else twiddle_table(twl[k], (float*)(host + o));
+            *dst[k] = dev + o;
+            o += al((size_t)twl[k] * twsz[k]);
+        }
+        HIPCHK(hipMemcpyAsync(dev, host, twbytes, hipMemcpyHostToDevice, stream_));
+    } else {
+        P.twM = twiddles(M, false); P.twLx = twiddles(P.Lx, false); P.twLy = twiddles(P.Ly, false);
+        P.twM32 = twiddles(M, true); P.twLx32 = twiddles(P.Lx, true); P.twLy32 = twiddles(P.Ly, true);
+        P.ptwM = twiddles(mx, false); P.ptwLx = twiddles(2 * mx, false); P.ptwLy = twiddles(2 * my, false);
+    }
+    // The element states and the first 8 grid functions (tractions, slip, previous tractions)
+    // carry a problem's state to its next case.  They are in this arena unless the last solve
+    // allocated nothing (then on the host) or used the other memory (arena vs fixed array).
+    const size_t lead = al(n) + 8 * n * 8;   // the arena's leading block: el, pn .. pvn
     if (newgrid) {
         HIPCHK(hipMemsetAsync(P.el, 0, n, stream_));
         HIPCHK(hipMemsetAsync(P.pn, 0, 8 * n * 8, stream_));
         P.have_solution = false;
+        P.state_on_host = false;
+    } else if (P.state_on_host && P.host_state.size() >= lead) {
+        HIPCHK(hipMemcpyAsync(arena, P.host_state.data(), lead, hipMemcpyHostToDevice, stream_));
+    } else if (noalloc_ && P.mem.base && P.mem.bytes >= lead) {
+        HIPCHK(hipMemcpyAsync(arena, P.mem.base, lead, hipMemcpyDeviceToDevice, stream_));
+    } else if (noalloc_) {
+        HIPCHK(hipMemsetAsync(P.el, 0, n, stream_));
+        HIPCHK(hipMemsetAsync(P.pn, 0, 8 * n * 8, stream_));
+        P.have_solution = false;
     }
+    if (!noalloc_) P.state_on_host = false;
     if (ibase == 9)
         HIPCHK(hipMemcpyAsync(P.h, P.h_host.data(), n * 8, hipMemcpyHostToDevice, stream_));
     if (in.rztang == 1) {
@@ -2777,6 +2918,53 @@ static Task calculate(Ctx C)
     co_return R.ierror;
 }
 
+int Engine::solve_noalloc(Patch* p)
+{
+    last_error.clear();
+    if (!noalloc_base_) {
+        last_error = "the no-allocation array is not available on this GPU";
+        return -1;
+    }
+    if (serving_) {
+        last_error = "the serving loop is running";
+        return -1;
+    }
+    char* base = noalloc_base_ + NOALLOC_ARENA;
+    FlushBufs fb;
+    fb.stage = base;
+    fb.stage_cap = NOALLOC_STAGE;
+    fb.part = (double*)(base + NOALLOC_STAGE);
+    fb.part_cap = NOALLOC_PART / sizeof(double);
+    fb.res = (double*)(base + NOALLOC_STAGE + NOALLOC_PART);
+    fb.res_cap = NOALLOC_RES / sizeof(double);
+    fb.cnt = (unsigned*)(base + NOALLOC_STAGE + NOALLOC_PART + NOALLOC_RES);
+    fb.cnt_cap = NOALLOC_CNT / sizeof(unsigned);
+    // the array keeps the last call's contents: the arrival counters must start at zero
+    launch_fill_u32(fb.cnt, fb.cnt_cap, 0u, stream_);
+    HIPCHK(hipGetLastError());
+    sched_->use_fixed(fb);
+    noalloc_ = true;
+    int rc = 0;
+    try {
+        rc = solve({p});
+    } catch (...) {
+        noalloc_ = false;
+        sched_->restore();
+        throw;
+    }
+    noalloc_ = false;
+    sched_->restore();
+    if (p->out.ierror == 0 || p->have_solution) {
+        // keep the solution with the problem: the array is reused by the next call
+        const size_t n = p->n, lead = ((n + 255) & ~(size_t)255) + 40 * n * 8;
+        p->host_state.resize(lead);
+        HIPCHK(hipMemcpyAsync(p->host_state.data(), p->el, lead, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        p->state_on_host = true;
+    }
+    return rc;
+}
+
 int Engine::solve(const std::vector<Patch*>& patches)
 {
     last_error.clear();
@@ -3011,6 +3199,14 @@ long Engine::rounds() const
 
 void Engine::get_field(const Patch& p, int what, double* out) const
 {
+    if (p.state_on_host && p.n > 0) {
+        // grid functions in arena order: pn px py sx sy pvx pvy pvn h hx hy ws.. w[24] un ux uy
+        static const int slot[11] = {0, 1, 2, 3, 4, 8, 37, 38, 39, 9, 10};
+        if (what < 0 || what > 10) return;
+        const size_t n = p.n, off = ((n + 255) & ~(size_t)255) + (size_t)slot[what] * n * 8;
+        if (p.host_state.size() >= off + n * 8) std::memcpy(out, p.host_state.data() + off, n * 8);
+        return;
+    }
     const double* src = nullptr;
     switch (what) {
     case 0: src = p.pn; break;
@@ -3033,6 +3229,10 @@ void Engine::get_field(const Patch& p, int what, double* out) const
 void Engine::get_eldiv(const Patch& p, int* out) const
 {
     if (p.n <= 0) return;
+    if (p.state_on_host && p.host_state.size() >= (size_t)p.n) {
+        for (int i = 0; i < p.n; i++) out[i] = (int8_t)p.host_state[i];
+        return;
+    }
     std::vector<int8_t> e(p.n);
     HIPCHK(hipMemcpyAsync(e.data(), p.el, p.n, hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));

@@ -121,6 +121,11 @@ struct Patch {
     const void *twM = nullptr, *twLx = nullptr, *twLy = nullptr;
     const void *twM32 = nullptr, *twLx32 = nullptr, *twLy32 = nullptr;
     const void *ptwM = nullptr, *ptwLx = nullptr, *ptwLy = nullptr;   // preconditioner sizes
+    // After a solve that allocated nothing (Engine::solve_noalloc) the solution lives here, as
+    // the leading block of the arena (element states, then the 40 grid functions) laid out as
+    // in device memory; the next solve that uses the arena loads it back.
+    std::vector<char> host_state;
+    bool state_on_host = false;
 };
 
 class Scheduler;
@@ -144,6 +149,14 @@ public:
     // Solve one case for every patch (each with its own inputs).  Returns 0 when every case
     // ran; individual outcomes are in Patch::out.
     int solve(const std::vector<Patch*>& patches);
+    // Solve one case allocating no device memory: every device buffer of the solve (the
+    // patch's arena, twiddle tables, staging, partials, counters, results) is carved from a
+    // fixed array of noalloc_bytes() per GPU, set up with the engine.  Returns 0 when the case
+    // ran (outcome in Patch::out), -1 when it cannot run this way (last_error says why: the
+    // patch does not fit, the serving loop is running).
+    int solve_noalloc(Patch* p);
+    static size_t noalloc_bytes();
+    static size_t noalloc_patch_limit();   // bytes of arena and twiddle tables that fit
     hipStream_t stream() const { return stream_; }
     // download the element data of a patch
     void get_field(const Patch& p, int what, double* out) const;
@@ -171,6 +184,8 @@ public:
 
 private:
     bool prepare(Patch& p);
+    char* noalloc_base_ = nullptr;   // device address of the fixed array on this engine's GPU
+    bool noalloc_ = false;           // prepare carves the arena from the fixed array
     void serve_loop();
     Patch* tickets_at(int id);
     std::thread server_;

@@ -3186,6 +3186,20 @@ size_t op_lds(int kind, const void* item, int stage)
     return 0;
 }
 
+__global__ void k_fill_u32(unsigned* p, size_t n, unsigned v)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s)
+{
+    if (n == 0) return;
+    size_t g = (n + 255) / 256;
+    if (g > 1024) g = 1024;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)g), dim3(256), 0, s, p, n, v);
+}
+
 void launch_rollcoef(int n, const double* cs, double* cv, hipStream_t s)
 {
     int g = (n + 255) / 256;

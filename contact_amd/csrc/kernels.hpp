@@ -29,6 +29,8 @@ void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, i
                RedOut ro, int conv_stage, size_t lds_bytes, hipStream_t s);
 
 void launch_rollcoef(int n, const double* cs, double* cv, hipStream_t s);
+// p[0..n) = v, by a kernel (usable where a caller may capture the stream into a graph)
+void launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s);
 void launch_to_float(size_t n, const double* in, float* out, hipStream_t s);
 
 }  // namespace cntc

@@ -97,6 +97,19 @@ void cntc_calculate(int *ire, int *icp, int *ierror);
    (src/test_table.f90:196-292); ierror[k] is what cntc_calculate would return. */
 void cntc_calculatebatch(int *nprob, const int *ire, const int *icp, int *ierror);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops).  Solves the current case of (ire, icp) as
+   cntc_calculate does, allocating nothing on the device: every device buffer of the solve
+   (the patch's grid functions, coefficients, spectra and FFT workspaces, twiddle tables, and
+   the staging, partials, counters and results of the batched launches) is carved from one
+   fixed array of 64 MiB per GPU, reserved when the library sets up (cntc_initialize must have
+   been called).  Limit: patches whose buffers exceed 56 MiB (about 160 x 160 elements) return
+   CNTC_err_other, as do calls while the serving loop runs.  The solution is kept in host memory
+   with the problem, so getters and later cases (continuation, warm starts) work as after
+   cntc_calculate.  Calls share the array: two calls overlapping on one GPU would race (the
+   library serialises its calls within a process). */
+void cntc_calculatenoalloc(int *ire, int *icp, int *ierror);
+
 /* ---- continuous batching (no counterpart in the reference) ------------------------------ */
 
 /* For a multibody caller that issues patch solves as they arise.  cntc_servestart starts the
