@@ -407,7 +407,7 @@ def main():
         return submit_batch(cntc, idl, label)
 
     # time kept back for the legs after the timed steps (plain FP64 subset, serving, report)
-    reserve = 45.0 if args.plain > 0 or args.serve_seconds > 0 else 5.0
+    reserve = 60.0 if args.plain > 0 or args.serve_seconds > 0 else 5.0
 
     def run_steps(nsteps, label, can_start):
         """Up to `nsteps` batches, two in flight; `can_start(done_latencies)` decides whether
@@ -470,7 +470,8 @@ def main():
     print("# plain fp64 subset: up to %d patches" % len(sub_all), file=sys.stderr, flush=True)
     t_plain, FP, sub = [], [], []
     for k in sub_all:
-        if left() < 12.0:
+        # keep time for the serving leg
+        if left() < (30.0 if args.serve_seconds > 0 else 12.0):
             break
         W.apply_case(cntc, *fp64[k], cases_fp64[k])
         t1 = time.perf_counter()

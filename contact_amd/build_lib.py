@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcntc_amd.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["kernels.hip", "engine.cpp", "cntc_api.cpp"]
+SOURCES = ["kernels.hip", "engine.cpp", "cntc_api.cpp", "inp_writer.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++20", "-fPIC", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -29,7 +29,7 @@ def _compile(src):
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    with concurrent.futures.ThreadPoolExecutor(max_workers=3) as ex:
+    with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
         results = list(ex.map(_compile, SOURCES))
     for obj, out in results:
         if out and verbose:

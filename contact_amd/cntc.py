@@ -408,7 +408,8 @@ OP_NAMES = ["vec", "ncg_negpn", "ncg_dd", "ncg_expand", "ncg_hmin", "sn_contract
             "sn_final", "aij_ones", "tcg_nt", "tcg_rset", "tcg_zprep", "tcg_qpost", "tcg_pupd",
             "tcg_as", "tcg_sa", "st_as", "st_sa", "gd_dp", "gd_project", "gd_probe", "gd_update",
             "gd_moves", "gd_diagres", "conv", "conv32", "elascf", "embed", "extract", "setup",
-            "forces", "stdygs", "guard", "sens", "sens_ext", "ledge"]
+            "forces", "stdygs", "guard", "sens", "sens_ext", "ledge", "gsb_setup", "gsb_walk",
+            "gsb_check"]
 
 
 def amdprofile(on):
@@ -435,6 +436,8 @@ def amdprofileget(lenarr=256):
             name = "stdygs_staged"
         elif name == "stdygs" and stage == 2:
             name = "stdygs_overlap"
+        elif name == "stdygs" and stage == 3:
+            name = "stdygs_block"
         out[name] = (float(sec[k]), float(byt[k]), int(nl[k]))
     return out
 

@@ -13,10 +13,12 @@
 #include <memory>
 #include <mutex>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "../../include/cntc_amd.h"
 #include "engine.hpp"
+#include "inp_writer.hpp"
 
 namespace {
 
@@ -58,6 +60,16 @@ struct Store {
     std::map<int, int> ire_module;
     int precision = 0;
     int idebug = 0;
+    // output folder and experiment name of the first initialise call, and the .inp file
+    // (src/contact_addon.f90:289-318, src/m_global_data.f90:41)
+    bool names_set = false;
+    std::string outdir, expnam = "contact_addon";
+    std::FILE* inp = nullptr;
+    int inp_open = 0;                // 1 open, -1 could not be opened
+    ~Store()
+    {
+        if (inp) std::fclose(inp);
+    }
 };
 
 Store& store()
@@ -67,6 +79,29 @@ Store& store()
 }
 
 const int MAX_REID = 999, MAX_CPID = 9;
+
+// the folder names of the first initialise call (copy_c_dirnam and the experiment name,
+// src/contact_addon.f90:289-318, 576-601): a separator is appended to a folder name, and a
+// directory part of the experiment name is dropped
+void set_names(const char* c_outdir, int len_outdir, const char* c_expnam, int len_expnam)
+{
+    Store& S = store();
+    if (S.names_set) return;
+    S.names_set = true;
+    auto cstr = [](const char* c, int len) {
+        std::string r;
+        for (int i = 0; c && i < len && c[i]; i++) r += c[i];
+        while (!r.empty() && r.back() == ' ') r.pop_back();
+        return r;
+    };
+    S.outdir = cstr(c_outdir, len_outdir);
+    if (S.outdir.size() > 1 && S.outdir.back() != '/') S.outdir += '/';
+    const std::string e = cstr(c_expnam, len_expnam);
+    if (!e.empty() && e[0] != ' ') {
+        const size_t k = e.find_last_of('/');
+        S.expnam = k == std::string::npos ? e : e.substr(k + 1);
+    }
+}
 
 // wait for the in-flight request of P (serving loop) and record its outcome
 void settle(Problem* P)
@@ -122,17 +157,22 @@ Engine* engine()
 
 extern "C" {
 
-void cntc_initializefirst(int* ifcver, int* ierror, int* ioutput, const char*, const char*, const char*,
-                          int*, int*, int*)
+void cntc_initializefirst(int* ifcver, int* ierror, int* ioutput, const char* c_wrkdir, const char* c_outdir,
+                          const char* c_expnam, int* len_wrkdir, int* len_outdir, int* len_expnam)
 {
     (void)ioutput;
+    (void)c_wrkdir;
+    (void)len_wrkdir;
+    std::lock_guard<std::mutex> g(store().mtx);
+    set_names(c_outdir, len_outdir ? *len_outdir : 0, c_expnam, len_expnam ? *len_expnam : 0);
     if (ifcver) *ifcver = 2400;
     if (ierror) *ierror = 0;
 }
 
-void cntc_initialize(int* ire, int* imodul, int* ifcver, int* ierror, const char*, int*)
+void cntc_initialize(int* ire, int* imodul, int* ifcver, int* ierror, const char* c_outdir, int* len_outdir)
 {
     std::lock_guard<std::mutex> g(store().mtx);
+    set_names(c_outdir, len_outdir ? *len_outdir : 0, nullptr, 0);
     if (ifcver) *ifcver = 2400;
     if (!ire || !imodul || *ire <= 0 || *ire > MAX_REID || *imodul != 3) {
         if (ierror) *ierror = CNTC_err_other;
@@ -488,6 +528,44 @@ void cntc_setmetadata(int* ire, int* icp, int* lenmta, int* params, double* valu
     }
 }
 
+// With CNTC_if_wrtinp >= 1: the case description in the .inp file, opened on first use in the
+// output folder (src/contact_addon.f90:3829-3867)
+static void write_inp(int ire, int icp, Problem* P)
+{
+    Store& S = store();
+    auto f = P->flags.find(CNTC_if_wrtinp);
+    if (f == P->flags.end() || f->second < 1 || S.inp_open < 0) return;
+    if (!S.inp) {
+        const std::string fname = S.outdir + S.expnam + ".inp";
+        S.inp = std::fopen(fname.c_str(), "w");
+        if (!S.inp) {
+            S.inp_open = -1;
+            if (S.idebug >= 1) std::fprintf(stderr, "cntc_calculate: could not open \"%s\" for writing\n", fname.c_str());
+            return;
+        }
+        S.inp_open = 1;
+    }
+    std::vector<int8_t> el;
+    Engine* E = S.engine.get();
+    const Patch& pt = P->patch;
+    if (E && P->calculated && pt.have_solution && pt.n > 0) {
+        std::vector<int> e((size_t)pt.n);
+        E->get_eldiv(pt, e.data());
+        el.assign(e.begin(), e.end());
+    }
+    cntc::InpDigits dg;
+    auto flag = [&](int code, int& v) {
+        auto it = P->flags.find(code);
+        if (it != P->flags.end()) v = it->second;
+    };
+    flag(CNTC_ic_matfil, dg.matfil);
+    flag(CNTC_ic_output, dg.output);
+    flag(CNTC_ic_flow, dg.flow);
+    flag(CNTC_ic_return, dg.ret);
+    cntc::write_inp_case(S.inp, ire, icp, P->ncase, pt, dg, P->fkin, el);
+    std::fflush(S.inp);
+}
+
 static void calculate_many(int n, const int* ire, const int* icp, int* ierror)
 {
     Store& S = store();
@@ -497,7 +575,12 @@ static void calculate_many(int n, const int* ire, const int* icp, int* ierror)
     for (int k = 0; k < n; k++) {
         Problem* P = activate(ire[k], icp[k]);
         if (!P) { ierror[k] = CNTC_err_icp; continue; }
-        if (!E) { ierror[k] = CNTC_err_other; continue; }
+        if (!E) {
+            ierror[k] = CNTC_err_other;
+            P->ncase++;
+            write_inp(ire[k], icp[k], P);
+            continue;
+        }
         ps.push_back(&P->patch);
         pr.push_back(P);
     }
@@ -536,6 +619,7 @@ static void calculate_many(int n, const int* ire, const int* icp, int* ierror)
         P->ncase++;
         P->twall += P->patch.out.t_done > 0 ? P->patch.out.t_done : tw;
         P->tcpu += tc / pr.size();
+        write_inp(ire[k], icp[k], P);
         j++;
     }
     if (S.idebug > 0 && !E->last_error.empty()) std::fprintf(stderr, "cntc_calculate: %s\n", E->last_error.c_str());
@@ -586,6 +670,7 @@ void cntc_calculatenoalloc(int* ire, int* icp, int* ierror)
     P->ncase++;
     P->twall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     P->tcpu += double(std::clock() - c0) / CLOCKS_PER_SEC;
+    write_inp(*ire, *icp, P);
     if (S.idebug > 0 && !E->last_error.empty()) std::fprintf(stderr, "cntc_calculatenoalloc: %s\n", E->last_error.c_str());
 }
 

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <coroutine>
 #include <cstdio>
@@ -373,6 +374,7 @@ private:
             }
     }
 
+    struct GsbGroup;
     // one launch of side-stream operations (all items of one flush)
     struct AsyncOp {
         hipStream_t stream = nullptr;
@@ -390,6 +392,7 @@ private:
         std::vector<long> ns;        // element counts of the items (CNTC_GS_TIMING)
         double t_launch = 0.0;       // host clock at launch (CNTC_GS_TIMING)
         long max_n = 0;
+        std::shared_ptr<GsbGroup> gsb;   // block SteadyGS: an operation in steps
     };
     void free_async(AsyncOp& op)
     {
@@ -410,6 +413,7 @@ private:
                 op.slots.clear();
                 op.dest.clear();
                 op.ns.clear();
+                op.gsb.reset();
                 return op;
             }
         AsyncOp op;
@@ -445,6 +449,12 @@ private:
                 hipError_t e = hipEventQuery(op.ev_done);
                 if (e == hipErrorNotReady) { k++; continue; }
                 HIPCHK(e);
+            }
+            if (op.gsb && !gsb_advance(op)) {
+                // the next step is queued on the operation's stream
+                k++;
+                block = false;
+                continue;
             }
             const double* r = (const double*)(op.host + op.res_off);
             if (op.timed) {
@@ -500,10 +510,11 @@ private:
     {
         (void)main_idle;
         if (pending_.empty()) return;
-        // CNTC_GS_KERNEL=1 selects the variant without row staging for every size, 2 the staged
-        // variant without the overlap of row sums and walk (testing)
+        // variants: 3 blocks of rows with FFT products, 2 row sums overlapped with the walk,
+        // 1 staged row sums, 0 direct sums; the highest whose LDS fits.  CNTC_GS_KERNEL=1..4
+        // caps the variant at 0..3; the default cap is 2
         const char* forced = std::getenv("CNTC_GS_KERNEL");
-        const int maxvar = forced && forced[0] == '1' ? 0 : forced && forced[0] == '2' ? 1 : 2;
+        const int maxvar = forced && forced[0] >= '1' && forced[0] <= '4' ? forced[0] - '1' : 2;
         // largest patches first; one launch per idle side stream, each of patches of similar
         // size (down to a quarter of the launch's largest, at most 64), because a launch
         // returns when its longest item ends and a launch queued behind another on the same
@@ -518,7 +529,7 @@ private:
             if (k >= order.size()) break;
             if (!side_idle(st)) continue;
             const long top = order[k]->item.gs.n;
-            std::vector<Req*> byvar[3];
+            std::vector<Req*> byvar[4];
             for (size_t c = 0; k < order.size() && c < 64 && 4L * order[k]->item.gs.n >= top; k++, c++) {
                 Req* r = order[k];
                 int v = maxvar;
@@ -526,6 +537,7 @@ private:
                 while (v > 0 && op_lds(r->kind, &r->item, v) + 2048 > MAX_LDS) v--;
                 byvar[v].push_back(r);
             }
+            if (!byvar[3].empty()) launch_gsb(byvar[3], st);
             for (int v = 2; v >= 0; v--)
                 if (!byvar[v].empty()) launch_async(byvar[v], v, st);
         }
@@ -579,6 +591,275 @@ private:
                               op.stream));
         HIPCHK(hipEventRecord(op.ev_done, op.stream));
         async_.push_back(op);
+    }
+
+
+    // SteadyGS by blocks of rows (variant 3) on a side stream, as one side-stream operation in
+    // steps: the set-up (differences, row ranges), then one step per sweep -- the product
+    // z = A dp, for each block the walk and the product of its change into z (none after
+    // the last block), and the convergence test.  complete_async advances it each time the
+    // step's event has completed; patches that have converged leave the later sweeps.
+    struct GsbGroup {
+        int B = 8;                            // rows per block
+        std::vector<GsItem> gs;
+        std::vector<int> first, last, nb;     // interior rows and blocks per patch
+        std::vector<char> active;
+        std::vector<std::array<double, 10>> fin;
+        std::vector<int> act;                 // patches of the sweep in progress
+        int nbmax = 0, sweeps = 0;
+        size_t off_st = 0, off_sw = 0, cap_sw = 0;
+        bool filled = false;                  // the sweep area holds the launches of `act`
+        struct Launch {
+            int kind, n, ng[3];
+            size_t items, gst[3], lds[3];
+        };
+        std::vector<Launch> seq;
+        double seconds = 0.0;
+        bool timed = false;
+    };
+    static GsbItem gsb_item(const GsItem& g, int mode, double* st)
+    {
+        GsbItem b{};
+        b.n = g.n; b.mx = g.mx; b.my = g.my; b.k = g.k;
+        b.mode = mode; b.maxgs = g.maxgs;
+        b.eps = g.eps; b.omegah = g.omegah; b.omegas = g.omegas; b.ga_inv = g.ga_inv; b.mu = g.mu;
+        b.facnel = g.facnel;
+        b.el = g.el; b.px = g.px; b.py = g.py; b.pn = g.pn; b.sx = g.sx; b.sy = g.sy;
+        b.wsx = g.wsx; b.wsy = g.wsy; b.dpx = g.dpx; b.dpy = g.dpy;
+        b.zx = g.zx; b.zy = g.zy; b.ddx = g.ddx; b.ddy = g.ddy;
+        b.rng = g.list;
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++) b.cf[i][j] = g.cf[i][j];
+        b.p0x = g.p0x; b.p0y = g.p0y;
+        b.st = st;
+        return b;
+    }
+    static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+    void launch_gsb(const std::vector<Req*>& reqs, hipStream_t stream)
+    {
+        const int nitems = (int)reqs.size(), K = 10;
+        auto G = std::make_shared<GsbGroup>();
+        if (const char* e = std::getenv("CNTC_GSB_ROWS"); e && std::atoi(e) > 0) G->B = std::atoi(e);
+        int mymax = 0;
+        for (auto* r : reqs) {
+            G->gs.push_back(r->item.gs);
+            mymax = std::max(mymax, r->item.gs.my);
+        }
+        const int nbb = (mymax + G->B - 1) / G->B;
+        const size_t isz = std::max(sizeof(ConvItem), sizeof(GsbItem));
+        const size_t one = align256(isz * nitems) + 3 * align256(sizeof(int) * (nitems + 1));
+        G->cap_sw = (size_t)(2 * nbb + 2) * one;
+        const size_t items_b = align256(sizeof(GsbItem) * nitems), gs_b = align256(sizeof(int) * (nitems + 1));
+        const size_t res_b = align256(sizeof(double) * K * nitems);
+        G->off_st = items_b + gs_b + res_b;
+        G->off_sw = G->off_st + align256(sizeof(double) * 4 * nitems);
+        AsyncOp op = take_async(G->off_sw + G->cap_sw);
+        op.stream = stream;
+        op.nitems = nitems;
+        op.K = K;
+        op.res_off = items_b + gs_b;
+        op.kind = OP_STDYGS;
+        op.variant = 3;
+        op.timed = false;   // timed per sweep by the group itself
+        op.t_launch = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count() - t_origin;
+        op.max_n = 0;
+        int* gst = (int*)(op.host + items_b);
+        for (int i = 0; i < nitems; i++) {
+            double* st = (double*)(op.dev + G->off_st) + 4 * i;
+            GsbItem b = gsb_item(G->gs[i], 0, st);
+            std::memcpy(op.host + sizeof(GsbItem) * i, &b, sizeof(GsbItem));
+            gst[i] = i;
+            op.slots.push_back(reqs[i]->slot);
+            op.dest.push_back(reqs[i]->res);
+            op.ns.push_back((long)G->gs[i].n);
+            op.max_n = std::max(op.max_n, (long)G->gs[i].n);
+            async_wait_[reqs[i]->slot] = 1;
+        }
+        gst[nitems] = nitems;
+        G->timed = timed_kind(OP_STDYGS);
+        op.gsb = G;
+        HIPCHK(hipEventRecord(op.ev_start, s_));
+        HIPCHK(hipStreamWaitEvent(op.stream, op.ev_start, 0));
+        HIPCHK(hipMemcpyAsync(op.dev, op.host, items_b + gs_b, hipMemcpyHostToDevice, op.stream));
+        launch_op(OP_GSB_SETUP, op.dev, (const int*)(op.dev + items_b), nitems, nitems,
+                  RedOut{(double*)(op.dev + op.res_off), nullptr, nullptr, nullptr}, 0, 0, op.stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(op.host + op.res_off, op.dev + op.res_off, res_b, hipMemcpyDeviceToHost,
+                              op.stream));
+        HIPCHK(hipEventRecord(op.ev_done, op.stream));
+        async_.push_back(op);
+    }
+    // the launches of a sweep over G.act, written to the sweep area of op (host and device)
+    void gsb_fill(AsyncOp& op, GsbGroup& G)
+    {
+        G.seq.clear();
+        size_t off = 0;
+        char* h = op.host + G.off_sw;
+        const int na = (int)G.act.size();
+        auto add_conv = [&](const std::vector<int>& who, bool change) {
+            if (who.empty()) return;
+            GsbGroup::Launch L{};
+            L.kind = OP_CONV;
+            L.n = (int)who.size();
+            L.items = off;
+            ConvItem* ci = (ConvItem*)(h + off);
+            for (int a = 0; a < L.n; a++) {
+                const GsItem& g = G.gs[who[a]];
+                ConvItem c = g.conv;
+                c.in[0] = change ? g.ddx : g.dpx;
+                c.in[1] = change ? g.ddy : g.dpy;
+                c.addend[0] = change ? g.zx : nullptr;
+                c.addend[1] = change ? g.zy : nullptr;
+                ci[a] = c;
+            }
+            off += align256(sizeof(ConvItem) * L.n);
+            for (int st = 0; st < 3; st++) {
+                L.gst[st] = off;
+                int* gp = (int*)(h + off);
+                int tot = 0;
+                size_t lds = 0;
+                for (int a = 0; a < L.n; a++) {
+                    gp[a] = tot;
+                    tot += op_groups(OP_CONV, &ci[a], st);
+                    lds = std::max(lds, op_lds(OP_CONV, &ci[a], st));
+                }
+                gp[L.n] = tot;
+                L.ng[st] = tot;
+                L.lds[st] = lds;
+                off += align256(sizeof(int) * (L.n + 1));
+            }
+            G.seq.push_back(L);
+        };
+        auto add_gsb = [&](int kind, const std::vector<int>& who, const std::vector<GsbItem>& its) {
+            if (who.empty()) return;
+            GsbGroup::Launch L{};
+            L.kind = kind;
+            L.n = (int)who.size();
+            L.items = off;
+            std::memcpy(h + off, its.data(), sizeof(GsbItem) * L.n);
+            off += align256(sizeof(GsbItem) * L.n);
+            L.gst[0] = off;
+            int* gp = (int*)(h + off);
+            size_t lds = 0;
+            for (int a = 0; a <= L.n; a++) gp[a] = a;
+            for (auto& b : its) lds = std::max(lds, op_lds(kind, &b, 0));
+            L.ng[0] = L.n;
+            L.lds[0] = lds;
+            off += align256(sizeof(int) * (L.n + 1));
+            G.seq.push_back(L);
+        };
+        auto st_of = [&](int i) { return (double*)(op.dev + G.off_st) + 4 * i; };
+        // z = A dp for every patch of the sweep
+        add_conv(G.act, false);
+        for (int b = 0; b < G.nbmax; b++) {
+            std::vector<int> who, whoc;
+            std::vector<GsbItem> its;
+            for (int a = 0; a < na; a++) {
+                const int i = G.act[a];
+                if (G.nb[i] <= b) continue;
+                GsbItem w = gsb_item(G.gs[i], 1, st_of(i));
+                w.y0 = G.first[i] + b * G.B;
+                w.y1 = std::min(G.last[i] + 1, w.y0 + G.B);
+                // d of the previous block (for the first block: the last block of the sweep
+                // before, whose change no product used)
+                const int pb = b > 0 ? b - 1 : G.nb[i] - 1;
+                w.z0 = G.first[i] + pb * G.B;
+                w.z1 = std::min(G.last[i] + 1, w.z0 + G.B);
+                who.push_back(i);
+                its.push_back(w);
+                if (G.nb[i] > b + 1) whoc.push_back(i);
+            }
+            add_gsb(OP_GSB_WALK, who, its);
+            add_conv(whoc, true);
+        }
+        std::vector<GsbItem> chk;
+        for (int a = 0; a < na; a++) chk.push_back(gsb_item(G.gs[G.act[a]], 2, st_of(G.act[a])));
+        add_gsb(OP_GSB_CHECK, G.act, chk);
+        if (off > G.cap_sw) throw std::runtime_error("steadygs blocks: sweep area too small");
+        HIPCHK(hipMemcpyAsync(op.dev + G.off_sw, h, off, hipMemcpyHostToDevice, op.stream));
+        G.filled = true;
+    }
+    // after a step's event: take its results; queue the next sweep and return false, or write
+    // the final results (op order, 10 per patch) and return true
+    bool gsb_advance(AsyncOp& op)
+    {
+        GsbGroup& G = *op.gsb;
+        const int nitems = op.nitems, K = op.K;
+        double* r = (double*)(op.host + op.res_off);
+        if (G.timed && G.sweeps > 0) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, op.t0, op.t1));
+            G.seconds += ms * 1e-3;
+        }
+        if (G.first.empty()) {
+            // set-up done: interior rows and blocks per patch
+            G.first.assign(nitems, 0);
+            G.last.assign(nitems, -1);
+            G.nb.assign(nitems, 0);
+            G.active.assign(nitems, 1);
+            G.fin.assign(nitems, std::array<double, 10>{});
+            for (int i = 0; i < nitems; i++) {
+                const double f = r[(size_t)i * K], l = r[(size_t)i * K + 1];
+                if (std::isfinite(f) && std::isfinite(l) && l >= f) {
+                    G.first[i] = (int)f;
+                    G.last[i] = (int)l;
+                    G.nb[i] = (G.last[i] - G.first[i]) / G.B + 1;
+                }
+                G.nbmax = std::max(G.nbmax, G.nb[i]);
+                G.act.push_back(i);
+            }
+        } else {
+            // a sweep done: patches that stop leave
+            std::vector<int> keep;
+            for (int a = 0; a < (int)G.act.size(); a++) {
+                const int i = G.act[a];
+                const double* q = r + (size_t)a * K;
+                if (q[6] != 0.0) {
+                    keep.push_back(i);
+                } else {
+                    for (int j = 0; j < 10; j++) G.fin[i][j] = q[j];
+                    G.fin[i][6] = 0.0;
+                    G.fin[i][7] = 0.0;
+                    G.active[i] = 0;
+                }
+            }
+            if (keep.size() != G.act.size()) G.filled = false;
+            G.act.swap(keep);
+        }
+        if (G.act.empty()) {
+            for (int i = 0; i < nitems; i++)
+                for (int j = 0; j < K; j++) r[(size_t)i * K + j] = G.fin[i][j];
+            if (G.timed) {
+                std::lock_guard<std::mutex> lk(prof_mtx);
+                auto& pe = prof[OP_STDYGS * 4 + 3];
+                pe.seconds += G.seconds;
+                for (int i = 0; i < nitems; i++) pe.bytes += G.fin[i][2];
+                pe.launches++;
+            }
+            return true;
+        }
+        if (!G.filled) gsb_fill(op, G);
+        if (G.timed) HIPCHK(hipEventRecord(op.t0, op.stream));
+        for (const auto& L : G.seq) {
+            const char* d = op.dev + G.off_sw;
+            if (L.kind == OP_CONV) {
+                for (int st = 0; st < 3; st++)
+                    launch_op(OP_CONV, d + L.items, (const int*)(d + L.gst[st]), L.n, L.ng[st], RedOut{},
+                              st, L.lds[st], op.stream);
+            } else {
+                RedOut ro{};
+                if (L.kind == OP_GSB_CHECK) ro.partials = (double*)(op.dev + op.res_off);
+                launch_op(L.kind, d + L.items, (const int*)(d + L.gst[0]), L.n, L.ng[0], ro, 0, L.lds[0],
+                          op.stream);
+            }
+            HIPCHK(hipGetLastError());
+        }
+        if (G.timed) HIPCHK(hipEventRecord(op.t1, op.stream));
+        HIPCHK(hipMemcpyAsync(op.host + op.res_off, op.dev + op.res_off, sizeof(double) * K * G.act.size(),
+                              hipMemcpyDeviceToHost, op.stream));
+        HIPCHK(hipEventRecord(op.ev_done, op.stream));
+        G.sweeps++;
+        return false;
     }
 
     struct Group {
@@ -838,6 +1119,54 @@ struct ResAwait {
 };
 
 // per-patch issuing context
+// the item of the FFT product u[ik] (mask) = sum_jk C[ik][jk] * p[jk] on patch P,
+// src/m_aijpj.f90:258-451.  which: 0 = cs, 1 = cv, 2 = ms; fp32 selects the single-precision
+// spectra.
+static ConvItem conv_item(const Patch* P, int which, int mask, int nout, const int* ik,
+                          double* const* out, int nin, const int* jk, double* const* in,
+                          const double* const* addend, bool fp32)
+{
+    ConvItem c{};
+    c.mx = P->mx;
+    c.my = P->my;
+    c.Lx = P->Lx;
+    c.Ly = P->Ly;
+    c.nin = nin;
+    c.nout = nout;
+    c.mask = mask;
+    c.mode = 0;
+    int M = P->Lx / 2, H = M + 1;
+    size_t bsz = (size_t)H * P->Ly;
+    const char* base;
+    size_t esz = fp32 ? 8 : 16;
+    if (which == 0) base = (const char*)(fp32 ? P->spec32 : P->spec);
+    else if (which == 1) base = (const char*)P->specv;
+    else base = (const char*)(fp32 ? P->specm32 : P->specm);
+    for (int j = 0; j < nin; j++) c.in[j] = in[j];
+    for (int i = 0; i < nout; i++) {
+        c.out[i] = out[i];
+        c.addend[i] = addend ? addend[i] : nullptr;
+        for (int j = 0; j < nin; j++) {
+            int a = ik[i], b = jk[j];   // 0-based directions
+            const void* sp = nullptr;
+            if (which == 2) {
+                if (a == b) sp = base + (size_t)a * bsz * esz;
+            } else if (P->nt_cpl || !((a == 2) != (b == 2))) {
+                sp = base + (size_t)(a * 3 + b) * bsz * esz;
+            }
+            c.spec[i][j] = sp;
+        }
+    }
+    c.el = P->el;
+    c.wsS = P->wsS;
+    c.wsT = P->wsT;
+    c.twM = fp32 ? P->twM32 : P->twM;
+    c.twLx = fp32 ? P->twLx32 : P->twLx;
+    c.twLy = fp32 ? P->twLy32 : P->twLy;
+    c.scale = 1.0;
+    return c;
+}
+
 struct Ctx {
     Scheduler* S;
     Patch* P;
@@ -887,50 +1216,12 @@ struct Ctx {
     ResAwait vmax(int mask, double* x) { return opr(OP_VEC, vitem(V_MAX, mask, x, nullptr, nullptr, nullptr, 0, 0)); }
     ResAwait count(int mask) { return opr(OP_VEC, vitem(V_COUNT, mask, P->pn, nullptr, nullptr, nullptr, 0, 0)); }
 
-    // FFT product u[ik] (mask) = sum_jk C[ik][jk] * p[jk], src/m_aijpj.f90:258-451.
-    // which: 0 = cs, 1 = cv, 2 = ms; fp32 selects the single-precision spectra.
+    // FFT product u[ik] (mask) = sum_jk C[ik][jk] * p[jk] (conv_item)
     void conv(int which, int mask, int nout, const int* ik, double* const* out, int nin, const int* jk,
               double* const* in, const double* const* addend = nullptr, bool fp32 = false)
     {
         AnyItem it;
-        ConvItem& c = it.conv;
-        c.mx = P->mx;
-        c.my = P->my;
-        c.Lx = P->Lx;
-        c.Ly = P->Ly;
-        c.nin = nin;
-        c.nout = nout;
-        c.mask = mask;
-        c.mode = 0;
-        int M = P->Lx / 2, H = M + 1;
-        size_t bsz = (size_t)H * P->Ly;
-        const char* base;
-        size_t esz = fp32 ? 8 : 16;
-        if (which == 0) base = (const char*)(fp32 ? P->spec32 : P->spec);
-        else if (which == 1) base = (const char*)P->specv;
-        else base = (const char*)(fp32 ? P->specm32 : P->specm);
-        for (int j = 0; j < nin; j++) c.in[j] = in[j];
-        for (int i = 0; i < nout; i++) {
-            c.out[i] = out[i];
-            c.addend[i] = addend ? addend[i] : nullptr;
-            for (int j = 0; j < nin; j++) {
-                int a = ik[i], b = jk[j];   // 0-based directions
-                const void* sp = nullptr;
-                if (which == 2) {
-                    if (a == b) sp = base + (size_t)a * bsz * esz;
-                } else if (P->nt_cpl || !((a == 2) != (b == 2))) {
-                    sp = base + (size_t)(a * 3 + b) * bsz * esz;
-                }
-                c.spec[i][j] = sp;
-            }
-        }
-        c.el = P->el;
-        c.wsS = P->wsS;
-        c.wsT = P->wsT;
-        c.twM = fp32 ? P->twM32 : P->twM;
-        c.twLx = fp32 ? P->twLx32 : P->twLx;
-        c.twLy = fp32 ? P->twLy32 : P->twLy;
-        c.scale = 1.0;
+        it.conv = conv_item(P, which, mask, nout, ik, out, nin, jk, in, addend, fp32);
         op(fp32 ? OP_CONV32 : OP_CONV, it);
     }
     // normal-normal product
@@ -2399,6 +2690,14 @@ static GsItem gs_item(Patch& P, int k, double omegah, double omegas, int maxgs, 
     g.cf[0][1] = P.cf + blk;
     g.cf[1][0] = P.cf + 3 * blk;
     g.cf[1][1] = P.cf + 4 * blk;
+    // the block variant: z = A dp and the block change d, and the item of their product
+    g.zx = P.w[3]; g.zy = P.w[4];
+    g.ddx = P.w[5]; g.ddy = P.w[6];
+    const int ik[2] = {0, 1}, jk[2] = {0, 1};
+    double* out[2] = {g.zx, g.zy};
+    double* in[2] = {g.ddx, g.ddy};
+    const double* add[2] = {g.zx, g.zy};
+    g.conv = conv_item(&P, 0, M_ALL, 2, ik, out, 2, jk, in, add, false);
     return g;
 }
 

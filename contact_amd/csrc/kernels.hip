@@ -2430,6 +2430,175 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs2(const GsItem* items, con
     }
 }
 
+// LDS arrays of the SteadyGS row being walked
+struct GsRow {
+    double* px; double* py;       // p
+    double* dpx; double* dpy;     // differences along x
+    double* g;                    // mu pn
+    double* pn;
+    double* wx; double* wy;       // ws
+    double* ucx; double* ucy;     // influence of other rows, unscaled (times ga_inv)
+    const double* zx; const double* zy;   // influence of other rows, scaled (null: none)
+    const double* stc;            // own-row coefficients, 4 blocks of 2 mx (offsets q - mx)
+    int8_t* el;
+};
+
+// Wavefront 0's walk along elements [ixw, iend) of the row starting at r0 (src/m_solvpt.f90:
+// 2950-3127): each interior element's 2x2 problem in order from the trailing edge, then the
+// rebuild of the upstream part of the row.  The own-row influence is summed over the
+// summation range [j0, j1].  lastna carries the first upstream element not in adhesion along
+// the row; difacc (lane 0) sums the squared changes; gsx, gsy receive s of each element.
+// Called by all 64 lanes of wavefront 0.
+__device__ void gs_walk_span(const GsRow& R, int mx, int j0, int j1, size_t r0, int& ixw, int iend,
+                             int& lastna, double& difacc, double ga_inv, double eps, double mu,
+                             double omegah, double omegas, double* gsx, double* gsy)
+{
+    const int lane = threadIdx.x, M2 = 2 * mx;
+    for (; ixw < iend; ixw++) {
+        const int ix = ixw;
+        if (R.el[ix] < ADHES) {
+            if (ix >= 1) lastna = ix;
+            continue;
+        }
+        double qsx = 0.0, qsy = 0.0;
+        for (int jx = j0 + lane; jx <= j1; jx += 64) {
+            int o = ix - jx + mx;
+            double qx = R.dpx[jx], qy = R.dpy[jx];
+            qsx += R.stc[o] * qx + R.stc[M2 + o] * qy;
+            qsy += R.stc[2 * M2 + o] * qx + R.stc[3 * M2 + o] * qy;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            qsx += __shfl_xor(qsx, off);
+            qsy += __shfl_xor(qsy, off);
+        }
+        const int jx = ix >= 1 ? lastna : -1;
+        // the element problem, solved by every lane on the same (broadcast) data so
+        // that its change D is known wavefront-wide; lane 0 stores
+        const int8_t eup = ix >= 1 ? R.el[ix - 1] : EXTER;
+        double Dx, Dy;
+        {
+            const int b0 = mx, bj = mx + (jx - ix);
+            double a11 = ga_inv * (R.stc[b0] - R.stc[bj]);
+            double a12 = ga_inv * (R.stc[M2 + b0] - R.stc[M2 + bj]);
+            double a21 = ga_inv * (R.stc[2 * M2 + b0] - R.stc[2 * M2 + bj]);
+            double a22 = ga_inv * (R.stc[3 * M2 + b0] - R.stc[3 * M2 + bj]);
+            double sv[2];
+            if (R.zx) {
+                sv[0] = (R.wx[ix] + R.zx[ix]) + ga_inv * (R.ucx[ix] + qsx);
+                sv[1] = (R.wy[ix] + R.zy[ix]) + ga_inv * (R.ucy[ix] + qsy);
+            } else {
+                sv[0] = R.wx[ix] + ga_inv * (R.ucx[ix] + qsx);
+                sv[1] = R.wy[ix] + ga_inv * (R.ucy[ix] + qsy);
+            }
+            const double opx = R.px[ix], opy = R.py[ix];
+            double pr[3] = {opx, opy, R.pn[ix]};
+            int8_t e8 = R.el[ix];
+            elmtrc(e8, a11, a12, a21, a22, eps, pr, mu, sv, omegah, omegas, true);
+            Dx = pr[0] - opx;
+            Dy = pr[1] - opy;
+            if (lane == 0) {
+                const size_t ii = r0 + ix;
+                difacc += Dx * Dx + Dy * Dy;
+                R.dpx[ix] += Dx;
+                R.dpy[ix] += Dy;
+                R.px[ix] = pr[0];
+                R.py[ix] = pr[1];
+                R.el[ix] = e8;
+                gsx[ii] = sv[0];
+                gsy[ii] = sv[1];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // Rebuild the upstream part of the row (:3086-3127).  The adhesion elements
+        // upstream satisfy p(k) = p(k+1) + dp(k), so they all move by the change D
+        // of p(ix) until one exceeds its traction bound (clipped there, D becomes
+        // that element's change) or a non-adhesion element ends the chain.  The
+        // wavefront handles 64 upstream elements per step.
+        int hi = ix - 1;
+        if (ix >= 1 && eup != ADHES) {
+            // the upstream neighbour ends the chain at once
+            if (lane == 0) {
+                const int kx = ix - 1;
+                if (eup == SLIP || eup == PLAST) {
+                    R.dpx[kx] = R.px[kx] - R.px[ix];
+                    R.dpy[kx] = R.py[kx] - R.py[ix];
+                } else {
+                    R.dpx[kx] = -R.px[ix];
+                    R.dpy[kx] = -R.py[ix];
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            hi = -1;
+        }
+        while (hi >= 0) {
+            const int kx = hi - lane;
+            const bool valid = kx >= 0;
+            const int8_t ek = valid ? R.el[kx] : EXTER;
+            const unsigned long long smask = __ballot(!valid || ek != ADHES);
+            const int sl = smask ? __ffsll((long long)smask) - 1 : 64;
+            double ox = 0.0, oy = 0.0, qx = 0.0, qy = 0.0;
+            bool clip = false;
+            if (lane < sl) {
+                ox = R.px[kx];
+                oy = R.py[kx];
+                qx = ox + Dx;
+                qy = oy + Dy;
+                clip = sqrt(qx * qx + qy * qy) > R.g[kx];
+            }
+            const unsigned long long cmask = __ballot(clip);
+            const int cl = cmask ? __ffsll((long long)cmask) - 1 : 64;
+            const int nmove = sl < cl ? sl : cl;
+            if (lane < nmove) {
+                R.px[kx] = qx;
+                R.py[kx] = qy;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (cl < sl) {
+                // clip at lane cl and continue below it with that element's change
+                double nDx = 0.0, nDy = 0.0;
+                if (lane == cl) {
+                    double ptabs = sqrt(qx * qx + qy * qy), ptbnd = R.g[kx];
+                    qx = qx * ptbnd / ptabs;
+                    qy = qy * ptbnd / ptabs;
+                    R.dpx[kx] = qx - R.px[kx + 1];
+                    R.dpy[kx] = qy - R.py[kx + 1];
+                    R.px[kx] = qx;
+                    R.py[kx] = qy;
+                    nDx = qx - ox;
+                    nDy = qy - oy;
+                }
+                Dx = __shfl(nDx, cl);
+                Dy = __shfl(nDy, cl);
+                hi -= cl + 1;
+            } else if (sl < 64) {
+                // the first non-adhesion element ends the chain
+                if (lane == sl && valid) {
+                    if (ek == SLIP || ek == PLAST) {
+                        R.dpx[kx] = R.px[kx] - R.px[kx + 1];
+                        R.dpy[kx] = R.py[kx] - R.py[kx + 1];
+                    } else if (R.el[kx + 1] >= ADHES) {
+                        R.dpx[kx] = -R.px[kx + 1];
+                        R.dpy[kx] = -R.py[kx + 1];
+                    }
+                }
+                hi = -1;
+            } else {
+                hi -= 64;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (ix >= 1 && R.el[ix] != ADHES) lastna = ix;
+    }
+}
+
 // SteadyGS with the row sums of the next row overlapped with the walk along the current row:
 // while wavefront 0 walks row iy element by element, the other wavefronts sum the influence
 // of every source row but iy and iy' on the next target row iy' (those rows do not change
@@ -2500,6 +2669,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs3(const GsItem* items, con
     int8_t* rel = (int8_t*)(ri1 + my);
     double* stg = gs_lds + gs3_stage_off(mx, my) / 8;
     const int SW = gs3_stage_stride(mx);
+    const GsRow R{rpx, rpy, rdpx, rdpy, rg, rpn, rwx, rwy, ucx, ucy, nullptr, nullptr, stc, rel};
     double* dpx = I.dpx;
     double* dpy = I.dpy;
     // the xx, xy and yy coefficient blocks: the coefficient kernel writes the xy and yx blocks
@@ -2734,142 +2904,8 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs3(const GsItem* items, con
                 const int iend = b >= nb ? t0 + nt : t0 + (int)(((long)nt * (b + 1) + nb - 1) / nb);
                 if (ixw >= iend) return;
                 const unsigned long long w0 = wall_clock64();
-                for (; ixw < iend; ixw++) {
-                    const int ix = ixw;
-                    if (rel[ix] < ADHES) {
-                        if (ix >= 1) lastna = ix;
-                        continue;
-                    }
-                    double qsx = 0.0, qsy = 0.0;
-                    for (int jx = j0 + lane; jx <= j1; jx += 64) {
-                        int o = ix - jx + mx;
-                        double qx = rdpx[jx], qy = rdpy[jx];
-                        qsx += stc[o] * qx + stc[M2 + o] * qy;
-                        qsy += stc[2 * M2 + o] * qx + stc[3 * M2 + o] * qy;
-                    }
-                    for (int off = 32; off > 0; off >>= 1) {
-                        qsx += __shfl_xor(qsx, off);
-                        qsy += __shfl_xor(qsy, off);
-                    }
-                    const int jx = ix >= 1 ? lastna : -1;
-                    // the element problem, solved by every lane on the same (broadcast) data so
-                    // that its change D is known wavefront-wide; lane 0 stores
-                    const int8_t eup = ix >= 1 ? rel[ix - 1] : EXTER;
-                    double Dx, Dy;
-                    {
-                        const int b0 = mx, bj = mx + (jx - ix);
-                        double a11 = I.ga_inv * (stc[b0] - stc[bj]);
-                        double a12 = I.ga_inv * (stc[M2 + b0] - stc[M2 + bj]);
-                        double a21 = I.ga_inv * (stc[2 * M2 + b0] - stc[2 * M2 + bj]);
-                        double a22 = I.ga_inv * (stc[3 * M2 + b0] - stc[3 * M2 + bj]);
-                        double sv[2] = {rwx[ix] + I.ga_inv * (ucx[ix] + qsx), rwy[ix] + I.ga_inv * (ucy[ix] + qsy)};
-                        const double opx = rpx[ix], opy = rpy[ix];
-                        double pr[3] = {opx, opy, rpn[ix]};
-                        int8_t e8 = rel[ix];
-                        elmtrc(e8, a11, a12, a21, a22, I.eps, pr, I.mu, sv, I.omegah, I.omegas, true);
-                        Dx = pr[0] - opx;
-                        Dy = pr[1] - opy;
-                        if (lane == 0) {
-                            const size_t ii = r0 + ix;
-                            difacc += Dx * Dx + Dy * Dy;
-                            rdpx[ix] += Dx;
-                            rdpy[ix] += Dy;
-                            rpx[ix] = pr[0];
-                            rpy[ix] = pr[1];
-                            rel[ix] = e8;
-                            I.sx[ii] = sv[0];
-                            I.sy[ii] = sv[1];
-                        }
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    // Rebuild the upstream part of the row (:3086-3127).  The adhesion elements
-                    // upstream satisfy p(k) = p(k+1) + dp(k), so they all move by the change D
-                    // of p(ix) until one exceeds its traction bound (clipped there, D becomes
-                    // that element's change) or a non-adhesion element ends the chain.  The
-                    // wavefront handles 64 upstream elements per step.
-                    int hi = ix - 1;
-                    if (ix >= 1 && eup != ADHES) {
-                        // the upstream neighbour ends the chain at once
-                        if (lane == 0) {
-                            const int kx = ix - 1;
-                            if (eup == SLIP || eup == PLAST) {
-                                rdpx[kx] = rpx[kx] - rpx[ix];
-                                rdpy[kx] = rpy[kx] - rpy[ix];
-                            } else {
-                                rdpx[kx] = -rpx[ix];
-                                rdpy[kx] = -rpy[ix];
-                            }
-                        }
-                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                        __builtin_amdgcn_wave_barrier();
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                        hi = -1;
-                    }
-                    while (hi >= 0) {
-                        const int kx = hi - lane;
-                        const bool valid = kx >= 0;
-                        const int8_t ek = valid ? rel[kx] : EXTER;
-                        const unsigned long long smask = __ballot(!valid || ek != ADHES);
-                        const int sl = smask ? __ffsll((long long)smask) - 1 : 64;
-                        double ox = 0.0, oy = 0.0, qx = 0.0, qy = 0.0;
-                        bool clip = false;
-                        if (lane < sl) {
-                            ox = rpx[kx];
-                            oy = rpy[kx];
-                            qx = ox + Dx;
-                            qy = oy + Dy;
-                            clip = sqrt(qx * qx + qy * qy) > rg[kx];
-                        }
-                        const unsigned long long cmask = __ballot(clip);
-                        const int cl = cmask ? __ffsll((long long)cmask) - 1 : 64;
-                        const int nmove = sl < cl ? sl : cl;
-                        if (lane < nmove) {
-                            rpx[kx] = qx;
-                            rpy[kx] = qy;
-                        }
-                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                        __builtin_amdgcn_wave_barrier();
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                        if (cl < sl) {
-                            // clip at lane cl and continue below it with that element's change
-                            double nDx = 0.0, nDy = 0.0;
-                            if (lane == cl) {
-                                double ptabs = sqrt(qx * qx + qy * qy), ptbnd = rg[kx];
-                                qx = qx * ptbnd / ptabs;
-                                qy = qy * ptbnd / ptabs;
-                                rdpx[kx] = qx - rpx[kx + 1];
-                                rdpy[kx] = qy - rpy[kx + 1];
-                                rpx[kx] = qx;
-                                rpy[kx] = qy;
-                                nDx = qx - ox;
-                                nDy = qy - oy;
-                            }
-                            Dx = __shfl(nDx, cl);
-                            Dy = __shfl(nDy, cl);
-                            hi -= cl + 1;
-                        } else if (sl < 64) {
-                            // the first non-adhesion element ends the chain
-                            if (lane == sl && valid) {
-                                if (ek == SLIP || ek == PLAST) {
-                                    rdpx[kx] = rpx[kx] - rpx[kx + 1];
-                                    rdpy[kx] = rpy[kx] - rpy[kx + 1];
-                                } else if (rel[kx + 1] >= ADHES) {
-                                    rdpx[kx] = -rpx[kx + 1];
-                                    rdpy[kx] = -rpy[kx + 1];
-                                }
-                            }
-                            hi = -1;
-                        } else {
-                            hi -= 64;
-                        }
-                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                        __builtin_amdgcn_wave_barrier();
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    }
-                    if (ix >= 1 && rel[ix] != ADHES) lastna = ix;
-                }
+                gs_walk_span(R, mx, j0, j1, r0, ixw, iend, lastna, difacc, I.ga_inv, I.eps, I.mu,
+                             I.omegah, I.omegas, I.sx, I.sy);
                 if (lane == 0) t_walk += wall_clock64() - w0;
             };
             const unsigned long long tr0 = wall_clock64();
@@ -2929,6 +2965,332 @@ __global__ void __launch_bounds__(GS_THREADS) k_stdygs3(const GsItem* items, con
 }
 
 // ------------------------------------------------------------------------------------------
+// SteadyGS by blocks of rows (GsbItem): set-up, the walk of one block, the end-of-sweep test
+// ------------------------------------------------------------------------------------------
+
+constexpr int GSB_THREADS = 256;
+
+// set-up: dp = differences along x (the leading-edge column keeps p, src/m_solvpt.f90:
+// 2893-2908), the summation and interior range of each row, d = 0 and the sweep state.
+// Returns (first interior row, last interior row, summed range lengths).
+__global__ void __launch_bounds__(GSB_THREADS) k_gsb_setup(const GsbItem* items, const int* gstart,
+                                                           int nitems, RedOut ro)
+{
+    __shared__ double red[16 * MAXK];
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GsbItem& I = items[it];
+    const int mx = I.mx, my = I.my, n = I.n, tid = threadIdx.x, T = blockDim.x;
+    for (int i = tid; i < n; i += T) {
+        const int ix = i % mx;
+        if (ix != mx - 1) {
+            I.dpx[i] = I.px[i] - I.px[i + 1];
+            I.dpy[i] = I.py[i] - I.py[i + 1];
+        } else {
+            I.dpx[i] = I.px[i];
+            I.dpy[i] = I.py[i];
+        }
+        I.ddx[i] = 0.0;
+        I.ddy[i] = 0.0;
+    }
+    // summation range [first - 1, last + 1] (src/m_aijpj.f90:173-185) and interior range
+    double acc[3] = {INFINITY, -INFINITY, 0.0};   // -first row, last row, lengths
+    for (int jy = tid; jy < my; jy += T) {
+        const int8_t* row = I.el + (size_t)jy * mx;
+        int first = 0;
+        while (first < mx - 1 && row[first] <= EXTER) first++;
+        int last = -1, lo = mx;
+        for (int ix = 0; ix < mx; ix++)
+            if (row[ix] >= ADHES) {
+                last = ix;
+                if (ix < lo) lo = ix;
+            }
+        const int j0 = first - 1 < 0 ? 0 : first - 1, j1 = last + 1 > mx - 1 ? mx - 1 : last + 1;
+        I.rng[jy] = j0;
+        I.rng[my + jy] = j1;
+        I.rng[2 * my + jy] = lo;
+        I.rng[3 * my + jy] = last;
+        if (last >= lo) {
+            acc[0] = fmin(acc[0], (double)jy);
+            acc[1] = fmax(acc[1], (double)jy);
+        }
+        if (j1 >= j0) acc[2] += (double)(j1 - j0 + 1);
+    }
+    // (the three values reduce with min, max and sum)
+    double a0[1] = {acc[0]}, a1[1] = {acc[1]}, a2[1] = {acc[2]};
+    group_reduce<1>(a0, 2, red);
+    group_reduce<1>(a1, 1, red);
+    group_reduce<1>(a2, 0, red);
+    if (tid == 0) {
+        for (int q = 0; q < 4; q++) I.st[q] = 0.0;
+        double* r = ro.partials + (size_t)blockIdx.x * 10;
+        r[0] = a0[0];
+        r[1] = a1[0];
+        r[2] = a2[0];
+        for (int q = 3; q < 10; q++) r[q] = 0.0;
+    }
+}
+
+// LDS of k_gsb_walk: 14 row arrays, the own-row coefficients (4 x 2 mx), the staging of a
+// row product (3 coefficient windows and 2 source rows), the states of the row
+__host__ __device__ inline int gsb_window(int mx) { return (2 * mx + GS3_TB + GS3_QB + 3) / 4 * 4; }
+__host__ __device__ inline size_t gsb_lds(int mx)
+{
+    return ((size_t)14 * mx + 8 * mx + 3 * gsb_window(mx) + 2 * (mx + GS3_QB) + 2 * GSB_THREADS) *
+               sizeof(double) + mx + 64;
+}
+
+// The walk of one block of rows.  For row t of the block, the influence of every other row is
+//   z_t - A(0) dp_t + sum over the block's rows j < t of A(t - j) d_j
+// (z = A dp at the block's start includes row t itself and the block's rows at their
+// start values; d_j = the change of row j's differences in this block), then wavefront 0
+// walks the row with the own-row influence of the current differences, as k_stdygs3 does.
+// The row products run on all threads: a thread owns GS3_TB consecutive targets and takes
+// the sources in chunks of GS3_QB from one coefficient window per block (xx, xy = yx, yy).
+__global__ void __launch_bounds__(GSB_THREADS) k_gsb_walk(const GsbItem* items, const int* gstart,
+                                                          int nitems, RedOut ro)
+{
+    extern __shared__ double gsb_lds_[];
+    (void)ro;
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GsbItem& I = items[it];
+    const int mx = I.mx, my = I.my, tid = threadIdx.x, T = blockDim.x, M2 = 2 * mx;
+    double* rpx = gsb_lds_;
+    double* rpy = rpx + mx;
+    double* rdpx = rpy + mx;
+    double* rdpy = rdpx + mx;
+    double* r0x = rdpy + mx;               // differences at the row's start
+    double* r0y = r0x + mx;
+    double* rg = r0y + mx;
+    double* rpn = rg + mx;
+    double* rwx = rpn + mx;
+    double* rwy = rwx + mx;
+    double* ucx = rwy + mx;
+    double* ucy = ucx + mx;
+    double* rzx = ucy + mx;
+    double* rzy = rzx + mx;
+    double* stc = rzy + mx;                // own-row coefficients, offsets (q - mx, 0)
+    double* sc = stc + 4 * M2;             // product staging: 3 windows, then 2 source rows
+    double* red = sc + 3 * gsb_window(mx) + 2 * (mx + GS3_QB);
+    int8_t* rel = (int8_t*)(red + 2 * GSB_THREADS);
+    const int* rj0 = I.rng;
+    const int* rj1 = rj0 + my;
+    const int* ri0 = rj1 + my;
+    const int* ri1 = ri0 + my;
+    const double* cfs[3] = {I.cf[0][0], I.cf[0][1], I.cf[1][1]};   // yx = xy
+    const double* cfb[4] = {I.cf[0][0], I.cf[0][1], I.cf[1][0], I.cf[1][1]};
+    const GsRow R{rpx, rpy, rdpx, rdpy, rg, rpn, rwx, rwy, ucx, ucy, rzx, rzy, stc, rel};
+
+    // d of the previous block has been used by its product: clear it
+    for (size_t e = (size_t)I.z0 * mx + tid; e < (size_t)I.z1 * mx; e += T) {
+        I.ddx[e] = 0.0;
+        I.ddy[e] = 0.0;
+    }
+    for (int e = tid; e < 4 * M2; e += T) {
+        const int b = e / M2, q = e - b * M2;
+        stc[e] = cfb[b][(size_t)my * M2 + q];
+    }
+    double difacc = 0.0, flops = 0.0;   // thread 0
+    double ax[GS3_TB], ay[GS3_TB];
+    // ax, ay (targets of row tr) += A(tr - sy) src over the source range [j0, j0 + nq), the
+    // source row given in global memory (sgx, sgy) or, for the row itself, in LDS negated
+    auto row_prod = [&](int tr, int sy, const double* sgx, const double* sgy, bool self) {
+        const int t0 = ri0[tr], nt = ri1[tr] - t0 + 1;
+        const int j0 = rj0[sy], nq = rj1[sy] - j0 + 1;
+        if (nq <= 0) return;
+        const int ntg = (nt + GS3_TB - 1) / GS3_TB, ntp = ntg * GS3_TB;
+        const int P = T / ntg > 0 ? T / ntg : 1;
+        const int tg = tid % ntg, part = tid / ntg;
+        const int Q = (nq + GS3_QB - 1) / GS3_QB * GS3_QB;
+        const int Wn = ntp + Q - 1, Wp = (Wn + 1 + 3) / 4 * 4;
+        // window S[k], k = t - q + Q - 1: coefficient of target t0 + t and source j0 + q
+        const int col0 = t0 - j0 - (Q - 1) + mx;
+        __syncthreads();
+#pragma unroll
+        for (int bb = 0; bb < 3; bb++) {
+            const double* crow = cfs[bb] + (size_t)(tr - sy + my) * M2;
+            for (int k = tid; k < Wp; k += T) {
+                const int col = col0 + k;
+                sc[bb * Wp + k] = (k < Wn && col >= 0 && col < M2) ? crow[col] : 0.0;
+            }
+        }
+        double* sdx = sc + 3 * Wp;
+        double* sdy = sdx + Q;
+        for (int e = tid; e < Q; e += T) {
+            const bool in = e < nq;
+            if (self) {
+                sdx[e] = in ? -r0x[j0 + e] : 0.0;
+                sdy[e] = in ? -r0y[j0 + e] : 0.0;
+            } else {
+                sdx[e] = in ? sgx[(size_t)sy * mx + j0 + e] : 0.0;
+                sdy[e] = in ? sgy[(size_t)sy * mx + j0 + e] : 0.0;
+            }
+        }
+        __syncthreads();
+        if (part < P) {
+            const int nch = Q / GS3_QB;
+            for (int c = part; c < nch; c += P) {
+                const int kmin = tg * GS3_TB - c * GS3_QB + (Q - GS3_QB);
+                double qx[GS3_QB], qy[GS3_QB];
+#pragma unroll
+                for (int j = 0; j < GS3_QB; j++) {
+                    qx[j] = sdx[c * GS3_QB + j];
+                    qy[j] = sdy[c * GS3_QB + j];
+                }
+                const double2* w0 = (const double2*)(sc + kmin);
+                const double2* w1 = (const double2*)(sc + Wp + kmin);
+                const double2* w3 = (const double2*)(sc + 2 * Wp + kmin);
+                double v0[GS3_TB + GS3_QB], v1[GS3_TB + GS3_QB], v3[GS3_TB + GS3_QB];
+#pragma unroll
+                for (int m = 0; m < (GS3_TB + GS3_QB) / 2; m++) {
+                    const double2 a = w0[m], b2 = w1[m], d = w3[m];
+                    v0[2 * m] = a.x;
+                    v0[2 * m + 1] = a.y;
+                    v1[2 * m] = b2.x;
+                    v1[2 * m + 1] = b2.y;
+                    v3[2 * m] = d.x;
+                    v3[2 * m + 1] = d.y;
+                }
+#pragma unroll
+                for (int i = 0; i < GS3_TB; i++) {
+                    double accx = ax[i], accy = ay[i];
+#pragma unroll
+                    for (int j = 0; j < GS3_QB; j++) {
+                        const int m = i + GS3_QB - 1 - j;
+                        accx += v0[m] * qx[j] + v1[m] * qy[j];
+                        accy += v1[m] * qx[j] + v3[m] * qy[j];
+                    }
+                    ax[i] = accx;
+                    ay[i] = accy;
+                }
+            }
+        }
+        if (tid == 0) flops += 8.0 * nt * nq;
+    };
+    // ucx, ucy over the interior of row tr = the sum of the parts' accumulators
+    auto reduce_parts = [&](int tr) {
+        const int t0 = ri0[tr], nt = ri1[tr] - t0 + 1;
+        const int ntg = (nt + GS3_TB - 1) / GS3_TB;
+        const int P = T / ntg > 0 ? T / ntg : 1;
+        const int tg = tid % ntg, part = tid / ntg;
+        __syncthreads();
+        for (int i = 0; i < GS3_TB; i++) {
+            if (part < P) {
+                red[2 * (part * ntg + tg)] = ax[i];
+                red[2 * (part * ntg + tg) + 1] = ay[i];
+            }
+            __syncthreads();
+            for (int g = tid; g < ntg; g += T) {
+                const int c = g * GS3_TB + i;
+                if (c < nt) {
+                    double sx = 0.0, sy = 0.0;
+                    for (int p = 0; p < P; p++) {
+                        sx += red[2 * (p * ntg + g)];
+                        sy += red[2 * (p * ntg + g) + 1];
+                    }
+                    ucx[t0 + c] = sx;
+                    ucy[t0 + c] = sy;
+                }
+            }
+            __syncthreads();
+        }
+    };
+
+    for (int t = I.y0; t < I.y1; t++) {
+        if (ri1[t] < ri0[t]) continue;   // no interior element in this row
+        const size_t r0 = (size_t)t * mx;
+        __syncthreads();
+        for (int ix = tid; ix < mx; ix += T) {
+            rpx[ix] = I.px[r0 + ix];
+            rpy[ix] = I.py[r0 + ix];
+            const double dx = I.dpx[r0 + ix], dy = I.dpy[r0 + ix];
+            rdpx[ix] = dx;
+            rdpy[ix] = dy;
+            r0x[ix] = dx;
+            r0y[ix] = dy;
+            const double pn = I.pn[r0 + ix];
+            rpn[ix] = pn;
+            rg[ix] = I.mu * pn;
+            rwx[ix] = I.wsx[r0 + ix];
+            rwy[ix] = I.wsy[r0 + ix];
+            rzx[ix] = I.zx[r0 + ix];
+            rzy[ix] = I.zy[r0 + ix];
+            rel[ix] = I.el[r0 + ix];
+        }
+#pragma unroll
+        for (int i = 0; i < GS3_TB; i++) ax[i] = ay[i] = 0.0;
+        // minus the row's own influence at its start (z holds it), plus the changes of the
+        // block's rows already walked
+        row_prod(t, t, nullptr, nullptr, true);
+        for (int j = I.y0; j < t; j++)
+            if (rj1[j] >= rj0[j]) row_prod(t, j, I.ddx, I.ddy, false);
+        reduce_parts(t);
+        if (tid < 64) {
+            const int t0 = ri0[t], nt = ri1[t] - t0 + 1;
+            int lastna = t0 - 1 >= 1 ? t0 - 1 : 0;
+            int ixw = t0;
+            gs_walk_span(R, mx, rj0[t], rj1[t], r0, ixw, t0 + nt, lastna, difacc, I.ga_inv, I.eps,
+                         I.mu, I.omegah, I.omegas, I.sx, I.sy);
+        }
+        __syncthreads();
+        for (int ix = tid; ix < mx; ix += T) {
+            I.px[r0 + ix] = rpx[ix];
+            I.py[r0 + ix] = rpy[ix];
+            I.dpx[r0 + ix] = rdpx[ix];
+            I.dpy[r0 + ix] = rdpy[ix];
+            I.ddx[r0 + ix] = rdpx[ix] - r0x[ix];
+            I.ddy[r0 + ix] = rdpy[ix] - r0y[ix];
+            I.el[r0 + ix] = rel[ix];
+        }
+    }
+    if (tid == 0) {
+        I.st[0] += difacc;
+        I.st[3] += flops;
+    }
+}
+
+// end of a sweep (src/m_solvpt.f90:3130-3150): dif = sqrt(sum |dp|^2 / 2k), difid from the rms
+// of p (of p - p0 in a sensitivity solve); counts the sweep and clears its sum.  Returns
+// (itgs, dif, flops, 0, 0, dif1, still sweeping, difid, 0, 0).
+__global__ void __launch_bounds__(GSB_THREADS) k_gsb_check(const GsbItem* items, const int* gstart,
+                                                           int nitems, RedOut ro)
+{
+    __shared__ double red[16 * MAXK];
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const GsbItem& I = items[it];
+    const int n = I.n, tid = threadIdx.x, T = blockDim.x;
+    double sq[1] = {0.0};
+    if (I.p0x) {
+        for (int i = tid; i < n; i += T) {
+            double bx = I.px[i] + -1.0 * I.p0x[i], by = I.py[i] + -1.0 * I.p0y[i];
+            sq[0] += bx * bx + by * by;
+        }
+    } else {
+        for (int i = tid; i < n; i += T) sq[0] += I.px[i] * I.px[i] + I.py[i] * I.py[i];
+    }
+    group_reduce<1>(sq, 0, red);
+    if (tid == 0) {
+        const double rms = sqrt(sq[0] / (double)(2 * n > 1 ? 2 * n : 1));
+        const double dif = sqrt(I.st[0] / (2.0 * I.k));
+        const double difid = I.eps * fmax(1e-6, I.facnel * rms);
+        const double itgs = I.st[1] + 1.0;
+        if (itgs == 1.0) I.st[2] = dif;
+        I.st[1] = itgs;
+        I.st[0] = 0.0;
+        double* r = ro.partials + (size_t)blockIdx.x * 10;
+        r[0] = itgs;
+        r[1] = dif;
+        r[2] = I.st[3];
+        r[3] = 0.0;
+        r[4] = 0.0;
+        r[5] = I.st[2];
+        r[6] = (dif >= difid && itgs < (double)I.maxgs) ? 1.0 : 0.0;
+        r[7] = difid;
+        r[8] = 0.0;
+        r[9] = 0.0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // host-side launchers
 // ------------------------------------------------------------------------------------------
 
@@ -2955,6 +3317,7 @@ int op_nred(int kind, const void* item)
     case OP_ST_AS: return 1;
     case OP_ST_SA: return 1;
     case OP_GD_PROBE: return 10;
+    case OP_GSB_SETUP: case OP_GSB_CHECK: return 10;
     case OP_GD_MOVES: return 1;
     case OP_GD_DIAGRES: return 3;
     case OP_SETUP: return 2;
@@ -2999,6 +3362,7 @@ int op_groups(int kind, const void* item, int conv_stage)
     case OP_GD_MOVES: case OP_GD_DIAGRES: return ngroups_elem(((const GdItem*)item)->n);
     case OP_SETUP: case OP_FORCES: return ngroups_elem(((const SetupItem*)item)->n);
     case OP_STDYGS: return 1;
+    case OP_GSB_SETUP: case OP_GSB_WALK: case OP_GSB_CHECK: return 1;
     case OP_SENS: case OP_SENS_EXT: return ngroups_elem(((const SensItem*)item)->n);
     case OP_GUARD: return (((const GsItem*)item)->my + ELEM_THREADS - 1) / ELEM_THREADS;
     case OP_LEDGE: {
@@ -3061,6 +3425,9 @@ void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, i
     case OP_ST_AS: hipLaunchKernelGGL(k_st_as, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, ro); break;
     case OP_ST_SA: hipLaunchKernelGGL(k_st_sa, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, ro); break;
     case OP_GD_DP: hipLaunchKernelGGL(k_gd_dp, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_GSB_SETUP: hipLaunchKernelGGL(k_gsb_setup, ge, dim3(GSB_THREADS), 0, s, (const GsbItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_GSB_WALK: hipLaunchKernelGGL(k_gsb_walk, ge, dim3(GSB_THREADS), lds_bytes, s, (const GsbItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_GSB_CHECK: hipLaunchKernelGGL(k_gsb_check, ge, dim3(GSB_THREADS), 0, s, (const GsbItem*)d_items, d_gstart, nitems, ro); break;
     case OP_GD_PROJECT: hipLaunchKernelGGL(k_gd_project, ge, dim3(GD_THREADS), lds_bytes, s, (const GdItem*)d_items, d_gstart, nitems, ro); break;
     case OP_GD_PROBE: hipLaunchKernelGGL(k_gd_probe, ge, dim3(GD_THREADS), lds_bytes, s, (const GdItem*)d_items, d_gstart, nitems, ro); break;
     case OP_GD_UPDATE: hipLaunchKernelGGL(k_gd_update, ge, dim3(GD_THREADS), lds_bytes, s, (const GdItem*)d_items, d_gstart, nitems, ro); break;
@@ -3171,6 +3538,7 @@ double op_bytes(int kind, const void* item, int stage)
 size_t op_lds(int kind, const void* item, int stage)
 {
     if (kind == OP_CONV || kind == OP_CONV32) return conv_lds(kind, (const ConvItem*)item, stage);
+    if (kind == OP_GSB_WALK) return gsb_lds(((const GsbItem*)item)->mx);
     if (kind == OP_GD_PROJECT || kind == OP_GD_PROBE || kind == OP_GD_UPDATE) {
         const GdItem* g = (const GdItem*)item;
         int R = gd_rows_per_tile(g->mx, g->my, kind == OP_GD_PROJECT ? GD_TILE_PROJ : GD_TILE);
@@ -3179,6 +3547,7 @@ size_t op_lds(int kind, const void* item, int stage)
     }
     if (kind == OP_STDYGS) {
         const GsItem* g = (const GsItem*)item;
+        if (stage == 3) return g->mx >= 2 ? gsb_lds(g->mx) : MAX_LDS + 1;   // rng needs 4 my <= 2 n
         if (stage == 2) return gs3_lds(g->mx, g->my);
         if (stage == 1) return gs2_lds(g->mx, g->my);
         return (size_t)g->mx * (5 * sizeof(double) + 1) + (size_t)(2 * g->my + 1) * sizeof(int) + 16;

@@ -64,6 +64,9 @@ enum OpKind : int {
     OP_SENS,
     OP_SENS_EXT,
     OP_LEDGE,
+    OP_GSB_SETUP,
+    OP_GSB_WALK,
+    OP_GSB_CHECK,
     OP_COUNT_KINDS
 };
 
@@ -166,20 +169,6 @@ struct GdItem {
     double d_ifc, d_lin, d_cns, d_slp, pow_s;
 };
 
-// SteadyGS (src/m_solvpt.f90:2825-3254): one work-group runs every sweep of one patch
-struct GsItem {
-    int n, mx, my, k;          // k = number of interior elements
-    int maxgs, pad;
-    double eps, omegah, omegas, ga_inv, mu, facnel;
-    int8_t* el;
-    double* px; double* py; const double* pn;
-    double* sx; double* sy;
-    const double* wsx; const double* wsy;
-    double* dpx; double* dpy;  // work: differences along x
-    int* list;                 // work: 2n ints, (element, jy*2mx + jx) of the summation range
-    const double* cf[2][2];    // tangential blocks of cs, layout [(iy+my)*2mx + ix+mx]
-    const double* p0x; const double* p0y;   // sensitivity solve: stopping test on p - p0
-};
 
 // sensitivity solves (sens_norm, src/m_solvpn.f90:531-650; sens_tang, src/m_solvpt.f90:544-796)
 //   OP_SENS, mode 0: copy a patch state (element division and up to 6 grid functions)
@@ -229,6 +218,53 @@ struct ConvItem {
     const void* twLx;          // exp(-2 pi i e/Lx), e < Lx
     const void* twLy;          // exp(-2 pi i e/Ly), e < Ly
     double scale;              // mode 1/2: factor applied to the spectrum
+};
+
+// SteadyGS (src/m_solvpt.f90:2825-3254): one work-group runs every sweep of one patch
+struct GsItem {
+    int n, mx, my, k;          // k = number of interior elements
+    int maxgs, pad;
+    double eps, omegah, omegas, ga_inv, mu, facnel;
+    int8_t* el;
+    double* px; double* py; const double* pn;
+    double* sx; double* sy;
+    const double* wsx; const double* wsy;
+    double* dpx; double* dpy;  // work: differences along x
+    int* list;                 // work: 2n ints, (element, jy*2mx + jx) of the summation range
+    const double* cf[2][2];    // tangential blocks of cs, layout [(iy+my)*2mx + ix+mx]
+    const double* p0x; const double* p0y;   // sensitivity solve: stopping test on p - p0
+    // block variant (GsbItem): the FFT product z = A dp and the block change d (work vectors),
+    // and the product's item (spectra of cs, workspaces, twiddles; in, out, addend set per use)
+    double* zx; double* zy;
+    double* ddx; double* ddy;
+    ConvItem conv;
+};
+
+// SteadyGS by blocks of rows (the sweeps and element updates of OP_STDYGS, src/m_solvpt.f90:
+// 2825-3254).  The influence of the rows outside the block being walked comes from the FFT
+// product z = A dp (cs), formed at each sweep's start and updated by the change d of each
+// walked block; only rows inside a block are summed directly.  One work-group per patch.
+//   mode 0 (set-up): dp = differences along x, the row ranges, d = 0, the sweep state;
+//     returns (first interior row, last interior row)
+//   mode 1 (walk): rows [y0, y1) in order; first clears d on rows [z0, z1)
+//   mode 2 (check, after a sweep): the convergence test; returns
+//     (itgs, dif, flops of the direct sums, 0, 0, dif of the first sweep, still sweeping, 0..)
+struct GsbItem {
+    int n, mx, my, k;
+    int mode, maxgs;
+    int y0, y1, z0, z1;
+    double eps, omegah, omegas, ga_inv, mu, facnel;
+    int8_t* el;
+    double* px; double* py; const double* pn;
+    double* sx; double* sy;
+    const double* wsx; const double* wsy;
+    double* dpx; double* dpy;
+    double* zx; double* zy;
+    double* ddx; double* ddy;
+    int* rng;                  // 4 my ints: summation range [rj0, rj1], interior [ri0, ri1]
+    const double* cf[2][2];
+    const double* p0x; const double* p0y;
+    double* st;                // 4 doubles: squared changes of the sweep, sweeps, dif1, flops
 };
 
 // per-case fields from the case parameters (set_prev_data, set_norm_rhs, set_tang_rhs and

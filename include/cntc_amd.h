@@ -25,11 +25,14 @@ extern "C" {
 
 /* ---- initialisation -------------------------------------------------------------------- */
 
-/* src/contact_addon.f90:237 cntc_initializeFirst */
+/* src/contact_addon.f90:237 cntc_initializeFirst.  The output folder and experiment name of
+   the first call name the .inp file written for CNTC_if_wrtinp (<outdir>/<expnam>.inp, expnam
+   "contact_addon" when none is given). */
 void cntc_initializefirst(int *ifcver, int *ierror, int *ioutput, const char *c_wrkdir,
                           const char *c_outdir, const char *c_expnam, int *len_wrkdir,
                           int *len_outdir, int *len_expnam);
-/* src/contact_addon.f90:476-576 cntc_initialize (imodul must be 3) */
+/* src/contact_addon.f90:476-576 cntc_initialize (imodul must be 3); the output folder of the
+   first initialising call is kept, as in the reference */
 void cntc_initialize(int *ire, int *imodul, int *ifcver, int *ierror, const char *c_outdir,
                      int *len_outdir);
 /* src/m_caddon_data.f90:310-332 cntc_getMaxNumThreads (1: calls are serialised; batches of
@@ -229,6 +232,8 @@ void cntc_finalizelast(void);
 #define CNTC_ic_sbsfil  1992
 #define CNTC_ic_npomax  1993
 #define CNTC_if_idebug  2000
+/* >= 1 per problem: cntc_calculate appends the case to the .inp file in the layout of the
+   reference's wrtinp (src/m_sinput.f90:1506-1862), for replay by the stand-alone program */
 #define CNTC_if_wrtinp  2002
 #define CNTC_if_openmp  2003
 #define CNTC_if_ncase   2005

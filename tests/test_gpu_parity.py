@@ -140,18 +140,23 @@ def test_steadygs_table_golden(cntc, oracle):
     print("table G=0: worst golden difference %.2e" % worst)
 
 
-@pytest.mark.parametrize("kernel", ["overlapped", "staged", "direct"])
+@pytest.mark.parametrize("kernel", ["blocks", "overlapped", "staged", "direct"])
 def test_gdsteady_stagnation_fallback(cntc, oracle, kernel, monkeypatch):
     """Hertzian patches on which GDsteady stagnates: both solvers fall back to SteadyGS
-    (src/m_solvpt.f90:466-485) and must agree to the solver tolerance.  The three SteadyGS
-    kernels are checked: row sums overlapped with the walk along the previous row (the
-    default where its LDS fits, rows up to ~480 elements), row sums before each row's walk,
-    and direct summation (rows too wide for staging)."""
+    (src/m_solvpt.f90:466-485) and must agree to the solver tolerance.  The four SteadyGS
+    variants are checked: blocks of rows with the other rows' influence from FFT products (rows
+    up to ~650 elements), row sums overlapped with the walk along the previous row (the
+    default where its LDS fits), row sums before each row's walk, and direct summation (rows
+    too wide for staging)."""
     from contact_amd import workloads as W
-    if kernel == "direct":
+    if kernel == "blocks":
+        monkeypatch.setenv("CNTC_GS_KERNEL", "4")
+    elif kernel == "direct":
         monkeypatch.setenv("CNTC_GS_KERNEL", "1")
     elif kernel == "staged":
         monkeypatch.setenv("CNTC_GS_KERNEL", "2")
+    elif kernel == "overlapped":
+        monkeypatch.setenv("CNTC_GS_KERNEL", "3")
     cases = W.hertz_batch(n=2048)
     for k in (381, 1505):
         c = cases[k]

@@ -170,3 +170,71 @@ def test_fortran_caller_through_iso_c_binding(cntc, tmp_path):
     else:
         assert rc == -99
     assert "FORTRAN_CALLER_DONE" in r.stdout
+
+
+_INP_SCRIPT = r'''
+import ctypes, sys
+sys.path.insert(0, sys.argv[1])
+from contact_amd import cntc, workloads as W
+i = ctypes.c_int
+L = cntc.lib()
+out, exp = sys.argv[2].encode(), b"replay/exp1"
+L.cntc_initializefirst(ctypes.byref(i(0)), ctypes.byref(i(0)), ctypes.byref(i(0)), b" ", out, exp,
+                       ctypes.byref(i(1)), ctypes.byref(i(len(out))), ctypes.byref(i(len(exp))))
+c = W.base_case(ipotcn=-3, mx=40, my=40, aa=7.193, bb=5.411, scale=1.3, fn=1e5, cksi=-0.001,
+                ceta=0.0, cphi=0.0, eps=1e-8, maxgs=999, maxin=20, fstat=0.3)
+W.apply_case(cntc, 5, 1, c)
+cntc.setflags(5, 1, [cntc.CNTC["if_wrtinp"]], [1])
+print(cntc.calculate(5, 1))
+W.apply_case(cntc, 6, 2, W.rough_patch(n=12))
+cntc.setflags(6, 2, [cntc.CNTC["if_wrtinp"]], [1])
+print(cntc.calculate(6, 2))
+W.apply_case(cntc, 7, 1, c)      # no wrtinp flag: not written
+print(cntc.calculate(7, 1))
+'''
+
+
+def test_inp_writer_replay_file(cntc, tmp_path):
+    """CNTC_if_wrtinp: the case descriptions in <outdir>/<expnam>.inp in the layout of wrtinp
+    (src/m_sinput.f90:1506-1862); written after the solve whatever its outcome (here without a
+    GPU).  Checks the Fortran edit descriptors on known values (Gw.d switches between the F
+    form with four trailing blanks and the E form at 0.1 and 10^d)."""
+    from contact_amd import workloads as W
+    r = subprocess.run([sys.executable, "-c", _INP_SCRIPT, ROOT, str(tmp_path)], capture_output=True,
+                       text=True, timeout=300, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    lines = (tmp_path / "exp1.inp").read_text().split("\n")
+    # case 1: Hertzian 40 x 40 steady rolling with GDsteady (G=5), Fn prescribed
+    assert lines[0] == "   3   5   1  MODULE, RE, CP: case 1 for result element  5, contact problem  1"
+    assert lines[1] == "  203100          P-B-T-N-F-S           PVTIME, BOUND,  TANG,   NORM,   FORCE,  STRESS"
+    assert lines[2] == "  022020          L-D-C-M-Z-E           FRCLAW, DISCNS, INFLCF, MATER,  RZNORM, EXRHS"
+    assert lines[3] == " 0501241        H-G-I-A-O-W-R     HEAT, GAUSEI, IESTIM, MATFIL, OUTPUT, FLOW,   RETURN"
+    assert lines[4] == "   999    20   30    1    1.0E-08        MAXGS , MAXIN , MAXNR , MAXOUT, EPS"
+    g = W.GD_PARAMS
+    f = lines[5].split()
+    assert [float(x) for x in f[:2]] == [round(g["fdecay"], 3), round(g["betath"], 3)] and f[2] == str(g["kdowfb"])
+    assert lines[6] == "  0.1000E+06 -0.1000E-02   0.000       0.000              FN, CKSI, CETA, CPHI"
+    assert lines[7] == "  0.3000      0.3000    " + " " * 34 + "FSTAT, FKIN"
+    chi, dq, veloc = (float(x) for x in lines[8].split()[:3])
+    assert chi == 0.0 and dq > 0.0 and veloc > 0.0 and lines[8].endswith("CHI, DQ, VELOC")
+    assert lines[9] == "  0.2800      0.2800      0.8200E+05  0.8200E+05          POISS 1,2,  GG 1,2"
+    assert lines[10] == "   -3                                                     IPOTCN"
+    assert lines[11] == "   40   40   7.193       5.411       1.300                MX,MY,AA,BB,SCALE"
+    assert lines[12] == " "
+    # case 2: an element-wise gap (ibase 9) on a 12 x 12 grid: no solution, so the tight box is
+    # the centre of the grid (4..8), 5 values per row
+    k = lines.index("   3   6   2  MODULE, RE, CP: case 1 for result element  6, contact problem  2")
+    body = lines[k:]
+    ip = body.index("    1                                                     IPOTCN")
+    assert body[ip + 1].startswith("    5    5") and body[ip + 1].endswith("MX,MY,XL,YL,DX,DY")
+    assert body[ip + 2] == "    9    1                                                IBASE, IPLAN"
+    assert body[ip + 3] == "%  UNDEF.DISTANCE, (1)-(2): SPECIFIED PER ELEMENT"
+    # rows 4..8, each after a blank record (the first row is not row 1)
+    rows = body[ip + 4: ip + 14]
+    assert all(x == " " for x in rows[0::2])
+    assert all(len(x) == 70 and len(x.split()) == 5 for x in rows[1::2])
+    assert body[ip + 14] == "%  UNRESTRICTED PLANFORM"
+    # Gw.d on known values: E form below 0.1 (0.0338 -> 0.338080E-01), F form from 0.1
+    assert rows[1].split()[0] == "0.338080E-01"
+    # problem 7 did not ask for the file
+    assert not any("result element  7" in x for x in lines)
