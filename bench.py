@@ -45,7 +45,15 @@ sys.path.insert(0, ROOT)
 if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
     os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
-IRE_MIXED, IRE_FP64 = 1, 460       # two mixed-precision sets of 228 result elements, then FP64
+IRE_MIXED, IRE_FP64 = 1, 700       # mixed-precision sets of 228 result elements, then FP64
+
+
+def side_streams():
+    """The engine's SteadyGS side streams: CNTC_SIDE_STREAMS, else 7, at most one less than
+    the hardware queues."""
+    hwq = max(2, int(os.environ.get("GPU_MAX_HW_QUEUES", "8")))
+    v = int(os.environ.get("CNTC_SIDE_STREAMS", "0"))
+    return max(1, min(hwq - 1, v)) if v > 0 else min(7, hwq - 1)
 
 
 def ids(k, base):
@@ -60,7 +68,8 @@ def kernel_name(op):
     conv = {"conv32_fwd": "k_rowfwd<float>", "conv32_mul": "k_colmul<float>",
             "conv32_inv": "k_rowinv<float>", "conv_fwd": "k_rowfwd<double>",
             "conv_mul": "k_colmul<double>", "conv_inv": "k_rowinv<double>",
-            "stdygs_staged": "k_stdygs2", "stdygs_overlap": "k_stdygs3"}
+            "stdygs_staged": "k_stdygs2", "stdygs_overlap": "k_stdygs3",
+            "stdygs_block": "steadygs sweeps by blocks (k_gsb_walk, FFT products, k_gsb_check)"}
     return conv.get(op, "k_" + op)
 
 
@@ -69,7 +78,7 @@ def roofline_of(name, sec, amount, nl, total_sec):
     for SteadyGS (amount = algorithmic flops of its direct influence sums)."""
     common = {"kernel": kernel_name(name), "avg_launch_us": round(sec / nl * 1e6, 2),
               "launches": nl, "share_of_device_time": round(sec / total_sec, 3)}
-    if name.startswith("stdygs"):
+    if name.startswith("stdygs") or name == "gsb_walk":
         achieved = amount / sec / 1e12
         r = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
              "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
@@ -354,6 +363,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--inflight", type=int, default=2,
+                    help="batches in flight (each on its own set of 2048 problems, 1-3)")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
     ap.add_argument("--budget", type=float, default=480.0,
@@ -387,11 +398,12 @@ def main():
 
     from contact_amd import cntc
     cntc.lib()
-    # two sets of problems, so that a batch can be submitted while the previous one finishes
-    sets = [[ids(k, IRE_MIXED + 229 * h) for k in range(npatch)] for h in range(2)]
+    # one set of problems per batch in flight, so that a batch can be submitted while the
+    # previous ones finish
+    nfl = max(1, min(3, args.inflight))
+    sets = [[ids(k, IRE_MIXED + 229 * h) for k in range(npatch)] for h in range(nfl)]
     mixed = sets[0]
-    fp64 = [ids(k, IRE_FP64) for k in range(npatch)]
-    for h in range(2):
+    for h in range(nfl):
         for k in range(npatch):
             W.apply_case(cntc, *sets[h][k], cases[k])
 
@@ -401,8 +413,8 @@ def main():
     cntc.servestart(200e9)
 
     def submit_step(s, label):
-        idl = sets[s % 2]
-        for k in range(npatch):       # waits for this set's previous batch, two steps back
+        idl = sets[s % nfl]
+        for k in range(npatch):       # waits for this set's previous batch, nfl steps back
             new_case(cntc, *idl[k], cases[k])
         return submit_batch(cntc, idl, label)
 
@@ -410,14 +422,14 @@ def main():
     reserve = 60.0 if args.plain > 0 or args.serve_seconds > 0 else 5.0
 
     def run_steps(nsteps, label, can_start):
-        """Up to `nsteps` batches, two in flight; `can_start(done_latencies)` decides whether
+        """Up to `nsteps` batches, nfl in flight; `can_start(done_latencies)` decides whether
         another batch may be submitted.  Returns (steps run, per-batch latencies, last
         batch's per-patch latencies)."""
         open_, done, last_lat = [], [], np.array([np.nan])
         for s in range(nsteps):
             if not can_start(done, s):
                 break
-            if len(open_) == 2:
+            if len(open_) == nfl:
                 _, last_lat, tb = finish_batch(cntc, open_.pop(0))
                 done.append(tb)
             open_.append(submit_step(s, "%s %d" % (label, s)))
@@ -431,11 +443,11 @@ def main():
     # warm-up: both sets once (their coefficient spectra are built on first use); more only
     # while everything asked for still fits the budget
     def warm_ok(done, s):
-        if s < 2:
+        if s < nfl:
             return True
         per = (time.perf_counter() - t0) / max(1, s)
         return left() - reserve >= (args.steps + args.warmup - s) * per * 1.1
-    warm_done, t_warm, _ = run_steps(max(2, args.warmup), "warmup", warm_ok)
+    warm_done, t_warm, _ = run_steps(max(nfl, args.warmup), "warmup", warm_ok)
     t_first = time.perf_counter() - t0
 
     cntc.amdprofile(2)
@@ -458,7 +470,7 @@ def main():
     elapsed = time.perf_counter() - t0
     prof = cntc.amdprofileget()
     cntc.amdprofile(False)
-    mixed = sets[(steps_done - 1) % 2]
+    mixed = sets[(steps_done - 1) % nfl]
 
     FM = forces_of(cntc, mixed)
     its = np.array([cntc.getiterations(i, j) for i, j in mixed])
@@ -473,13 +485,14 @@ def main():
         # keep time for the serving leg
         if left() < (30.0 if args.serve_seconds > 0 else 12.0):
             break
-        W.apply_case(cntc, *fp64[k], cases_fp64[k])
+        fp = ids(len(t_plain), IRE_FP64)
+        W.apply_case(cntc, *fp, cases_fp64[k])
         t1 = time.perf_counter()
-        rc = cntc.calculate(*fp64[k])
+        rc = cntc.calculate(*fp)
         t_plain.append(time.perf_counter() - t1)
         if rc != 0:
             raise SystemExit("plain solve of patch %d failed: %d" % (k, rc))
-        FP.append(cntc.getcontactforces(*fp64[k]))
+        FP.append(cntc.getcontactforces(*fp))
         sub.append(k)
     t_plain = np.array(t_plain) if sub else np.array([np.nan])
     # the accuracy of the mixed-precision batch: against the plain FP64 solves of the subset
@@ -496,7 +509,7 @@ def main():
 
     # dominant kernel (device time) over the timed steps, and the dominant one on the main stream
     traffic = read_traffic()
-    kernel_sec = sum(v[0] for v in prof.values())
+    kernel_sec = sum(v[0] for k, v in prof.items() if k != "stdygs_block")
 
     def roof(name, v):
         r = roofline_of(name, v[0], v[1], v[2], kernel_sec)
@@ -505,9 +518,12 @@ def main():
         r["traffic_unit"] = "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/traffic_per_dispatch.txt)"
         return r
 
-    name, top1 = max(prof.items(), key=lambda kv: kv[1][0])
+    # "stdygs_block" times whole sweeps of the block variant (walk, FFT products, tests): not a
+    # kernel; its walk kernel is recorded on its own as "gsb_walk"
+    single = {k: v for k, v in prof.items() if k != "stdygs_block"}
+    name, top1 = max(single.items(), key=lambda kv: kv[1][0])
     roofline = roof(name, top1)
-    main_k = {k: v for k, v in prof.items() if not k.startswith("stdygs")}
+    main_k = {k: v for k, v in prof.items() if not k.startswith("stdygs") and not k.startswith("gsb")}
     roofline_main = roof(*max(main_k.items(), key=lambda kv: kv[1][0])) if main_k else None
     fft = {k: v for k, v in prof.items() if k.startswith("conv")}
     roofline_fft = [roof(k, v) for k, v in sorted(fft.items())]
@@ -549,7 +565,7 @@ def main():
         "batch_latency_ms": {"mean": round(float(np.mean(t_step_a)) * 1e3, 1),
                              "min": round(float(np.min(t_step_a)) * 1e3, 1),
                              "max": round(float(np.max(t_step_a)) * 1e3, 1),
-                             "what": "submit of a batch to its last result; batches overlap by up to one"},
+                             "what": "submit of a batch to its last result; up to %d batches in flight" % nfl},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -560,6 +576,8 @@ def main():
             "patches": npatch, "elements": int(n_elem.sum()), "grid_sides": [20, 256],
             "solver": "NormCG + GDsteady (SteadyGS fall-back), eps 1e-8",
             "precision": "FP32 FFT products in FP64 refinement", "seed": 2024,
+            "batches_in_flight": nfl,
+            "side_streams": side_streams(),
         },
         "force_err_max": float(np.max(e_mixed)),
         "force_err_p99": float(np.percentile(np.max(e_mixed, axis=1), 99)),
@@ -576,7 +594,7 @@ def main():
         "speedup_vs_plain_fp64": round(value / plain_rate, 2),
         "warmup_patches_per_s": round(npatch * max(1, warm_done) / t_first, 2) if warm_done else None,
         "device_time_per_wall": round(kernel_sec / elapsed, 3),
-        "device_time_what": "HIP-event time of the recorded kernels (SteadyGS, GD probe, FFT products) / wall",
+        "device_time_what": "HIP-event time of the recorded kernels (SteadyGS walk, GD probe, FFT products) / wall",
         "kernels": kernels,
         "serving": serving,
         "roofline": roofline,

@@ -409,7 +409,7 @@ OP_NAMES = ["vec", "ncg_negpn", "ncg_dd", "ncg_expand", "ncg_hmin", "sn_contract
             "tcg_as", "tcg_sa", "st_as", "st_sa", "gd_dp", "gd_project", "gd_probe", "gd_update",
             "gd_moves", "gd_diagres", "conv", "conv32", "elascf", "embed", "extract", "setup",
             "forces", "stdygs", "guard", "sens", "sens_ext", "ledge", "gsb_setup", "gsb_walk",
-            "gsb_check"]
+            "gsb_check", "cnvxgs", "dense"]
 
 
 def amdprofile(on):

@@ -151,6 +151,8 @@ union AnyItem {
     ExtractItem ext;
     SetupItem setup;
     GsItem gs;
+    CgsItem cgs;
+    DenseItem dense;
     SensItem sens;
     LedgeItem ledge;
     AnyItem() { std::memset(this, 0, sizeof(*this)); }
@@ -174,6 +176,8 @@ static size_t item_size(int kind)
     case OP_EXTRACT: return sizeof(ExtractItem);
     case OP_SETUP: case OP_FORCES: return sizeof(SetupItem);
     case OP_STDYGS: case OP_GUARD: return sizeof(GsItem);
+    case OP_CNVXGS: return sizeof(CgsItem);
+    case OP_DENSE: return sizeof(DenseItem);
     case OP_SENS: case OP_SENS_EXT: return sizeof(SensItem);
     case OP_LEDGE: return sizeof(LedgeItem);
     }
@@ -512,9 +516,9 @@ private:
         if (pending_.empty()) return;
         // variants: 3 blocks of rows with FFT products, 2 row sums overlapped with the walk,
         // 1 staged row sums, 0 direct sums; the highest whose LDS fits.  CNTC_GS_KERNEL=1..4
-        // caps the variant at 0..3; the default cap is 2
+        // caps the variant at 0..3 (testing)
         const char* forced = std::getenv("CNTC_GS_KERNEL");
-        const int maxvar = forced && forced[0] >= '1' && forced[0] <= '4' ? forced[0] - '1' : 2;
+        const int maxvar = forced && forced[0] >= '1' && forced[0] <= '4' ? forced[0] - '1' : 3;
         // largest patches first; one launch per idle side stream, each of patches of similar
         // size (down to a quarter of the launch's largest, at most 64), because a launch
         // returns when its longest item ends and a launch queued behind another on the same
@@ -616,6 +620,14 @@ private:
         std::vector<Launch> seq;
         double seconds = 0.0;
         bool timed = false;
+        // profiling: events around each walk launch of the sweep in flight, their summed time
+        std::vector<hipEvent_t> wev;
+        int nwev = 0, nwalk = 0;
+        double walk_seconds = 0.0;
+        ~GsbGroup()
+        {
+            for (auto e : wev) (void)hipEventDestroy(e);
+        }
     };
     static GsbItem gsb_item(const GsItem& g, int mode, double* st)
     {
@@ -790,7 +802,13 @@ private:
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, op.t0, op.t1));
             G.seconds += ms * 1e-3;
+            for (int j = 0; j < G.nwev; j += 2) {
+                HIPCHK(hipEventElapsedTime(&ms, G.wev[j], G.wev[j + 1]));
+                G.walk_seconds += ms * 1e-3;
+                G.nwalk++;
+            }
         }
+        G.nwev = 0;
         if (G.first.empty()) {
             // set-up done: interior rows and blocks per patch
             G.first.assign(nitems, 0);
@@ -830,11 +848,16 @@ private:
             for (int i = 0; i < nitems; i++)
                 for (int j = 0; j < K; j++) r[(size_t)i * K + j] = G.fin[i][j];
             if (G.timed) {
+                // the sweeps as a whole (walks, products and tests), and the walk kernel alone
+                // with the flops of its direct sums
                 std::lock_guard<std::mutex> lk(prof_mtx);
                 auto& pe = prof[OP_STDYGS * 4 + 3];
                 pe.seconds += G.seconds;
-                for (int i = 0; i < nitems; i++) pe.bytes += G.fin[i][2];
                 pe.launches++;
+                auto& pw = prof[OP_GSB_WALK * 4];
+                pw.seconds += G.walk_seconds;
+                for (int i = 0; i < nitems; i++) pw.bytes += G.fin[i][2];
+                pw.launches += G.nwalk;
             }
             return true;
         }
@@ -849,8 +872,21 @@ private:
             } else {
                 RedOut ro{};
                 if (L.kind == OP_GSB_CHECK) ro.partials = (double*)(op.dev + op.res_off);
+                const bool tw = G.timed && L.kind == OP_GSB_WALK;
+                if (tw) {
+                    while ((int)G.wev.size() < G.nwev + 2) {
+                        hipEvent_t e;
+                        HIPCHK(hipEventCreate(&e));
+                        G.wev.push_back(e);
+                    }
+                    HIPCHK(hipEventRecord(G.wev[G.nwev], op.stream));
+                }
                 launch_op(L.kind, d + L.items, (const int*)(d + L.gst[0]), L.n, L.ng[0], ro, 0, L.lds[0],
                           op.stream);
+                if (tw) {
+                    HIPCHK(hipEventRecord(G.wev[G.nwev + 1], op.stream));
+                    G.nwev += 2;
+                }
             }
             HIPCHK(hipGetLastError());
         }
@@ -1017,8 +1053,9 @@ private:
             }
             RedOut ro{dpart_ + L.part_off, L.K > 0 ? dcnt_ + L.cnt_off : nullptr, dres_ + L.res_off,
                       (const int*)(dstage_ + L.modes_off)};
-            // SteadyGS writes its one group's values as the item's results
-            if (L.kind == OP_STDYGS) ro = RedOut{dres_ + L.res_off, nullptr, nullptr, nullptr};
+            // SteadyGS and ConvexGS write their one group's values as the item's results
+            if (L.kind == OP_STDYGS || L.kind == OP_CNVXGS)
+                ro = RedOut{dres_ + L.res_off, nullptr, nullptr, nullptr};
             launch_op(L.kind, dstage_ + L.items_off, (const int*)(dstage_ + L.gstart_off), L.nitems,
                       L.ngroups, ro, L.stage, L.lds, s_);
             if (hipError_t le = hipGetLastError(); le != hipSuccess) {
@@ -1122,6 +1159,48 @@ struct ResAwait {
 // the item of the FFT product u[ik] (mask) = sum_jk C[ik][jk] * p[jk] on patch P,
 // src/m_aijpj.f90:258-451.  which: 0 = cs, 1 = cv, 2 = ms; fp32 selects the single-precision
 // spectra.
+// the dense product of the same blocks as conv_item (which 0: cs, 1: cv, 2: preconditioner)
+static DenseItem dense_item(const Patch* P, int which, int mask, int nout, const int* ik,
+                           double* const* out, int nin, const int* jk, double* const* in,
+                           const double* const* addend)
+{
+    DenseItem d{};
+    d.mx = P->mx;
+    d.my = P->my;
+    d.nin = nin;
+    d.nout = nout;
+    d.mask = mask;
+    const size_t nb = (size_t)(2 * P->mx) * (2 * P->my);
+    const double* base = which == 0 ? P->cf : (which == 1 ? P->cfv : P->cfm);
+    for (int j = 0; j < nin; j++) d.in[j] = in[j];
+    for (int i = 0; i < nout; i++) {
+        d.out[i] = out[i];
+        d.addend[i] = addend ? addend[i] : nullptr;
+        for (int j = 0; j < nin; j++) {
+            const int a = ik[i], b = jk[j];
+            const double* c = nullptr;
+            if (which == 2) {
+                if (a == b) c = base + (size_t)a * nb;
+            } else if (P->nt_cpl || !((a == 2) != (b == 2))) {
+                c = base + (size_t)(a * 3 + b) * nb;
+            }
+            d.cf[i][j] = c;
+        }
+    }
+    d.el = P->el;
+    d.ga_inv = 1.0 / P->ga;   // every product, the preconditioner's included, is (1/G) cf * in
+    return d;
+}
+
+// Where the dense FP64-MFMA product replaces the FFT product on patches of at most
+// DENSE_MAX_N elements (CNTC_DENSE): 1 (default) in the sensitivity solves, 2 in every
+// product, 0 nowhere.
+static int dense_mode()
+{
+    const char* v = std::getenv("CNTC_DENSE");
+    return v ? std::max(0, std::min(2, std::atoi(v))) : 1;
+}
+
 static ConvItem conv_item(const Patch* P, int which, int mask, int nout, const int* ik,
                           double* const* out, int nin, const int* jk, double* const* in,
                           const double* const* addend, bool fp32)
@@ -1221,6 +1300,13 @@ struct Ctx {
               double* const* in, const double* const* addend = nullptr, bool fp32 = false)
     {
         AnyItem it;
+        if (P->dense) {
+            // small patch: the dense product on FP64 MFMA from the coefficient tables (FP64
+            // also where the FFT product would run in FP32)
+            it.dense = dense_item(P, which, mask, nout, ik, out, nin, jk, in, addend);
+            op(OP_DENSE, it);
+            return;
+        }
         it.conv = conv_item(P, which, mask, nout, ik, out, nin, jk, in, addend, fp32);
         op(fp32 ? OP_CONV32 : OP_CONV, it);
     }
@@ -1608,7 +1694,12 @@ bool Engine::prepare(Patch& P)
         R.ierror = ERR_OTHER;
         return false;
     }
-    if (mx < 2 || my < 2) { R.ierror = ERR_OTHER; last_error = "grids need mx, my >= 2"; return false; }
+    // a single row (2-D problems) only with ConvexGS: the row-walk solvers are not set up for it
+    if (mx < 2 || my < 1 || (my < 2 && in.gausei != 2)) {
+        R.ierror = ERR_OTHER;
+        last_error = "grids need mx >= 2 and my >= 2 (my = 1 with ConvexGS, G=2)";
+        return false;
+    }
     bool newgrid = mx != P.mx || my != P.my;
     if (newgrid && P.have_solution && (in.pvtime != 2 || in.iestim != 0)) {
         R.ierror = ERR_OTHER;
@@ -1621,7 +1712,8 @@ bool Engine::prepare(Patch& P)
     bool is_roll = in.tang == 2 || in.tang == 3;
     P.chi = in.chi;
     P.dq = in.dq > 0 ? in.dq : dx;
-    if (in.tang == 3) {
+    if (in.tang == 3 && in.gausei != 2) {
+        // steady rolling with SteadyGS or GDsteady: dq = dx; ConvexGS (G=2) keeps dq
         if (std::fabs(P.chi) > 0.01 && std::fabs(P.chi - PI) > 0.01) P.chi = 0.0;
         P.dq = dx;
     }
@@ -1630,7 +1722,7 @@ bool Engine::prepare(Patch& P)
     // transient rolling with dq > dx: elements entering the contact area near the leading edge
     // spend part of the step in it (sxbnd/subnd, src/m_leadedge.f90); with dq <= dx every
     // interior element has facdt = 1 and no leading-edge position is used
-    P.ledge = in.tang == 2 && P.dq > dx * (1.0 + 1e-12);
+    P.ledge = (in.tang == 2 || (in.tang == 3 && in.gausei == 2)) && P.dq > dx * (1.0 + 1e-12);
     // set_norm_rhs, src/m_sdis.f90:321-494: the quadratic gap is formed on the GPU; the host
     // keeps a copy only where it is needed (element-wise input, or eldiv0's initial estimate)
     int ihertz = in.ipotcn < 0 ? -in.ipotcn : 0;
@@ -1815,9 +1907,23 @@ bool Engine::prepare(Patch& P)
 // src/m_aijpj.f90:457-708)
 // ------------------------------------------------------------------------------------------
 
-static Task setup_coefficients(Ctx C, bool need_cv)
+// need_cv: the coefficients cv of the previous time (transient rolling) with their spectra;
+// cv_table: cv without spectra (steady rolling, for ConvexGS's cs - cv).  The tables and spectra
+// of the previous case are kept when grid, material, rolling step and precision are unchanged
+// (the reference keeps its spectra c%fft_cf the same way, src/m_aijpj.f90:873-914); the
+// solve that allocates nothing shares its memory between problems and always recomputes.
+static Task setup_coefficients(Ctx C, bool need_cv, bool cv_table = false)
 {
     Patch& P = *C.P;
+    Patch::CoefKey key;
+    key.cf = P.cf; key.mx = P.mx; key.my = P.my; key.Lx = P.Lx; key.Ly = P.Ly;
+    key.cvmode = (need_cv ? 1 : 0) + (cv_table ? 2 : 0);
+    key.precision = P.in.precision;
+    key.dx = P.dx; key.dy = P.dy; key.ak = P.ak; key.nu = P.nu; key.ga = P.ga;
+    key.chi = P.chi; key.dq = P.dq;
+    if (P.coef_valid && key == P.coef_key) co_return 0;
+    P.coef_valid = false;
+    P.prec_valid = 0;
     int mx = P.mx, my = P.my;
     size_t nb = (size_t)(2 * mx) * (2 * my);
     int M = P.Lx / 2, H = M + 1;
@@ -1829,7 +1935,7 @@ static Task setup_coefficients(Ctx C, bool need_cv)
     it.coef.akv = P.ak; it.coef.nuv = P.nu;
     it.coef.cf = P.cf;
     C.op(OP_ELASCF, it);
-    if (need_cv) {
+    if (need_cv || cv_table) {
         AnyItem iv = it;
         iv.coef.xshft = std::cos(P.chi) * P.dq;
         iv.coef.yshft = std::sin(P.chi) * P.dq;
@@ -1837,7 +1943,7 @@ static Task setup_coefficients(Ctx C, bool need_cv)
         C.op(OP_ELASCF, iv);
     }
     co_await C.count(M_ALL);   // completes the coefficient generation before the roll step
-    if (need_cv) launch_rollcoef((int)nb, P.cf, P.cfv, C.E->stream());
+    if (need_cv || cv_table) launch_rollcoef((int)nb, P.cf, P.cfv, C.E->stream());
 
     // spectra of the blocks: embed (wrap, scaled by 1/(G M Ly)) then forward transform
     double scale = (1.0 / P.ga) / ((double)M * (double)P.Ly);
@@ -1877,6 +1983,8 @@ static Task setup_coefficients(Ctx C, bool need_cv)
             launch_to_float(2 * bsz, (const double*)((char*)P.spec + (size_t)b * bsz * 16),
                             (float*)((char*)P.spec32 + (size_t)b * bsz * 8), C.E->stream());
     }
+    P.coef_key = key;
+    P.coef_valid = !C.E->noalloc_active();
     co_return 0;
 }
 
@@ -1885,6 +1993,7 @@ static Task setup_coefficients(Ctx C, bool need_cv)
 static Task setup_prec(Ctx C, int kk)
 {
     Patch& P = *C.P;
+    if (P.coef_valid && (P.prec_valid >> kk & 1u)) co_return 0;
     int mx = P.mx, my = P.my;
     size_t nb = (size_t)(2 * mx) * (2 * my);
     int M = P.Lx / 2, H = M + 1;
@@ -1936,6 +2045,7 @@ static Task setup_prec(Ctx C, int kk)
     if (P.in.precision == 1)
         launch_to_float(2 * bsz, (const double*)((char*)P.specm + (size_t)kk * bsz * 16),
                         (float*)((char*)P.specm32 + (size_t)kk * bsz * 8), C.E->stream());
+    if (P.coef_valid) P.prec_valid |= 1u << kk;
     co_return 0;
 }
 
@@ -2187,7 +2297,10 @@ static Task sens_norm(Ctx C, double* unn)
     snap(C, P.el, el0, {P.pn}, {ps0});
     int it = 0;
     double err = 0.0;
+    const bool dense0 = P.dense;
+    P.dense = dense0 || (dense_mode() >= 1 && P.n <= DENSE_MAX_N);
     int rc = co_await normcg(C, 0, P.in.mxsens, P.in.epsens, &penloc, &it, &err, ps0);
+    P.dense = dense0;
     r = co_await C.sum(M_ALL, P.pn);
     double fnloc = P.dxdy * r.v[0];
     sens_at(P, IOUT_FN, IIN_PEN) = (fnloc - P.fntrue) / dpen;
@@ -2730,6 +2843,50 @@ static Task stdygs(Ctx C, int k, double omegah, double omegas, int maxgs, double
     co_return 0;
 }
 
+// ConvexGS, src/m_solvpt.f90:2446-2821: all sweeps in one launch per patch (on the main
+// stream: the solver of G=2 and of steady rolling with contact in the trailing-edge column).
+// p0x, p0y: initial estimate of a sensitivity solve; info as for SteadyGS.
+static Task cnvxgs(Ctx C, int k, double omegah, double omegas, int maxgs, double eps, int* itgs_out,
+                   double* err_out, int* info, const double* p0x = nullptr, const double* p0y = nullptr)
+{
+    Patch& P = *C.P;
+    AnyItem it;
+    CgsItem& g = it.cgs;
+    g.n = P.n; g.mx = P.mx; g.my = P.my; g.k = k;
+    g.maxgs = maxgs; g.steady = P.in.tang == 3 ? 1 : 0;
+    g.eps = eps; g.omegah = omegah; g.omegas = omegas;
+    g.ga_inv = 1.0 / P.ga;
+    g.mu = P.in.fstat;
+    g.facnel = (double)std::sqrt((float)P.n / (float)std::max(1, k));
+    g.el = P.el;
+    g.px = P.px; g.py = P.py; g.pn = P.pn;
+    g.sx = P.sx; g.sy = P.sy;
+    g.wsx = P.wsx; g.wsy = P.wsy;
+    g.lix = P.lix;
+    g.ubx = P.w[3]; g.uby = P.w[4];
+    g.list = (int*)P.w[2];
+    const size_t blk = (size_t)(2 * P.mx) * (2 * P.my);
+    const size_t off[2][2] = {{0, blk}, {3 * blk, 4 * blk}};
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            g.cf[a][b] = P.cf + off[a][b];
+            g.cv[a][b] = P.cfv + off[a][b];
+        }
+    g.p0x = p0x; g.p0y = p0y;
+    Res r = co_await C.opr(OP_CNVXGS, it);
+    const int itgs = (int)r.v[0];
+    const double dif = r.v[1], dif1 = r.v[5];
+    double conv = 1.0;
+    if (dif * dif1 != 0.0 && itgs > 1) conv = std::exp(std::log(dif / dif1) / (itgs - 1));
+    *info = 0;
+    if (itgs >= maxgs) *info = 1;
+    if (itgs >= maxgs && conv > sp(0.997f)) *info = 2;
+    if (itgs >= maxgs && conv > 1.0) *info = 3;
+    *itgs_out = itgs;
+    *err_out = dif;
+    co_return 0;
+}
+
 // tang_solver, src/m_solvpt.f90:382-493: GDsteady for T=3 with G=5, restarted by SteadyGS when it
 // stagnates (:466-485); SteadyGS for other G in steady rolling; TangCG for T=1, 2.  *it counts
 // the iterations of both solvers of a fall-back; *info is set by SteadyGS only.  p0x, p0y: the
@@ -2740,7 +2897,10 @@ static Task tang_solver(Ctx C, int k, const double cf00[4], double omegah, doubl
 {
     Patch& P = *C.P;
     int it = 0, rc;
-    if (P.in.tang == 3 && P.in.gausei == 5) {
+    if (P.cnvx) {
+        rc = co_await cnvxgs(C, k, omegah, omegas, maxit, eps, &it, err_out, info, p0x, p0y);
+        *it_out = std::abs(it);
+    } else if (P.in.tang == 3 && P.in.gausei == 5) {
         int lstagn = 0;
         rc = co_await gdsteady(C, cf00, maxit, eps, &it, err_out, &lstagn);
         P.out.gd_stagnated = lstagn;
@@ -2891,6 +3051,8 @@ static Task sens_tang(Ctx C, int k, const double cf00[4], double omegah, double 
         if (F >= 2) add_rigid_slip(C, wbx, wby, wbx, wby, 1, P.in.ceta);
     }
     int stagn0 = P.out.gd_stagnated, used0 = P.out.used_stdygs, ret = 0;
+    const bool dense0 = P.dense;
+    P.dense = dense0 || (dense_mode() >= 1 && P.n <= DENSE_MAX_N);
     for (int icrp = 0; icrp < 3 && !ret; icrp++) {
         double dcrp = 0.0001 * ch[icrp];
         add_rigid_slip(C, wbx, wby, P.wsx, P.wsy, icrp, dcrp);
@@ -2910,6 +3072,7 @@ static Task sens_tang(Ctx C, int k, const double cf00[4], double omegah, double 
         }
         snap(C, el0, P.el, {p0x, p0y, s0x, s0y}, {P.px, P.py, P.sx, P.sy});
     }
+    P.dense = dense0;
     snap(C, nullptr, nullptr, {wbx, wby}, {P.wsx, P.wsy});
     P.out.gd_stagnated = stagn0;
     P.out.used_stdygs = used0;
@@ -3032,20 +3195,20 @@ static Task stang(Ctx C, int* ittang_out)
     const int k = (int)r.v[0];
     double cf00[4] = {0.0, 0.0, 0.0, 0.0};
     {
-        // steady rolling needs exterior elements at the trailing edge, else the reference
-        // switches to ConvexGS (src/m_stang.f90:131-143, 185-190); ConvexGS (G=2) is not
-        // provided.  The same launch returns the coefficients of element (0,0).
-        if (P.in.tang == 3 && P.in.gausei == 2) co_return ERR_OTHER;
+        // the solver (src/m_stang.f90:145-190): ConvexGS for G=2, and for steady rolling with
+        // contact in the trailing-edge column (no exterior element to start a row's walk).  The
+        // same launch returns the coefficients of element (0,0).
         AnyItem gi;
         gi.gs = gs_item(P, k, 1.0, 1.0, P.in.maxgs, P.in.eps);
         Res g = co_await C.opr(OP_GUARD, gi);
-        if (P.in.tang == 3 && g.v[0] > 0) co_return ERR_OTHER;
+        P.cnvx = P.in.gausei == 2 || (P.in.tang == 3 && g.v[0] > 0);
         for (int q = 0; q < 4; q++) cf00[q] = g.v[1 + q];
     }
-    // relaxation of SteadyGS (src/m_stang.f90:192-223)
+    // relaxation (src/m_stang.f90:192-223)
     double omegah = P.in.omegah, omegas = P.in.omegas;
     if (P.in.gausei == 0 || P.in.gausei == 4 || P.in.gausei == 5) {
         if (k <= 25) { omegah = 1.0; omegas = 1.0; }
+        else if (P.in.tang == 3 && P.cnvx) { omegah = 0.5; omegas = 0.5; }
         else if (P.in.tang == 3) {
             double ar = P.dx / P.dy;
             if (ar <= 5.0) { omegah = 0.9; omegas = 1.0; }
@@ -3055,7 +3218,7 @@ static Task stang(Ctx C, int* ittang_out)
     }
     // stang_rhs: ws = -facdt h_t (- A'_tt(cv) pv_t for T=2) on the interior; facdt = 1 for
     // interior elements when dq = dx (src/m_leadedge.f90:290-318)
-    if (P.ledge) {
+    if (P.ledge && P.in.tang == 2) {
         // transient rolling with dq > dx: facdt and the leading-edge positions from the element
         // division (sxbnd), then (1a) in the interior and (1b) at the leading edge, where the
         // previous-time displacement is replaced by ubnd, interpolated from A pv (cs)
@@ -3070,6 +3233,20 @@ static Task stang(Ctx C, int* ittang_out)
         C.conv_t(0, M_ALL, bx, by, P.pvx, P.pvy);
         L.mode = 1;
         L.hx = P.hx; L.hy = P.hy; L.ux = ux; L.uy = uy; L.bx = bx; L.by = by;
+        L.wsx = P.wsx; L.wsy = P.wsy;
+        C.op(OP_LEDGE, li);
+    } else if (P.in.tang == 3 && P.cnvx) {
+        // steady rolling with ConvexGS: facdt and the leading-edge positions (sxbnd, the edge
+        // 1 dx beyond ix0), ws = -facdt h_t; ubnd enters ConvexGS's shift at each sweep
+        AnyItem li;
+        LedgeItem& L = li.ledge;
+        L.n = P.n; L.mx = P.mx; L.my = P.my; L.mode = 0;
+        L.el = P.el; L.facdt = P.facdt; L.lix = P.lix;
+        L.xc1 = P.xc1; L.dx = P.dx; L.dq = P.dq; L.fxdfac = 1.0;
+        C.op(OP_LEDGE, li);
+        L.mode = 1;
+        L.steady = 1;
+        L.hx = P.hx; L.hy = P.hy;
         L.wsx = P.wsx; L.wsy = P.wsy;
         C.op(OP_LEDGE, li);
     } else {
@@ -3130,6 +3307,7 @@ static Task calculate(Ctx C)
 {
     Patch& P = *C.P;
     CaseResult& R = P.out;
+    P.dense = dense_mode() == 2 && P.n <= DENSE_MAX_N;
     {
         // set_prev_data, set_norm_rhs, set_tang_rhs, init_curr_data (src/m_sdis.f90:208-768)
         AnyItem it;
@@ -3173,7 +3351,7 @@ static Task calculate(Ctx C)
             for (int ii = IIN_KSI; ii <= IIN_PHI; ii++)
                 for (int io = IOUT_FN; io <= IOUT_MZ; io++) sens_at(P, io, ii) = 0.0;
     }
-    co_await setup_coefficients(C, P.in.tang == 2);
+    co_await setup_coefficients(C, P.in.tang == 2, P.in.tang == 3);
     int itnorm = 0, ittang = 0;
     // B=2: the Hertzian pressure is the normal solution (panprc skips NORM, src/m_scontc.f90:409-411)
     int rc = P.in.bound == 2 ? 0 : co_await snorm(C, &itnorm);

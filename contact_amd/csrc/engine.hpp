@@ -89,6 +89,25 @@ struct Patch {
     double dx = 1, dy = 1, dxdy = 1, xc1 = 0, yc1 = 0;
     int Lx = 0, Ly = 0;
     bool have_solution = false;
+    bool cnvx = false;               // the tangential solver of the case in progress is ConvexGS
+    bool dense = false;              // influence products by the dense FP64-MFMA kernel (n <= 2500)
+    // What the coefficient tables and spectra in the arena (cf, cfv, spec, specv, spec32 and the
+    // preconditioner's cfm, specm, specm32) were computed for.  A case with the same key reuses
+    // them; prec_valid has bit kk set when preconditioner block kk is there too.
+    struct CoefKey {
+        const void* cf = nullptr;
+        int mx = 0, my = 0, Lx = 0, Ly = 0, cvmode = 0, precision = 0;
+        double dx = 0, dy = 0, ak = 0, nu = 0, ga = 0, chi = 0, dq = 0;
+        bool operator==(const CoefKey& o) const
+        {
+            return cf == o.cf && mx == o.mx && my == o.my && Lx == o.Lx && Ly == o.Ly &&
+                   cvmode == o.cvmode && precision == o.precision && dx == o.dx && dy == o.dy &&
+                   ak == o.ak && nu == o.nu && ga == o.ga && chi == o.chi && dq == o.dq;
+        }
+    };
+    CoefKey coef_key;
+    bool coef_valid = false;
+    unsigned prec_valid = 0;
     int slot = -1;                   // scheduler slot during a solve
     // sensitivities sens(iout, iin) in internal units, iout + 4 iin: outputs fn, fx/(mu fn),
     // fy/(mu fn), mz; inputs pen, cksi, ceta, cphi (src/m_hierarch_data.f90:122-124)
@@ -158,6 +177,7 @@ public:
     static size_t noalloc_bytes();
     static size_t noalloc_patch_limit();   // bytes of arena and twiddle tables that fit
     hipStream_t stream() const { return stream_; }
+    bool noalloc_active() const { return noalloc_; }
     // download the element data of a patch
     void get_field(const Patch& p, int what, double* out) const;
     void get_eldiv(const Patch& p, int* out) const;

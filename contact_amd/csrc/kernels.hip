@@ -1711,7 +1711,12 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_ledge(const LedgeItem* items, 
         const int i = base + e * ELEM_THREADS + threadIdx.x;
         if (i >= I.n) break;
         double wx = 0.0, wy = 0.0;
-        if (I.el[i] >= ADHES) {
+        if (I.el[i] >= ADHES && I.steady) {
+            // steady rolling (stang_rhs (1a), (1b) with T=3): the rigid slip times facdt; the
+            // displacement at the leading edge enters the solver's shift instead
+            wx = -I.facdt[i] * I.hx[i];
+            wy = -I.facdt[i] * I.hy[i];
+        } else if (I.el[i] >= ADHES) {
             const double f = I.facdt[i];
             wx = -f * I.hx[i];
             wy = -f * I.hy[i];
@@ -3291,6 +3296,215 @@ __global__ void __launch_bounds__(GSB_THREADS) k_gsb_check(const GsbItem* items,
 }
 
 // ------------------------------------------------------------------------------------------
+// ConvexGS, src/m_solvpt.f90:2446-2821 (CgsItem)
+// ------------------------------------------------------------------------------------------
+
+// One wavefront per patch runs every sweep (the element updates are sequential; each shift is
+// a direct sum over the interior elements, split over the 64 lanes and reduced by shuffles).
+__global__ void __launch_bounds__(64) k_cnvxgs(const CgsItem* items, const int* gstart, int nitems,
+                                               RedOut ro)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const CgsItem& I = items[it];
+    const int mx = I.mx, my = I.my, n = I.n, lane = threadIdx.x, M2 = 2 * mx;
+    // the interior elements in grid order (iel, src/m_stang.f90:115-124)
+    int k = 0;
+    for (int base = 0; base < n; base += 64) {
+        const int i = base + lane;
+        const bool in = i < n && I.el[i] >= ADHES;
+        const unsigned long long m = __ballot(in);
+        const int pos = __popcll(m & ((1ull << lane) - 1ull));
+        if (in) I.list[k + pos] = i;
+        k += __popcll(m);
+    }
+    __syncthreads();
+    const size_t c0 = (size_t)my * M2 + mx;   // offset (0, 0)
+    double coefs[2][2], coefsv[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            coefs[a][b] = I.ga_inv * I.cf[a][b][c0];
+            coefsv[a][b] = I.steady ? I.ga_inv * (I.cf[a][b][c0] - I.cv[a][b][c0]) : coefs[a][b];
+        }
+    // the influence at element i of the current tractions, A(cs) or A(cs - cv) (unscaled)
+    auto shift = [&](int i, bool csv, double& ux, double& uy) {
+        const int ix = i % mx, iy = i / mx;
+        double ax = 0.0, ay = 0.0;
+        for (int q = lane; q < k; q += 64) {
+            const int j = I.list[q], jx = j % mx, jy = j / mx;
+            const size_t c = (size_t)(iy - jy + my) * M2 + (ix - jx + mx);
+            double cxx = I.cf[0][0][c], cxy = I.cf[0][1][c], cyx = I.cf[1][0][c], cyy = I.cf[1][1][c];
+            if (csv) {
+                cxx -= I.cv[0][0][c];
+                cxy -= I.cv[0][1][c];
+                cyx -= I.cv[1][0][c];
+                cyy -= I.cv[1][1][c];
+            }
+            const double qx = I.px[j], qy = I.py[j];
+            ax += cxx * qx + cxy * qy;
+            ay += cyx * qx + cyy * qy;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            ax += __shfl_xor(ax, off);
+            ay += __shfl_xor(ay, off);
+        }
+        ux = ax;
+        uy = ay;
+    };
+    int itgs = 0;
+    double dif = 2.0, difid = 1.0, dif1 = 0.0, flops = 0.0;
+    while (dif >= difid && itgs < I.maxgs) {
+        itgs++;
+        double difacc = 0.0;
+        if (I.steady) {
+            // subnd (src/m_leadedge.f90:336-394) with the edge 1 dx beyond ix0: ubnd = u(ix0 + 1)
+            for (int q = 0; q < k; q++) {
+                const int i = I.list[q];
+                const int j0 = (int)I.lix[i];
+                if (j0 < 0) continue;
+                double bx = 0.0, by = 0.0;
+                if (j0 + 3 <= mx) {
+                    shift((i / mx) * mx + j0 + 1, false, bx, by);
+                    bx *= I.ga_inv;
+                    by *= I.ga_inv;
+                }
+                if (lane == 0) {
+                    I.ubx[i] = bx;
+                    I.uby[i] = by;
+                }
+            }
+            __syncthreads();
+        }
+        for (int q = 0; q < k; q++) {
+            const int i = I.list[q];
+            const int8_t e0 = I.el[i];
+            // after 1000 sweeps the slip elements are skipped on odd sweeps (:2603)
+            if (!(itgs <= 1000 || itgs % 2 == 0 || e0 == ADHES)) continue;
+            const bool zledge = I.steady && I.lix[i] >= 0.0;
+            double ux, uy;
+            shift(i, I.steady && !zledge, ux, uy);
+            double sv[2] = {I.wsx[i] + I.ga_inv * ux, I.wsy[i] + I.ga_inv * uy};
+            if (zledge) {
+                sv[0] -= I.ubx[i];
+                sv[1] -= I.uby[i];
+            }
+            const double (*cc)[2] = (!I.steady || zledge) ? coefs : coefsv;
+            const double opx = I.px[i], opy = I.py[i];
+            double pr[3] = {opx, opy, I.pn[i]};
+            int8_t e8 = e0;
+            elmtrc(e8, cc[0][0], cc[0][1], cc[1][0], cc[1][1], I.eps, pr, I.mu, sv, I.omegah, I.omegas, false);
+            difacc += (opx - pr[0]) * (opx - pr[0]) + (opy - pr[1]) * (opy - pr[1]);
+            if (lane == 0) {
+                I.px[i] = pr[0];
+                I.py[i] = pr[1];
+                I.sx[i] = sv[0];
+                I.sy[i] = sv[1];
+                I.el[i] = e8;
+            }
+            __syncthreads();
+        }
+        flops += 8.0 * (double)k * (double)k;
+        double sq = 0.0;
+        for (int i = lane; i < n; i += 64) {
+            if (I.p0x) {
+                const double bx = I.px[i] + -1.0 * I.p0x[i], by = I.py[i] + -1.0 * I.p0y[i];
+                sq += bx * bx + by * by;
+            } else {
+                sq += I.px[i] * I.px[i] + I.py[i] * I.py[i];
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off);
+        const double rms = sqrt(sq / (double)(2 * n > 1 ? 2 * n : 1));
+        dif = sqrt(difacc / (2.0 * I.k));
+        difid = I.eps * fmax(1e-6, I.facnel * rms);
+        if (itgs == 1) dif1 = dif;
+    }
+    if (lane == 0) {
+        double* r = ro.partials + (size_t)blockIdx.x * 10;
+        r[0] = (double)itgs;
+        r[1] = dif;
+        r[2] = flops;
+        r[3] = 0.0;
+        r[4] = 0.0;
+        r[5] = dif1;
+        for (int q = 6; q < 10; q++) r[q] = 0.0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Dense influence product on FP64 MFMA (DenseItem)
+// ------------------------------------------------------------------------------------------
+
+typedef double dense_v4 __attribute__((ext_vector_type(4)));
+
+// One wavefront per tile of 16 x 16 targets (ix0 + i, iy0 + j).  With d = iy - jy fixed the
+// product is an ordinary matrix product: the Toeplitz block T_d (T_d[ix][jx] = cf(ix - jx, d))
+// times the source rows jy = iy - d.  v_mfma_f64_16x16x4f64 takes A = 16 targets x 4 sources
+// (lane i + 16 k holds A[i][k]) and B = 4 sources x 16 target rows (lane j + 16 k holds B[k][j]);
+// lane j + 16 q holds D[4 q + r][j], r = 0..3.  The k loop runs over d and the sources in
+// steps of 4; coefficients outside the tables and sources outside the grid enter as zeros.
+__global__ void __launch_bounds__(64) k_dense(const DenseItem* items, const int* gstart, int nitems)
+{
+    const int it = find_item(gstart, nitems, blockIdx.x);
+    const DenseItem& I = items[it];
+    const int mx = I.mx, my = I.my, M2 = 2 * mx, lane = threadIdx.x;
+    const int ntx = (mx + 15) / 16;
+    const int t = blockIdx.x - gstart[it];
+    const int ix0 = (t % ntx) * 16, iy0 = (t / ntx) * 16;
+    const int i = lane & 15, q = lane >> 4;
+    dense_v4 acc[3];
+#pragma unroll
+    for (int o = 0; o < 3; o++) acc[o] = (dense_v4){0.0, 0.0, 0.0, 0.0};
+    const int ia = ix0 + i;       // this lane's target in A (row i)
+    const int jb = iy0 + i;       // this lane's target row in B (column i)
+    const int d0 = max(iy0 - (my - 1), -(my - 1)), d1 = min(iy0 + 15, my - 1);
+    for (int d = d0; d <= d1; d++) {
+        const int jy = jb - d;
+        const bool rowok = jb < my && jy >= 0 && jy < my;
+        const size_t crow = (size_t)(d + my) * M2;
+        for (int jx0 = 0; jx0 < mx; jx0 += 4) {
+            const int jx = jx0 + q;
+            const bool aok = jx < mx && ia < mx;
+            const bool bok = jx < mx && rowok;
+            const size_t ca = aok ? crow + (size_t)(ia - jx + mx) : 0;
+            const size_t sb = bok ? (size_t)jy * mx + jx : 0;
+            double b[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++) b[j] = (j < I.nin && bok) ? I.in[j][sb] : 0.0;
+#pragma unroll
+            for (int o = 0; o < 3; o++) {
+                if (o >= I.nout) break;
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    if (j >= I.nin) break;
+                    const double* c = I.cf[o][j];
+                    if (!c) continue;
+                    const double a = aok ? c[ca] : 0.0;
+                    acc[o] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[j], acc[o], 0, 0, 0);
+                }
+            }
+        }
+    }
+    const int iy = iy0 + i;
+    if (iy >= my) return;
+#pragma unroll
+    for (int o = 0; o < 3; o++) {
+        if (o >= I.nout) break;
+        double* dst = I.out[o];
+        const double* add = I.addend[o];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int ix = ix0 + 4 * q + r;
+            if (ix >= mx) continue;
+            const size_t ii = (size_t)iy * mx + ix;
+            if (I.mask != M_ALL && !in_mask(I.mask, I.el[ii])) continue;
+            double u = acc[o][r] * I.ga_inv;
+            if (add) u += add[ii];
+            dst[ii] = u;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // host-side launchers
 // ------------------------------------------------------------------------------------------
 
@@ -3318,6 +3532,7 @@ int op_nred(int kind, const void* item)
     case OP_ST_SA: return 1;
     case OP_GD_PROBE: return 10;
     case OP_GSB_SETUP: case OP_GSB_CHECK: return 10;
+    case OP_CNVXGS: return 10;
     case OP_GD_MOVES: return 1;
     case OP_GD_DIAGRES: return 3;
     case OP_SETUP: return 2;
@@ -3363,6 +3578,11 @@ int op_groups(int kind, const void* item, int conv_stage)
     case OP_SETUP: case OP_FORCES: return ngroups_elem(((const SetupItem*)item)->n);
     case OP_STDYGS: return 1;
     case OP_GSB_SETUP: case OP_GSB_WALK: case OP_GSB_CHECK: return 1;
+    case OP_CNVXGS: return 1;
+    case OP_DENSE: {
+        const DenseItem* d = (const DenseItem*)item;
+        return ((d->mx + 15) / 16) * ((d->my + 15) / 16);
+    }
     case OP_SENS: case OP_SENS_EXT: return ngroups_elem(((const SensItem*)item)->n);
     case OP_GUARD: return (((const GsItem*)item)->my + ELEM_THREADS - 1) / ELEM_THREADS;
     case OP_LEDGE: {
@@ -3425,6 +3645,8 @@ void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, i
     case OP_ST_AS: hipLaunchKernelGGL(k_st_as, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, ro); break;
     case OP_ST_SA: hipLaunchKernelGGL(k_st_sa, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, ro); break;
     case OP_GD_DP: hipLaunchKernelGGL(k_gd_dp, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_CNVXGS: hipLaunchKernelGGL(k_cnvxgs, ge, dim3(64), 0, s, (const CgsItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_DENSE: hipLaunchKernelGGL(k_dense, ge, dim3(64), 0, s, (const DenseItem*)d_items, d_gstart, nitems); break;
     case OP_GSB_SETUP: hipLaunchKernelGGL(k_gsb_setup, ge, dim3(GSB_THREADS), 0, s, (const GsbItem*)d_items, d_gstart, nitems, ro); break;
     case OP_GSB_WALK: hipLaunchKernelGGL(k_gsb_walk, ge, dim3(GSB_THREADS), lds_bytes, s, (const GsbItem*)d_items, d_gstart, nitems, ro); break;
     case OP_GSB_CHECK: hipLaunchKernelGGL(k_gsb_check, ge, dim3(GSB_THREADS), 0, s, (const GsbItem*)d_items, d_gstart, nitems, ro); break;
@@ -3521,6 +3743,15 @@ double op_bytes(int kind, const void* item, int stage)
         return (c->mode == 0 ? c->nout : c->nin) * npt * outb;
     }
     case OP_ELASCF: { const CoefItem* c = (const CoefItem*)item; return 9.0 * 8.0 * (2 * c->mx) * (2 * c->my); }
+    case OP_DENSE: {
+        // FP64 flops of the dense matrix-vector products: 2 n^2 per nonzero block
+        const DenseItem* d = (const DenseItem*)item;
+        int nblk = 0;
+        for (int o = 0; o < d->nout; o++)
+            for (int j = 0; j < d->nin; j++) nblk += d->cf[o][j] != nullptr;
+        const double n = (double)d->mx * d->my;
+        return 2.0 * n * n * nblk;
+    }
     case OP_STDYGS: { const GsItem* g = (const GsItem*)item; return (double)g->n * 73.0; }
     case OP_GUARD: { const GsItem* g = (const GsItem*)item; return (double)g->my; }
     case OP_SENS: {

@@ -67,6 +67,8 @@ enum OpKind : int {
     OP_GSB_SETUP,
     OP_GSB_WALK,
     OP_GSB_CHECK,
+    OP_CNVXGS,
+    OP_DENSE,
     OP_COUNT_KINDS
 };
 
@@ -194,6 +196,7 @@ struct SensItem {
 // interpolated from b = A pv (cs) between ix0 and ix0 + 1 where lix >= 0.
 struct LedgeItem {
     int n, mx, my, mode;
+    int steady, pad;           // mode 1, steady rolling: ws = -facdt h only
     const int8_t* el;
     double* facdt; double* lix;
     const double* hx; const double* hy;
@@ -265,6 +268,43 @@ struct GsbItem {
     const double* cf[2][2];
     const double* p0x; const double* p0y;
     double* st;                // 4 doubles: squared changes of the sweep, sweeps, dif1, flops
+};
+
+// ConvexGS (src/m_solvpt.f90:2446-2821, chi = 0, no plasticity): one wavefront per patch runs
+// every sweep.  Point Gauss-Seidel over the interior elements in grid order, each shift by
+// direct summation with the current tractions: s = ws + A(cs) p (T=1, 2), s = ws + A(cs - cv) p
+// in steady rolling, and near its leading edge (lix >= 0) s = ws + A(cs) p - ubnd with ubnd
+// from A(cs) p at the sweep's start.  Returns (itgs, dif, flops, 0, 0, dif1, 0, 0, 0, 0).
+struct CgsItem {
+    int n, mx, my, k;
+    int maxgs, steady;
+    double eps, omegah, omegas, ga_inv, mu, facnel;
+    int8_t* el;
+    double* px; double* py; const double* pn;
+    double* sx; double* sy;
+    const double* wsx; const double* wsy;
+    const double* lix;         // steady rolling: 0-based leading-edge position, or -1
+    double* ubx; double* uby;  // work: ubnd per element
+    int* list;                 // work: the interior elements (k)
+    const double* cf[2][2];    // tangential blocks of cs, layout [(iy+my)*2mx + ix+mx]
+    const double* cv[2][2];    // the same blocks of cv (steady rolling)
+    const double* p0x; const double* p0y;   // sensitivity solve: stopping test on p - p0
+};
+
+// Dense influence product of a small patch (n <= DENSE_MAX_N) on FP64 MFMA: the same product
+// as OP_CONV mode 0, out_i = addend_i + (1/G) sum_j A_ij in_j over the (out, in) blocks, written
+// where the mask holds (src/m_aijpj.f90:99-254, 424-448: the reference's direct summation).
+// The block-Toeplitz matrix is applied as a sum over the row offsets d = iy - jy of ordinary
+// matrix products (the Toeplitz block of offset d times the source rows jy = iy - d).
+constexpr int DENSE_MAX_N = 2500;
+struct DenseItem {
+    int mx, my, nin, nout, mask, pad;
+    const double* in[3];
+    double* out[3];
+    const double* addend[3];   // added to the product where written (may be null)
+    const double* cf[3][3];    // [out][in] coefficient tables [(iy+my)*2mx + ix+mx], null = zero
+    const int8_t* el;
+    double ga_inv;
 };
 
 // per-case fields from the case parameters (set_prev_data, set_norm_rhs, set_tang_rhs and

@@ -9,12 +9,12 @@
  * reference routine it replaces.
  *
  * Scope of the solver behind cntc_calculate: module 3 (basic contact), elastic half-space with
- * quasi-identical materials (K = 0), Coulomb friction (L = 0), uniform grids, creepages
- * prescribed (F = 0), T = 0..3 with chi = 0, normal problem by NormCG, steady rolling by
- * GDsteady (G=5, with the reference's SteadyGS fall-back) or SteadyGS (G=0, 3, 4), shift and
- * transient rolling (dq = dx) by TangCG.  Other settings (ConvexGS: G=2, or steady rolling
- * without exterior elements at the trailing edge) make cntc_calculate return
- * CNTC_err_other (-99) instead of solving.
+ * quasi-identical materials (K = 0), Coulomb friction (L = 0), uniform grids, creepages or
+ * tangential forces prescribed (F = 0, 1, 2), T = 0..3 with chi = 0, normal problem by NormCG,
+ * steady rolling by GDsteady (G=5, with the reference's SteadyGS fall-back) or SteadyGS (G=0,
+ * 3, 4), shift and transient rolling by TangCG, and ConvexGS for G=2 and for steady rolling
+ * with contact in the trailing-edge column.  Settings outside this subset make cntc_calculate
+ * return CNTC_err_other (-99) instead of solving.
  */
 #ifndef CNTC_AMD_H
 #define CNTC_AMD_H

@@ -346,6 +346,7 @@ struct orc_problem {
     int ierror;
     int gd_stagnated;
     int used_stdygs;
+    int use_cnvxgs;     /* the tangential solver of this case is ConvexGS (set by stang) */
     /* sensitivities sens(iout, iin), column-major as in the reference (outpt%sens) */
     double sens[16];
     /* initial estimates of a sensitivity solve: the stopping tests measure the change from
@@ -2154,14 +2155,98 @@ static int set_facdt(orc_problem *P, int tang, double fxdfac)
 /* subnd, src/m_leadedge.f90:336-394: the tangential displacement difference at the leading edge
    of element ii's row, interpolated between ix0 and ix0 + 1 from the previous tractions pv (all
    components, direct sums with the coefficients cs and pv's element division igv) */
-static double subnd(orc_problem *P, int ii, int ik, double fxdfac)
+static double subnd_of(orc_problem *P, int ii, int ik, double fxdfac, double **p, const eldiv *pel)
 {
     int mx = P->mx, iy = ii / mx, ix0 = P->lix[ii] + 1;   /* 1-based */
     if (ix0 + 2 > mx) return 0.0;
     int i0 = ix0 + iy * mx;                                /* 1-based index of ix0 */
-    double u0 = aijpj(P, i0, ik, P->pv, -3, &P->cs, &P->igv);
-    double u1 = aijpj(P, i0 + 1, ik, P->pv, -3, &P->cs, &P->igv);
+    double u0 = aijpj(P, i0, ik, p, -3, &P->cs, pel);
+    double u1 = aijpj(P, i0 + 1, ik, p, -3, &P->cs, pel);
     return (1.0 - fxdfac) * u0 + fxdfac * u1;
+}
+
+static double subnd(orc_problem *P, int ii, int ik, double fxdfac)
+{
+    return subnd_of(P, ii, ik, fxdfac, P->pv, &P->igv);
+}
+
+/* cnvxgs, src/m_solvpt.f90:2446-2821 (chi = 0, no plasticity, no third body): point
+   Gauss-Seidel over the interior elements in grid order, each element's shift by direct
+   summation with the current tractions.  Transient (T=1, 2): s = ws + A(cs) p.  Steady rolling
+   (T=3): s = ws + A(csv) p, and near the leading edge (lix >= 0) s = ws + A(cs) p - ubnd with
+   ubnd interpolated from A(cs) p at the sweep's start (subnd, src/m_leadedge.f90:336-394).
+   After 1000 sweeps the slip elements are skipped on odd sweeps (:2603). */
+static int cnvxgs(orc_problem *P, double **ws, const double *mus, int k, double eps, int maxgs,
+                  double omegah, double omegas, int is_ssrol, int *itgs_out, double *err_out,
+                  int *info)
+{
+    int npot = P->npot, *el = P->igs.el;
+    double *px = P->ps[0], *py = P->ps[1], *pn = P->ps[2];
+    double *sx = P->ss[0], *sy = P->ss[1];
+    double coefs[2][2], coefsv[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            coefs[a][b] = P->cs.ga_inv * cf_at(&P->cs, a, b, 0, 0);
+            coefsv[a][b] = P->csv.ga_inv * cf_at(&P->csv, a, b, 0, 0);
+        }
+    int *iel = ialloc(npot), n = 0;
+    for (int i = 0; i < npot; i++)
+        if (el[i] >= ORC_ADHES) iel[n++] = i;
+    double *ubx = dalloc(npot), *uby = dalloc(npot);
+    int nadh, nslip, nplast, nexter;
+    count_el(el, npot, &nadh, &nslip, &nplast, &nexter);
+    double facnel = (double)sqrtf((float)npot / (float)(nadh + nslip + nplast));
+    int itgs = 0;
+    double dif = 2.0, difid = 1.0, dif1 = 0.0;
+    while (dif >= difid && itgs < maxgs) {
+        itgs++;
+        dif = 0.0;
+        if (is_ssrol)
+            for (int i = 0; i < n; i++) {
+                int ii = iel[i];
+                if (P->lix[ii] < 0) continue;
+                ubx[ii] = subnd_of(P, ii, 1, 1.0, P->ps, &P->igs);
+                uby[ii] = subnd_of(P, ii, 2, 1.0, P->ps, &P->igs);
+            }
+        for (int i = 0; i < n; i++) {
+            int ii = iel[i];
+            if (!(itgs <= 1000 || itgs % 2 == 0 || el[ii] == ORC_ADHES)) continue;
+            int zledge = is_ssrol && P->lix[ii] >= 0;
+            double s[2];
+            if (!is_ssrol) {
+                s[0] = ws[0][ii] + aijpj(P, ii + 1, 1, P->ps, -2, &P->cs, &P->igs);
+                s[1] = ws[1][ii] + aijpj(P, ii + 1, 2, P->ps, -2, &P->cs, &P->igs);
+            } else if (!zledge) {
+                s[0] = ws[0][ii] + aijpj(P, ii + 1, 1, P->ps, -2, &P->csv, &P->igs);
+                s[1] = ws[1][ii] + aijpj(P, ii + 1, 2, P->ps, -2, &P->csv, &P->igs);
+            } else {
+                s[0] = ws[0][ii] + aijpj(P, ii + 1, 1, P->ps, -2, &P->cs, &P->igs) - ubx[ii];
+                s[1] = ws[1][ii] + aijpj(P, ii + 1, 2, P->ps, -2, &P->cs, &P->igs) - uby[ii];
+            }
+            double pr[3] = {px[ii], py[ii], pn[ii]};
+            elmtrc(&el[ii], (!is_ssrol || zledge) ? coefs : coefsv, eps, omegah, omegas, pr, mus[ii], s);
+            dif += sqr(px[ii] - pr[0]) + sqr(py[ii] - pr[1]);
+            px[ii] = pr[0];
+            py[ii] = pr[1];
+            sx[ii] = s[0];
+            sy[ii] = s[1];
+        }
+        dif = sqrt(dif / (2.0 * k));
+        double rms = P->p0[0] ? gf_rms_diff(ALLELM, el, npot, px, P->p0[0], py, P->p0[1])
+                              : gf_rms(ALLELM, el, npot, px, py);
+        difid = eps * dmax(1e-6, facnel * rms);
+        if (itgs == 1) dif1 = dif;
+    }
+    double conv = 1.0;
+    if (dif * dif1 != 0.0 && itgs > 1) conv = exp(log(dif / dif1) / (itgs - 1));
+    *info = 0;
+    if (itgs >= maxgs) *info = 1;
+    if (itgs >= maxgs && conv > sp(0.997f)) *info = 2;
+    if (itgs >= maxgs && conv > 1.0) *info = 3;
+    *itgs_out = itgs;
+    *err_out = dif;
+    free(iel); free(ubx); free(uby);
+    return 0;
 }
 
 /* tang_solver, src/m_solvpt.f90:382-493: GDsteady for T=3 with G=5, restarted by SteadyGS when
@@ -2174,7 +2259,10 @@ static int tang_solver(orc_problem *P, const orc_input *inp, double **ws, const 
 {
     int rc, it1 = 0;
     areas(&P->igs, P->mx, P->my);
-    if (inp->tang == 3 && inp->gausei == 5) {
+    if (P->use_cnvxgs) {
+        rc = cnvxgs(P, ws, mus, k, eps, maxit, omegah, omegas, inp->tang == 3, &it1, err, info);
+        *it = abs(it1);
+    } else if (inp->tang == 3 && inp->gausei == 5) {
         int lstagn = 0;
         rc = gdsteady(P, G, maxit, eps, ws, mus, &it1, err, &lstagn);
         P->gd_stagnated = lstagn;
@@ -2467,19 +2555,21 @@ static int stang(orc_problem *P, const orc_input *inp, int *ittang_out)
     for (int i = 0; i < npot; i++)
         if (el[i] >= ORC_ADHES) k++;
     if (k <= 0) { free(mus); return CNTC_ERR_OTHER; }
-    int rc = set_facdt(P, inp->tang, 1.0);
+    /* the solver (src/m_stang.f90:145-190): ConvexGS for G=2, and for steady rolling with
+       contact in the trailing-edge column (no exterior element to start the row's walk) */
+    int icount = 0;
+    for (int iy = 0; iy < my; iy++)
+        if (el[iy * mx] >= ORC_ADHES) icount++;
+    P->use_cnvxgs = inp->gausei == 2 || (inp->tang == 3 && icount > 0);
+    /* sxbnd: the leading edge 1 dx beyond the last interior element, 2 dx for SteadyGS */
+    int is_stdygs = inp->tang == 3 && !P->use_cnvxgs && inp->gausei != 5;
+    int rc = set_facdt(P, inp->tang, is_stdygs ? 2.0 : 1.0);
     if (rc) { free(mus); return rc; }
-    /* steady rolling needs exterior elements at the trailing edge; otherwise the reference
-       switches to ConvexGS (src/m_stang.f90:131-143, 185-190), which is not restated here */
-    if (inp->tang == 3) {
-        for (int iy = 0; iy < my; iy++)
-            if (el[iy * mx] >= ORC_ADHES) { free(mus); return CNTC_ERR_OTHER; }
-        if (inp->gausei == 2) { free(mus); return CNTC_ERR_OTHER; }
-    }
-    /* relaxation parameters of SteadyGS, src/m_stang.f90:192-223 */
+    /* relaxation parameters, src/m_stang.f90:192-223 */
     double omegah = inp->omegah, omegas = inp->omegas;
     if (inp->gausei == 0 || inp->gausei == 4 || inp->gausei == 5) {
         if (k <= 25) { omegah = 1.0; omegas = 1.0; }
+        else if (inp->tang == 3 && P->use_cnvxgs) { omegah = 0.5; omegas = 0.5; }
         else if (inp->tang == 3) {
             if (P->dx / P->dy <= 5.0) { omegah = 0.9; omegas = 1.0; }
             else if (P->dx / P->dy <= 15.0) { omegah = 0.8; omegas = 0.8; }
@@ -2775,10 +2865,11 @@ int orc_calculate(orc_problem *P, const orc_input *inp)
             P->y[ii] = yc1 + (iy - 1) * dy;
         }
 
-    /* check_roll_stepsize, src/m_sdis.f90:125-204 (GDsteady is not ConvexGS, G != 2) */
+    /* check_roll_stepsize, src/m_sdis.f90:125-204: steady rolling with SteadyGS or GDsteady
+       uses dq = dx; ConvexGS (G=2) keeps dq */
     int is_roll = inp->tang == 2 || inp->tang == 3;
     P->dq = inp->dq > 0.0 ? inp->dq : dx;
-    if (inp->tang == 3) {
+    if (inp->tang == 3 && inp->gausei != 2) {
         if (fabs(P->chi) > 0.01 && fabs(P->chi - PI) > 0.01) P->chi = 0.0;
         P->dq = dx;
     }

@@ -273,6 +273,8 @@ def case_input(c):
     inp.fdecay, inp.betath, inp.kdowfb = g["fdecay"], g["betath"], g["kdowfb"]
     inp.d_ifc, inp.d_lin, inp.d_cns, inp.d_slp, inp.pow_s = (g["d_ifc"], g["d_lin"], g["d_cns"],
                                                              g["d_slp"], g["pow_s"])
+    if inp.gausei in (2, 3):
+        inp.omegah, inp.omegas = c.get("omegah", 0.5), c.get("omegas", 0.5)
     if c.get("pvtime") is not None:
         inp.pvtime = c["pvtime"]
         inp.iestim = c.get("iestim", 0)

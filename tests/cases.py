@@ -148,3 +148,21 @@ TESTBANK_CHAIN_GOLDEN = {
     15: (-2.185e-03, 0.6570, 652, 321, 22, -5.21),      # :1514, :1536-1543
     16: (-2.124e-03, 0.6563, 610, 363, 15, None),       # :1605, :1642
 }
+
+
+def carter_convexgs_case():
+    """testbank/testbank.inp:88-103, case 6: the Carter 2-D example (one row, dy = 200) in
+    steady rolling with ConvexGS (G=2, omegah = omegas = 0.8) and dq = 5 dx; Fn prescribed,
+    h = 0.001 x^2."""
+    from contact_amd import workloads as W
+    dx, dy = 0.04, 200.0
+    return W.base_case(tang=3, norm=1, fn=35780.0, cksi=-0.00024, ceta=0.0, cphi=0.0, fstat=0.3,
+                       g=82000.0, nu=0.28, eps=1e-5, maxgs=999, maxin=100, gausei=2, omegah=0.8,
+                       omegas=0.8, ipotcn=3, mx=55, my=1, xl=-1.0450 - 0.5 * dx, yl=-0.5 * dy,
+                       dx=dx, dy=dy, prmudf=[0.001, 0.0, 0.0, 0.0, 0.0, 0.0], dq=0.2,
+                       veloc=30000.0, pvtime=2)
+
+
+# testbank/testbank.ref_out:551-601 (case 6): Fx, Fx/(mu Fn), contact, adhesion, slip,
+# NormCG iterations, ConvexGS sweeps, approach, pmax as printed
+CARTER_GOLDEN = (6974.0, 0.6497, 51, 30, 21, 11, 163, 6.492e-3, 113.9)

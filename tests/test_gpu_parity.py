@@ -144,10 +144,9 @@ def test_steadygs_table_golden(cntc, oracle):
 def test_gdsteady_stagnation_fallback(cntc, oracle, kernel, monkeypatch):
     """Hertzian patches on which GDsteady stagnates: both solvers fall back to SteadyGS
     (src/m_solvpt.f90:466-485) and must agree to the solver tolerance.  The four SteadyGS
-    variants are checked: blocks of rows with the other rows' influence from FFT products (rows
-    up to ~650 elements), row sums overlapped with the walk along the previous row (the
-    default where its LDS fits), row sums before each row's walk, and direct summation (rows
-    too wide for staging)."""
+    variants are checked: blocks of rows with the other rows' influence from FFT products (the
+    default, rows up to ~650 elements), row sums overlapped with the walk along the previous
+    row, row sums before each row's walk, and direct summation (rows too wide for staging)."""
     from contact_amd import workloads as W
     if kernel == "blocks":
         monkeypatch.setenv("CNTC_GS_KERNEL", "4")
@@ -170,8 +169,8 @@ def test_gdsteady_stagnation_fallback(cntc, oracle, kernel, monkeypatch):
 
 
 def test_steadygs_wide_rows(cntc, oracle):
-    """SteadyGS (G=0) on rows of 520 elements: the row-staged kernel then needs more than 64 KB
-    of LDS per work-group (gfx950 allows 160 KB), and the overlapped one does not fit."""
+    """SteadyGS (G=0) on rows of 520 elements: the block variant then needs about 126 KB of LDS
+    per work-group (gfx950 allows 160 KB)."""
     c = C.hertz_case(12.0, 3.0, 520, 16, 82000.0, cksi=0.0008, ceta=-0.0004, cphi=0.0002,
                      eps=1e-6, maxgs=299, maxin=5, gausei=0)
     g = run_gpu(cntc, c)

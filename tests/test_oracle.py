@@ -423,3 +423,39 @@ def test_line_contact_hertz_options(oracle):
     assert pn.sum() * dx * dy == pytest.approx(5e4, rel=2e-2)
     inside = np.abs(yc1 + np.arange(20) * dy) <= 3.0
     assert np.allclose(pn[inside], pn[inside][0], rtol=0, atol=0) and np.all(pn[~inside] == 0)
+
+
+def test_convexgs_carter_golden(oracle):
+    """testbank case 6 (ConvexGS, steady rolling with dq = 5 dx and the leading-edge
+    correction): every printed figure of testbank.ref_out, and Fy below the reference's print
+    filter 0.5 eps mu Fn (src/m_soutpt.f90:536)."""
+    c = C.carter_convexgs_case()
+    P = oracle.Problem()
+    assert P.calculate(*C.to_oracle(oracle, c)) == 0
+    fn, fx, fy, mz = P.forces()
+    el = P.eldiv()
+    its = P.iters()
+    g = C.CARTER_GOLDEN
+    print("carter: fx", fx, "A/S", (el == 1).sum(), (el == 2).sum(), "iters", its, "pen", P.pen())
+    assert round(fx) == g[0] and round(fx / (0.3 * fn), 4) == g[1]
+    assert ((el > 0).sum(), (el == 1).sum(), (el == 2).sum()) == g[2:5]
+    assert its[2] == g[5] and its[3] == g[6]
+    assert float("%.3e" % P.pen()) == g[7] and round(P.tractions()[0].max(), 1) == g[8]
+    assert abs(fy) < 0.5 * c["eps"] * 0.3 * fn
+
+
+def test_convexgs_equals_other_solvers(oracle):
+    """ConvexGS solves the same equations as TangCG (shift, T=1) and as SteadyGS (steady
+    rolling at dq = dx, where A(cs - cv) p = A(cs) dp): at eps 1e-11 the converged forces agree
+    to 1e-8 of Fn and the element states are identical."""
+    for tang, other in ((1, 0), (3, 0)):
+        base = C.hertz_case(5.0, 4.0, 22, 18, 6e4, cksi=-0.0009, ceta=0.0003, cphi=0.1e-3,
+                            eps=1e-11, maxgs=3000, maxin=30, tang=tang, gausei=other)
+        cg = dict(base, gausei=2, omegah=0.6, omegas=0.6)
+        Pa, Pb = oracle.Problem(), oracle.Problem()
+        assert Pa.calculate(*C.to_oracle(oracle, base)) == 0
+        assert Pb.calculate(*C.to_oracle(oracle, cg)) == 0
+        fa, fb = np.array(Pa.forces()[:3]), np.array(Pb.forces()[:3])
+        print("T=%d" % tang, "G=%d" % other, fa, "ConvexGS", fb, Pb.iters())
+        assert np.abs(fa - fb).max() <= 1e-8 * fa[0]
+        assert np.array_equal(Pa.eldiv(), Pb.eldiv())

@@ -37,7 +37,7 @@ def both(cntc, c, ire_ref, ire_na, name):
     print(name, "forces", cntc.getcontactforces(ire_na, 1), flush=True)
 
 
-def test_noalloc_equals_calculate(cntc):
+def test_noalloc_equals_calculate(cntc, monkeypatch):
     for ire in range(90, 100):
         cntc.initialize(ire)
     # two calls in a row, the second with a smaller patch
@@ -46,10 +46,14 @@ def test_noalloc_equals_calculate(cntc):
     both(cntc, big, 90, 91, "40x40 GDsteady")
     small = C.cattaneo_case()
     both(cntc, small, 92, 93, "17x17 TangCG")
-    # SteadyGS (G=0), run on the main stream by this entry point
+    # SteadyGS (G=0), run on the main stream by this entry point with the one-work-group kernel
+    # (row sums overlapped with the walk); cntc_calculate is pinned to the same kernel, since
+    # its default (blocks of rows with FFT products) sums in another order
     gs = C.hertz_case(5.0, 3.5, 26, 20, 6e4, cksi=-0.0012, ceta=0.0003, eps=1e-8, maxgs=999,
                       maxin=20, gausei=0)
+    monkeypatch.setenv("CNTC_GS_KERNEL", "3")
     both(cntc, gs, 94, 95, "26x20 SteadyGS")
+    monkeypatch.delenv("CNTC_GS_KERNEL")
     # transient rolling: continuation and warm start from the solution kept on the host
     for step in range(3):
         c = C.hertz_case(6.0, 4.0, 30, 24, 8e4, cksi=-0.0005 * (step + 1), ceta=0.0001,
