@@ -285,6 +285,40 @@ def serve_leg(cntc, W, seconds, rate, nchan=64, seed=77, ire0=801, sens_frac=0.1
             "elements_per_channel_mean": round(float(np.mean([c["mx"] * c["my"] for c in chans])), 1)}
 
 
+def sens_leg(cntc, W, n=256, seed=31, ire0=920):
+    """Sensitivities of small patches: `n` Hertzian steady-rolling patches of at most 2500
+    elements (the batch's size distribution below that bound) solved in one batch with
+    CNTC_ic_sens = 3, so that one NormCG and three GDsteady perturbation solves per patch run
+    their influence products on the dense FP64-MFMA kernel.  Reports the batch's patches/s
+    and that kernel's FP64 rate from HIP events on its stream."""
+    cases = [c for c in W.hertz_batch(n=4096, seed=seed) if c["mx"] * c["my"] <= 2500][:n]
+    ids = [(ire0 + k // 9, 1 + k % 9) for k in range(len(cases))]
+    for (ire, icp), c in zip(ids, cases):
+        W.apply_case(cntc, ire, icp, dict(c, sens=3))
+    cntc.amdprofile(2)
+    t0 = time.perf_counter()
+    err = cntc.calculatebatch([i for i, _ in ids], [j for _, j in ids])
+    dt = time.perf_counter() - t0
+    prof = cntc.amdprofileget()
+    cntc.amdprofile(False)
+    if np.any(err != 0):
+        raise SystemExit("sensitivity leg failed: %s" % sorted(set(err)))
+    out = {"workload": "%d Hertzian steady-rolling patches of 80-2500 elements with sensitivities "
+                       "(S=3), one batch" % len(cases),
+           "patches": len(cases), "seconds": round(dt, 3), "patches_per_s": round(len(cases) / dt, 1),
+           "elements_mean": round(float(np.mean([c["mx"] * c["my"] for c in cases])), 1)}
+    if "dense" in prof:
+        sec, flops, nl = prof["dense"]
+        achieved = flops / sec / 1e12
+        out["roofline_dense"] = {
+            "bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "kernel": "k_dense",
+            "flops_per_launch": round(flops / nl), "avg_launch_us": round(sec / nl * 1e6, 2), "launches": nl,
+            "note": "algorithmic FP64 flops of the dense products (2 n^2 per nonzero 2x2 or normal "
+                    "block) over the launch time; v_mfma_f64_16x16x4f64"}
+    return out
+
+
 def forces_of(cntc, idl):
     return np.array([cntc.getcontactforces(i, j) for i, j in idl])
 
@@ -366,6 +400,8 @@ def main():
     ap.add_argument("--inflight", type=int, default=2,
                     help="batches in flight (each on its own set of 2048 problems, 1-3)")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--sens-patches", type=int, default=256,
+                    help="small patches in the sensitivity leg (dense FP64-MFMA products); 0 skips it")
     ap.add_argument("--serve-rate", type=float, default=1000.0)
     ap.add_argument("--budget", type=float, default=480.0,
                     help="wall-clock seconds for the whole run: warm-up and timed steps stop "
@@ -419,7 +455,7 @@ def main():
         return submit_batch(cntc, idl, label)
 
     # time kept back for the legs after the timed steps (plain FP64 subset, serving, report)
-    reserve = 60.0 if args.plain > 0 or args.serve_seconds > 0 else 5.0
+    reserve = 75.0 if args.plain > 0 or args.serve_seconds > 0 or args.sens_patches > 0 else 5.0
 
     def run_steps(nsteps, label, can_start):
         """Up to `nsteps` batches, nfl in flight; `can_start(done_latencies)` decides whether
@@ -483,7 +519,7 @@ def main():
     t_plain, FP, sub = [], [], []
     for k in sub_all:
         # keep time for the serving leg
-        if left() < (30.0 if args.serve_seconds > 0 else 12.0):
+        if left() < (45.0 if args.serve_seconds > 0 else 25.0):
             break
         fp = ids(len(t_plain), IRE_FP64)
         W.apply_case(cntc, *fp, cases_fp64[k])
@@ -503,6 +539,10 @@ def main():
         print("# serving leg", file=sys.stderr, flush=True)
         serving = serve_leg(cntc, W, args.serve_seconds, args.serve_rate)
     cntc.servestop()
+    sens = None
+    if args.sens_patches > 0 and left() > 12.0:
+        print("# sensitivity leg", file=sys.stderr, flush=True)
+        sens = sens_leg(cntc, W, n=args.sens_patches)
 
     value = npatch * steps_done / elapsed
     plain_rate = len(sub) / float(np.sum(t_plain)) if sub else float("nan")
@@ -597,6 +637,7 @@ def main():
         "device_time_what": "HIP-event time of the recorded kernels (SteadyGS walk, GD probe, FFT products) / wall",
         "kernels": kernels,
         "serving": serving,
+        "sensitivity_leg": sens,
         "roofline": roofline,
         "roofline_main_stream": roofline_main,
         "roofline_fft": roofline_fft,

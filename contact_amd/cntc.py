@@ -409,11 +409,11 @@ OP_NAMES = ["vec", "ncg_negpn", "ncg_dd", "ncg_expand", "ncg_hmin", "sn_contract
             "tcg_as", "tcg_sa", "st_as", "st_sa", "gd_dp", "gd_project", "gd_probe", "gd_update",
             "gd_moves", "gd_diagres", "conv", "conv32", "elascf", "embed", "extract", "setup",
             "forces", "stdygs", "guard", "sens", "sens_ext", "ledge", "gsb_setup", "gsb_walk",
-            "gsb_check", "cnvxgs", "dense"]
+            "gsb_check", "cnvxgs", "dense", "tcg_vupd"]
 
 
 def amdprofile(on):
-    """Start (True: every kernel, or 2: only the SteadyGS, GD probe and FFT product kernels;
+    """Start (True: every kernel, or 2: only the SteadyGS, GD probe, FFT and dense product kernels;
     clears earlier records) or stop (False) per-kernel HIP-event timing."""
     lib().cntc_amdprofile(_i(2 if on == 2 else (1 if on else 0)))
 

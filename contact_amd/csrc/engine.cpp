@@ -166,7 +166,7 @@ static size_t item_size(int kind)
     case OP_SN_CONTRACT: case OP_SN_EXPAND: case OP_SN_FINAL: return sizeof(NcgItem);
     case OP_AIJ_ONES: return sizeof(AijItem);
     case OP_TCG_NT: case OP_TCG_RSET: case OP_TCG_ZPREP: case OP_TCG_QPOST: case OP_TCG_PUPD:
-    case OP_TCG_AS: case OP_TCG_SA: return sizeof(TcgItem);
+    case OP_TCG_AS: case OP_TCG_SA: case OP_TCG_VUPD: return sizeof(TcgItem);
     case OP_ST_AS: case OP_ST_SA: return sizeof(StItem);
     case OP_GD_DP: case OP_GD_PROJECT: case OP_GD_PROBE: case OP_GD_UPDATE: case OP_GD_MOVES:
     case OP_GD_DIAGRES: return sizeof(GdItem);
@@ -314,7 +314,8 @@ public:
     bool timed_kind(int kind) const
     {
         return profile == 1 || (profile == 2 && (kind == OP_STDYGS || kind == OP_GD_PROBE ||
-                                                 kind == OP_CONV || kind == OP_CONV32));
+                                                 kind == OP_CONV || kind == OP_CONV32 ||
+                                                 kind == OP_DENSE));
     }
     std::mutex prof_mtx;             // prof and nflush are read by the caller's thread
     std::map<int, ProfEntry> prof;   // key kind*4 + stage
@@ -950,6 +951,9 @@ private:
             for (auto& g : lev) {
                 int kind = g.kind;
                 size_t isz = item_size(kind);
+                // an operation without an item record would launch on whatever the staging
+                // buffer holds
+                if (isz == 0) throw std::runtime_error("scheduler: operation kind without an item size");
                 int nitems = (int)g.reqs.size();
                 size_t items_off = off;
                 if (off + isz * nitems > stage_cap()) throw std::runtime_error("scheduler: staging overflow");
@@ -1282,6 +1286,33 @@ struct Ctx {
         it.vec.b = b;
         return it;
     }
+    // device-scalar forms: a reduction whose results stay on the device (sout, 4 values) for
+    // later launches, and an update whose scalar the device forms from such results
+    void vdev(int sub, int mask, double* x, double* y, double* sout)
+    {
+        AnyItem it = vitem(sub, mask, x, nullptr, y, nullptr, 0, 0);
+        it.vec.sout = sout;
+        op(OP_VEC, it);
+    }
+    void vfn(int sub, int mask, double* x, double* y, int sfn, const double* s0, const double* s1 = nullptr,
+             const double* s2 = nullptr, int flag = 0)
+    {
+        AnyItem it = vitem(sub, mask, x, nullptr, y, nullptr, 0, 0);
+        it.vec.sfn = sfn;
+        it.vec.sin[0] = s0;
+        it.vec.sin[1] = s1;
+        it.vec.sin[2] = s2;
+        it.vec.flag = flag;
+        op(OP_VEC, it);
+    }
+    // a sum of squares that also returns the device scalars e0[0], e1[0] as results 2 and 3
+    ResAwait sumsq_echo(int mask, double* x, const double* e0, const double* e1)
+    {
+        AnyItem it = vitem(V_SUMSQ, mask, x, nullptr, nullptr, nullptr, 0, 0);
+        it.vec.echo[0] = e0;
+        it.vec.echo[1] = e1;
+        return opr(OP_VEC, it);
+    }
     void set(int mask, double* x, double a, double* x1 = nullptr) { op(OP_VEC, vitem(V_SET, mask, x, x1, nullptr, nullptr, a, 0)); }
     void copy(int mask, double* x, double* y, double* x1 = nullptr, double* y1 = nullptr) { op(OP_VEC, vitem(V_COPY, mask, x, x1, y, y1, 0, 0)); }
     void scal(int mask, double* x, double a, double* x1 = nullptr) { op(OP_VEC, vitem(V_SCAL, mask, x, x1, nullptr, nullptr, a, 0)); }
@@ -1581,7 +1612,8 @@ size_t patch_bytes(int mx, int my)
     size_t ws = std::max((size_t)3 * Ly * H, pH * pL) * 16;
     size_t emb = std::max((size_t)Lx * Ly, nb) * 8;
     return al(n) + al(42 * n * 8) + al(9 * nb * 8) * 2 + al(3 * nb * 8) + al(9 * bsz * 16) * 2 +
-           al(3 * bsz * 16) + al(9 * bsz * 8) + al(3 * bsz * 8) + al(3 * emb) + 2 * al(ws) + 4096;
+           al(3 * bsz * 16) + al(9 * bsz * 8) + al(3 * bsz * 8) + al(3 * emb) + 2 * al(ws) + al(64 * 8) +
+           4096;
 }
 
 bool Engine::prepare(Patch& P)
@@ -1807,6 +1839,7 @@ bool Engine::prepare(Patch& P)
     P.embed = (double*)take(3 * emb);
     P.wsS = take(ws);
     P.wsT = take(ws);
+    P.sc = (double*)take(64 * 8);
     if (noalloc_) {
         // tables computed into the pinned staging tail and copied into the array
         char* host = sched_->host_tail(twbytes);
@@ -2145,44 +2178,51 @@ static Task normcg(Ctx C, int norm, int maxcg, double eps, double* pen, int* itc
         r = co_await C.sum(M_INT, res);
         C.addc(M_INT, res, -r.v[0] / std::max(1.0, r.v[1]));
     }
-    double rz1 = 0.0, rz2 = 0.0, rms_xk = 1.0, rms_upd = 2.0 * eps, rms_upd1 = 0.0, alpha = 0.0;
+    double rms_xk = 1.0, rms_upd = 2.0 * eps, rms_upd1 = 0.0, alpha = 0.0;
     int itinn = 0;
     bool lchanged = false;
+    // The iteration's scalars stay on the device (slots of 4 in the patch's scalar area): the
+    // means that project z, v, q on zero sum (N=1), rz2 = z.res (two slots by the parity of
+    // the iteration, so that the previous one is rz1), z.r_prv, r.v and v.A v.  The updates
+    // form beta and alpha from them as the host formed them (src/m_solvpn.f90:130-250); the
+    // iteration's one host round trip is the sum of squares of v, which also returns r.v and
+    // v.A v so that the host forms the same alpha.
+    double* sc = P.sc;
+    double *s_zm = sc + 16, *s_rzr = sc + 28, *s_vm = sc + 32, *s_qm = sc + 36, *s_rv = sc + 40,
+           *s_vav = sc + 44;
     while ((lchanged || rms_upd > eps * rms_xk) && itcg < maxcg) {
         itcg++;
         itinn++;
+        double* s_rz2 = sc + 20 + 4 * (itcg & 1);
+        double* s_rz1 = sc + 20 + 4 * ((itcg + 1) & 1);
         C.conv_n(2, M_INT, z, res, fp32);
         if (norm == 1) {
-            r = co_await C.sum(M_INT, z);
-            C.addc(M_INT, z, -r.v[0] / std::max(1.0, r.v[1]));
+            C.vdev(V_SUM, M_INT, z, nullptr, s_zm);
+            C.vfn(V_ADDC, M_INT, z, nullptr, SF_NEG_MEAN, s_zm);
         }
-        rz1 = rz2;
-        r = co_await C.dot(M_INT, z, res);
-        rz2 = r.v[0];
-        if (itcg <= 1 || rz1 < TINY) {
+        C.vdev(V_DOT, M_INT, z, res, s_rz2);
+        if (itcg <= 1) {
             C.copy(M_INT, z, v);
         } else {
-            r = co_await C.dot(M_INT, z, r_prv);
-            double beta = std::max(0.0, (rz2 - r.v[0]) / std::max(TINY, rz1));
-            C.axpby(M_INT, 1.0, z, beta, v);
+            C.vdev(V_DOT, M_INT, z, r_prv, s_rzr);
+            C.vfn(V_AXPBY, M_INT, z, v, SF_NCG_BETA, s_rz2, s_rzr, s_rz1);
         }
         if (norm == 1) {
-            r = co_await C.sum(M_INT, v);
-            C.addc(M_INT, v, -r.v[0] / std::max(1.0, r.v[1]));
+            C.vdev(V_SUM, M_INT, v, nullptr, s_vm);
+            C.vfn(V_ADDC, M_INT, v, nullptr, SF_NEG_MEAN, s_vm);
         }
         C.conv_n(0, M_INT, q, v, fp32);
         if (norm == 1) {
-            r = co_await C.sum(M_INT, q);
-            C.addc(M_INT, q, -r.v[0] / std::max(1.0, r.v[1]));
+            C.vdev(V_SUM, M_INT, q, nullptr, s_qm);
+            C.vfn(V_ADDC, M_INT, q, nullptr, SF_NEG_MEAN, s_qm);
         }
-        r = co_await C.dot(M_INT, res, v);
-        double rv = r.v[0];
-        r = co_await C.dot(M_INT, q, v);
-        double vav = r.v[0];
+        C.vdev(V_DOT, M_INT, res, v, s_rv);
+        C.vdev(V_DOT, M_INT, q, v, s_vav);
+        C.vfn(V_AXPY, M_INT, v, ps, SF_NCG_ALPHA, s_rv, s_vav, nullptr, ncon == 1 ? 1 : 0);
+        r = co_await C.sumsq_echo(M_INT, v, s_rv, s_vav);
+        const double rv = r.v[2], vav = r.v[3];
         if (std::fabs(vav) > 1e-32 && ncon == 1) alpha = rv / vav;
         else alpha = rv / std::max(TINY, vav);
-        C.axpy(M_INT, alpha, v, ps);
-        r = co_await C.sumsq(M_INT, v);
         rms_upd = std::fabs(alpha) * std::sqrt(r.v[0] / std::max(1.0, r.v[1]));
         if (itcg == 1) rms_upd1 = rms_upd;
         if (itcg <= 3 || itcg % 10 == 0) {
@@ -2685,6 +2725,7 @@ static Task tangcg(Ctx C, const double cf00[4], int maxcg, double eps, int* itcg
     T.vx = P.w[6]; T.vy = P.w[7]; T.qx = P.w[8]; T.qy = P.w[9]; T.poldx = P.w[10]; T.poldy = P.w[11];
     T.mu = P.in.fstat; T.ga = P.ga; T.c11 = c11; T.c22 = c22;
     T.p0x = p0x; T.p0y = p0y;
+    T.sc = P.sc;
     AnyItem t;
     t.tcg = T;
     Res r;
@@ -2722,25 +2763,20 @@ static Task tangcg(Ctx C, const double cf00[4], int maxcg, double eps, int* itcg
         } else {
             C.copy(M_ALL, T.rx, T.zx, T.ry, T.zy);
         }
-        r = co_await C.opr(OP_TCG_ZPREP, t);
-        double zq = r.v[0], vq0 = r.v[1];
-        if (itcg <= 1 || lchanged) {
-            C.copy(M_ALL, T.zx, T.vx, T.zy, T.vy);
-        } else {
-            double beta;
-            if (std::fabs(zq) < 1e-60 || std::fabs(vq0) < small * std::fabs(zq)) beta = 0.0;
-            else beta = -zq / vq0;
-            C.axpby(M_ALL, 1.0, T.zx, beta, T.vx, T.zy, T.vy);
-        }
+        // zprep leaves (z.q, v.q) in the patch's device scalars, from which vupd forms beta
+        // (src/m_solvpt.f90:2150-2160); qpost leaves (r.v, v.q), from which pupd forms alpha
+        // (:2190-2200).  The iteration's only host round trip is pupd's result, which also
+        // returns (r.v, v.q) so that the host forms the same alpha.
+        C.op(OP_TCG_ZPREP, t);
+        if (itcg <= 1 || lchanged) C.copy(M_ALL, T.zx, T.vx, T.zy, T.vy);
+        else C.op(OP_TCG_VUPD, t);
         C.conv_t(0, M_INT, T.qx, T.qy, T.vx, T.vy, nullptr, nullptr, fp32);
-        r = co_await C.opr(OP_TCG_QPOST, t);
-        double rv = r.v[0], vq = r.v[1];
+        C.op(OP_TCG_QPOST, t);
+        r = co_await C.opr(OP_TCG_PUPD, t);
+        const double rv = r.v[2], vq = r.v[3];
         if (std::fabs(rv) < 1e-60) alpha = 0.0;
         else if (std::fabs(vq) < small * std::fabs(rv)) alpha = 1.0;
         else alpha = rv / vq;
-        AnyItem tu = t;
-        tu.tcg.alpha = alpha;
-        r = co_await C.opr(OP_TCG_PUPD, tu);
         dif = alpha * facnel * std::sqrt(r.v[0] / (2.0 * npot));
         difinn += dif;
         if (itcg <= 5 || itcg % 5 == 1) {

@@ -137,6 +137,7 @@ struct Patch {
     void *spec = nullptr, *specv = nullptr, *specm = nullptr, *spec32 = nullptr, *specm32 = nullptr;
     double* embed = nullptr;         // padded real scratch for spectra (Lx*Ly, or 4*cf_mx*cf_my)
     void *wsS = nullptr, *wsT = nullptr;
+    double* sc = nullptr;            // device scalars that one launch leaves for the next (64)
     const void *twM = nullptr, *twLx = nullptr, *twLy = nullptr;
     const void *twM32 = nullptr, *twLx32 = nullptr, *twLy32 = nullptr;
     const void *ptwM = nullptr, *ptwLx = nullptr, *ptwLy = nullptr;   // preconditioner sizes

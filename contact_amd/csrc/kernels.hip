@@ -90,7 +90,7 @@ __device__ inline double combine_identity(int mode)
 // that depends only on the item's group count, so an item's result is the same whichever
 // other items share the launch.
 template <int K>
-__device__ inline void final_reduce(const RedOut& ro, const int* gstart, int it)
+__device__ inline void final_reduce(const RedOut& ro, const int* gstart, int it, double* extra = nullptr)
 {
     __threadfence();   // acquire: the other groups' partials
     if (threadIdx.x >= 64) return;
@@ -100,7 +100,10 @@ __device__ inline void final_reduce(const RedOut& ro, const int* gstart, int it)
         double x = combine_identity(mode);
         for (int g = g0 + lane; g < g1; g += 64) x = combine(x, ro.partials[(size_t)g * K + k], mode);
         for (int off = 32; off > 0; off >>= 1) x = combine(x, __shfl_xor(x, off), mode);
-        if (lane == 0) ro.results[(size_t)it * K + k] = x;
+        if (lane == 0) {
+            ro.results[(size_t)it * K + k] = x;
+            if (extra) extra[k] = x;   // the item's device copy, read by later launches
+        }
     }
     if (lane == 0) ro.count[it] = 0u;   // ready for the next launch that uses this counter
 }
@@ -109,7 +112,8 @@ __device__ inline void final_reduce(const RedOut& ro, const int* gstart, int it)
 // item's last group also forms the item's results (no separate finalize launch).  Must be
 // reached by every thread of the group.
 template <int K>
-__device__ inline void write_partials(double (&v)[K], int mode, const RedOut& ro, const int* gstart, int it)
+__device__ inline void write_partials(double (&v)[K], int mode, const RedOut& ro, const int* gstart, int it,
+                                      double* extra = nullptr)
 {
     __shared__ double red_lds[16 * MAXK];
     __shared__ int s_last;
@@ -125,7 +129,7 @@ __device__ inline void write_partials(double (&v)[K], int mode, const RedOut& ro
         s_last = last;
     }
     __syncthreads();
-    if (s_last) final_reduce<K>(ro, gstart, it);
+    if (s_last) final_reduce<K>(ro, gstart, it, extra);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -140,19 +144,37 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_vec(const VecItem* items, cons
     int g0 = gstart[it];
     int base = (blockIdx.x - g0) * ELEM_THREADS * ELEM_PER_THREAD;
     int mode = I.sub == V_MAXABS || I.sub == V_MAX ? 1 : (I.sub == V_MIN ? 2 : 0);
-    double acc[2] = {mode == 0 ? 0.0 : (mode == 1 ? -INFINITY : INFINITY), 0.0};
+    double acc[4] = {mode == 0 ? 0.0 : (mode == 1 ? -INFINITY : INFINITY), 0.0, 0.0, 0.0};
+    // the scalars formed on the device from earlier launches' results, as the host would
+    double a = I.a, b = I.b;
+    int sub = I.sub;
+    if (I.sfn == SF_NEG_MEAN) {
+        a = -I.sin[0][0] / fmax(1.0, I.sin[0][1]);
+    } else if (I.sfn == SF_NCG_BETA) {
+        const double rz2 = I.sin[0][0], rzr = I.sin[1][0], rz1 = I.sin[2][0];
+        if (rz1 < NCG_TINY) {
+            sub = V_COPY;
+        } else {
+            a = 1.0;
+            b = fmax(0.0, (rz2 - rzr) / fmax(NCG_TINY, rz1));
+        }
+    } else if (I.sfn == SF_NCG_ALPHA) {
+        const double rv = I.sin[0][0], vav = I.sin[1][0];
+        if (fabs(vav) > 1e-32 && I.flag) a = rv / vav;
+        else a = rv / fmax(NCG_TINY, vav);
+    }
     for (int e = 0; e < ELEM_PER_THREAD; e++) {
         int i = base + e * ELEM_THREADS + threadIdx.x;
         if (i >= I.n) break;
         if (!in_mask(I.mask, I.el ? I.el[i] : (int8_t)0)) continue;
         for (int c = 0; c < I.ncomp; c++) {
-            switch (I.sub) {
-            case V_SET: I.x[c][i] = I.a; break;
+            switch (sub) {
+            case V_SET: I.x[c][i] = a; break;
             case V_COPY: I.y[c][i] = I.x[c][i]; break;
-            case V_SCAL: I.x[c][i] *= I.a; break;
-            case V_AXPY: I.y[c][i] += I.a * I.x[c][i]; break;
-            case V_AXPBY: I.y[c][i] = I.a * I.x[c][i] + I.b * I.y[c][i]; break;
-            case V_ADDC: I.x[c][i] += I.a; break;
+            case V_SCAL: I.x[c][i] *= a; break;
+            case V_AXPY: I.y[c][i] += a * I.x[c][i]; break;
+            case V_AXPBY: I.y[c][i] = a * I.x[c][i] + b * I.y[c][i]; break;
+            case V_ADDC: I.x[c][i] += a; break;
             case V_DOT: acc[0] += I.x[c][i] * I.y[c][i]; acc[1] += 1.0; break;
             case V_SUM: acc[0] += I.x[c][i]; acc[1] += 1.0; break;
             case V_SUMSQ: acc[0] += I.x[c][i] * I.x[c][i]; acc[1] += 1.0; break;
@@ -164,9 +186,14 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_vec(const VecItem* items, cons
         }
     }
     // max/min: the second value is 0 (the count of the sums); the group reduction of
-    // zeros under max or min stays 0
+    // zeros under max or min stays 0.  The echoed scalars enter the sums once, from the
+    // item's first group's first thread.
     if (mode != 0) acc[1] = 0.0;
-    write_partials<2>(acc, mode, ro, gstart, it);
+    if (mode == 0 && I.echo[0] && blockIdx.x == gstart[it] && threadIdx.x == 0) {
+        acc[2] = I.echo[0][0];
+        acc[3] = I.echo[1][0];
+    }
+    write_partials<4>(acc, mode, ro, gstart, it, I.sout);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -447,7 +474,27 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_zprep(const TcgItem* items
         acc[0] += zx * I.qx[i] + zy * I.qy[i];
         acc[1] += I.vx[i] * I.qx[i] + I.vy[i] * I.qy[i];
     }
-    write_partials<2>(acc, 0, ro, gstart, it);
+    write_partials<2>(acc, 0, ro, gstart, it, I.sc);
+}
+
+// v = z + beta v with beta from zprep's (z.q, v.q) in sc[0..1] (src/m_solvpt.f90:2150-2160)
+__global__ void __launch_bounds__(ELEM_THREADS) k_tcg_vupd(const TcgItem* items, const int* gstart,
+                                                           int nitems, RedOut ro)
+{
+    int it = find_item(gstart, nitems, blockIdx.x);
+    const TcgItem& I = items[it];
+    (void)ro;
+    const double small = 1e-6, zq = I.sc[0], vq0 = I.sc[1];
+    double beta;
+    if (fabs(zq) < 1e-60 || fabs(vq0) < small * fabs(zq)) beta = 0.0;
+    else beta = -zq / vq0;
+    int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
+    for (int e = 0; e < ELEM_PER_THREAD; e++) {
+        int i = base + e * ELEM_THREADS + threadIdx.x;
+        if (i >= I.n) break;
+        I.vx[i] = 1.0 * I.zx[i] + beta * I.vx[i];
+        I.vy[i] = 1.0 * I.zy[i] + beta * I.vy[i];
+    }
 }
 
 // q: slip projection and curvature term; returns (r.v, v.q)
@@ -477,18 +524,30 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_qpost(const TcgItem* items
         acc[0] += I.rx[i] * I.vx[i] + I.ry[i] * I.vy[i];
         acc[1] += I.vx[i] * qx + I.vy[i] * qy;
     }
-    write_partials<2>(acc, 0, ro, gstart, it);
+    write_partials<2>(acc, 0, ro, gstart, it, I.sc ? I.sc + 2 : nullptr);
 }
 
 // pold = p; p += alpha v (interior); slip clipped to the bound; returns (sum v^2, sum p^2), or
-// sum (p - p0)^2 in a sensitivity solve (src/m_solvpt.f90:2209-2216)
+// sum (p - p0)^2 in a sensitivity solve (src/m_solvpt.f90:2209-2216), then (r.v, v.q).  With
+// sc set, alpha is formed here from qpost's (r.v, v.q) in sc[2..3] (:2190-2200).
 __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_pupd(const TcgItem* items, const int* gstart,
                                                            int nitems, RedOut ro)
 {
     int it = find_item(gstart, nitems, blockIdx.x);
     const TcgItem& I = items[it];
     int base = (blockIdx.x - gstart[it]) * ELEM_THREADS * ELEM_PER_THREAD;
-    double acc[2] = {0.0, 0.0};
+    double alpha = I.alpha, rv = 0.0, vq = 0.0;
+    if (I.sc) {
+        const double small = 1e-6;
+        rv = I.sc[2];
+        vq = I.sc[3];
+        if (fabs(rv) < 1e-60) alpha = 0.0;
+        else if (fabs(vq) < small * fabs(rv)) alpha = 1.0;
+        else alpha = rv / vq;
+    }
+    // (r.v, v.q) enter the sums once, from the item's first group's first thread
+    const bool first = blockIdx.x == gstart[it] && threadIdx.x == 0;
+    double acc[4] = {0.0, 0.0, first ? rv : 0.0, first ? vq : 0.0};
     for (int e = 0; e < ELEM_PER_THREAD; e++) {
         int i = base + e * ELEM_THREADS + threadIdx.x;
         if (i >= I.n) break;
@@ -497,8 +556,8 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_pupd(const TcgItem* items,
         I.poldy[i] = py;
         int8_t s = I.el[i];
         if (s >= ADHES) {
-            px += I.alpha * I.vx[i];
-            py += I.alpha * I.vy[i];
+            px += alpha * I.vx[i];
+            py += alpha * I.vy[i];
         }
         if (s == SLIP) {
             double g = I.mu * I.pn[i];
@@ -515,7 +574,7 @@ __global__ void __launch_bounds__(ELEM_THREADS) k_tcg_pupd(const TcgItem* items,
         }
         acc[1] += px * px + py * py;
     }
-    write_partials<2>(acc, 0, ro, gstart, it);
+    write_partials<4>(acc, 0, ro, gstart, it);
 }
 
 // adhesion elements above the traction bound move to slip; returns #moved
@@ -964,12 +1023,25 @@ __global__ void __launch_bounds__(GD_THREADS) k_gd_update(const GdItem* items, c
 // plstrc without plasticity (elmtrc), src/m_solvpt.f90:3258-3807; the plastic branch only
 // reports a violated constraint when tau_c0 = 1e20 and k_tau = 0
 // scale_dir: with omegas = 1 put the slip traction on the bound along the Newton solution
-// directly instead of through atan2, cos and sin (the same direction up to rounding); used on
-// the sequential path of SteadyGS, where the transcendental chain is latency
+// directly instead of through atan2, cos and sin (the same direction up to rounding), and
+// divide a pair of values by a common denominator through one reciprocal (the same values up
+// to rounding); used on the sequential path of SteadyGS, where the chain of FP64 divisions
+// and transcendentals is the latency of every element
 __device__ void elmtrc(int8_t& el, const double c11, const double c12, const double c21,
                        const double c22, double eps, double pr[3], double mus, double s[2],
                        double omegah = 1.0, double omegas = 1.0, bool scale_dir = false)
 {
+    // (a / d, b / d), or through 1 / d on the latency path
+    auto pair_div = [scale_dir](double a, double b, double d, double& qa, double& qb) {
+        if (scale_dir) {
+            const double r = 1.0 / d;
+            qa = a * r;
+            qb = b * r;
+        } else {
+            qa = a / d;
+            qb = b / d;
+        }
+    };
     const int maxnr = 10;
     const double prc = 0.0001, taucs = 1e20, PI = 3.14159265358979323846;
     double pold1 = pr[0], pold2 = pr[1];
@@ -993,8 +1065,7 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
             } else {
                 double pabs1 = sqrt((pold1 + dp0) * (pold1 + dp0) + (pold2 + dp1) * (pold2 + dp1));
                 if (pabs1 <= bnd) {
-                    pr[0] = pr[0] * bnd / pabs2;
-                    pr[1] = pr[1] * bnd / pabs2;
+                    pair_div(pr[0] * bnd, pr[1] * bnd, pabs2, pr[0], pr[1]);
                     violat = 0;
                 } else {
                     violat = 1;
@@ -1006,8 +1077,7 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
             violat = 0;
             double trc_bound = mus * pr[2];
             double sabs = sqrt(si0x * si0x + si0y * si0y);
-            pr[0] = -trc_bound * si0x / sabs;
-            pr[1] = -trc_bound * si0y / sabs;
+            pair_div(-trc_bound * si0x, -trc_bound * si0y, sabs, pr[0], pr[1]);
             six = si0x + c11 * pr[0] + c12 * pr[1];
             siy = si0y + c21 * pr[0] + c22 * pr[1];
             double fk0 = pr[1] * six - pr[0] * siy;
@@ -1023,8 +1093,8 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
                 if (det == 0.0) {
                     itnr = maxnr;
                 } else {
-                    double d0 = -(g11 * fk0 - g01 * fk1) / det;
-                    double d1 = -(-g10 * fk0 + g00 * fk1) / det;
+                    double d0, d1;
+                    pair_div(-(g11 * fk0 - g01 * fk1), -(-g10 * fk0 + g00 * fk1), det, d0, d1);
                     pr[0] += d0;
                     pr[1] += d1;
                 }
@@ -1038,8 +1108,10 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
                 // direction): the traction at the bound in the Newton solution's direction
                 double pabs = sqrt(pr[0] * pr[0] + pr[1] * pr[1]);
                 if (pabs > 0.0) {
-                    pr[0] = trc_bound * (pr[0] / pabs);
-                    pr[1] = trc_bound * (pr[1] / pabs);
+                    double ux, uy;
+                    pair_div(pr[0], pr[1], pabs, ux, uy);
+                    pr[0] = trc_bound * ux;
+                    pr[1] = trc_bound * uy;
                 } else {
                     pr[0] = trc_bound;
                     pr[1] = 0.0;
@@ -2448,6 +2520,32 @@ struct GsRow {
     int8_t* el;
 };
 
+
+// Wavefront sum on the latency path: butterflies within each row of 16 lanes by DPP moves
+// (quad permutes [1,0,3,2] and [2,3,0,1], then the half-row and row mirrors: every lane of a
+// row ends with the row's sum, the same bits by commutativity), then the four row sums read
+// from lanes 0, 16, 32, 48 and added in that order.  Needs all 64 lanes active.
+template <int CTRL>
+__device__ inline double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double lane_f64(double v, int l)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ inline double wave_sum(double v)
+{
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0x141>(v);
+    v += dpp_f64<0x140>(v);
+    return ((lane_f64(v, 0) + lane_f64(v, 16)) + lane_f64(v, 32)) + lane_f64(v, 48);
+}
+
 // Wavefront 0's walk along elements [ixw, iend) of the row starting at r0 (src/m_solvpt.f90:
 // 2950-3127): each interior element's 2x2 problem in order from the trailing edge, then the
 // rebuild of the upstream part of the row.  The own-row influence is summed over the
@@ -2472,10 +2570,8 @@ __device__ void gs_walk_span(const GsRow& R, int mx, int j0, int j1, size_t r0, 
             qsx += R.stc[o] * qx + R.stc[M2 + o] * qy;
             qsy += R.stc[2 * M2 + o] * qx + R.stc[3 * M2 + o] * qy;
         }
-        for (int off = 32; off > 0; off >>= 1) {
-            qsx += __shfl_xor(qsx, off);
-            qsy += __shfl_xor(qsy, off);
-        }
+        qsx = wave_sum(qsx);
+        qsy = wave_sum(qsy);
         const int jx = ix >= 1 ? lastna : -1;
         // the element problem, solved by every lane on the same (broadcast) data so
         // that its change D is known wavefront-wide; lane 0 stores
@@ -3440,8 +3536,9 @@ typedef double dense_v4 __attribute__((ext_vector_type(4)));
 // product is an ordinary matrix product: the Toeplitz block T_d (T_d[ix][jx] = cf(ix - jx, d))
 // times the source rows jy = iy - d.  v_mfma_f64_16x16x4f64 takes A = 16 targets x 4 sources
 // (lane i + 16 k holds A[i][k]) and B = 4 sources x 16 target rows (lane j + 16 k holds B[k][j]);
-// lane j + 16 q holds D[4 q + r][j], r = 0..3.  The k loop runs over d and the sources in
-// steps of 4; coefficients outside the tables and sources outside the grid enter as zeros.
+// register r of lane j + 16 q holds D[4 r + q][j], r = 0..3 (measured on gfx950).  The k loop
+// runs over d and the sources in steps of 4; coefficients outside the tables and sources
+// outside the grid enter as zeros.
 __global__ void __launch_bounds__(64) k_dense(const DenseItem* items, const int* gstart, int nitems)
 {
     const int it = find_item(gstart, nitems, blockIdx.x);
@@ -3493,7 +3590,7 @@ __global__ void __launch_bounds__(64) k_dense(const DenseItem* items, const int*
         const double* add = I.addend[o];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int ix = ix0 + 4 * q + r;
+            const int ix = ix0 + 4 * r + q;
             if (ix >= mx) continue;
             const size_t ii = (size_t)iy * mx + ix;
             if (I.mask != M_ALL && !in_mask(I.mask, I.el[ii])) continue;
@@ -3514,7 +3611,7 @@ static int ngroups_rows(int my) { return (my + ROW_THREADS - 1) / ROW_THREADS; }
 int op_nred(int kind, const void* item)
 {
     switch (kind) {
-    case OP_VEC: return 2;
+    case OP_VEC: return 4;
     case OP_NCG_NEGPN: return 3;
     case OP_NCG_DD: return 2;
     case OP_NCG_EXPAND: return 1;
@@ -3525,7 +3622,7 @@ int op_nred(int kind, const void* item)
     case OP_AIJ_ONES: return 1;
     case OP_TCG_ZPREP: return 2;
     case OP_TCG_QPOST: return 2;
-    case OP_TCG_PUPD: return 2;
+    case OP_TCG_PUPD: return 4;
     case OP_TCG_AS: return 1;
     case OP_TCG_SA: return 2;
     case OP_ST_AS: return 1;
@@ -3565,7 +3662,7 @@ int op_groups(int kind, const void* item, int conv_stage)
         return ngroups_elem(((const NcgItem*)item)->n);
     case OP_AIJ_ONES: return ngroups_rows(((const AijItem*)item)->my);
     case OP_TCG_NT: case OP_TCG_RSET: case OP_TCG_ZPREP: case OP_TCG_QPOST: case OP_TCG_PUPD:
-    case OP_TCG_AS: case OP_TCG_SA:
+    case OP_TCG_AS: case OP_TCG_SA: case OP_TCG_VUPD:
         return ngroups_elem(((const TcgItem*)item)->n);
     case OP_ST_AS: case OP_ST_SA: return ngroups_elem(((const StItem*)item)->n);
     case OP_GD_DP: return ngroups_elem(((const GdItem*)item)->n);
@@ -3640,6 +3737,7 @@ void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, i
     case OP_TCG_ZPREP: hipLaunchKernelGGL(k_tcg_zprep, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
     case OP_TCG_QPOST: hipLaunchKernelGGL(k_tcg_qpost, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
     case OP_TCG_PUPD: hipLaunchKernelGGL(k_tcg_pupd, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
+    case OP_TCG_VUPD: hipLaunchKernelGGL(k_tcg_vupd, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
     case OP_TCG_AS: hipLaunchKernelGGL(k_tcg_as, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
     case OP_TCG_SA: hipLaunchKernelGGL(k_tcg_sa, ge, te, 0, s, (const TcgItem*)d_items, d_gstart, nitems, ro); break;
     case OP_ST_AS: hipLaunchKernelGGL(k_st_as, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, ro); break;

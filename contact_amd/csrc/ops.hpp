@@ -69,6 +69,7 @@ enum OpKind : int {
     OP_GSB_CHECK,
     OP_CNVXGS,
     OP_DENSE,
+    OP_TCG_VUPD,
     OP_COUNT_KINDS
 };
 
@@ -96,7 +97,26 @@ struct VecItem {
     double* x[2];
     double* y[2];
     double a, b;
+    // device scalars (the patch's scalar area): sout receives the results (value, count) for
+    // later launches; with sfn set, a (and b) are formed on the device from sin as the host
+    // would form them (VecScalarFn; flag carries a host-side condition); echo[0..1] are
+    // returned as results 2 and 3 of a sum, so that the host can form the same values
+    double* sout;
+    const double* sin[3];
+    const double* echo[2];
+    int sfn, flag;
 };
+
+// scalar functions of VecItem.sfn (NormCG, src/m_solvpn.f90:130-250)
+enum VecScalarFn : int {
+    SF_NONE = 0,
+    SF_NEG_MEAN,    // a = -sin[0][0] / max(1, sin[0][1])  (minus the mean of a sum)
+    SF_NCG_BETA,    // rz2 = sin[0][0], rzr = sin[1][0], rz1 = sin[2][0]: y = x if rz1 < TINY, else
+                    // y = x + max(0, (rz2 - rzr) / max(TINY, rz1)) y
+    SF_NCG_ALPHA,   // rv = sin[0][0], vav = sin[1][0]: a = rv / vav if |vav| > 1e-32 and flag,
+                    // else rv / max(TINY, vav)
+};
+constexpr double NCG_TINY = 1e-20;
 
 // NormCG / snorm element steps (src/m_solvpn.f90:298-393, src/m_snorm.f90:195-325)
 struct NcgItem {
@@ -139,6 +159,10 @@ struct TcgItem {
     const double* p0x; const double* p0y;   // sensitivity solve: stopping test on p - p0
     double mu, ga, c11, c22;
     double alpha;
+    // device scalars of the iteration (the patch's scalar area): zprep stores (z.q, v.q) in
+    // sc[0..1], qpost (r.v, v.q) in sc[2..3]; vupd forms beta and pupd alpha from them, as the
+    // host would, so that one iteration needs no host round trip before pupd's results
+    double* sc;
 };
 
 // stang outer-loop moves (src/m_stang.f90:433-508)
