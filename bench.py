@@ -397,7 +397,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--inflight", type=int, default=2,
+    ap.add_argument("--inflight", type=int, default=3,
                     help="batches in flight (each on its own set of 2048 problems, 1-3)")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--sens-patches", type=int, default=256,

@@ -2554,7 +2554,8 @@ __device__ inline double wave_sum(double v)
 // Called by all 64 lanes of wavefront 0.
 __device__ void gs_walk_span(const GsRow& R, int mx, int j0, int j1, size_t r0, int& ixw, int iend,
                              int& lastna, double& difacc, double ga_inv, double eps, double mu,
-                             double omegah, double omegas, double* gsx, double* gsy)
+                             double omegah, double omegas, double* gsx, double* gsy,
+                             bool s_lds = false)
 {
     const int lane = threadIdx.x, M2 = 2 * mx;
     for (; ixw < iend; ixw++) {
@@ -2605,8 +2606,15 @@ __device__ void gs_walk_span(const GsRow& R, int mx, int j0, int j1, size_t r0, 
                 R.px[ix] = pr[0];
                 R.py[ix] = pr[1];
                 R.el[ix] = e8;
-                gsx[ii] = sv[0];
-                gsy[ii] = sv[1];
+                if (s_lds) {
+                    // kept in LDS in place of this element's other-row influence (read once,
+                    // above) and stored with the row: no global store on the element chain
+                    R.ucx[ix] = sv[0];
+                    R.ucy[ix] = sv[1];
+                } else {
+                    gsx[ii] = sv[0];
+                    gsy[ii] = sv[1];
+                }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -3330,10 +3338,15 @@ __global__ void __launch_bounds__(GSB_THREADS) k_gsb_walk(const GsbItem* items, 
             int lastna = t0 - 1 >= 1 ? t0 - 1 : 0;
             int ixw = t0;
             gs_walk_span(R, mx, rj0[t], rj1[t], r0, ixw, t0 + nt, lastna, difacc, I.ga_inv, I.eps,
-                         I.mu, I.omegah, I.omegas, I.sx, I.sy);
+                         I.mu, I.omegah, I.omegas, I.sx, I.sy, true);
         }
         __syncthreads();
+        const int t1 = ri1[t];
         for (int ix = tid; ix < mx; ix += T) {
+            if (ix >= ri0[t] && ix <= t1 && rel[ix] >= ADHES) {
+                I.sx[r0 + ix] = ucx[ix];
+                I.sy[r0 + ix] = ucy[ix];
+            }
             I.px[r0 + ix] = rpx[ix];
             I.py[r0 + ix] = rpy[ix];
             I.dpx[r0 + ix] = rdpx[ix];

@@ -45,9 +45,9 @@ CASES = {
 def test_dense_products_vs_oracle_and_fft(cntc, oracle, monkeypatch, name):
     c = CASES[name]
     monkeypatch.setenv("CNTC_DENSE", "2")
-    g = run_gpu(cntc, c, ire=40)
+    g = run_gpu(cntc, c, ire=611)
     monkeypatch.setenv("CNTC_DENSE", "0")
-    f = run_gpu(cntc, c, ire=41)
+    f = run_gpu(cntc, c, ire=612)
     monkeypatch.delenv("CNTC_DENSE")
     o = run_oracle(oracle, c)
     assert g[0] == f[0] == o[0] == 0
@@ -67,7 +67,7 @@ def test_dense_sensitivities_equal_fft(cntc, oracle, monkeypatch):
                      maxgs=2000, maxin=30)
     c.update(sens=3, mxsens=200, epsens=1e-10)
     out = {}
-    for mode, ire in (("1", 42), ("0", 43)):
+    for mode, ire in (("1", 613), ("0", 614)):
         monkeypatch.setenv("CNTC_DENSE", mode)
         C.apply_cntc(cntc, ire, 1, c)
         assert cntc.calculate(ire, 1) == 0
@@ -88,14 +88,14 @@ def test_dense_kernel_runs_in_sensitivity_solves(cntc):
     are counted by the per-kernel timing) and leaves the main solve on the FFT products."""
     c = C.hertz_case(6.0, 4.0, 24, 24, 8200.0, cksi=0.0004, cphi=0.001, eps=1e-8, maxgs=999,
                      maxin=30)
-    C.apply_cntc(cntc, 44, 1, c)
+    C.apply_cntc(cntc, 615, 1, c)
     cntc.amdprofile(True)
-    assert cntc.calculate(44, 1) == 0
+    assert cntc.calculate(615, 1) == 0
     plain = cntc.amdprofileget()
     c["sens"] = 3
-    C.apply_cntc(cntc, 45, 1, c)
+    C.apply_cntc(cntc, 616, 1, c)
     cntc.amdprofile(True)
-    assert cntc.calculate(45, 1) == 0
+    assert cntc.calculate(616, 1) == 0
     sens = cntc.amdprofileget()
     cntc.amdprofile(False)
     assert "dense" not in plain

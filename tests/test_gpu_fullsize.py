@@ -33,13 +33,13 @@ def test_rough_patch_256_tight_vs_oracle(cntc):
     from tests.test_gpu_configs import force_err, traction_err, states_agree
     g = np.load(os.path.join(HERE, "golden", "rough_patch_256_eps12.npz"))
     c = W.rough_patch(n=256, eps=1e-12)
-    rc, f, pn, px, py, el = solve(cntc, 42, c)
+    rc, f, pn, px, py, el = solve(cntc, 601, c)
     assert rc == 0 and int(g["rc"]) == 0
     ref = [g["pn"], g["px"], g["py"]]
     ef = force_err(f, g["forces"], c["fstat"], c["fn"])
     ep = traction_err([pn, px, py], ref)
     ndiff, bad = states_agree(el, g["el"], [pn, px, py], ref, c["fstat"], 1e-8)
-    print("256^2 eps 1e-12: gpu", f, cntc.getiterations(42, 1), "oracle", g["forces"], g["iters"],
+    print("256^2 eps 1e-12: gpu", f, cntc.getiterations(601, 1), "oracle", g["forces"], g["iters"],
           "force err", ef, "traction err %.2e" % ep, "states differing", ndiff)
     assert ef.max() <= 1e-9
     assert ep <= 1e-8
@@ -53,7 +53,7 @@ def test_rough_patch_512_tight_vs_oracle(cntc):
     from tests.test_gpu_configs import force_err, traction_err, states_agree
     g = np.load(os.path.join(HERE, "golden", "rough_patch_512_eps12.npz"))
     c = W.rough_patch(n=512, eps=1e-12)
-    rc, f, pn, px, py, el = solve(cntc, 43, c)
+    rc, f, pn, px, py, el = solve(cntc, 602, c)
     assert rc == 0 and int(g["rc"]) == 0
     s = int(g["stride"])
     got = [pn[::s], px[::s], py[::s]]
@@ -62,7 +62,7 @@ def test_rough_patch_512_tight_vs_oracle(cntc):
     ep = traction_err(got, ref)
     ndiff, bad = states_agree(el[::s], g["el"], got, ref, c["fstat"], 1e-8)
     counts = np.bincount(el, minlength=4)
-    print("512^2 eps 1e-12: gpu", f, cntc.getiterations(43, 1), "oracle", g["forces"], g["iters"],
+    print("512^2 eps 1e-12: gpu", f, cntc.getiterations(602, 1), "oracle", g["forces"], g["iters"],
           "force err", ef, "traction err %.2e" % ep, "counts", counts, g["counts"])
     assert ef.max() <= 1e-9
     assert ep <= 1e-8
@@ -72,11 +72,11 @@ def test_rough_patch_512_tight_vs_oracle(cntc):
 
 def test_rough_patch_1024_mixed_vs_fp64(cntc):
     c = W.rough_patch(n=1024)
-    r64 = solve(cntc, 50, c)
+    r64 = solve(cntc, 603, c)
     cm = dict(c)
     cm["precision"] = 1
-    rmx = solve(cntc, 51, cm)
-    print("1024 fp64", r64[1], cntc.getiterations(50, 1), "mixed", rmx[1], cntc.getiterations(51, 1))
+    rmx = solve(cntc, 604, cm)
+    print("1024 fp64", r64[1], cntc.getiterations(603, 1), "mixed", rmx[1], cntc.getiterations(604, 1))
     assert r64[0] == 0 and rmx[0] == 0
     rel = np.abs(rmx[1][:3] - r64[1][:3]) / np.abs(r64[1][:3])
     pmax = np.max(r64[2])
