@@ -248,6 +248,8 @@ public:
     {
         drop_async();
         q_.assign(nslots, {});
+        dirty_.clear();
+        in_dirty_.assign(nslots, 0);
         waiter_.assign(nslots, nullptr);
         async_wait_.assign(nslots, 0);
         prio_.assign(nslots, 0);
@@ -293,13 +295,14 @@ public:
         r.slot = slot;
         r.item = it;
         q_[slot].push_back(r);
+        if (!in_dirty_[slot]) {
+            in_dirty_[slot] = 1;
+            dirty_.push_back(slot);
+        }
     }
-    bool any_queued() const
-    {
-        for (auto& q : q_)
-            if (!q.empty()) return true;
-        return false;
-    }
+    // dirty_ lists exactly the slots with queued operations, so that a flush visits those
+    // and not every slot of the serving loop's 16384
+    bool any_queued() const { return !dirty_.empty(); }
     bool any_async() const { return !async_.empty(); }
     void set_result_dest(int slot, double* dst) { q_[slot].back().res = dst; }
 
@@ -350,8 +353,8 @@ public:
         for (;;) {
             if (any_queued()) {
                 bool anyp = false;
-                for (size_t sl = 0; sl < q_.size(); sl++)
-                    if (prio_[sl] && !q_[sl].empty()) anyp = true;
+                for (int sl : dirty_)
+                    if (prio_[sl]) anyp = true;
                 const bool only = anyp && (prio_turn_++ % (PRIO_EXTRA + 1)) != 0;
                 flush(only);
                 resume_ready(slots, only);
@@ -913,11 +916,14 @@ private:
         std::vector<std::vector<Group>> levels;
         size_t maxlen = 0;
         auto taken = [&](size_t sl) { return !prio_only || prio_[sl]; };
-        for (size_t sl = 0; sl < q_.size(); sl++)
+        // the slots with queued work, in slot order (the order of a launch's items)
+        std::vector<int> work(dirty_);
+        std::sort(work.begin(), work.end());
+        for (int sl : work)
             if (taken(sl)) maxlen = std::max(maxlen, q_[sl].size());
         for (size_t lev = 0; lev < maxlen; lev++) {
             std::map<int, Group> byk;
-            for (size_t sl = 0; sl < q_.size(); sl++) {
+            for (int sl : work) {
                 auto& q = q_[sl];
                 if (taken(sl) && lev < q.size()) {
                     if (is_async_kind(q[lev].kind) && !fixed_) {
@@ -1096,8 +1102,15 @@ private:
                 int K = op_nred(rm.first->kind, &rm.first->item);
                 for (int k = 0; k < K; k++) rm.first->res[k] = hres_[rm.second + k];
             }
-        for (size_t sl = 0; sl < q_.size(); sl++)
-            if (taken(sl)) q_[sl].clear();
+        dirty_.clear();
+        for (int sl : work) {
+            if (taken(sl)) {
+                q_[sl].clear();
+                in_dirty_[sl] = 0;
+            } else {
+                dirty_.push_back(sl);
+            }
+        }
         nflush++;
     }
 
@@ -1115,6 +1128,8 @@ private:
     int PRIO_DIV = 4;          // CNTC_PRIO_DIV
     long prio_turn_ = 0;
     std::vector<std::vector<Req>> q_;
+    std::vector<int> dirty_;        // slots whose queue is not empty
+    std::vector<char> in_dirty_;
     std::vector<std::coroutine_handle<>> waiter_;
     char* hstage_ = nullptr;
     char* dstage_ = nullptr;

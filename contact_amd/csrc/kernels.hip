@@ -3561,36 +3561,65 @@ __global__ void __launch_bounds__(64) k_dense(const DenseItem* items, const int*
     const int t = blockIdx.x - gstart[it];
     const int ix0 = (t % ntx) * 16, iy0 = (t / ntx) * 16;
     const int i = lane & 15, q = lane >> 4;
-    dense_v4 acc[3];
+    const int nin = I.nin, nout = I.nout;
+    // one accumulator per (out, in) block: independent MFMA chains, summed at the end
+    dense_v4 acc[3][3];
 #pragma unroll
-    for (int o = 0; o < 3; o++) acc[o] = (dense_v4){0.0, 0.0, 0.0, 0.0};
+    for (int o = 0; o < 3; o++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) acc[o][j] = (dense_v4){0.0, 0.0, 0.0, 0.0};
     const int ia = ix0 + i;       // this lane's target in A (row i)
     const int jb = iy0 + i;       // this lane's target row in B (column i)
     const int d0 = max(iy0 - (my - 1), -(my - 1)), d1 = min(iy0 + 15, my - 1);
-    for (int d = d0; d <= d1; d++) {
+    const int nstep = (mx + 3) / 4;
+    // the operands of step (d, jx0) for this lane: A of every block, B of every input
+    auto fetch = [&](int d, int jx0, double (&a)[3][3], double (&b)[3]) {
         const int jy = jb - d;
-        const bool rowok = jb < my && jy >= 0 && jy < my;
-        const size_t crow = (size_t)(d + my) * M2;
-        for (int jx0 = 0; jx0 < mx; jx0 += 4) {
-            const int jx = jx0 + q;
-            const bool aok = jx < mx && ia < mx;
-            const bool bok = jx < mx && rowok;
-            const size_t ca = aok ? crow + (size_t)(ia - jx + mx) : 0;
-            const size_t sb = bok ? (size_t)jy * mx + jx : 0;
-            double b[3];
+        const int jx = jx0 + q;
+        const bool aok = jx < mx && ia < mx;
+        const bool bok = jx < mx && jb < my && jy >= 0 && jy < my;
+        const size_t ca = aok ? (size_t)(d + my) * M2 + (size_t)(ia - jx + mx) : 0;
+        const size_t sb = bok ? (size_t)jy * mx + jx : 0;
 #pragma unroll
-            for (int j = 0; j < 3; j++) b[j] = (j < I.nin && bok) ? I.in[j][sb] : 0.0;
+        for (int j = 0; j < 3; j++) b[j] = (j < nin && bok) ? I.in[j][sb] : 0.0;
+#pragma unroll
+        for (int o = 0; o < 3; o++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const double* c = (o < nout && j < nin) ? I.cf[o][j] : nullptr;
+                a[o][j] = (c && aok) ? c[ca] : 0.0;
+            }
+    };
+    if (d0 <= d1) {
+        // software pipeline: the loads of the next step are issued before this step's MFMAs
+        double a[3][3], b[3], an[3][3], bn[3];
+        int d = d0, k = 0;
+        fetch(d, 0, a, b);
+        for (;;) {
+            int dn = d, kn = k + 1;
+            if (kn == nstep) {
+                kn = 0;
+                dn = d + 1;
+            }
+            const bool more = dn <= d1;
+            if (more) fetch(dn, 4 * kn, an, bn);
 #pragma unroll
             for (int o = 0; o < 3; o++) {
-                if (o >= I.nout) break;
+                if (o >= nout) break;
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
-                    if (j >= I.nin) break;
-                    const double* c = I.cf[o][j];
-                    if (!c) continue;
-                    const double a = aok ? c[ca] : 0.0;
-                    acc[o] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[j], acc[o], 0, 0, 0);
+                    if (j >= nin || !I.cf[o][j]) continue;
+                    acc[o][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[o][j], b[j], acc[o][j], 0, 0, 0);
                 }
+            }
+            if (!more) break;
+            d = dn;
+            k = kn;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                b[j] = bn[j];
+#pragma unroll
+                for (int o = 0; o < 3; o++) a[o][j] = an[o][j];
             }
         }
     }
@@ -3598,7 +3627,7 @@ __global__ void __launch_bounds__(64) k_dense(const DenseItem* items, const int*
     if (iy >= my) return;
 #pragma unroll
     for (int o = 0; o < 3; o++) {
-        if (o >= I.nout) break;
+        if (o >= nout) break;
         double* dst = I.out[o];
         const double* add = I.addend[o];
 #pragma unroll
@@ -3607,7 +3636,11 @@ __global__ void __launch_bounds__(64) k_dense(const DenseItem* items, const int*
             if (ix >= mx) continue;
             const size_t ii = (size_t)iy * mx + ix;
             if (I.mask != M_ALL && !in_mask(I.mask, I.el[ii])) continue;
-            double u = acc[o][r] * I.ga_inv;
+            double u = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                if (j < nin) u += acc[o][j][r];
+            u *= I.ga_inv;
             if (add) u += add[ii];
             dst[ii] = u;
         }
