@@ -3545,18 +3545,22 @@ __global__ void __launch_bounds__(64) k_cnvxgs(const CgsItem* items, const int* 
 
 typedef double dense_v4 __attribute__((ext_vector_type(4)));
 
-// One wavefront per tile of 16 x 16 targets (ix0 + i, iy0 + j).  With d = iy - jy fixed the
+// One work-group per tile of 16 x 16 targets (ix0 + i, iy0 + j).  With d = iy - jy fixed the
 // product is an ordinary matrix product: the Toeplitz block T_d (T_d[ix][jx] = cf(ix - jx, d))
 // times the source rows jy = iy - d.  v_mfma_f64_16x16x4f64 takes A = 16 targets x 4 sources
 // (lane i + 16 k holds A[i][k]) and B = 4 sources x 16 target rows (lane j + 16 k holds B[k][j]);
 // register r of lane j + 16 q holds D[4 r + q][j], r = 0..3 (measured on gfx950).  The k loop
 // runs over d and the sources in steps of 4; coefficients outside the tables and sources
-// outside the grid enter as zeros.
-__global__ void __launch_bounds__(64) k_dense(const DenseItem* items, const int* gstart, int nitems)
+// outside the grid enter as zeros.  Each of the group's wavefronts takes a contiguous part of
+// the d range (a serial chain an eighth as long); wavefront 0 adds the parts in wave order.
+constexpr int DENSE_WAVES = 8;
+
+__global__ void __launch_bounds__(64 * DENSE_WAVES) k_dense(const DenseItem* items, const int* gstart, int nitems)
 {
+    __shared__ double part[DENSE_WAVES][3][4][64];
     const int it = find_item(gstart, nitems, blockIdx.x);
     const DenseItem& I = items[it];
-    const int mx = I.mx, my = I.my, M2 = 2 * mx, lane = threadIdx.x;
+    const int mx = I.mx, my = I.my, M2 = 2 * mx, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ntx = (mx + 15) / 16;
     const int t = blockIdx.x - gstart[it];
     const int ix0 = (t % ntx) * 16, iy0 = (t / ntx) * 16;
@@ -3570,7 +3574,10 @@ __global__ void __launch_bounds__(64) k_dense(const DenseItem* items, const int*
         for (int j = 0; j < 3; j++) acc[o][j] = (dense_v4){0.0, 0.0, 0.0, 0.0};
     const int ia = ix0 + i;       // this lane's target in A (row i)
     const int jb = iy0 + i;       // this lane's target row in B (column i)
-    const int d0 = max(iy0 - (my - 1), -(my - 1)), d1 = min(iy0 + 15, my - 1);
+    // the tile's row offsets, split in DENSE_WAVES contiguous parts
+    const int dlo = max(iy0 - (my - 1), -(my - 1)), dhi = min(iy0 + 15, my - 1);
+    const int nd = dhi - dlo + 1, per = (nd + DENSE_WAVES - 1) / DENSE_WAVES;
+    const int d0 = dlo + wave * per, d1 = min(dhi, d0 + per - 1);
     const int nstep = (mx + 3) / 4;
     // the operands of step (d, jx0) for this lane: A of every block, B of every input
     auto fetch = [&](int d, int jx0, double (&a)[3][3], double (&b)[3]) {
@@ -3623,6 +3630,19 @@ __global__ void __launch_bounds__(64) k_dense(const DenseItem* items, const int*
             }
         }
     }
+    // each wavefront's sums over its blocks, then wavefront 0 adds the parts in wave order
+#pragma unroll
+    for (int o = 0; o < 3; o++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            double u = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                if (j < nin) u += acc[o][j][r];
+            part[wave][o][r][lane] = u;
+        }
+    __syncthreads();
+    if (wave != 0) return;
     const int iy = iy0 + i;
     if (iy >= my) return;
 #pragma unroll
@@ -3638,8 +3658,7 @@ __global__ void __launch_bounds__(64) k_dense(const DenseItem* items, const int*
             if (I.mask != M_ALL && !in_mask(I.mask, I.el[ii])) continue;
             double u = 0.0;
 #pragma unroll
-            for (int j = 0; j < 3; j++)
-                if (j < nin) u += acc[o][j][r];
+            for (int w = 0; w < DENSE_WAVES; w++) u += part[w][o][r][lane];
             u *= I.ga_inv;
             if (add) u += add[ii];
             dst[ii] = u;
@@ -3790,7 +3809,7 @@ void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, i
     case OP_ST_SA: hipLaunchKernelGGL(k_st_sa, ge, te, 0, s, (const StItem*)d_items, d_gstart, nitems, ro); break;
     case OP_GD_DP: hipLaunchKernelGGL(k_gd_dp, ge, te, 0, s, (const GdItem*)d_items, d_gstart, nitems, ro); break;
     case OP_CNVXGS: hipLaunchKernelGGL(k_cnvxgs, ge, dim3(64), 0, s, (const CgsItem*)d_items, d_gstart, nitems, ro); break;
-    case OP_DENSE: hipLaunchKernelGGL(k_dense, ge, dim3(64), 0, s, (const DenseItem*)d_items, d_gstart, nitems); break;
+    case OP_DENSE: hipLaunchKernelGGL(k_dense, ge, dim3(64 * DENSE_WAVES), 0, s, (const DenseItem*)d_items, d_gstart, nitems); break;
     case OP_GSB_SETUP: hipLaunchKernelGGL(k_gsb_setup, ge, dim3(GSB_THREADS), 0, s, (const GsbItem*)d_items, d_gstart, nitems, ro); break;
     case OP_GSB_WALK: hipLaunchKernelGGL(k_gsb_walk, ge, dim3(GSB_THREADS), lds_bytes, s, (const GsbItem*)d_items, d_gstart, nitems, ro); break;
     case OP_GSB_CHECK: hipLaunchKernelGGL(k_gsb_check, ge, dim3(GSB_THREADS), 0, s, (const GsbItem*)d_items, d_gstart, nitems, ro); break;
