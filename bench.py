@@ -319,6 +319,34 @@ def sens_leg(cntc, W, n=256, seed=31, ire0=920):
     return out
 
 
+def rough_table(cntc, W):
+    """The single rough patches alone (512^2 FP64, and 1024^2 with FP32 products in the FP64
+    refinement): after a first solve on the grid (coefficient spectra built), a second solve
+    with every kernel timed by HIP events; prints per kernel the launches, time, algorithmic
+    bytes per launch and GB/s against the 8 TB/s HBM peak."""
+    for n, prec, ire in ((512, 0, 961), (1024, 1, 962)):
+        c = W.rough_patch(n=n, precision=prec)
+        W.apply_case(cntc, ire, 1, c)
+        if cntc.calculate(ire, 1) != 0:
+            raise SystemExit("rough patch %d failed" % n)
+        W.apply_case(cntc, ire, 1, c)
+        cntc.amdprofile(True)
+        t0 = time.perf_counter()
+        rc = cntc.calculate(ire, 1)
+        wall = time.perf_counter() - t0
+        prof = cntc.amdprofileget()
+        cntc.amdprofile(False)
+        print("== %d^2 %s: rc %d, wall %.3f s, iterations (itnorm, itcg, ittang, itgs, flags) %s"
+              % (n, "mixed" if prec else "fp64", rc, wall, cntc.getiterations(ire, 1)))
+        print("%-16s %8s %10s %13s %9s %6s" % ("kernel", "launches", "total ms", "alg MB/launch", "GB/s", "frac"))
+        for k, (sec, byt, nl) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+            if nl == 0 or sec <= 0:
+                continue
+            gbs = byt / sec / 1e9
+            print("%-16s %8d %10.3f %13.3f %9.1f %6.3f" % (k, nl, sec * 1e3, byt / nl / 1e6, gbs, gbs / HBM_PEAK_GBS))
+        print("device time in the timed kernels %.3f s" % sum(v[0] for v in prof.values()), flush=True)
+
+
 def forces_of(cntc, idl):
     return np.array([cntc.getcontactforces(i, j) for i, j in idl])
 
@@ -400,6 +428,8 @@ def main():
     ap.add_argument("--inflight", type=int, default=3,
                     help="batches in flight (each on its own set of 2048 problems, 1-3)")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--rough-table", action="store_true",
+                    help="only the single rough patches (512^2 FP64, 1024^2 mixed): per-kernel GB/s")
     ap.add_argument("--sens-patches", type=int, default=256,
                     help="small patches in the sensitivity leg (dense FP64-MFMA products); 0 skips it")
     ap.add_argument("--serve-rate", type=float, default=1000.0)
@@ -408,6 +438,10 @@ def main():
                          "after the last whole step that fits, and the JSON line reports the "
                          "steps actually run")
     args = ap.parse_args()
+    if args.rough_table:
+        from contact_amd import cntc, workloads as W
+        rough_table(cntc, W)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: patches are independent, so a multi-GPU run would "
               "split the batch by patch with no exchange; --gpus %d is not supported" % args.gpus,
