@@ -1354,6 +1354,9 @@ struct Ctx {
             return;
         }
         it.conv = conv_item(P, which, mask, nout, ik, out, nin, jk, in, addend, fp32);
+        const auto sh = E->conv_shape(fp32, P->Lx, P->Ly, nin, nout);
+        it.conv.rpg = sh.first;
+        it.conv.cpg = sh.second;
         op(fp32 ? OP_CONV32 : OP_CONV, it);
     }
     // normal-normal product
@@ -1410,9 +1413,13 @@ Engine::Engine()
 Engine::~Engine()
 {
     serve_stop();
-    if (const char* v = std::getenv("CNTC_FLUSH_STATS"); v && v[0] == '1' && sched_)
+    if (const char* v = std::getenv("CNTC_FLUSH_STATS"); v && v[0] == '1' && sched_) {
         std::fprintf(stderr, "flushes %ld, launches %ld, host issue %.3f s, wait %.3f s\n", sched_->nflush,
                      sched_->flush_launches, sched_->flush_issue_s, sched_->flush_wait_s);
+        int tuned = 0;
+        for (auto& kv : shapes_) tuned += kv.second.first != 0 || kv.second.second != 0;
+        std::fprintf(stderr, "FFT size classes %zu, %d with tuned tile shapes\n", shapes_.size(), tuned);
+    }
     sched_.reset();
     for (auto& t : tw_) (void)hipFree(t.second);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -1427,6 +1434,112 @@ static void twiddle_table(int L, T* h)
         h[2 * e] = (T)cosl(a);
         h[2 * e + 1] = (T)sinl(a);
     }
+}
+
+// FFT tile-shape tuner.  For one size class (precision, Lx, Ly, nin, nout) it runs the
+// product on synthetic zero data of that size (mx = Lx/2, my = Ly/2, every (out, in) block)
+// for the default shapes and for half and twice the rows per row-stage work-group and the
+// columns per column-stage work-group (within the LDS), three times each, timed by HIP
+// events on the engine's stream, and keeps the fastest.  Every shape computes the same bits.
+std::pair<int, int> Engine::conv_shape(bool fp32, int Lx, int Ly, int nin, int nout)
+{
+    const long key = ((((long)(fp32 ? 1 : 0) * 8192 + Lx) * 8192 + Ly) * 4 + nin) * 4 + nout;
+    auto f = shapes_.find(key);
+    if (f != shapes_.end()) return f->second;
+    if (noalloc_) return {0, 0};
+    if (const char* v = std::getenv("CNTC_FFT_TUNE"); v && v[0] == '0') return shapes_[key] = {0, 0};
+    const int kind = fp32 ? OP_CONV32 : OP_CONV;
+    const int M = Lx / 2, H = M + 1, mx = M, my = std::max(1, Ly / 2);
+    const size_t es = fp32 ? 8 : 16, npt = (size_t)mx * my, bsz = (size_t)H * Ly;
+    // device buffers: inputs, outputs, spectra of every block, the two row-spectrum
+    // workspaces (3 components of Ly rows of H bins, as a patch arena has), item and group table
+    const size_t b_in = 8 * npt * nin, b_out = 8 * npt * nout, b_spec = es * bsz * nin * nout,
+                 b_ws = 16 * 3 * bsz, b_item = 4096;
+    char* base = nullptr;
+    const size_t total = b_in + b_out + b_spec + 2 * b_ws + b_item + 5 * 256;
+    if (hipMalloc(&base, total) != hipSuccess) {
+        (void)hipGetLastError();
+        return shapes_[key] = {0, 0};
+    }
+    HIPCHK(hipMemsetAsync(base, 0, total, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    char* q = base;
+    auto take = [&](size_t b) { char* r = q; q += al(b); return r; };
+    ConvItem c{};
+    c.mx = mx; c.my = my; c.Lx = Lx; c.Ly = Ly;
+    c.nin = nin; c.nout = nout; c.mask = M_ALL; c.mode = 0;
+    double* in = (double*)take(b_in);
+    double* out = (double*)take(b_out);
+    char* spec = take(b_spec);
+    for (int j = 0; j < nin; j++) c.in[j] = in + (size_t)j * npt;
+    for (int i = 0; i < nout; i++) {
+        c.out[i] = out + (size_t)i * npt;
+        for (int j = 0; j < nin; j++) c.spec[i][j] = spec + (size_t)(i * nin + j) * bsz * es;
+    }
+    c.wsS = take(b_ws);
+    c.wsT = take(b_ws);
+    c.twM = twiddles(M, fp32);
+    c.twLx = twiddles(Lx, fp32);
+    c.twLy = twiddles(Ly, fp32);
+    c.scale = 1.0;
+    char* d_item = take(b_item);   // the item at 0, its group table at 2048
+    static_assert(sizeof(ConvItem) <= 2048, "the tuner's item slot");
+    int r0 = 0, c0 = 0;
+    conv_default_shape(&c, &r0, &c0);
+    const int nc = std::max(nin, nout);
+    // the stages of one shape, best of three, in milliseconds (stages: {0, 2} rows, {1} columns)
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto timed = [&](int rpg, int cpg, std::initializer_list<int> stages) {
+        c.rpg = rpg;
+        c.cpg = cpg;
+        float best = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            float tot = 0.f;
+            for (int st : stages) {
+                const int ng = op_groups(kind, &c, st);
+                const int gs[2] = {0, ng};
+                // (the previous launch has ended: its event was waited for)
+                HIPCHK(hipMemcpy(d_item, &c, sizeof c, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(d_item + 2048, gs, sizeof gs, hipMemcpyHostToDevice));
+                HIPCHK(hipEventRecord(e0, stream_));
+                launch_op(kind, d_item, (const int*)(d_item + 2048), 1, ng, RedOut{}, st, op_lds(kind, &c, st),
+                          stream_);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipEventRecord(e1, stream_));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                tot += ms;
+            }
+            best = std::min(best, tot);
+        }
+        return best;
+    };
+    int br = 0, bc = 0;
+    {
+        float tbest = timed(0, 0, {0, 2});
+        for (int r : {r0 / 2, r0 * 2}) {
+            if (r < 1 || r > 32 || r == r0 || 2 * (size_t)r * M * es + 2048 > MAX_LDS) continue;
+            const float t = timed(r, 0, {0, 2});
+            if (t < tbest) { tbest = t; br = r; }
+        }
+    }
+    {
+        float tbest = timed(0, 0, {1});
+        for (int cc : {c0 / 2, c0 * 2}) {
+            if (cc < 1 || cc > 16 || cc == c0 || 2 * (size_t)nc * cc * Ly * es + 2048 > MAX_LDS) continue;
+            const float t = timed(0, cc, {1});
+            if (t < tbest) { tbest = t; bc = cc; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipStreamSynchronize(stream_));
+    HIPCHK(hipFree(base));
+    return shapes_[key] = {br, bc};
 }
 
 const void* Engine::twiddles(int L, bool fp32)

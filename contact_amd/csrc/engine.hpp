@@ -184,6 +184,10 @@ public:
     void get_eldiv(const Patch& p, int* out) const;
     // twiddle tables exp(-2 pi i e / L), e < L, cached per length and precision
     const void* twiddles(int L, bool fp32);
+    // tile shapes of the FFT products of one size class (CNTC_FFT_TUNE=0: the defaults), timed
+    // once per class on synthetic data of that size: {rows per work-group, columns per
+    // work-group}, 0 for a default
+    std::pair<int, int> conv_shape(bool fp32, int Lx, int Ly, int nin, int nout);
     std::string last_error;
     std::chrono::steady_clock::time_point t_solve0;   // start of the current solve call
     // per-kernel timing (HIP events around every launch); key = kind*4 + stage
@@ -220,6 +224,7 @@ private:
     hipStream_t stream_ = nullptr;
     std::unique_ptr<Scheduler> sched_;
     std::vector<std::pair<long, void*>> tw_;
+    std::map<long, std::pair<int, int>> shapes_;   // conv_shape by (fp32, Lx, Ly, nin, nout)
 };
 
 int opt_fft_size(int n);

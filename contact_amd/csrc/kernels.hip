@@ -1296,6 +1296,9 @@ struct LdsSync {
 __host__ __device__ inline int rows_per_group(int M) { int r = 2048 / M; return r < 1 ? 1 : (r > 32 ? 32 : r); }
 // columns per work-group for a column length L and nc components
 __host__ __device__ inline int cols_per_group(int L, int nc) { int r = 2048 / (L * nc); return r < 1 ? 1 : (r > 16 ? 16 : r); }
+// the shapes of one product item: its own (chosen by the engine's tuner) or the defaults
+__host__ __device__ inline int conv_rows(const ConvItem& c) { return c.rpg > 0 ? c.rpg : rows_per_group(c.Lx / 2); }
+__host__ __device__ inline int conv_cols(const ConvItem& c, int nc) { return c.cpg > 0 ? c.cpg : cols_per_group(c.Ly, nc); }
 
 template <typename R>
 __device__ inline R to_r(double v) { return (R)v; }
@@ -1313,7 +1316,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_rowfwd(const ConvItem* items, c
     // mode 0: inputs are grid functions (rows my, length mx); modes 1/2: dense Lx*Ly arrays
     const int nrow = I.mode == 0 ? I.my : I.Ly;
     const int nxr = I.mode == 0 ? I.mx : I.Lx;
-    const int rpg = rows_per_group(M);
+    const int rpg = conv_rows(I);
     const int ngy = (nrow + rpg - 1) / rpg;
     int t = blockIdx.x - gstart[it];
     int c = t / ngy;
@@ -1363,7 +1366,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_colmul(const ConvItem* items, c
     const int nrow_out = I.mode == 0 ? I.my : I.Ly;
     const int nin = I.nin, nout = I.mode == 0 ? I.nout : nin;
     const int nc = nin > nout ? nin : nout;
-    const int cpg = cols_per_group(Ly, nc);
+    const int cpg = conv_cols(I, nc);
     int kx0 = (blockIdx.x - gstart[it]) * cpg;
     int ncol = min(cpg, H - kx0);
     cpx<R>* a = (cpx<R>*)lds_raw;
@@ -1435,7 +1438,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_rowinv(const ConvItem* items, c
     const int Lx = I.Lx, M = Lx / 2, H = M + 1;
     const int nrow = I.mode == 0 ? I.my : I.Ly;
     const int nxr = I.mode == 0 ? I.mx : I.Lx;
-    const int rpg = rows_per_group(M);
+    const int rpg = conv_rows(I);
     const int ngy = (nrow + rpg - 1) / rpg;
     int t = blockIdx.x - gstart[it];
     int c = t / ngy;
@@ -3759,14 +3762,16 @@ int op_groups(int kind, const void* item, int conv_stage)
         int M = c->Lx / 2, H = M + 1;
         int nrow = c->mode == 0 ? c->my : c->Ly;
         int nout = c->mode == 0 ? c->nout : c->nin;
-        if (conv_stage == 0) return c->nin * ((nrow + rows_per_group(M) - 1) / rows_per_group(M));
+        const int rpg = conv_rows(*c);
+        (void)M;
+        if (conv_stage == 0) return c->nin * ((nrow + rpg - 1) / rpg);
         if (conv_stage == 1) {
             int nc = c->nin > nout ? c->nin : nout;
-            int cpg = cols_per_group(c->Ly, nc);
+            int cpg = conv_cols(*c, nc);
             return (H + cpg - 1) / cpg;
         }
         if (c->mode == 1) return 0;
-        return nout * ((nrow + rows_per_group(M) - 1) / rows_per_group(M));
+        return nout * ((nrow + rpg - 1) / rpg);
     }
     }
     return 0;
@@ -3776,10 +3781,10 @@ static size_t conv_lds(int kind, const ConvItem* c, int stage)
 {
     size_t es = kind == OP_CONV32 ? 8 : 16;
     int M = c->Lx / 2;
-    if (stage == 0 || stage == 2) return 2 * (size_t)rows_per_group(M) * M * es;
+    if (stage == 0 || stage == 2) return 2 * (size_t)conv_rows(*c) * M * es;
     int nout = c->mode == 0 ? c->nout : c->nin;
     int nc = c->nin > nout ? c->nin : nout;
-    return 2 * (size_t)nc * cols_per_group(c->Ly, nc) * c->Ly * es;
+    return 2 * (size_t)nc * conv_cols(*c, nc) * c->Ly * es;
 }
 
 void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, int ngroups,
@@ -3961,6 +3966,14 @@ void launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s)
     size_t g = (n + 255) / 256;
     if (g > 1024) g = 1024;
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)g), dim3(256), 0, s, p, n, v);
+}
+
+void conv_default_shape(const ConvItem* c, int* rpg, int* cpg)
+{
+    const int nout = c->mode == 0 ? c->nout : c->nin;
+    const int nc = c->nin > nout ? c->nin : nout;
+    *rpg = rows_per_group(c->Lx / 2);
+    *cpg = cols_per_group(c->Ly, nc);
 }
 
 void launch_rollcoef(int n, const double* cs, double* cv, hipStream_t s)

@@ -28,6 +28,10 @@ size_t op_lds(int kind, const void* item, int stage);
 void launch_op(int kind, const void* d_items, const int* d_gstart, int nitems, int ngroups,
                RedOut ro, int conv_stage, size_t lds_bytes, hipStream_t s);
 
+// the default tile shapes of a product item (rows per work-group of the row stages, columns
+// per work-group of the column stage)
+void conv_default_shape(const ConvItem* c, int* rpg, int* cpg);
+
 void launch_rollcoef(int n, const double* cs, double* cv, hipStream_t s);
 // p[0..n) = v, by a kernel (usable where a caller may capture the stream into a graph)
 void launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s);

@@ -234,6 +234,9 @@ struct LedgeItem {
 struct ConvItem {
     int mx, my, Lx, Ly;        // grid and padded real sizes; complex row length M = Lx/2
     int nin, nout, mask, mode; // mode 0: product, 1: forward transform only, 2: reciprocal
+    int rpg, cpg;              // rows per work-group of the row stages, columns per work-group
+                               // of the column stage (0: the default shapes); any shape gives
+                               // the same bits, each transform's arithmetic is its own
     const double* in[3];
     double* out[3];
     const double* addend[3];   // added to the product where written (may be null)
