@@ -1020,13 +1020,34 @@ __global__ void __launch_bounds__(GD_THREADS) k_gd_update(const GdItem* items, c
     }
 }
 
+// 1/d and 1/sqrt(x) on the latency path of SteadyGS: the hardware estimates refined by two
+// Newton steps each (within an ulp or two of the correctly rounded values, without the IEEE
+// division and square-root sequences)
+__device__ inline double fast_rcp(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ inline double fast_rsqrt(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double t = fma(-h * y, y, 0.5);   // y <- y + y (1/2 - x y^2 / 2)
+    y = fma(y, t, y);
+    t = fma(-h * y, y, 0.5);
+    return fma(y, t, y);
+}
+
 // plstrc without plasticity (elmtrc), src/m_solvpt.f90:3258-3807; the plastic branch only
 // reports a violated constraint when tau_c0 = 1e20 and k_tau = 0
 // scale_dir: with omegas = 1 put the slip traction on the bound along the Newton solution
-// directly instead of through atan2, cos and sin (the same direction up to rounding), and
-// divide a pair of values by a common denominator through one reciprocal (the same values up
-// to rounding); used on the sequential path of SteadyGS, where the chain of FP64 divisions
-// and transcendentals is the latency of every element
+// directly instead of through atan2, cos and sin (the same direction up to rounding), divide
+// by refined reciprocals and reciprocal square roots, and test the adhesion bound on squares
+// (the same values up to rounding); used on the sequential path of SteadyGS, where the chain
+// of FP64 divisions, square roots and transcendentals is the latency of every element
 __device__ void elmtrc(int8_t& el, const double c11, const double c12, const double c21,
                        const double c22, double eps, double pr[3], double mus, double s[2],
                        double omegah = 1.0, double omegas = 1.0, bool scale_dir = false)
@@ -1034,7 +1055,7 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
     // (a / d, b / d), or through 1 / d on the latency path
     auto pair_div = [scale_dir](double a, double b, double d, double& qa, double& qb) {
         if (scale_dir) {
-            const double r = 1.0 / d;
+            const double r = fast_rcp(d);
             qa = a * r;
             qb = b * r;
         } else {
@@ -1051,7 +1072,32 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
     int itry = 0, violat = 1;
     while (itry <= 3 && violat == 1) {
         itry++;
-        if (el == ADHES) {
+        if (el == ADHES && scale_dir) {
+            // the same steps with the bound compared on squares and the reciprocals refined
+            const double trc_bound = mus * pr[2];
+            const double detinv = fast_rcp(c11 * c22 - c12 * c21);
+            const double dp0 = (-c22 * s[0] + c12 * s[1]) * detinv;
+            const double dp1 = (c21 * s[0] - c11 * s[1]) * detinv;
+            pr[0] = pold1 + omegah * dp0;
+            pr[1] = pold2 + omegah * dp1;
+            const double p2 = pr[0] * pr[0] + pr[1] * pr[1];
+            const double bnd = fmin(trc_bound, taucs);
+            if (p2 <= bnd * bnd) {
+                violat = 0;
+            } else {
+                const double q0 = pold1 + dp0, q1 = pold2 + dp1;
+                if (q0 * q0 + q1 * q1 <= bnd * bnd) {
+                    const double f = bnd * fast_rsqrt(p2);
+                    pr[0] *= f;
+                    pr[1] *= f;
+                    violat = 0;
+                } else {
+                    violat = 1;
+                }
+            }
+            six = si0x + c11 * pr[0] + c12 * pr[1];
+            siy = si0y + c21 * pr[0] + c22 * pr[1];
+        } else if (el == ADHES) {
             double trc_bound = mus * pr[2];
             double detinv = 1.0 / (c11 * c22 - c12 * c21);
             double dp0 = (-c22 * s[0] + c12 * s[1]) * detinv;
@@ -1076,8 +1122,14 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
         } else if (el == SLIP) {
             violat = 0;
             double trc_bound = mus * pr[2];
-            double sabs = sqrt(si0x * si0x + si0y * si0y);
-            pair_div(-trc_bound * si0x, -trc_bound * si0y, sabs, pr[0], pr[1]);
+            if (scale_dir) {
+                const double f = trc_bound * fast_rsqrt(si0x * si0x + si0y * si0y);
+                pr[0] = -f * si0x;
+                pr[1] = -f * si0y;
+            } else {
+                double sabs = sqrt(si0x * si0x + si0y * si0y);
+                pair_div(-trc_bound * si0x, -trc_bound * si0y, sabs, pr[0], pr[1]);
+            }
             six = si0x + c11 * pr[0] + c12 * pr[1];
             siy = si0y + c21 * pr[0] + c22 * pr[1];
             double fk0 = pr[1] * six - pr[0] * siy;
@@ -1106,12 +1158,11 @@ __device__ void elmtrc(int8_t& el, const double c11, const double c12, const dou
             if (scale_dir && omegas == 1.0) {
                 // alph0 + (alph1 - alph0) is alph1 up to rounding (the 2 pi wrap keeps the
                 // direction): the traction at the bound in the Newton solution's direction
-                double pabs = sqrt(pr[0] * pr[0] + pr[1] * pr[1]);
-                if (pabs > 0.0) {
-                    double ux, uy;
-                    pair_div(pr[0], pr[1], pabs, ux, uy);
-                    pr[0] = trc_bound * ux;
-                    pr[1] = trc_bound * uy;
+                const double p2 = pr[0] * pr[0] + pr[1] * pr[1];
+                if (p2 > 0.0) {
+                    const double f = trc_bound * fast_rsqrt(p2);
+                    pr[0] *= f;
+                    pr[1] *= f;
                 } else {
                     pr[0] = trc_bound;
                     pr[1] = 0.0;
