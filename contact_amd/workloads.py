@@ -126,7 +126,8 @@ def apply_case(cntc, ire, icp, c):
     cntc.setflags(ire, icp, [1972, 1973, 2009], [c["tang"], c["norm"], c.get("precision", 0)])
     if c.get("pvtime") is not None:
         cntc.setflags(ire, icp, [1970, 1983], [c["pvtime"], c.get("iestim", 0)])
-    cntc.setmaterialparameters(ire, icp, 0, [c["nu"], c["nu"], c["g"], c["g"]])
+    # poiss 1, 2 and gg 1, 2: "nu2" and "g2" set the second body of dissimilar materials (K != 0)
+    cntc.setmaterialparameters(ire, icp, 0, [c["nu"], c.get("nu2", c["nu"]), c["g"], c.get("g2", c["g"])])
     cntc.setfrictionmethod(ire, icp, 0, [c["fstat"], c["fstat"]])
     cntc.setreferencevelocity(ire, icp, c.get("veloc", 10000.0))
     if c.get("gausei", 5) == 5:

@@ -259,8 +259,8 @@ def case_input(c):
     inp.norm = c["norm"]
     inp.fn = c.get("fn", 0.0)
     inp.pen = c.get("pen", 0.0)
-    inp.poiss1 = inp.poiss2 = c["nu"]
-    inp.gg1 = inp.gg2 = c["g"]
+    inp.poiss1, inp.poiss2 = c["nu"], c.get("nu2", c["nu"])
+    inp.gg1, inp.gg2 = c["g"], c.get("g2", c["g"])
     inp.fstat = c["fstat"]
     inp.veloc = c.get("veloc", 10000.0)
     inp.maxgs = c["maxgs"]

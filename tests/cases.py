@@ -166,3 +166,22 @@ def carter_convexgs_case():
 # testbank/testbank.ref_out:551-601 (case 6): Fx, Fx/(mu Fn), contact, adhesion, slip,
 # NormCG iterations, ConvexGS sweeps, approach, pmax as printed
 CARTER_GOLDEN = (6974.0, 0.6497, 51, 30, 21, 11, 163, 6.492e-3, 113.9)
+
+
+def panag_case():
+    """testbank/testbank.inp:187-203, case 9: dissimilar materials (poiss 0.1 / 0.28, gg 1 /
+    81000, so K = 0.4) in steady rolling with SteadyGS, Fn and Fx/(mu Fn) = -0.4718 prescribed,
+    h = 0.3332e-3 (x^2 + y^2) on a 25 x 25 grid: the Panagiotopoulos alternation of the normal
+    and tangential problems (src/m_scontc.f90:356-553, up to 10 outer iterations)."""
+    from contact_amd import workloads as W
+    return W.base_case(tang=3, norm=1, fn=2.468, cksi=0.0, ceta=0.0, cphi=1.0e-4, fstat=0.6,
+                       g=1.0, g2=0.81e5, nu=0.1, nu2=0.28, eps=1e-6, maxgs=80, maxin=100,
+                       gausei=0, ipotcn=1, mx=25, my=25, xl=-13.75, yl=-13.75, dx=1.1, dy=1.1,
+                       prmudf=[0.3332e-3, 0.0, 0.3332e-3, 0.0, 0.0, 0.0], dq=1.1, veloc=1000.0,
+                       pvtime=2, force=1, fxrel=-0.4718)
+
+
+# testbank/testbank.ref_out:951-963, 965-969 (case 9): Fn, Fy, Mz, cksi, Fy/(mu Fn), approach,
+# pmax, contact, adhesion, slip, outer iterations, dFx/dcksi, dFy/dcksi as printed
+PANAG_GOLDEN = (2.468, -0.2610, -1.470, 2.827e-4, -0.1762, 7.607e-2, 1.199e-2, 292, 140, 152,
+                10, -272.0, 30.3)

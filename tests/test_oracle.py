@@ -459,3 +459,25 @@ def test_convexgs_equals_other_solvers(oracle):
         print("T=%d" % tang, "G=%d" % other, fa, "ConvexGS", fb, Pb.iters())
         assert np.abs(fa - fb).max() <= 1e-8 * fa[0]
         assert np.array_equal(Pa.eldiv(), Pb.eldiv())
+
+
+def test_testbank_panagiotopoulos_golden(oracle):
+    """testbank case 9 (dissimilar materials, K = 0.4: the normal and tangential problems
+    coupled through the cross influence coefficients and alternated by panprc,
+    src/m_scontc.f90:356-553): every figure testbank.ref_out:951-969 prints for it."""
+    c = C.panag_case()
+    P = oracle.Problem()
+    assert P.calculate(*C.to_oracle(oracle, c)) == 0
+    g = C.PANAG_GOLDEN
+    fn, fx, fy, mz = P.forces()
+    assert round(fn, 3) == g[0] and round(fy, 4) == g[1] and round(mz, 3) == g[2]
+    assert round(fx, 4) == round(-0.4718 * 0.6 * 2.468, 4)
+    cksi = P.creepages()[0]
+    assert float("%.3e" % cksi) == g[3] and round(fy / (0.6 * fn), 4) == g[4]
+    assert float("%.3e" % P.pen()) == g[5]
+    assert float("%.3e" % P.tractions()[0].max()) == g[6]
+    el = P.eldiv()
+    assert ((el > 0).sum(), (el == 1).sum(), (el == 2).sum()) == g[7:10]
+    assert P.iters()[0] == P.iters()[1] == g[10]
+    S = P.sens()
+    assert round(S[1, 1]) == g[11] and round(S[2, 1], 1) == g[12]
