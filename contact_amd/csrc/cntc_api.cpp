@@ -293,6 +293,12 @@ void cntc_setmaterialparameters(int* ire, int* icp, int* mdigit, int* nparam, do
     in.poiss2 = rparam[1];
     in.gg1 = rparam[2] * P->scl.forc / P->scl.area;
     in.gg2 = rparam[3] * P->scl.forc / P->scl.area;
+    // MAXOUT for quasi-identity or not (src/contact_addon.f90:1459-1462): the Panagiotopoulos
+    // alternation runs up to 10 times for dissimilar materials, once otherwise
+    const double ga = 2.0 / (1.0 / in.gg1 + 1.0 / in.gg2);
+    const double ak = (ga / 4.0) * ((1.0 - 2.0 * in.poiss1) / in.gg1 - (1.0 - 2.0 * in.poiss2) / in.gg2);
+    if (in.maxout >= 2 && std::fabs(ak) < 1e-4) in.maxout = 1;
+    if (in.maxout <= 1 && std::fabs(ak) >= 1e-4) in.maxout = 10;
 }
 
 void cntc_settimestep(int* ire, int* icp, double* dt)

@@ -1739,7 +1739,7 @@ size_t patch_bytes(int mx, int my)
     size_t pH = (size_t)mx + 1, pL = 2 * (size_t)my;
     size_t ws = std::max((size_t)3 * Ly * H, pH * pL) * 16;
     size_t emb = std::max((size_t)Lx * Ly, nb) * 8;
-    return al(n) + al(42 * n * 8) + al(9 * nb * 8) * 2 + al(3 * nb * 8) + al(9 * bsz * 16) * 2 +
+    return al(n) + al(46 * n * 8) + al(9 * nb * 8) * 2 + al(3 * nb * 8) + al(9 * bsz * 16) * 2 +
            al(3 * bsz * 16) + al(9 * bsz * 8) + al(3 * bsz * 8) + al(3 * emb) + 2 * al(ws) + al(64 * 8) +
            4096;
 }
@@ -1783,7 +1783,14 @@ bool Engine::prepare(Patch& P)
     P.nu = P.ga * (in.poiss1 / in.gg1 + in.poiss2 / in.gg2) / 2.0;
     P.ak = (P.ga / 4.0) * ((1.0 - 2.0 * in.poiss1) / in.gg1 - (1.0 - 2.0 * in.poiss2) / in.gg2);
     P.nt_cpl = std::fabs(P.ak) >= 1e-6;
-    if (P.nt_cpl) { R.ierror = ERR_OTHER; last_error = "dissimilar materials (K != 0) are not supported"; return false; }
+    // K != 0 couples the normal and tangential problems.  Its right-hand side terms of previous
+    // tractions (A'_tn pv_n for shift and transient rolling with a previous case, and the
+    // leading-edge form (1b) at dq != dx) are not built: those combinations are refused.
+    if (P.nt_cpl && ((P.in.tang == 1 || P.in.tang == 2) && P.in.pvtime == 0)) {
+        R.ierror = ERR_OTHER;
+        last_error = "dissimilar materials (K != 0) with the previous tractions of shift or transient rolling are not supported";
+        return false;
+    }
     P.muscal = in.fstat;
     P.fntrue = in.fn;
     P.pen = in.pen;
@@ -1949,13 +1956,15 @@ bool Engine::prepare(Patch& P)
     char* const arena = b;
     auto take = [&](size_t bytes) { char* r = b; b += al(bytes); return r; };
     P.el = (int8_t*)take(n);
-    double* d = (double*)take(42 * n * 8);
+    double* d = (double*)take(46 * n * 8);
     P.pn = d; P.px = d + n; P.py = d + 2 * n; P.sx = d + 3 * n; P.sy = d + 4 * n;
     P.pvx = d + 5 * n; P.pvy = d + 6 * n; P.pvn = d + 7 * n;
     P.h = d + 8 * n; P.hx = d + 9 * n; P.hy = d + 10 * n; P.wsx = d + 11 * n; P.wsy = d + 12 * n;
     for (int k = 0; k < 24; k++) P.w[k] = d + (13 + k) * n;
     P.un = d + 37 * n; P.ux = d + 38 * n; P.uy = d + 39 * n;
     P.facdt = d + 40 * n; P.lix = d + 41 * n;
+    P.hn = P.nt_cpl ? d + 42 * n : P.h;
+    for (int k = 0; k < 3; k++) P.po[k] = d + (43 + k) * n;
     P.cf = (double*)take(9 * nb * 8);
     P.cfv = (double*)take(9 * nb * 8);
     P.cfm = (double*)take(3 * nb * 8);
@@ -2109,7 +2118,9 @@ static Task setup_coefficients(Ctx C, bool need_cv, bool cv_table = false)
     // spectra of the blocks: embed (wrap, scaled by 1/(G M Ly)) then forward transform
     double scale = (1.0 / P.ga) / ((double)M * (double)P.Ly);
     std::vector<std::pair<const double*, void*>> blocks;
-    int bl[5] = {0, 1, 3, 4, 8};   // xx, xy, yx, yy, zz (normal-tangential blocks are zero, K=0)
+    // xx, xy, yx, yy, zz; the normal-tangential blocks xz, yz, zx, zy are zero unless K != 0
+    std::vector<int> bl = {0, 1, 3, 4, 8};
+    if (P.nt_cpl) bl = {0, 1, 2, 3, 4, 5, 6, 7, 8};
     for (int b : bl) blocks.push_back({P.cf + (size_t)b * nb, (char*)P.spec + (size_t)b * bsz * 16});
     if (need_cv)
         for (int b : bl) blocks.push_back({P.cfv + (size_t)b * nb, (char*)P.specv + (size_t)b * bsz * 16});
@@ -2239,7 +2250,7 @@ static NcgItem ncg_item(Patch& P)
     I.res = P.w[1];
     I.dd = P.w[3];
     I.v = P.w[5];
-    I.h = P.h;
+    I.h = P.hn;
     I.unn = P.w[7];
     I.hmin = P.hmin;
     return I;
@@ -2263,7 +2274,7 @@ static Task normcg(Ctx C, int norm, int maxcg, double eps, double* pen, int* itc
 
     if (norm == 1) *pen = 0.0;
     C.set(M_ALL, rhs, *pen);
-    C.axpy(M_ALL, -1.0, P.h, rhs);
+    C.axpy(M_ALL, -1.0, P.hn, rhs);
     C.set(M_ALL, res, 0.0);
     C.set(M_ALL, v, 0.0);
     C.set(M_ALL, r_prv, 0.0);
@@ -2485,6 +2496,15 @@ static Task snorm(Ctx C, int* itnorm_out)
     double* unn = P.w[7];
     bool use_vecfft = true;   // the product of the whole area serves the expansion test
     co_await setup_prec(C, 2);
+    if (P.nt_cpl) {
+        // K != 0: the normal displacement of the current tangential tractions joins the gap,
+        // hn = h + A_nt p_t on every element (src/m_snorm.f90:112-119)
+        int ik[1] = {2}, jk[2] = {0, 1};
+        double* o[1] = {P.hn};
+        double* in[2] = {P.px, P.py};
+        const double* ad[1] = {P.h};
+        C.conv(0, M_ALL, 1, ik, o, 2, jk, in, ad);
+    }
     int itnorm = 0, itcg = 0, ret = 0;
     bool zready;
     Res r;
@@ -3422,6 +3442,21 @@ static Task stang(Ctx C, int* ittang_out)
             C.axpy(M_INT, -1.0, ux, P.wsx, uy, P.wsy);
         }
     }
+    if (P.nt_cpl) {
+        // K != 0: the tangential displacements of the normal tractions, ws += A_tn ps_n - A'_tn
+        // pv_n on the interior, with pv = ps in steady rolling; pv = 0 in shift and transient
+        // rolling without a previous case (stang_rhs, src/m_stang.f90:828-869)
+        double *ux = P.w[20], *uy = P.w[21];
+        int ik[2] = {0, 1}, jk[1] = {2};
+        double* o[2] = {ux, uy};
+        double* in[1] = {P.pn};
+        C.conv(0, M_INT, 2, ik, o, 1, jk, in);
+        C.axpy(M_INT, 1.0, ux, P.wsx, uy, P.wsy);
+        if (P.in.tang == 3) {
+            C.conv(1, M_INT, 2, ik, o, 1, jk, in);
+            C.axpy(M_INT, -1.0, ux, P.wsx, uy, P.wsy);
+        }
+    }
     // with forces prescribed, solvpt rebuilds its right-hand side from wsfix1 (kept in w[22],
     // w[23]) and the current creepages at every call
     if (P.in.force3 >= 1) snap(C, nullptr, nullptr, {P.wsx, P.wsy}, {P.w[22], P.w[23]});
@@ -3515,29 +3550,63 @@ static Task calculate(Ctx C)
             for (int ii = IIN_KSI; ii <= IIN_PHI; ii++)
                 for (int io = IOUT_FN; io <= IOUT_MZ; io++) sens_at(P, io, ii) = 0.0;
     }
-    co_await setup_coefficients(C, P.in.tang == 2, P.in.tang == 3);
-    int itnorm = 0, ittang = 0;
-    // B=2: the Hertzian pressure is the normal solution (panprc skips NORM, src/m_scontc.f90:409-411)
-    int rc = P.in.bound == 2 ? 0 : co_await snorm(C, &itnorm);
-    if (rc == 0) {
-        R.itnorm = itnorm;
+    // steady rolling with K != 0 also needs the spectra of the shifted coefficients (A'_tn)
+    co_await setup_coefficients(C, P.in.tang == 2 || (P.in.tang == 3 && P.nt_cpl), P.in.tang == 3);
+    // panprc (src/m_scontc.f90:356-553): NORM and TANG alternate until the tractions change by
+    // at most 5 eps of their RMS on the interior, up to maxout times (1 unless K != 0)
+    const int maxout = P.nt_cpl ? std::max(1, P.in.maxout) : 1;
+    int itnorm_tot = 0, ittang_tot = 0, itout = 0, rc = 0;
+    double difid = 1.0, dif = 200.0 * difid;
+    if (maxout > 1) C.copy(M_ALL, P.px, P.po[0], P.py, P.po[1]), C.copy(M_ALL, P.pn, P.po[2]);
+    while (dif > difid && itout < maxout && itnorm_tot >= 0 && ittang_tot >= 0) {
+        itout++;
+        int itnorm = 0, ittang = 0;
+        // B=2: the Hertzian pressure is the normal solution (panprc skips NORM, :409-411)
+        rc = P.in.bound == 2 ? 0 : co_await snorm(C, &itnorm);
+        if (rc) break;
+        itnorm_tot = itnorm >= 0 ? itnorm_tot + itnorm : -1;
         C.set(M_EXT, P.px, 0.0, P.py);
         C.set(M_EXT, P.pn, 0.0);
         Res r = co_await C.count(M_INT);
         if (P.in.tang == 0 || r.v[0] <= 0) {
             C.set(M_ALL, P.sx, 0.0, P.sy);
+            dif = 0.0;
         } else {
             rc = co_await stang(C, &ittang);
-            R.ittang = ittang;
+            if (rc) break;
+            ittang_tot = ittang >= 0 ? ittang_tot + ittang : -1;
+            if (maxout > 1) {
+                // dif = RMS of the change of (px, py, pn) over the interior, difid = 5 eps RMS(p)
+                C.axpy(M_ALL, -1.0, P.px, P.po[0], P.py, P.po[1]);
+                C.axpy(M_ALL, -1.0, P.pn, P.po[2]);
+                Res d2 = co_await C.sumsq(M_INT, P.po[0], P.po[1]);
+                Res d1 = co_await C.sumsq(M_INT, P.po[2]);
+                Res p2 = co_await C.sumsq(M_INT, P.px, P.py);
+                Res p1 = co_await C.sumsq(M_INT, P.pn);
+                const double cnt = 3.0 * std::max(1.0, d1.v[1]);
+                dif = std::sqrt((d2.v[0] + d1.v[0]) / cnt);
+                difid = 5.0 * P.in.eps * std::sqrt((p2.v[0] + p1.v[0]) / cnt);
+                C.copy(M_ALL, P.px, P.po[0], P.py, P.po[1]);
+                C.copy(M_ALL, P.pn, P.po[2]);
+            }
         }
     }
+    R.itnorm = itnorm_tot;
+    R.ittang = ittang_tot;
     // displacement differences of the solution (soutpt, src/m_soutpt.f90:371-397): us = A ps on
     // the interior, 0 elsewhere (init_curr_data clears us for every case, src/m_sdis.f90:624);
     // with K = 0 the normal and tangential parts decouple.  Read by cntc_getdisplacements.
     C.set(M_ALL, P.un, 0.0);
     C.set(M_ALL, P.ux, 0.0, P.uy);
-    if (P.in.bound != 2) C.conv_n(0, M_INT, P.un, P.pn);   // B=2: normal part 0 (:379-380)
-    if (P.in.tang != 0) C.conv_t(0, M_INT, P.ux, P.uy, P.px, P.py);
+    if (P.nt_cpl && P.in.bound != 2 && P.in.tang != 0) {
+        int ik[3] = {0, 1, 2}, jk[3] = {0, 1, 2};
+        double* o[3] = {P.ux, P.uy, P.un};
+        double* in[3] = {P.px, P.py, P.pn};
+        C.conv(0, M_INT, 3, ik, o, 3, jk, in);
+    } else {
+        if (P.in.bound != 2) C.conv_n(0, M_INT, P.un, P.pn);   // B=2: normal part 0 (:379-380)
+        if (P.in.tang != 0) C.conv_t(0, M_INT, P.ux, P.uy, P.px, P.py);
+    }
     // forces (soutpt, src/m_soutpt.f90:424-450)
     AnyItem fi;
     fi.setup = setup_item(P);

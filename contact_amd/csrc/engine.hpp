@@ -130,6 +130,8 @@ struct Patch {
     double *pvx = nullptr, *pvy = nullptr, *pvn = nullptr;
     double *h = nullptr, *hx = nullptr, *hy = nullptr, *wsx = nullptr, *wsy = nullptr;
     double* w[24] = {};              // work vectors
+    double* hn = nullptr;            // the normal gap NormCG solves for: h, plus A_nt p_t when K != 0
+    double* po[3] = {};              // tractions of the previous outer (Panagiotopoulos) iteration
     double *un = nullptr, *ux = nullptr, *uy = nullptr;   // displacement differences A ps
     double *facdt = nullptr, *lix = nullptr;  // leading edge of transient rolling (P.ledge)
     bool ledge = false;              // T=2 with dq > dx

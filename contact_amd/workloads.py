@@ -132,16 +132,16 @@ def apply_case(cntc, ire, icp, c):
     cntc.setreferencevelocity(ire, icp, c.get("veloc", 10000.0))
     if c.get("gausei", 5) == 5:
         g = GD_PARAMS
-        cntc.setsolverflags(ire, icp, 5, [c["maxgs"], c["maxin"], 30, 1, g["kdowfb"]],
+        cntc.setsolverflags(ire, icp, 5, [c["maxgs"], c["maxin"], 30, c.get("maxout", 1), g["kdowfb"]],
                             [c["eps"], g["fdecay"], g["betath"], g["d_ifc"], g["d_lin"],
                              g["d_cns"], g["d_slp"], g["pow_s"]])
     elif c["gausei"] in (2, 3):
         # G=2 ConvexGS, G=3 fixed relaxation: omegah, omegas (inislp, omgslp unused with L=0)
-        cntc.setsolverflags(ire, icp, c["gausei"], [c["maxgs"], c["maxin"], 30, 1, 1],
+        cntc.setsolverflags(ire, icp, c["gausei"], [c["maxgs"], c["maxin"], 30, c.get("maxout", 1), 1],
                             [c["eps"], c.get("omegah", 0.5), c.get("omegas", 0.5), 0.9])
     else:
         # G=0: the default solvers (SteadyGS for T=3), src/test_table.f90:151-160
-        cntc.setsolverflags(ire, icp, 0, [c["maxgs"], c["maxin"], 30, 1], [c["eps"]])
+        cntc.setsolverflags(ire, icp, 0, [c["maxgs"], c["maxin"], 30, c.get("maxout", 1)], [c["eps"]])
     if c["ipotcn"] == -3:
         cntc.sethertzcontact(ire, icp, -3, [c["mx"], c["my"], c["aa"], c["bb"], c["scale"]])
     elif c["ipotcn"] == -2:

@@ -178,7 +178,7 @@ def panag_case():
                        g=1.0, g2=0.81e5, nu=0.1, nu2=0.28, eps=1e-6, maxgs=80, maxin=100,
                        gausei=0, ipotcn=1, mx=25, my=25, xl=-13.75, yl=-13.75, dx=1.1, dy=1.1,
                        prmudf=[0.3332e-3, 0.0, 0.3332e-3, 0.0, 0.0, 0.0], dq=1.1, veloc=1000.0,
-                       pvtime=2, force=1, fxrel=-0.4718)
+                       pvtime=2, force=1, fxrel=-0.4718, maxout=10)
 
 
 # testbank/testbank.ref_out:951-963, 965-969 (case 9): Fn, Fy, Mz, cksi, Fy/(mu Fn), approach,

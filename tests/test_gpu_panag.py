@@ -1,0 +1,55 @@
+"""Dissimilar materials (K != 0) through the cntc_* C ABI: the normal and tangential problems
+coupled through the cross influence coefficients and alternated by the Panagiotopoulos loop
+(src/m_scontc.f90:356-553), with the coupling terms in the normal gap (src/m_snorm.f90:112-119)
+and in the tangential right-hand side (src/m_stang.f90:859-864).
+
+testbank case 9 (eps 1e-6, as the reference runs it) is checked against every figure that
+testbank.ref_out prints for it.  At eps 1e-10 the forces agree with the oracle to 1e-8 of Fn,
+the tractions to 1e-7 of max pn and the element states are identical, for SteadyGS, GDsteady
+and a shift by TangCG.
+"""
+import numpy as np
+import pytest
+
+from tests import cases as C
+from tests.test_gpu_parity import compare, run_gpu, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def test_panagiotopoulos_golden(cntc, oracle):
+    c = C.panag_case()
+    g = run_gpu(cntc, c, ire=640)
+    o = run_oracle(oracle, c)
+    compare(g, o, 1e-5, 1e-4, "case 9")
+    gold = C.PANAG_GOLDEN
+    fn, fx, fy, mz = g[1]
+    its = cntc.getiterations(640, 1)
+    print("case 9 gpu: forces", g[1], "iterations", its, "pen", cntc.getpenetration(640, 1),
+          flush=True)
+    assert round(fn, 3) == gold[0] and round(fy, 4) == gold[1] and round(mz, 3) == gold[2]
+    assert round(fy / (0.6 * fn), 4) == gold[4]
+    assert float("%.3e" % cntc.getpenetration(640, 1)) == gold[5]
+    assert float("%.3e" % g[2][0].max()) == gold[6]
+    el = g[3]
+    assert ((el > 0).sum(), (el == 1).sum(), (el == 2).sum()) == gold[7:10]
+    assert its[0] == its[2] == gold[10]
+
+
+CASES = {
+    "steady SteadyGS": dict(C.panag_case(), eps=1e-10, maxgs=999),
+    "steady GDsteady": dict(C.panag_case(), eps=1e-10, maxgs=999, gausei=5),
+    "shift TangCG": dict(C.panag_case(), eps=1e-10, maxgs=999, tang=1, force=0, cksi=2.0e-4),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_coupled_vs_oracle(cntc, oracle, name):
+    c = CASES[name]
+    ire = 641 + list(CASES).index(name)
+    g = run_gpu(cntc, c, ire=ire)
+    o = run_oracle(oracle, c)
+    compare(g, o, 1e-8, 1e-7, name)
+    its = cntc.getiterations(ire, 1)
+    print(name, "outer iterations gpu", its[0], "oracle", o[4].iters()[0], flush=True)
+    assert its[0] > 1
