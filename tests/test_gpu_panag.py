@@ -53,3 +53,12 @@ def test_coupled_vs_oracle(cntc, oracle, name):
     its = cntc.getiterations(ire, 1)
     print(name, "outer iterations gpu", its[0], "oracle", o[4].iters()[0], flush=True)
     assert its[0] > 1
+
+
+def test_coupled_shift_after_previous_case_refused(cntc):
+    """K != 0 in a shift that continues a previous case (P=0) needs A'_tn pv_n in the
+    right-hand side, which is not built: the call returns CNTC_err_other (-99) without
+    solving, and the same case with P=2 solves."""
+    c = dict(C.panag_case(), tang=1, force=0, cksi=2.0e-4, pvtime=0)
+    assert run_gpu(cntc, c, ire=644)[0] == -99
+    assert run_gpu(cntc, dict(c, pvtime=2), ire=645)[0] == 0
