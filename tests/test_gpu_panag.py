@@ -6,7 +6,8 @@ and in the tangential right-hand side (src/m_stang.f90:859-864).
 testbank case 9 (eps 1e-6, as the reference runs it) is checked against every figure that
 testbank.ref_out prints for it.  At eps 1e-10 the forces agree with the oracle to 1e-8 of Fn,
 the tractions to 1e-7 of max pn and the element states are identical, for SteadyGS, GDsteady
-and a shift by TangCG.
+and a shift by TangCG, and for SteadyGS with FP32 products in the refinement or with every
+product dense.
 """
 import numpy as np
 import pytest
@@ -62,3 +63,17 @@ def test_coupled_shift_after_previous_case_refused(cntc):
     c = dict(C.panag_case(), tang=1, force=0, cksi=2.0e-4, pvtime=0)
     assert run_gpu(cntc, c, ire=644)[0] == -99
     assert run_gpu(cntc, dict(c, pvtime=2), ire=645)[0] == 0
+
+
+def test_coupled_fp32_refinement_and_dense(cntc, oracle, monkeypatch):
+    """The coupled steady-rolling case with FP32 products inside the FP64 refinement
+    (CNTC_if_precision 1), and with every product dense on FP64 MFMA (CNTC_DENSE=2, the
+    cross blocks included): both match the oracle's FP64 solution to the same bar."""
+    c = dict(C.panag_case(), eps=1e-10, maxgs=999)
+    o = run_oracle(oracle, c)
+    g = run_gpu(cntc, dict(c, precision=1), ire=646)
+    compare(g, o, 1e-8, 1e-7, "coupled, FP32 products in the refinement")
+    monkeypatch.setenv("CNTC_DENSE", "2")
+    g = run_gpu(cntc, c, ire=647)
+    monkeypatch.delenv("CNTC_DENSE")
+    compare(g, o, 1e-8, 1e-7, "coupled, dense products")
