@@ -1783,14 +1783,6 @@ bool Engine::prepare(Patch& P)
     P.nu = P.ga * (in.poiss1 / in.gg1 + in.poiss2 / in.gg2) / 2.0;
     P.ak = (P.ga / 4.0) * ((1.0 - 2.0 * in.poiss1) / in.gg1 - (1.0 - 2.0 * in.poiss2) / in.gg2);
     P.nt_cpl = std::fabs(P.ak) >= 1e-6;
-    // K != 0 couples the normal and tangential problems.  Its right-hand side terms of previous
-    // tractions (A'_tn pv_n for shift and transient rolling with a previous case, and the
-    // leading-edge form (1b) at dq != dx) are not built: those combinations are refused.
-    if (P.nt_cpl && ((P.in.tang == 1 || P.in.tang == 2) && P.in.pvtime == 0)) {
-        R.ierror = ERR_OTHER;
-        last_error = "dissimilar materials (K != 0) with the previous tractions of shift or transient rolling are not supported";
-        return false;
-    }
     P.muscal = in.fstat;
     P.fntrue = in.fn;
     P.pen = in.pen;
@@ -1871,6 +1863,14 @@ bool Engine::prepare(Patch& P)
     if (newgrid && P.have_solution && (in.pvtime != 2 || in.iestim != 0)) {
         R.ierror = ERR_OTHER;
         last_error = "continuation needs an unchanged grid";
+        return false;
+    }
+    // K != 0 couples the normal and tangential problems.  Transient rolling after a previous
+    // case at dq != dx would need A'_tn pv_n on the interior but not at the leading edge
+    // (form 1b), which is not built: that combination is refused before the patch changes.
+    if (P.nt_cpl && in.tang == 2 && in.pvtime == 0 && (in.dq > 0 ? in.dq : dx) > dx * (1.0 + 1e-12)) {
+        R.ierror = ERR_OTHER;
+        last_error = "dissimilar materials (K != 0) in transient rolling at dq != dx after a previous case are not supported";
         return false;
     }
     P.mx = mx; P.my = my; P.n = mx * my;
@@ -3436,24 +3436,27 @@ static Task stang(Ctx C, int* ittang_out)
     } else {
         C.set(M_ALL, P.wsx, 0.0, P.wsy);
         C.axpy(M_INT, -1.0, P.hx, P.wsx, P.hy, P.wsy);
-        if (P.in.tang == 2) {
+        if (P.in.tang == 2 || (P.in.tang == 1 && P.in.pvtime == 0)) {
+            // the previous tractions' displacements A' pv_t (src/m_stang.f90:873): shifted
+            // coefficients in transient rolling, cs itself in a shift (cv = cs)
             double *ux = P.w[20], *uy = P.w[21];
-            C.conv_t(1, M_INT, ux, uy, P.pvx, P.pvy);
+            C.conv_t(P.in.tang == 2 ? 1 : 0, M_INT, ux, uy, P.pvx, P.pvy);
             C.axpy(M_INT, -1.0, ux, P.wsx, uy, P.wsy);
         }
     }
     if (P.nt_cpl) {
         // K != 0: the tangential displacements of the normal tractions, ws += A_tn ps_n - A'_tn
         // pv_n on the interior, with pv = ps in steady rolling; pv = 0 in shift and transient
-        // rolling without a previous case (stang_rhs, src/m_stang.f90:828-869)
+        // rolling without a previous case (stang_rhs, src/m_stang.f90:855-861)
         double *ux = P.w[20], *uy = P.w[21];
         int ik[2] = {0, 1}, jk[1] = {2};
         double* o[2] = {ux, uy};
         double* in[1] = {P.pn};
         C.conv(0, M_INT, 2, ik, o, 1, jk, in);
         C.axpy(M_INT, 1.0, ux, P.wsx, uy, P.wsy);
-        if (P.in.tang == 3) {
-            C.conv(1, M_INT, 2, ik, o, 1, jk, in);
+        if (P.in.tang == 3 || P.in.pvtime == 0) {
+            double* prev[1] = {P.in.tang == 3 ? P.pn : P.pvn};
+            C.conv(P.in.tang == 1 ? 0 : 1, M_INT, 2, ik, o, 1, jk, prev);
             C.axpy(M_INT, -1.0, ux, P.wsx, uy, P.wsy);
         }
     }
@@ -3939,6 +3942,10 @@ void Engine::get_field(const Patch& p, int what, double* out) const
 void Engine::get_eldiv(const Patch& p, int* out) const
 {
     if (p.n <= 0) return;
+    if (!p.el) {   // never laid out: no solution yet
+        for (int i = 0; i < p.n; i++) out[i] = 0;
+        return;
+    }
     if (p.state_on_host && p.host_state.size() >= (size_t)p.n) {
         for (int i = 0; i < p.n; i++) out[i] = (int8_t)p.host_state[i];
         return;

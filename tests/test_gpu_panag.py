@@ -56,11 +56,43 @@ def test_coupled_vs_oracle(cntc, oracle, name):
     assert its[0] > 1
 
 
-def test_coupled_shift_after_previous_case_refused(cntc):
-    """K != 0 in a shift that continues a previous case (P=0) needs A'_tn pv_n in the
-    right-hand side, which is not built: the call returns CNTC_err_other (-99) without
-    solving, and the same case with P=2 solves."""
-    c = dict(C.panag_case(), tang=1, force=0, cksi=2.0e-4, pvtime=0)
+def _sequence(base, tang, dq=None):
+    """Three cases on one problem: no previous case (P=2), then two continuations (P=0)."""
+    c = dict(base, tang=tang, force=0, eps=1e-10, maxgs=999)
+    if dq is not None:
+        c["dq"] = dq
+    return [dict(c, pvtime=2, cksi=2.0e-4), dict(c, pvtime=0, cksi=3.0e-4),
+            dict(c, pvtime=0, cksi=1.0e-4)]
+
+
+K0 = dict(C.panag_case(), nu2=0.1, g2=1.0)
+SEQUENCES = {
+    "shift K=0": _sequence(K0, 1),
+    "shift K=0.4": _sequence(C.panag_case(), 1),
+    "transient K=0.4 dq=dx": _sequence(C.panag_case(), 2, dq=1.1),
+}
+
+
+@pytest.mark.parametrize("name", list(SEQUENCES))
+def test_continuation_vs_oracle(cntc, oracle, name):
+    """Shift and transient rolling continuing a previous case (P=0): the previous tractions'
+    displacements A' pv enter the tangential right-hand side (src/m_stang.f90:855-873), with
+    the normal-tangential blocks when K != 0.  Every case of the sequence matches the oracle
+    run through the same sequence."""
+    ire = 650 + list(SEQUENCES).index(name)
+    P = oracle.Problem()
+    for k, c in enumerate(SEQUENCES[name]):
+        g = run_gpu(cntc, c, ire=ire)
+        assert P.calculate(*C.to_oracle(oracle, c)) == 0
+        o = (0, P.forces(), P.tractions(), P.eldiv(), P)
+        compare(g, o, 1e-8, 1e-7, "%s, case %d" % (name, k))
+
+
+def test_coupled_transient_ledge_after_previous_case_refused(cntc):
+    """K != 0 in transient rolling at dq != dx continuing a previous case needs A'_tn pv_n
+    on the interior but not at the leading edge, which is not built: the call returns
+    CNTC_err_other (-99) without solving, and the same case with P=2 solves."""
+    c = dict(C.panag_case(), tang=2, force=0, cksi=2.0e-4, dq=2.2, pvtime=0)
     assert run_gpu(cntc, c, ire=644)[0] == -99
     assert run_gpu(cntc, dict(c, pvtime=2), ire=645)[0] == 0
 
