@@ -109,3 +109,24 @@ def test_coupled_fp32_refinement_and_dense(cntc, oracle, monkeypatch):
     g = run_gpu(cntc, c, ire=647)
     monkeypatch.delenv("CNTC_DENSE")
     compare(g, o, 1e-8, 1e-7, "coupled, dense products")
+
+
+def test_coupled_in_a_batch_equals_single(cntc, oracle):
+    """Coupled (K != 0) and uncoupled patches solved together by cntc_calculatebatch: each
+    coupled patch's forces and tractions are bitwise those of the same patch solved alone,
+    and the batch's products of different shapes (normal from tangential, tangential from
+    normal, all nine blocks) share launches with the uncoupled ones."""
+    cs = [dict(C.panag_case(), eps=1e-10, maxgs=999, cphi=1e-4 * (k + 1)) for k in range(3)]
+    cs += [C.hertz_case(4.0 + k, 6.0, 14 + 3 * k, 12, 5e4, cksi=-0.0005 * (k + 1), eps=1e-8,
+                        maxgs=999, maxin=20) for k in range(3)]
+    single = [run_gpu(cntc, c, ire=660 + k) for k, c in enumerate(cs)]
+    for k, c in enumerate(cs):
+        C.apply_cntc(cntc, 670 + k, 1, c)
+    err = cntc.calculatebatch([670 + k for k in range(len(cs))], [1] * len(cs))
+    assert (err == 0).all()
+    for k in range(len(cs)):
+        f = cntc.getcontactforces(670 + k, 1)
+        pn, px, py = cntc.gettractions(670 + k, 1)
+        assert f == single[k][1], k
+        assert np.array_equal(pn.ravel(), single[k][2][0]) and np.array_equal(px.ravel(), single[k][2][1]), k
+    compare(single[0], run_oracle(oracle, cs[0]), 1e-8, 1e-7, "coupled patch of the batch")
